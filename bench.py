@@ -1,0 +1,140 @@
+#!/usr/bin/env python3
+"""Throughput of the HelioCubed MHD RHS on one MI355X: cell-updates/s per RK stage.
+
+One step = one classical RK4 step (4 stage preparations + 4 RHS evaluations
++ the stage accumulate) of the whole six-panel shell, inputs resident in HBM.
+Prints one JSON line. The CPU baseline is the FP64 oracle (oracle/) timed on
+this host's cores on one RK4 step of the same grid.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+# Workloads: grid, operator options, and the algorithmic bytes per unit of
+# the kernels that can dominate (unit = face for the face-flux kernels, cell
+# for the others).
+WORKLOADS = {
+    "shell-6x128x64x64-o2-minmod-roe": dict(n_rad=128, n_ang=64, limiter=1, riemann=2),
+    "shell-6x32x32x32-o2-minmod-rusanov": dict(n_rad=32, n_ang=32, limiter=1, riemann=1),
+}
+# Algorithmic HBM bytes per unit for each kernel (8 FP64 components = 64 B):
+#   face flux: read the 8-component cell state once + write the 8-component face flux;
+#   divergence: read 3 face fluxes + write K + read-modify-write U;
+#   prepare: read U0 and K, write W.
+ALGO_BYTES = {
+    "k_face_flux_0": 128, "k_face_flux_1": 128, "k_face_flux_2": 128,
+    "k_divergence": 64 * 3 + 64 + 128,
+    "k_prepare_valid": 64 * 3,
+}
+
+
+def units_per_launch(name, nr, na):
+    if name == "k_face_flux_0":
+        return 6 * (nr + 1) * na * na
+    if name in ("k_face_flux_1", "k_face_flux_2"):
+        return 6 * nr * (na + 1) * na
+    return 6 * nr * na * na
+
+
+def cpu_baseline(wl, ju, inner, dt):
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
+    threads = min(threads, 16)
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc  # the oracle is the timed CPU baseline only
+
+    O = orc.Oracle(wl["n_rad"], wl["n_ang"], limiter=wl["limiter"], riemann=wl["riemann"])
+    O.rk4(ju, inner, dt, 1)  # first call builds the OpenMP pool and caches
+    t0 = time.perf_counter()
+    O.rk4(ju, inner, dt, 1)
+    t = time.perf_counter() - t0
+    cells = 6 * wl["n_rad"] * wl["n_ang"] ** 2
+    return {"value": cells * 4 / t, "unit": "cell-updates/s per RK stage", "cores": threads,
+            "kind": "port",
+            "sample": f"1 RK4 step of the same shell on the CPU FP64 oracle, {threads} OpenMP threads, {t:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="shell-6x128x64x64-o2-minmod-roe", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(the operator shards by boxes across GPUs, which this build does not run)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    import heliocubed_amd as hc
+
+    wl = WORKLOADS[args.workload]
+    shell = hc.Shell(wl["n_rad"], wl["n_ang"])
+    opts = hc.Options(limiter=wl["limiter"], riemann_solver=wl["riemann"])
+    ju, inner = hc.init_state(shell, opts, 0, 20261203)
+    dt = 0.25 * (shell.inner_radius * np.pi / 2 / shell.n_ang) / 1.6
+    dev = torch.device("cuda", 0)
+    U = torch.from_numpy(ju).to(dev)
+    G = torch.from_numpy(inner).to(dev)
+    L = hc.Level(shell, opts)
+
+    for _ in range(args.warmup):
+        L.rk4_advance(U, G, dt, 1)
+    torch.cuda.synchronize()
+    L.profile(True)
+    t0 = time.perf_counter()
+    L.rk4_advance(U, G, dt, args.steps)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    prof = L.profile_read()
+    L.profile(False)
+    if not torch.isfinite(U).all():
+        raise RuntimeError("non-finite state after the timed steps")
+
+    cells = 6 * wl["n_rad"] * wl["n_ang"] ** 2
+    value = cells * 4 * args.steps / elapsed
+    dominant = max((k for k in prof if k in ALGO_BYTES), key=lambda k: prof[k][0])
+    ms, launches = prof[dominant]
+    avg_s = ms / launches / 1e3
+    algo = ALGO_BYTES[dominant] * units_per_launch(dominant, wl["n_rad"], wl["n_ang"])
+    achieved = algo / avg_s / 1e9
+    line = {
+        "metric": "cell-updates/s per RK stage (FP64)",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic seeded radial-wind shell with a Gaussian pressure pulse",
+        "config": {"workload": args.workload, "cells": cells, "panels": 6, "n_rad": wl["n_rad"],
+                   "n_ang": wl["n_ang"], "limiter": wl["limiter"], "riemann_solver": wl["riemann"],
+                   "precision": "fp64"},
+        "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "avg_launch_us": avg_s * 1e6},
+        "kernels_ms_per_step": {k: v[0] / args.steps for k, v in prof.items()},
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(wl, ju, inner, dt)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

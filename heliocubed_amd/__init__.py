@@ -1,0 +1,239 @@
+"""HelioCubed per-box MHD right-hand side on MI355X.
+
+Python host side of the C ABI in ``include/heliocubed.h``. The compute path is
+the HIP library ``heliocubed_amd/libheliocubed_amd.so`` (hand-written gfx950
+kernels). Device memory and streams come from PyTorch; nothing here falls back
+to a CPU implementation: if the library is missing the import of
+:func:`lib` raises.
+
+Reference interface mirrored (src/ = HelioCubed reference repository):
+
+* :class:`Level` plays the role of ``BoxOp_EulerCubedSphere``'s level entry
+  points ``preStageLevel`` (src/BoxOp_PreStage.H:427-501) and ``StageLevel``
+  (src/BoxOp_EulerCubedSphereImplem.H:955-1012) plus ``MBLevelRK::advance``
+  (src/MHD_LevelRK.H:22-29).
+* ``Level.ghost()`` is ``BoxOp_EulerCubedSphere::ghost()``
+  (src/BoxOp_EulerCubedSphere.H:376-390).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libheliocubed_amd.so")
+
+
+class HcShell(ctypes.Structure):
+    _fields_ = [
+        ("n_rad", ctypes.c_int),
+        ("n_ang", ctypes.c_int),
+        ("inner_radius", ctypes.c_double),
+        ("outer_radius", ctypes.c_double),
+        ("radial_stretch", ctypes.c_double),
+    ]
+
+
+class HcOptions(ctypes.Structure):
+    _fields_ = [
+        ("spatial_order", ctypes.c_int),
+        ("limiter", ctypes.c_int),
+        ("riemann_solver", ctypes.c_int),
+        ("entropy_fix", ctypes.c_double),
+        ("gamma", ctypes.c_double),
+    ]
+
+
+# (name, restype, argtypes) of every exported C-ABI symbol.
+_P = ctypes.c_void_p
+_D = ctypes.c_double
+_I = ctypes.c_int
+_SIGNATURES = [
+    ("hc_level_create", _I, [ctypes.POINTER(HcShell), ctypes.POINTER(HcOptions), ctypes.POINTER(_P)]),
+    ("hc_level_destroy", None, [_P]),
+    ("hc_level_ghost", _I, [_P]),
+    ("hc_level_valid_count", ctypes.c_size_t, [_P]),
+    ("hc_level_stage_count", ctypes.c_size_t, [_P]),
+    ("hc_level_inner_count", ctypes.c_size_t, [_P]),
+    ("hc_level_face_count", ctypes.c_size_t, [_P, _I]),
+    ("hc_prepare_stage", _I, [_P, _P, _P, _D, _P, _P, _P]),
+    ("hc_stage_rhs", _I, [_P, _P, _P, _P, _D, _P, _P, _P, _P]),
+    ("hc_rk4_advance", _I, [_P, _P, _P, _D, _I, _P]),
+    ("hc_init_state", _I, [ctypes.POINTER(HcShell), ctypes.POINTER(HcOptions), _I, ctypes.c_ulonglong, _P, _P]),
+    ("hc_geom_cell_detA", _D, [_I, _I, _I]),
+    ("hc_geom_face_normal", None, [_I, _I, _I, _I, _I, _P]),
+    ("hc_geom_ghost_donors", _I, [_I, _I, _I, _I, _P, _P, _P, _P]),
+    ("hc_level_profile", _I, [_P, _I]),
+    ("hc_level_profile_read", _I, [_P, _I, ctypes.c_char_p, _I, ctypes.POINTER(_D), ctypes.POINTER(ctypes.c_longlong)]),
+    ("hc_last_error", ctypes.c_char_p, []),
+]
+EXPORTED_SYMBOLS = [s[0] for s in _SIGNATURES]
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """Load the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                "(the HIP path has no CPU fallback)")
+        handle = ctypes.CDLL(LIB_PATH)
+        for name, res, args in _SIGNATURES:
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+    return _lib
+
+
+class HcError(RuntimeError):
+    pass
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        raise HcError(lib().hc_last_error().decode())
+
+
+@dataclass
+class Shell:
+    """Shell geometry (ParseInputs inner_radius/outer_radius/radial_stretch)."""
+
+    n_rad: int
+    n_ang: int
+    inner_radius: float = 21.5
+    outer_radius: float = 215.0
+    radial_stretch: float = 0.0
+
+    def c(self) -> HcShell:
+        return HcShell(self.n_rad, self.n_ang, self.inner_radius, self.outer_radius, self.radial_stretch)
+
+
+@dataclass
+class Options:
+    """Hot-path option subset of ParseInputs (src/Inputs_Parsing.H:71-123)."""
+
+    spatial_order: int = 2
+    limiter: int = 1
+    riemann_solver: int = 1
+    entropy_fix: float = 0.3
+    gamma: float = 5.0 / 3.0
+
+    def c(self) -> HcOptions:
+        return HcOptions(self.spatial_order, self.limiter, self.riemann_solver, self.entropy_fix, self.gamma)
+
+
+def init_state(shell: Shell, options: Options, config: int = 0, seed: int = 20261203):
+    """Seeded initial J*U level and inner ghost primitives as numpy arrays."""
+    import numpy as np
+
+    g = 2
+    na, nr = shell.n_ang, shell.n_rad
+    ju = np.zeros((6, 8, na, na, nr))
+    inner = np.zeros((6, 8, na + 2 * g, na + 2 * g, g))
+    sh, op = shell.c(), options.c()
+    _check(lib().hc_init_state(ctypes.byref(sh), ctypes.byref(op), config, seed,
+                               ju.ctypes.data, inner.ctypes.data))
+    return ju, inner
+
+
+def _ptr(t) -> int | None:
+    return None if t is None else t.data_ptr()
+
+
+class Level:
+    """A six-panel shell level on the current HIP device."""
+
+    def __init__(self, shell: Shell, options: Options):
+        import torch
+
+        self.shell = shell
+        self.options = options
+        self._handle = ctypes.c_void_p()
+        sh, op = shell.c(), options.c()
+        _check(lib().hc_level_create(ctypes.byref(sh), ctypes.byref(op), ctypes.byref(self._handle)))
+        self.device = torch.device("cuda", torch.cuda.current_device())
+
+    def __del__(self):
+        if getattr(self, "_handle", None) and self._handle.value:
+            lib().hc_level_destroy(self._handle)
+            self._handle = ctypes.c_void_p()
+
+    def ghost(self) -> int:
+        return lib().hc_level_ghost(self._handle)
+
+    @property
+    def valid_shape(self):
+        s = self.shell
+        return (6, 8, s.n_ang, s.n_ang, s.n_rad)
+
+    @property
+    def stage_shape(self):
+        s, g = self.shell, self.ghost()
+        return (6, 8, s.n_ang + 2 * g, s.n_ang + 2 * g, s.n_rad + 2 * g)
+
+    def face_shape(self, d: int):
+        s = self.shell
+        return (6, 8, s.n_ang + (d == 2), s.n_ang + (d == 1), s.n_rad + (d == 0))
+
+    @staticmethod
+    def _stream():
+        import torch
+
+        return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+    @staticmethod
+    def _require(t, shape, name):
+        import torch
+
+        if t is None:
+            return
+        if t.dtype != torch.float64 or not t.is_cuda or not t.is_contiguous() or tuple(t.shape) != tuple(shape):
+            raise ValueError(f"{name}: need a contiguous float64 device tensor of shape {tuple(shape)}")
+
+    def prepare_stage(self, U0, inner, W, K=None, a_dt: float = 0.0):
+        """preStageLevel: W <- ghosted Cartesian primitives of U0 + a_dt*K."""
+        self._require(U0, self.valid_shape, "U0")
+        self._require(K, self.valid_shape, "K")
+        self._require(W, self.stage_shape, "W")
+        g = self.ghost()
+        self._require(inner, (6, 8, self.shell.n_ang + 2 * g, self.shell.n_ang + 2 * g, g), "inner")
+        _check(lib().hc_prepare_stage(self._handle, _ptr(U0), _ptr(K), a_dt, _ptr(inner), _ptr(W), self._stream()))
+
+    def stage_rhs(self, W, K, U=None, b_dt: float = 0.0, fluxes=None):
+        """StageLevel: K <- d<JU>/dt; optionally U += b_dt*K and the face fluxes."""
+        self._require(W, self.stage_shape, "W")
+        self._require(K, self.valid_shape, "K")
+        self._require(U, self.valid_shape, "U")
+        f = list(fluxes) if fluxes is not None else [None, None, None]
+        for d in range(3):
+            self._require(f[d], self.face_shape(d), f"F{d}")
+        _check(lib().hc_stage_rhs(self._handle, _ptr(W), _ptr(K), _ptr(U), b_dt,
+                                  _ptr(f[0]), _ptr(f[1]), _ptr(f[2]), self._stream()))
+
+    def profile(self, enable: bool = True):
+        """Turn per-kernel HIP-event timing on (clearing old records) or off."""
+        _check(lib().hc_level_profile(self._handle, int(enable)))
+
+    def profile_read(self):
+        """{kernel name: (total ms, launches)} of the recorded launches."""
+        out = {}
+        slot = 0
+        name = ctypes.create_string_buffer(128)
+        while True:
+            ms, n = ctypes.c_double(), ctypes.c_longlong()
+            rc = lib().hc_level_profile_read(self._handle, slot, name, 128, ctypes.byref(ms), ctypes.byref(n))
+            if rc == 1:
+                return out
+            _check(rc)
+            out[name.value.decode()] = (ms.value, n.value)
+            slot += 1
+
+    def rk4_advance(self, U, inner, dt: float, nsteps: int = 1):
+        """MBLevelRK4::advance: nsteps classical RK4 steps in place on U."""
+        self._require(U, self.valid_shape, "U")
+        _check(lib().hc_rk4_advance(self._handle, _ptr(U), _ptr(inner), dt, nsteps, self._stream()))

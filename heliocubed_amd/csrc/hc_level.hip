@@ -1,0 +1,753 @@
+// Level driver and O2 kernels of the HelioCubed MHD right-hand side on
+// MI355X (gfx950). The C ABI is declared in include/heliocubed.h.
+//
+// Data: one ghosted FP64 array per stage for the whole shell (panel-major,
+// component-major, radial index unit-stride), so intra-panel box halos need
+// no copies; only the seam ghost ring of each panel is gathered.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/heliocubed.h"
+#include "hc_geometry.hpp"
+#include "hc_physics.hpp"
+
+namespace
+{
+thread_local std::string g_lastError;
+
+int fail(const std::string& msg)
+{
+    g_lastError = msg;
+    return -1;
+}
+
+#define HC_CHECK(call)                                                              \
+    do {                                                                            \
+        hipError_t e_ = (call);                                                     \
+        if (e_ != hipSuccess)                                                       \
+            throw std::runtime_error(std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+constexpr int kBlock = 256;
+
+// Geometry and sizes handed to every kernel by value.
+struct LevelView
+{
+    int nr, na, g, NI, NJ, NK;
+    long long cs, ps;      // stage component / panel strides
+    long long vcs, vps;    // valid component / panel strides
+    int limiter, roe;
+    double gamma, entropy;
+    const double* radial;  // [nr+2g+1][4]: rmid, dvol, area0, areaT for i = -g ..
+    const double* detA;    // [na][na]
+    const double* frame[3];// [6][fk][fj][10]: 9 frame entries + |normal|
+    const double* cellA;   // [6][NK][NJ][9]
+    const double* cellAi;  // [6][NK][NJ][9]
+    const int* edge;       // [6][2][2][4]: q, dq, sq, rev
+};
+
+__device__ __forceinline__ long long stageIndex(const LevelView& v, int p, int c, int i, int j, int k)
+{
+    return p*v.ps + c*v.cs + ((long long)(k+v.g)*v.NJ + (j+v.g))*v.NI + (i+v.g);
+}
+
+__device__ __forceinline__ long long validIndex(const LevelView& v, int p, int c, int i, int j, int k)
+{
+    return p*v.vps + c*v.vcs + ((long long)k*v.na + j)*v.nr + i;
+}
+
+__device__ __forceinline__ const double* rad(const LevelView& v, int i) { return v.radial + 4*(i+v.g); }
+
+__device__ __forceinline__ void loadStage(const LevelView& v, const double* W, int p, int i, int j, int k, double w[8])
+{
+    const long long base = stageIndex(v, p, 0, i, j, k);
+#pragma unroll
+    for (int c = 0; c < 8; ++c) w[c] = W[base + c*v.cs];
+}
+
+// 1. Valid cells: stage state, conversion to Cartesian primitives.
+__global__ void __launch_bounds__(kBlock)
+k_prepare_valid(LevelView v, const double* __restrict__ U0, const double* __restrict__ K,
+                double a_dt, double* __restrict__ W)
+{
+    const long long n = 6LL*v.na*v.na*v.nr;
+    const long long t = (long long)blockIdx.x*blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int i = t % v.nr;
+    long long r = t / v.nr;
+    const int j = r % v.na; r /= v.na;
+    const int k = r % v.na;
+    const int p = r / v.na;
+    double u[8], w[8];
+    const long long vb = validIndex(v, p, 0, i, j, k);
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+        u[c] = K ? U0[vb + c*v.vcs] + a_dt*K[vb + c*v.vcs] : U0[vb + c*v.vcs];
+    const double invJ = 1.0/(rad(v,i)[1]*v.detA[k*v.na + j]);
+    hc::conservedToPrimitive(w, u, invJ, v.gamma);
+    const long long sb = stageIndex(v, p, 0, i, j, k);
+#pragma unroll
+    for (int c = 0; c < 8; ++c) W[sb + c*v.cs] = w[c];
+}
+
+// 2. Seam ghost columns: weighted sums of valid donor columns, then floors.
+__global__ void __launch_bounds__(kBlock)
+k_seam_gather(LevelView v, int ncol, const long long* __restrict__ colDst,
+              const int* __restrict__ colStart, const long long* __restrict__ donorSrc,
+              const double* __restrict__ donorW, double* __restrict__ W)
+{
+    const long long t = (long long)blockIdx.x*blockDim.x + threadIdx.x;
+    if (t >= (long long)ncol*v.nr) return;
+    const int i = t % v.nr;
+    const int col = t / v.nr;
+    const int d0 = colStart[col], d1 = colStart[col+1];
+    const long long dst = colDst[col] + i;
+    for (int c = 0; c < 8; ++c)
+    {
+        double s = 0;
+        for (int d = d0; d < d1; ++d) s += donorW[d]*W[donorSrc[d] + c*v.cs + i];
+        if (c == 0) s = fmax(s, hc::kRhoFloor);
+        if (c == 4) s = fmax(s, 0.0);
+        W[dst + c*v.cs] = s;
+    }
+}
+
+// 3. Radial boundary ghosts: inner fixed primitives, outer copy of the last
+// layer scaled by (r_last/r)^2 (r for the two angular B components) in the
+// native basis, src/BoxOp_PreStage.H:105-141, src/BoxOp_SecondOrderPreStage.H:169-215.
+__global__ void __launch_bounds__(kBlock)
+k_radial_bc(LevelView v, const double* __restrict__ inner, double* __restrict__ W)
+{
+    const int n = 6*v.NJ*v.NK;
+    const int t = blockIdx.x*blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int jj = t % v.NJ;
+    const int kk = (t / v.NJ) % v.NK;
+    const int p = t / (v.NJ*v.NK);
+    const int j = jj - v.g, k = kk - v.g;
+    for (int ig = 0; ig < v.g; ++ig)
+        for (int c = 0; c < 8; ++c)
+            W[stageIndex(v, p, c, -v.g+ig, j, k)] = inner[((((long long)p*8+c)*v.NK+kk)*v.NJ+jj)*v.g+ig];
+    const double* A = v.cellA + 9LL*(((long long)p*v.NK+kk)*v.NJ+jj);
+    const double* Ai = v.cellAi + 9LL*(((long long)p*v.NK+kk)*v.NJ+jj);
+    double w[8], u[8];
+    loadStage(v, W, p, v.nr-1, j, k, w);
+    double kinetic = 0, magnetic = 0;
+    u[0] = w[0];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+    {
+        kinetic += 0.5*w[0]*w[1+d]*w[1+d];
+        magnetic += w[5+d]*w[5+d]/(8*hc::kPi);
+        double m = 0, b = 0;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) { m += Ai[3*d+q]*w[0]*w[1+q]; b += Ai[3*d+q]*w[5+q]; }
+        u[1+d] = m; u[5+d] = b;
+    }
+    u[4] = w[4]/(v.gamma-1)+kinetic+magnetic;
+    const double rlast = rad(v, v.nr-1)[0];
+    for (int i = v.nr; i < v.nr+v.g; ++i)
+    {
+        const double ratio = rlast/rad(v,i)[0];
+        double ug[8], cg[8], wg[8];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) ug[c] = u[c]*(ratio*ratio);
+        ug[6] = u[6]*ratio; ug[7] = u[7]*ratio;
+        cg[0] = ug[0]; cg[4] = ug[4];
+#pragma unroll
+        for (int grp = 0; grp < 2; ++grp)
+        {
+            const int st = grp == 0 ? 1 : 5;
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+            {
+                double s = 0;
+#pragma unroll
+                for (int q = 0; q < 3; ++q) s += A[3*d+q]*ug[st+q];
+                cg[st+d] = s;
+            }
+        }
+        hc::conservedToPrimitive(wg, cg, 1.0, v.gamma);
+        const long long sb = stageIndex(v, p, 0, i, j, k);
+#pragma unroll
+        for (int c = 0; c < 8; ++c) W[sb + c*v.cs] = wg[c];
+    }
+}
+
+// 4. Interior reconstructed state of every panel at its own seam faces,
+// src/BoxOp_SecondOrder.H:80-97. PS[p][d-1][s][c][t][i].
+__global__ void __launch_bounds__(kBlock)
+k_panel_states(LevelView v, const double* __restrict__ W, double* __restrict__ PS)
+{
+    const long long n = 24LL*v.na*v.nr;
+    const long long t = (long long)blockIdx.x*blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int i = t % v.nr;
+    long long r = t / v.nr;
+    const int tt = r % v.na; r /= v.na;
+    const int s = r % 2; r /= 2;
+    const int d = 1 + (r % 2);
+    const int p = r / 2;
+    const int f = s ? v.na : 0;
+    double mm[8], m[8], c[8], pp[8], out[8];
+    auto at = [&](int off, double o[8]) {
+        if (d == 1) loadStage(v, W, p, i, f+off, tt, o); else loadStage(v, W, p, i, tt, f+off, o);
+    };
+    if (s == 0) { at(-1, m); at(0, c); at(1, pp); hc::plmHighSide(out, m, c, pp, v.limiter); }
+    else { at(-2, mm); at(-1, m); at(0, c); hc::plmLowSide(out, mm, m, c, v.limiter); }
+    const long long base = ((((long long)(p*2+(d-1))*2+s)*8)*v.na + tt)*v.nr + i;
+    const long long cstride = (long long)v.na*v.nr;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) PS[base + q*cstride] = out[q];
+}
+
+// 5. Face fluxes of direction D for every face of the level.
+template <int D>
+__global__ void __launch_bounds__(kBlock)
+k_face_flux(LevelView v, const double* __restrict__ W, const double* __restrict__ PS,
+            double* __restrict__ F)
+{
+    const int fi = v.nr + (D == 0), fj = v.na + (D == 1), fk = v.na + (D == 2);
+    const long long n = 6LL*fi*fj*fk;
+    const long long t = (long long)blockIdx.x*blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int i = t % fi;
+    long long r = t / fi;
+    const int j = r % fj; r /= fj;
+    const int k = r % fk;
+    const int p = r / fk;
+    constexpr int e0 = D == 0, e1 = D == 1, e2 = D == 2;
+    double mm[8], m[8], c[8], pp[8], l[8], rr[8];
+    loadStage(v, W, p, i-2*e0, j-2*e1, k-2*e2, mm);
+    loadStage(v, W, p, i-e0, j-e1, k-e2, m);
+    loadStage(v, W, p, i, j, k, c);
+    loadStage(v, W, p, i+e0, j+e1, k+e2, pp);
+    hc::plmFaceStates(l, rr, mm, m, c, pp, v.limiter);
+    if (D > 0)
+    {
+        const int f = D == 1 ? j : k, tt = D == 1 ? k : j;
+        if (f == 0 || f == v.na)
+        {
+            const int s = f == 0 ? 0 : 1;
+            const int* e = v.edge + 4*((p*2 + (D-1))*2 + s);
+            const int tq = e[3] ? v.na-1-tt : tt;
+            const long long base = ((((long long)(e[0]*2+(e[1]-1))*2+e[2])*8)*v.na + tq)*v.nr + i;
+            const long long cstride = (long long)v.na*v.nr;
+            double* dst = s == 0 ? l : rr;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) dst[q] = PS[base + q*cstride];
+        }
+    }
+    const double* fr = v.frame[D] + 10LL*(((long long)p*fk + k)*fj + j);
+    const double area = (D == 0 ? rad(v,i)[2] : rad(v,i)[3])*fr[9];
+    double out[8];
+    hc::o2FaceFlux(out, l, rr, fr, area, v.gamma, v.entropy, v.roe != 0);
+    const long long cstride = (long long)fi*fj*fk;
+    const long long base = (long long)p*8*cstride + ((long long)k*fj + j)*fi + i;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) F[base + q*cstride] = out[q];
+}
+
+// 6. Seam flux reconciliation 1/2 (F_local + sign F_adjacent), each physical
+// seam face once, src/MHD_AMR.H:1484-1496.
+__global__ void __launch_bounds__(kBlock)
+k_seam_reconcile(LevelView v, double* __restrict__ F1, double* __restrict__ F2)
+{
+    const long long n = 24LL*v.na*v.nr;
+    const long long t = (long long)blockIdx.x*blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int i = t % v.nr;
+    long long r = t / v.nr;
+    const int tt = r % v.na; r /= v.na;
+    const int s = r % 2; r /= 2;
+    const int d = 1 + (r % 2);
+    const int p = r / 2;
+    const int* e = v.edge + 4*((p*2 + (d-1))*2 + s);
+    const int q = e[0], dq = e[1], sq = e[2];
+    const int self = (p*2 + (d-1))*2 + s, other = (q*2 + (dq-1))*2 + sq;
+    if (self > other) return;
+    const double sign = (s != sq) ? 1.0 : -1.0;
+    const int tq = e[3] ? v.na-1-tt : tt;
+    auto faceBase = [&](int dd, int pp, int f, int tg, double*& arr, long long& cstr) {
+        const int fj = v.na + (dd == 1), fk = v.na + (dd == 2);
+        cstr = (long long)v.nr*fj*fk;
+        const int jj = dd == 1 ? f : tg, kk = dd == 1 ? tg : f;
+        arr = dd == 1 ? F1 : F2;
+        return (long long)pp*8*cstr + ((long long)kk*fj + jj)*v.nr + i;
+    };
+    double* Fa; double* Fb; long long csa, csb;
+    const long long a = faceBase(d, p, s ? v.na : 0, tt, Fa, csa);
+    const long long b = faceBase(dq, q, sq ? v.na : 0, tq, Fb, csb);
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+    {
+        const double fa = Fa[a + c*csa], fb = Fb[b + c*csb];
+        Fa[a + c*csa] = 0.5*(fa + sign*fb);
+        Fb[b + c*csb] = 0.5*(fb + sign*fa);
+    }
+}
+
+// 7. Flux divergence into K, fused with the RK accumulate U += b_dt*K.
+__global__ void __launch_bounds__(kBlock)
+k_divergence(LevelView v, const double* __restrict__ F0, const double* __restrict__ F1,
+             const double* __restrict__ F2, double* __restrict__ K, double* __restrict__ U, double b_dt)
+{
+    const long long n = 6LL*v.na*v.na*v.nr;
+    const long long t = (long long)blockIdx.x*blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int i = t % v.nr;
+    long long r = t / v.nr;
+    const int j = r % v.na; r /= v.na;
+    const int k = r % v.na;
+    const int p = r / v.na;
+    const double inv[3] = {-1.0/(1.0/v.nr), -1.0/(1.0/v.na), -1.0/(1.0/v.na)};
+    const long long cs0 = (long long)(v.nr+1)*v.na*v.na, cs1 = (long long)v.nr*(v.na+1)*v.na;
+    const long long b0 = p*8*cs0 + ((long long)k*v.na + j)*(v.nr+1) + i;
+    const long long b1 = p*8*cs1 + ((long long)k*(v.na+1) + j)*v.nr + i;
+    const long long b2 = p*8*cs1 + ((long long)k*v.na + j)*v.nr + i;
+    const long long vb = validIndex(v, p, 0, i, j, k);
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+    {
+        double s = 0;
+        s += inv[0]*(F0[b0 + c*cs0 + 1] - F0[b0 + c*cs0]);
+        s += inv[1]*(F1[b1 + c*cs1 + v.nr] - F1[b1 + c*cs1]);
+        s += inv[2]*(F2[b2 + c*cs1 + (long long)v.na*v.nr] - F2[b2 + c*cs1]);
+        K[vb + c*v.vcs] = s;
+        if (U) U[vb + c*v.vcs] += b_dt*s;
+    }
+}
+
+inline unsigned blocksFor(long long n) { return (unsigned)((n + kBlock - 1)/kBlock); }
+
+template <typename T>
+T* deviceCopy(const std::vector<T>& host)
+{
+    T* p = nullptr;
+    HC_CHECK(hipMalloc(&p, std::max<size_t>(1, host.size())*sizeof(T)));
+    if (!host.empty()) HC_CHECK(hipMemcpy(p, host.data(), host.size()*sizeof(T), hipMemcpyHostToDevice));
+    return p;
+}
+} // namespace
+
+// Optional per-kernel HIP-event timing on the launch stream.
+struct KernelProfile
+{
+    bool enabled = false;
+    std::vector<std::string> names;
+    std::vector<std::vector<std::pair<hipEvent_t, hipEvent_t>>> events;
+
+    int slot(const char* name)
+    {
+        for (size_t i = 0; i < names.size(); ++i) if (names[i] == name) return (int)i;
+        names.emplace_back(name);
+        events.emplace_back();
+        return (int)names.size()-1;
+    }
+    template <typename Launch>
+    void run(const char* name, hipStream_t s, Launch&& launch)
+    {
+        if (!enabled) { launch(); return; }
+        hipEvent_t a, b;
+        HC_CHECK(hipEventCreate(&a));
+        HC_CHECK(hipEventCreate(&b));
+        HC_CHECK(hipEventRecord(a, s));
+        launch();
+        HC_CHECK(hipEventRecord(b, s));
+        events[slot(name)].emplace_back(a, b);
+    }
+    void clear()
+    {
+        for (auto& v : events) for (auto& e : v) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
+        for (auto& v : events) v.clear();
+    }
+    ~KernelProfile() { clear(); }
+};
+
+struct hc_level
+{
+    KernelProfile profile;
+    hc::ShellDims dims{};
+    hc_options opt{};
+    LevelView view{};
+    int ncol = 0;
+    long long* colDst = nullptr;
+    int* colStart = nullptr;
+    long long* donorSrc = nullptr;
+    double* donorW = nullptr;
+    std::vector<void*> owned;
+    // workspace
+    double* W = nullptr;
+    double* F[3] = {nullptr, nullptr, nullptr};
+    double* PS = nullptr;
+    double* U0 = nullptr;
+    double* K = nullptr;
+
+    ~hc_level()
+    {
+        for (void* p : owned) (void)hipFree(p);
+    }
+    template <typename T> T* own(T* p) { owned.push_back(p); return p; }
+
+    size_t validCount() const { return (size_t)6*8*dims.na*dims.na*dims.nr; }
+    size_t stageCount() const { return (size_t)6*8*view.NI*view.NJ*view.NK; }
+    size_t innerCount() const { return (size_t)6*8*view.NJ*view.NK*dims.g; }
+    size_t faceCount(int d) const
+    { return (size_t)6*8*(dims.nr+(d==0))*(dims.na+(d==1))*(dims.na+(d==2)); }
+
+    void ensureWorkspace()
+    {
+        if (W) return;
+        HC_CHECK(hipMalloc(&W, stageCount()*sizeof(double))); owned.push_back(W);
+        HC_CHECK(hipMemset(W, 0, stageCount()*sizeof(double)));
+        for (int d = 0; d < 3; ++d) { HC_CHECK(hipMalloc(&F[d], faceCount(d)*sizeof(double))); owned.push_back(F[d]); }
+        HC_CHECK(hipMalloc(&PS, 24ull*8*dims.na*dims.nr*sizeof(double))); owned.push_back(PS);
+        HC_CHECK(hipMalloc(&U0, validCount()*sizeof(double))); owned.push_back(U0);
+        HC_CHECK(hipMalloc(&K, validCount()*sizeof(double))); owned.push_back(K);
+    }
+};
+
+namespace
+{
+void buildLevel(hc_level& L)
+{
+    const int nr = L.dims.nr, na = L.dims.na, g = L.dims.g;
+    LevelView& v = L.view;
+    v.nr = nr; v.na = na; v.g = g;
+    v.NI = nr+2*g; v.NJ = na+2*g; v.NK = na+2*g;
+    v.cs = (long long)v.NI*v.NJ*v.NK; v.ps = 8*v.cs;
+    v.vcs = (long long)nr*na*na; v.vps = 8*v.vcs;
+    v.limiter = L.opt.limiter;
+    v.roe = L.opt.riemann_solver == 2;
+    v.gamma = L.opt.gamma;
+    v.entropy = L.opt.entropy_fix;
+
+    std::vector<double> radial(4*(nr+2*g+1));
+    for (int i = -g; i <= nr+g; ++i)
+    {
+        const hc::RadialCell c = hc::radialCell(L.dims, i);
+        double* r = &radial[4*(i+g)];
+        r[0] = c.rmid; r[1] = c.dvol; r[2] = c.area0; r[3] = c.areaT;
+    }
+    v.radial = L.own(deviceCopy(radial));
+
+    std::vector<double> detA((size_t)na*na);
+    for (int k = 0; k < na; ++k) for (int j = 0; j < na; ++j) detA[(size_t)k*na+j] = hc::cellDetA(na, j, k);
+    v.detA = L.own(deviceCopy(detA));
+
+    for (int d = 0; d < 3; ++d)
+    {
+        const int fj = na + (d == 1), fk = na + (d == 2);
+        std::vector<double> fr(10ull*6*fj*fk);
+        for (int p = 0; p < 6; ++p) for (int k = 0; k < fk; ++k) for (int j = 0; j < fj; ++j)
+        {
+            const hc::Vec3 n = hc::faceNormal(na, p, d, j, k);
+            const hc::FaceFrame F = hc::makeFaceFrame(n.x);
+            double* o = &fr[10*(((size_t)p*fk+k)*fj+j)];
+            for (int a = 0; a < 3; ++a) for (int b = 0; b < 3; ++b) o[3*a+b] = F.f[a][b];
+            o[9] = F.norm;
+        }
+        v.frame[d] = L.own(deviceCopy(fr));
+    }
+
+    const int NJ = v.NJ, NK = v.NK;
+    std::vector<double> A(9ull*6*NJ*NK), Ai(9ull*6*NJ*NK);
+    const double h = 1.0/na;
+    for (int p = 0; p < 6; ++p) for (int kk = 0; kk < NK; ++kk) for (int jj = 0; jj < NJ; ++jj)
+    {
+        double a[3][3], ai[3][3];
+        hc::pointBasis(p, hc::xiToAngle((jj-g+0.5)*h), hc::xiToAngle((kk-g+0.5)*h), a);
+        hc::invert3(a, ai);
+        const size_t x = 9*(((size_t)p*NK+kk)*NJ+jj);
+        for (int r = 0; r < 3; ++r) for (int c = 0; c < 3; ++c) { A[x+3*r+c] = a[r][c]; Ai[x+3*r+c] = ai[r][c]; }
+    }
+    v.cellA = L.own(deviceCopy(A));
+    v.cellAi = L.own(deviceCopy(Ai));
+
+    std::vector<int> edge(6*2*2*4);
+    for (int p = 0; p < 6; ++p) for (int d = 1; d < 3; ++d) for (int s = 0; s < 2; ++s)
+    {
+        const hc::SeamEdge e = hc::seamEdge(na, p, d, s);
+        int* o = &edge[4*((p*2+(d-1))*2+s)];
+        o[0] = e.q; o[1] = e.dq; o[2] = e.sq; o[3] = e.rev;
+    }
+    v.edge = L.own(deviceCopy(edge));
+
+    std::vector<long long> colDst, donorSrc;
+    std::vector<int> colStart{0};
+    std::vector<double> donorW;
+    auto stageOff = [&](int p, int j, int k) {
+        return (long long)p*v.ps + ((long long)(k+g)*NJ + (j+g))*v.NI + g;
+    };
+    for (int p = 0; p < 6; ++p) for (int k = -g; k < na+g; ++k) for (int j = -g; j < na+g; ++j)
+    {
+        if (j >= 0 && j < na && k >= 0 && k < na) continue;
+        for (const auto& d : hc::ghostColumnDonors(na, p, j, k))
+        {
+            donorSrc.push_back(stageOff(d.q, d.j, d.k));
+            donorW.push_back(d.w);
+        }
+        colDst.push_back(stageOff(p, j, k));
+        colStart.push_back((int)donorW.size());
+    }
+    L.ncol = (int)colDst.size();
+    L.colDst = L.own(deviceCopy(colDst));
+    L.colStart = L.own(deviceCopy(colStart));
+    L.donorSrc = L.own(deviceCopy(donorSrc));
+    L.donorW = L.own(deviceCopy(donorW));
+}
+
+void prepareStage(hc_level& L, const double* U0, const double* K, double a_dt,
+                  const double* inner, double* W, hipStream_t s)
+{
+    const LevelView& v = L.view;
+    KernelProfile& P = L.profile;
+    P.run("k_prepare_valid", s, [&] {
+        hipLaunchKernelGGL(k_prepare_valid, dim3(blocksFor(6LL*v.na*v.na*v.nr)), dim3(kBlock), 0, s, v, U0, K, a_dt, W); });
+    P.run("k_seam_gather", s, [&] {
+        hipLaunchKernelGGL(k_seam_gather, dim3(blocksFor((long long)L.ncol*v.nr)), dim3(kBlock), 0, s,
+                           v, L.ncol, L.colDst, L.colStart, L.donorSrc, L.donorW, W); });
+    P.run("k_radial_bc", s, [&] {
+        hipLaunchKernelGGL(k_radial_bc, dim3(blocksFor(6LL*v.NJ*v.NK)), dim3(kBlock), 0, s, v, inner, W); });
+    HC_CHECK(hipGetLastError());
+}
+
+void stageRhs(hc_level& L, const double* W, double* K, double* U, double b_dt,
+              double* F0, double* F1, double* F2, hipStream_t s)
+{
+    L.ensureWorkspace();
+    const LevelView& v = L.view;
+    double* F[3] = {F0 ? F0 : L.F[0], F1 ? F1 : L.F[1], F2 ? F2 : L.F[2]};
+    KernelProfile& P = L.profile;
+    P.run("k_panel_states", s, [&] {
+        hipLaunchKernelGGL(k_panel_states, dim3(blocksFor(24LL*v.na*v.nr)), dim3(kBlock), 0, s, v, W, L.PS); });
+    P.run("k_face_flux_0", s, [&] {
+        hipLaunchKernelGGL(k_face_flux<0>, dim3(blocksFor(6LL*(v.nr+1)*v.na*v.na)), dim3(kBlock), 0, s, v, W, L.PS, F[0]); });
+    P.run("k_face_flux_1", s, [&] {
+        hipLaunchKernelGGL(k_face_flux<1>, dim3(blocksFor(6LL*v.nr*(v.na+1)*v.na)), dim3(kBlock), 0, s, v, W, L.PS, F[1]); });
+    P.run("k_face_flux_2", s, [&] {
+        hipLaunchKernelGGL(k_face_flux<2>, dim3(blocksFor(6LL*v.nr*v.na*(v.na+1))), dim3(kBlock), 0, s, v, W, L.PS, F[2]); });
+    P.run("k_seam_reconcile", s, [&] {
+        hipLaunchKernelGGL(k_seam_reconcile, dim3(blocksFor(24LL*v.na*v.nr)), dim3(kBlock), 0, s, v, F[1], F[2]); });
+    P.run("k_divergence", s, [&] {
+        hipLaunchKernelGGL(k_divergence, dim3(blocksFor(6LL*v.na*v.na*v.nr)), dim3(kBlock), 0, s,
+                           v, F[0], F[1], F[2], K, U, b_dt); });
+    HC_CHECK(hipGetLastError());
+}
+
+// ---- Seeded initial data ---------------------------------------------------
+struct SplitMix
+{
+    unsigned long long s;
+    unsigned long long next()
+    {
+        unsigned long long z = (s += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30))*0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27))*0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+    double uniform() { return (next() >> 11)*(1.0/9007199254740992.0); }
+};
+
+struct WindState
+{
+    double r0, r1, gamma;
+    double center[3], width, pulse;
+    bool hemispheric;
+    // Cartesian primitive state at position X.
+    void at(const double X[3], double w[8]) const
+    {
+        const double r = std::sqrt(X[0]*X[0]+X[1]*X[1]+X[2]*X[2]);
+        const double rho = (r0/r)*(r0/r);
+        const double sign = (!hemispheric || X[2] > 0) ? 1.0 : -1.0;
+        double d2 = 0;
+        for (int a = 0; a < 3; ++a) d2 += (X[a]-center[a])*(X[a]-center[a]);
+        const double p = 0.1*rho*(1.0 + pulse*std::exp(-d2/(width*width)));
+        w[0] = rho; w[4] = p;
+        for (int a = 0; a < 3; ++a) { w[1+a] = X[a]/r; w[5+a] = sign*rho*X[a]/r; }
+    }
+};
+} // namespace
+
+extern "C" {
+
+const char* hc_last_error(void) { return g_lastError.c_str(); }
+
+int hc_level_create(const hc_shell* shell, const hc_options* options, hc_level** out)
+{
+    if (!shell || !options || !out) return fail("hc_level_create: null argument");
+    if (shell->n_rad < 2 || shell->n_ang < 4) return fail("hc_level_create: need n_rad >= 2 and n_ang >= 4");
+    if (!(shell->outer_radius > shell->inner_radius && shell->inner_radius > 0))
+        return fail("hc_level_create: need 0 < inner_radius < outer_radius");
+    if (options->spatial_order != 2) return fail("hc_level_create: only spatial_order 2 is implemented");
+    if (options->riemann_solver != 1 && options->riemann_solver != 2)
+        return fail("hc_level_create: riemann_solver must be 1 (Rusanov) or 2 (Roe)");
+    if (options->limiter < 0 || options->limiter > 3) return fail("hc_level_create: limiter must be 0..3");
+    try
+    {
+        std::unique_ptr<hc_level> L(new hc_level());
+        L->dims = {shell->n_rad, shell->n_ang, 2, shell->inner_radius, shell->outer_radius, shell->radial_stretch};
+        L->opt = *options;
+        buildLevel(*L);
+        *out = L.release();
+        return 0;
+    }
+    catch (const std::exception& e) { return fail(std::string("hc_level_create: ") + e.what()); }
+}
+
+void hc_level_destroy(hc_level* level) { delete level; }
+
+int hc_level_ghost(const hc_level* level) { return level ? level->dims.g : -1; }
+size_t hc_level_valid_count(const hc_level* level) { return level ? level->validCount() : 0; }
+size_t hc_level_stage_count(const hc_level* level) { return level ? level->stageCount() : 0; }
+size_t hc_level_inner_count(const hc_level* level) { return level ? level->innerCount() : 0; }
+size_t hc_level_face_count(const hc_level* level, int d)
+{ return level && d >= 0 && d < 3 ? level->faceCount(d) : 0; }
+
+int hc_prepare_stage(hc_level* level, const double* U0, const double* K, double a_dt,
+                     const double* inner, double* W, hc_stream stream)
+{
+    if (!level || !U0 || !inner || !W) return fail("hc_prepare_stage: null argument");
+    try { prepareStage(*level, U0, K, a_dt, inner, W, static_cast<hipStream_t>(stream)); return 0; }
+    catch (const std::exception& e) { return fail(std::string("hc_prepare_stage: ") + e.what()); }
+}
+
+int hc_stage_rhs(hc_level* level, const double* W, double* K, double* U, double b_dt,
+                 double* F0, double* F1, double* F2, hc_stream stream)
+{
+    if (!level || !W || !K) return fail("hc_stage_rhs: null argument");
+    try { stageRhs(*level, W, K, U, b_dt, F0, F1, F2, static_cast<hipStream_t>(stream)); return 0; }
+    catch (const std::exception& e) { return fail(std::string("hc_stage_rhs: ") + e.what()); }
+}
+
+int hc_rk4_advance(hc_level* level, double* U, const double* inner, double dt, int nsteps, hc_stream stream)
+{
+    if (!level || !U || !inner) return fail("hc_rk4_advance: null argument");
+    if (nsteps < 0) return fail("hc_rk4_advance: nsteps must be >= 0");
+    try
+    {
+        hc_level& L = *level;
+        L.ensureWorkspace();
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        const double a[4] = {0.0, 0.5, 0.5, 1.0};
+        const double b[4] = {1.0/6, 1.0/3, 1.0/3, 1.0/6};
+        for (int n = 0; n < nsteps; ++n)
+        {
+            HC_CHECK(hipMemcpyAsync(L.U0, U, L.validCount()*sizeof(double), hipMemcpyDeviceToDevice, s));
+            for (int st = 0; st < 4; ++st)
+            {
+                prepareStage(L, L.U0, st ? L.K : nullptr, dt*a[st], inner, L.W, s);
+                stageRhs(L, L.W, L.K, U, dt*b[st], nullptr, nullptr, nullptr, s);
+            }
+        }
+        return 0;
+    }
+    catch (const std::exception& e) { return fail(std::string("hc_rk4_advance: ") + e.what()); }
+}
+
+int hc_init_state(const hc_shell* shell, const hc_options* options, int config,
+                  unsigned long long seed, double* JU, double* inner)
+{
+    if (!shell || !options || !JU || !inner) return fail("hc_init_state: null argument");
+    if (config < 0 || config > 2) return fail("hc_init_state: config must be 0, 1 or 2");
+    const int nr = shell->n_rad, na = shell->n_ang, g = 2;
+    const hc::ShellDims dims{nr, na, g, shell->inner_radius, shell->outer_radius, shell->radial_stretch};
+    SplitMix rng{seed};
+    WindState ws;
+    ws.r0 = shell->inner_radius; ws.r1 = shell->outer_radius; ws.gamma = options->gamma;
+    const double cz = 2.0*rng.uniform()-1.0, phi = 2.0*hc::kPi*rng.uniform();
+    const double rc = 0.5*(ws.r0+ws.r1), sz = std::sqrt(std::max(0.0, 1.0-cz*cz));
+    ws.center[0] = rc*sz*std::cos(phi); ws.center[1] = rc*sz*std::sin(phi); ws.center[2] = rc*cz;
+    ws.width = 0.1*(ws.r1-ws.r0);
+    ws.pulse = config == 2 ? 0.0 : 0.2;
+    ws.hemispheric = config == 0;
+    const double h = 1.0/na, gamma = options->gamma;
+    const int NJ = na+2*g, NK = na+2*g;
+    for (int p = 0; p < 6; ++p) for (int k = -g; k < na+g; ++k) for (int j = -g; j < na+g; ++j)
+    {
+        const hc::Vec3 dir = hc::panelPoint(p, hc::xiToAngle((j+0.5)*h), hc::xiToAngle((k+0.5)*h));
+        const bool valid = j >= 0 && j < na && k >= 0 && k < na;
+        const double detA = valid ? hc::cellDetA(na, j, k) : 0.0;
+        for (int i = -g; i < nr; ++i)
+        {
+            if (i >= 0 && !valid) continue;
+            const hc::RadialCell rc2 = hc::radialCell(dims, i);
+            const double X[3] = {rc2.rmid*dir.x[0], rc2.rmid*dir.x[1], rc2.rmid*dir.x[2]};
+            double w[8];
+            ws.at(X, w);
+            if (i < 0)
+            {
+                for (int c = 0; c < 8; ++c)
+                    inner[((((size_t)p*8+c)*NK+(k+g))*NJ+(j+g))*g+(i+g)] = w[c];
+                continue;
+            }
+            double u[8];
+            double v2 = 0, b2 = 0;
+            u[0] = w[0];
+            for (int a = 0; a < 3; ++a)
+            {
+                u[1+a] = w[0]*w[1+a]; u[5+a] = w[5+a];
+                v2 += w[1+a]*w[1+a]; b2 += w[5+a]*w[5+a];
+            }
+            u[4] = w[4]/(gamma-1.0) + 0.5*w[0]*v2 + b2/(8*hc::kPi);
+            const double J = rc2.dvol*detA;
+            for (int c = 0; c < 8; ++c)
+                JU[((((size_t)p*8+c)*na+k)*na+j)*nr+i] = J*u[c];
+        }
+    }
+    return 0;
+}
+
+int hc_level_profile(hc_level* level, int enable)
+{
+    if (!level) return fail("hc_level_profile: null level");
+    level->profile.clear();
+    level->profile.enabled = enable != 0;
+    return 0;
+}
+
+int hc_level_profile_read(hc_level* level, int slot, char* name, int name_len, double* total_ms, long long* launches)
+{
+    if (!level) return fail("hc_level_profile_read: null level");
+    KernelProfile& P = level->profile;
+    if (slot < 0 || slot >= (int)P.names.size()) return 1;
+    try
+    {
+        double ms = 0;
+        for (auto& e : P.events[slot])
+        {
+            HC_CHECK(hipEventSynchronize(e.second));
+            float t = 0;
+            HC_CHECK(hipEventElapsedTime(&t, e.first, e.second));
+            ms += t;
+        }
+        if (name && name_len > 0) { std::strncpy(name, P.names[slot].c_str(), name_len-1); name[name_len-1] = 0; }
+        if (total_ms) *total_ms = ms;
+        if (launches) *launches = (long long)P.events[slot].size();
+        return 0;
+    }
+    catch (const std::exception& e) { return fail(std::string("hc_level_profile_read: ") + e.what()); }
+}
+
+double hc_geom_cell_detA(int n_ang, int j, int k) { return hc::cellDetA(n_ang, j, k); }
+
+void hc_geom_face_normal(int n_ang, int p, int d, int j, int k, double* normal3)
+{
+    const hc::Vec3 n = hc::faceNormal(n_ang, p, d, j, k);
+    for (int a = 0; a < 3; ++a) normal3[a] = n.x[a];
+}
+
+int hc_geom_ghost_donors(int n_ang, int p, int j, int k, int* q, int* jj, int* kk, double* w)
+{
+    const auto d = hc::ghostColumnDonors(n_ang, p, j, k);
+    for (size_t x = 0; x < d.size(); ++x) { q[x] = d[x].q; jj[x] = d[x].j; kk[x] = d[x].k; w[x] = d[x].w; }
+    return (int)d.size();
+}
+} // extern "C"

@@ -1,0 +1,139 @@
+/*
+ * HelioCubed per-box MHD right-hand side on MI355X: C ABI.
+ *
+ * The reference (HelioCubed, C++17 on the LBNL Proto framework) evaluates the
+ * ideal-MHD RHS of a six-panel cubed-sphere shell through the BoxOp member
+ *   BoxOp_EulerCubedSphere::operator()(Rhs, fluxes, USph, dt, scale, counter,
+ *                                      cartesianPrimitives, activeFaces)
+ *   (src/BoxOp_EulerCubedSphere.H:412-421, src/BoxOp_EulerCubedSphereImplem.H:4-402)
+ * called once per box per RK stage by StageLevel
+ *   (src/BoxOp_EulerCubedSphereImplem.H:955-1012)
+ * after preStageLevel has filled the ghost cells
+ *   (src/BoxOp_PreStage.H:427-501, src/BoxOp_SecondOrderPreStage.H:6-77),
+ * with the classical RK4 of src/MHD_LevelRK.H:22-29 / src/MHD_AMRRungeKutta.H:5-41.
+ * The functions below replace those entry points with stream-ordered device
+ * calls. All array arguments are device pointers to double unless stated.
+ *
+ * Memory layout (FP64, component-major like Proto BoxData, radial index i
+ * fastest). Components: rho, v or rho*v (3, Cartesian), p or E, B (3, Cartesian).
+ *   valid level   V[6][8][na][na][nr]            index ((((p*8+c)*na+k)*na+j)*nr+i)
+ *   stage level   W[6][8][na+2g][na+2g][nr+2g]   cell (i,j,k) at ((((p*8+c)*NK+k+g)*NJ+j+g)*NI+i+g)
+ *   inner ghosts  G[6][8][na+2g][na+2g][g]       Cartesian primitives of the cells i=-g..-1
+ *   face fluxes   F_d[6][8][na+(d==2)][na+(d==1)][nr+(d==0)]; face index m is the low face of cell m
+ * Errors: every function returns 0 on success and a negative code on error;
+ * hc_last_error() returns the message of the calling thread's last error.
+ */
+#ifndef HELIOCUBED_H
+#define HELIOCUBED_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Shell geometry: inner_radius/outer_radius/radial_stretch are the
+ * reference's ParseInputs fields of the same names (src/Inputs_Parsing.H:63-65);
+ * n_rad x n_ang x n_ang cells per panel. */
+typedef struct hc_shell
+{
+    int n_rad;
+    int n_ang;
+    double inner_radius;
+    double outer_radius;
+    double radial_stretch;
+} hc_shell;
+
+/* Operator options: the hot-path subset of ParseInputs
+ * (src/Inputs_Parsing.H:71-123). spatial_order 2 selects the PLM operator
+ * (src/BoxOp_SecondOrder.H:155-370); limiter 0 none, 1 or 2 minmod, 3 MC;
+ * riemann_solver 1 Rusanov, 2 Roe 8-wave. */
+typedef struct hc_options
+{
+    int spatial_order;
+    int limiter;
+    int riemann_solver;
+    double entropy_fix;
+    double gamma;
+} hc_options;
+
+typedef struct hc_level hc_level;
+typedef void* hc_stream; /* hipStream_t; NULL is the default stream */
+
+/* Builds the level geometry (radial lines, face frames, seam gather tables)
+ * on the current device. Replaces the geometry caches of
+ * BoxOp_EulerCubedSphere::levelGeometry (src/BoxOp_EulerCubedSphere.H:317-368)
+ * and the seam plan of src/MHD_SecondOrderSeamPlan.H:51-87. */
+int hc_level_create(const hc_shell* shell, const hc_options* options, hc_level** out);
+void hc_level_destroy(hc_level* level);
+
+/* Ghost width of the stage level: BoxOp_EulerCubedSphere::ghost()
+ * (src/BoxOp_EulerCubedSphere.H:376-390). */
+int hc_level_ghost(const hc_level* level);
+
+/* Element counts (doubles) of the valid, stage and inner-ghost arrays, and of
+ * face array d. */
+size_t hc_level_valid_count(const hc_level* level);
+size_t hc_level_stage_count(const hc_level* level);
+size_t hc_level_inner_count(const hc_level* level);
+size_t hc_level_face_count(const hc_level* level, int d);
+
+/* Stage preparation: U = U0 + a_dt*K (K may be NULL), conversion of the
+ * mapped conserved averages J*U to Cartesian primitives, seam ghost
+ * interpolation, floors and radial boundary ghosts into W.
+ * Replaces preStageLevel / prepareSecondOrderPrimitives
+ * (src/BoxOp_PreStage.H:427-501, src/BoxOp_SecondOrderPreStage.H:6-77,103-218). */
+int hc_prepare_stage(hc_level* level, const double* U0, const double* K, double a_dt,
+                     const double* inner_ghosts, double* W, hc_stream stream);
+
+/* Level RHS: K = d<JU>/dt of every valid cell from the stage level W, with
+ * shared seam face states and seam flux reconciliation; face fluxes are
+ * written when F0/F1/F2 are non-NULL. If U is non-NULL it is incremented by
+ * b_dt*K in the same pass (the RK stage accumulate).
+ * Replaces StageLevel + the panel register reflux
+ * (src/BoxOp_EulerCubedSphereImplem.H:955-1012, src/MHD_AMR.H:1417-1499). */
+int hc_stage_rhs(hc_level* level, const double* W, double* K, double* U, double b_dt,
+                 double* F0, double* F1, double* F2, hc_stream stream);
+
+/* nsteps classical RK4 steps of the level in place on U (valid layout).
+ * Replaces MBLevelRK4::advance as driven by HelioCubed::MBLevelRK::advance
+ * (src/MHD_LevelRK.H:22-29; tableau src/MHD_AMRRungeKutta.H:8-12). */
+int hc_rk4_advance(hc_level* level, double* U, const double* inner_ghosts, double dt,
+                   int nsteps, hc_stream stream);
+
+/* Host-side seeded initial state (test problem data, not on the hot path):
+ * config 0 = radial wind rho=(r0/r)^2, v_r=1, B_r=+-(r0/r)^2 by hemisphere
+ * (sign of z), p=0.1 rho (1 + 0.2 exp(-|x-x_c|^2/w^2)), w = 0.1 (r1-r0), x_c at
+ * radius (r0+r1)/2 in a seeded direction; config 1 = the same with a
+ * monopole B_r=+(r0/r)^2; config 2 = config 1 without the pulse. Writes the valid J*U level and
+ * the inner ghost primitives into HOST arrays. Replaces the role of
+ * BoxOp_EulerCubedSphere::initialize/primToCons (src/BoxOp_Initialize.H:47-254,
+ * src/BoxOp_Utilities.H:67-219) for these problems. */
+int hc_init_state(const hc_shell* shell, const hc_options* options, int config,
+                  unsigned long long seed, double* JU_host, double* inner_host);
+
+/* Host-side geometry probes (no device use): cell-averaged det A of angular
+ * cell (j,k), and the face-averaged cofactor row of face (d; j,k) of panel p. */
+double hc_geom_cell_detA(int n_ang, int j, int k);
+void hc_geom_face_normal(int n_ang, int p, int d, int j, int k, double* normal3);
+
+/* Host-side seam gather probe: donors of ghost column (p,j,k) of the stage
+ * level; returns the donor count (<= 16). */
+int hc_geom_ghost_donors(int n_ang, int p, int j, int k, int* q, int* jj, int* kk, double* w);
+
+/* Per-kernel timing with HIP events recorded on the launch stream around
+ * every kernel launch of this level (enable != 0 turns it on and clears past
+ * records). hc_level_profile_read returns 0 and the kernel name, the summed
+ * milliseconds and the launch count of slot `slot`, or 1 past the last slot.
+ * No reference counterpart: the reference times with Proto PR_TIMERS
+ * (src/BoxOp_EulerCubedSphereImplem.H:20). */
+int hc_level_profile(hc_level* level, int enable);
+int hc_level_profile_read(hc_level* level, int slot, char* name, int name_len,
+                          double* total_ms, long long* launches);
+
+const char* hc_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,179 @@
+"""ctypes wrapper of the CPU FP64 oracle (oracle/hc_oracle.cpp).
+
+TEST INFRASTRUCTURE ONLY: tests/, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of bench.py use it as the checker or the timed CPU
+baseline. The product (heliocubed_amd/) never imports it.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_build", "libhcoracle.so")
+
+
+class OrcCfg(ctypes.Structure):
+    _fields_ = [
+        ("n_rad", ctypes.c_int),
+        ("n_ang", ctypes.c_int),
+        ("ghost", ctypes.c_int),
+        ("r0", ctypes.c_double),
+        ("r1", ctypes.c_double),
+        ("radial_stretch", ctypes.c_double),
+        ("gamma", ctypes.c_double),
+        ("limiter", ctypes.c_int),
+        ("riemann", ctypes.c_int),
+        ("entropy_fix", ctypes.c_double),
+    ]
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        P, D, I = ctypes.c_void_p, ctypes.c_double, ctypes.c_int
+        cfgp = ctypes.POINTER(OrcCfg)
+        sig = {
+            "orc_create": (P, [cfgp]),
+            "orc_destroy": (None, [P]),
+            "orc_o2_prepare": (I, [P, P, P, P]),
+            "orc_o2_rhs": (I, [P, P, P, P, P, P]),
+            "orc_o2_rk4": (I, [P, P, P, D, I]),
+            "orc_cell_detA": (D, [cfgp, I, I]),
+            "orc_face_adjA": (None, [cfgp, I, I, I, I, P]),
+            "orc_basis": (None, [I, D, D, P]),
+            "orc_roe8": (I, [P, P, P, D, D, I]),
+            "orc_rusanov": (None, [P, P, P, D]),
+            "orc_plm": (None, [P, P, P, P, P, P, I]),
+            "orc_face_flux": (None, [P, P, P, P, D, D, D, I]),
+            "orc_cartesian_primitive": (None, [P, P, D, D]),
+            "orc_ghost_donors": (I, [cfgp, I, I, I, P, P, P, P]),
+            "orc_seam_edge": (None, [cfgp, I, I, I, P]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data
+
+
+class Oracle:
+    """O2 operator of one shell configuration on the CPU (FP64)."""
+
+    def __init__(self, n_rad, n_ang, r0=21.5, r1=215.0, radial_stretch=0.0, gamma=5.0 / 3.0,
+                 limiter=1, riemann=1, entropy_fix=0.3, ghost=2):
+        self.cfg = OrcCfg(n_rad, n_ang, ghost, r0, r1, radial_stretch, gamma, limiter, riemann, entropy_fix)
+        self.nr, self.na, self.g = n_rad, n_ang, ghost
+        self._h = ctypes.c_void_p(lib().orc_create(ctypes.byref(self.cfg)))
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().orc_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    @property
+    def valid_shape(self):
+        return (6, 8, self.na, self.na, self.nr)
+
+    @property
+    def stage_shape(self):
+        g = self.g
+        return (6, 8, self.na + 2 * g, self.na + 2 * g, self.nr + 2 * g)
+
+    def face_shape(self, d):
+        return (6, 8, self.na + (d == 2), self.na + (d == 1), self.nr + (d == 0))
+
+    def prepare(self, ju, inner):
+        W = np.zeros(self.stage_shape)
+        lib().orc_o2_prepare(self._h, _p(ju), _p(inner), _p(W))
+        return W
+
+    def rhs(self, W, with_fluxes=False):
+        k = np.zeros(self.valid_shape)
+        if with_fluxes:
+            F = [np.zeros(self.face_shape(d)) for d in range(3)]
+            lib().orc_o2_rhs(self._h, _p(W), _p(k), _p(F[0]), _p(F[1]), _p(F[2]))
+            return k, F
+        lib().orc_o2_rhs(self._h, _p(W), _p(k), None, None, None)
+        return k
+
+    def rk4(self, ju, inner, dt, nsteps=1):
+        u = np.ascontiguousarray(ju.copy())
+        lib().orc_o2_rk4(self._h, _p(u), _p(inner), dt, nsteps)
+        return u
+
+    def cell_detA(self, j, k):
+        return lib().orc_cell_detA(ctypes.byref(self.cfg), j, k)
+
+    def face_adjA(self, p, d, j, k):
+        n = np.zeros(3)
+        lib().orc_face_adjA(ctypes.byref(self.cfg), p, d, j, k, _p(n))
+        return n
+
+    def ghost_donors(self, p, j, k):
+        q = np.zeros(16, dtype=np.int32)
+        jj = np.zeros(16, dtype=np.int32)
+        kk = np.zeros(16, dtype=np.int32)
+        w = np.zeros(16)
+        n = lib().orc_ghost_donors(ctypes.byref(self.cfg), p, j, k, q.ctypes.data, jj.ctypes.data,
+                                   kk.ctypes.data, _p(w))
+        return [(int(q[x]), int(jj[x]), int(kk[x]), float(w[x])) for x in range(n)]
+
+    def seam_edge(self, p, d, s):
+        out = np.zeros(4, dtype=np.int32)
+        lib().orc_seam_edge(ctypes.byref(self.cfg), p, d, s, out.ctypes.data)
+        return tuple(int(x) for x in out)
+
+
+def roe8(left, right, gamma=5.0 / 3.0, entropy_fix=0.3, check=True):
+    out = np.zeros(8)
+    ok = lib().orc_roe8(_p(out), _p(np.ascontiguousarray(left, dtype=np.float64)),
+                        _p(np.ascontiguousarray(right, dtype=np.float64)), gamma, entropy_fix, int(check))
+    return bool(ok), out
+
+
+def rusanov(left, right, gamma=5.0 / 3.0):
+    out = np.zeros(8)
+    lib().orc_rusanov(_p(out), _p(np.ascontiguousarray(left, dtype=np.float64)),
+                      _p(np.ascontiguousarray(right, dtype=np.float64)), gamma)
+    return out
+
+
+def plm(mm, m, c, p, limiter=1):
+    lo, hi = np.zeros(8), np.zeros(8)
+    args = [np.ascontiguousarray(x, dtype=np.float64) for x in (mm, m, c, p)]
+    lib().orc_plm(_p(lo), _p(hi), *[_p(a) for a in args], limiter)
+    return lo, hi
+
+
+def face_flux(left, right, normal, area, gamma=5.0 / 3.0, entropy_fix=0.3, roe=False):
+    out = np.zeros(8)
+    lib().orc_face_flux(_p(out), _p(np.ascontiguousarray(left, dtype=np.float64)),
+                        _p(np.ascontiguousarray(right, dtype=np.float64)),
+                        _p(np.ascontiguousarray(normal, dtype=np.float64)), area, gamma, entropy_fix, int(roe))
+    return out
+
+
+def basis(p, alpha, beta):
+    A = np.zeros(9)
+    lib().orc_basis(p, alpha, beta, _p(A))
+    return A.reshape(3, 3)
