@@ -42,7 +42,12 @@ class HcOptions(ctypes.Structure):
         ("riemann_solver", ctypes.c_int),
         ("entropy_fix", ctypes.c_double),
         ("gamma", ctypes.c_double),
+        ("precision", ctypes.c_int),
     ]
+
+
+class HcBox(ctypes.Structure):
+    _fields_ = [("block", ctypes.c_int), ("lo", ctypes.c_int * 3), ("hi", ctypes.c_int * 3)]
 
 
 # (name, restype, argtypes) of every exported C-ABI symbol.
@@ -64,6 +69,13 @@ _SIGNATURES = [
     ("hc_geom_cell_detA", _D, [_I, _I, _I]),
     ("hc_geom_face_normal", None, [_I, _I, _I, _I, _I, _P]),
     ("hc_geom_ghost_donors", _I, [_I, _I, _I, _I, _P, _P, _P, _P]),
+    ("hc_boxop_create", _I, [ctypes.POINTER(HcShell), ctypes.POINTER(HcOptions), ctypes.POINTER(HcBox), _I,
+                             ctypes.POINTER(_P)]),
+    ("hc_boxop_destroy", None, [_P]),
+    ("hc_boxop_count", ctypes.c_longlong, [_P, _I]),
+    ("hc_boxop_apply", _I, [_P, _P, _P, _P, _P, _P, _P]),
+    ("hc_init_native_boxes", _I, [ctypes.POINTER(HcShell), ctypes.POINTER(HcOptions), _I, ctypes.c_ulonglong,
+                                  ctypes.POINTER(HcBox), _I, _I, _P]),
     ("hc_level_profile", _I, [_P, _I]),
     ("hc_level_profile_read", _I, [_P, _I, ctypes.c_char_p, _I, ctypes.POINTER(_D), ctypes.POINTER(ctypes.c_longlong)]),
     ("hc_last_error", ctypes.c_char_p, []),
@@ -122,9 +134,11 @@ class Options:
     riemann_solver: int = 1
     entropy_fix: float = 0.3
     gamma: float = 5.0 / 3.0
+    precision: int = 0  # 0 fp64, 1 mixed (fp32 slopes and Riemann dissipation)
 
     def c(self) -> HcOptions:
-        return HcOptions(self.spatial_order, self.limiter, self.riemann_solver, self.entropy_fix, self.gamma)
+        return HcOptions(self.spatial_order, self.limiter, self.riemann_solver, self.entropy_fix, self.gamma,
+                         self.precision)
 
 
 def init_state(shell: Shell, options: Options, config: int = 0, seed: int = 20261203):
@@ -237,3 +251,79 @@ class Level:
         """MBLevelRK4::advance: nsteps classical RK4 steps in place on U."""
         self._require(U, self.valid_shape, "U")
         _check(lib().hc_rk4_advance(self._handle, _ptr(U), _ptr(inner), dt, nsteps, self._stream()))
+
+
+def box_table(boxes):
+    """ctypes array of HcBox from (block, lo, hi) triples."""
+    arr = (HcBox * len(boxes))()
+    for n, (blk, lo, hi) in enumerate(boxes):
+        arr[n].block = blk
+        for d in range(3):
+            arr[n].lo[d] = lo[d]
+            arr[n].hi[d] = hi[d]
+    return arr
+
+
+def shell_boxes(shell: Shell, size=(32, 32, 32)):
+    """Uniform decomposition of all six panels into boxes of the given size."""
+    out = []
+    for p in range(6):
+        for k in range(0, shell.n_ang, size[2]):
+            for j in range(0, shell.n_ang, size[1]):
+                for i in range(0, shell.n_rad, size[0]):
+                    out.append((p, (i, j, k), (min(i + size[0], shell.n_rad) - 1,
+                                               min(j + size[1], shell.n_ang) - 1,
+                                               min(k + size[2], shell.n_ang) - 1)))
+    return out
+
+
+def init_native_boxes(shell: Shell, options: Options, boxes, config: int = 3, seed: int = 20261203, quad: int = 3):
+    """Native conserved cell averages (ghost 5) of every box, concatenated, as numpy."""
+    import numpy as np
+
+    g = 5
+    total = sum(8 * (hi[0] - lo[0] + 1 + 2 * g) * (hi[1] - lo[1] + 1 + 2 * g) * (hi[2] - lo[2] + 1 + 2 * g)
+                for _, lo, hi in boxes)
+    out = np.zeros(total)
+    tab = box_table(boxes)
+    sh, op = shell.c(), options.c()
+    _check(lib().hc_init_native_boxes(ctypes.byref(sh), ctypes.byref(op), config, seed, tab, len(boxes), quad,
+                                      out.ctypes.data))
+    return out
+
+
+class BoxOp:
+    """Fourth-order BoxOp (BoxOp_EulerCubedSphere::operator()) over a box table.
+
+    apply(USph, rhs) evaluates every box in the same launches; rhs is
+    accumulated into, as the reference's a_Rhs (the caller zeroes it).
+    """
+
+    def __init__(self, shell: Shell, options: Options, boxes):
+        self.shell, self.options, self.boxes = shell, options, list(boxes)
+        self._handle = ctypes.c_void_p()
+        sh, op = shell.c(), options.c()
+        tab = box_table(self.boxes)
+        _check(lib().hc_boxop_create(ctypes.byref(sh), ctypes.byref(op), tab, len(self.boxes),
+                                     ctypes.byref(self._handle)))
+
+    def __del__(self):
+        if getattr(self, "_handle", None) and self._handle.value:
+            lib().hc_boxop_destroy(self._handle)
+            self._handle = ctypes.c_void_p()
+
+    def count(self, what: int) -> int:
+        return lib().hc_boxop_count(self._handle, what)
+
+    def apply(self, USph, rhs, fluxes=None):
+        import torch
+
+        for t, what, name in ((USph, 0, "USph"), (rhs, 1, "rhs")):
+            if t.dtype != torch.float64 or not t.is_cuda or not t.is_contiguous() or t.numel() != self.count(what):
+                raise ValueError(f"{name}: need a contiguous float64 device tensor of {self.count(what)} elements")
+        f = list(fluxes) if fluxes is not None else [None, None, None]
+        for d in range(3):
+            if f[d] is not None and (f[d].numel() != self.count(2 + d) or f[d].dtype != torch.float64):
+                raise ValueError(f"F{d}: need {self.count(2 + d)} float64 elements")
+        stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+        _check(lib().hc_boxop_apply(self._handle, _ptr(USph), _ptr(rhs), _ptr(f[0]), _ptr(f[1]), _ptr(f[2]), stream))

@@ -260,4 +260,70 @@ inline SeamEdge seamEdge(int na, int p, int d, int s)
     e.rev = tq[0] != 0 ? 1 : 0;
     return e;
 }
+
+// ---- Face averages for the fourth-order operator -------------------------
+// Gauss-Legendre nodes/weights on [0,1] (n points) by Newton iteration on
+// the Legendre polynomial.
+inline void gaussLegendre01(int n, std::vector<double>& x, std::vector<double>& w)
+{
+    x.resize(n); w.resize(n);
+    for (int i = 0; i < n; ++i)
+    {
+        double z = std::cos(kPi*(i+0.75)/(n+0.5)), dp = 0;
+        for (int it = 0; it < 100; ++it)
+        {
+            double p0 = 1, p1 = z;
+            for (int k = 2; k <= n; ++k) { const double p2 = ((2*k-1)*z*p1-(k-1)*p0)/k; p0 = p1; p1 = p2; }
+            dp = n*(z*p1-p0)/(z*z-1);
+            const double dz = p1/dp;
+            z -= dz;
+            if (std::fabs(dz) < 1e-16) break;
+        }
+        x[i] = 0.5*(1-z);
+        w[i] = 1.0/((1-z*z)*dp*dp);
+    }
+}
+
+// Averages over face (d; j,k) of det A (out[0]), det(A) A (out[1..9],
+// row-major) and the cofactor row d (out[10..12]), all in block coordinates
+// xi (A = d xhat / d xi). d = 0 is the angular cell (j,k), d = 1 the line
+// alpha = j h over cell k, d = 2 the line beta = k h over cell j. The det A
+// average of d = 0 is the exact solid angle; d = 1, 2 use the closed form
+// int det A dxi = (pi/2) [t / sqrt(1 + s^2 + t^2)] along the line (s the fixed
+// tangent); the cofactor rows use the great-circle closed forms.
+inline void faceMetricAverages(int na, int p, int d, int j, int k, double out[13])
+{
+    static std::vector<double> gx, gw;
+    if (gx.empty()) gaussLegendre01(12, gx, gw);
+    const double h = 1.0/na;
+    for (int a = 0; a < 13; ++a) out[a] = 0;
+    auto add = [&](double wgt, double al, double be) {
+        double A[3][3];
+        pointBasis(p, al, be, A);
+        const double det = A[0][0]*(A[1][1]*A[2][2]-A[1][2]*A[2][1])
+                         - A[0][1]*(A[1][0]*A[2][2]-A[1][2]*A[2][0])
+                         + A[0][2]*(A[1][0]*A[2][1]-A[1][1]*A[2][0]);
+        for (int a = 0; a < 3; ++a) for (int b = 0; b < 3; ++b) out[1+3*a+b] += wgt*det*A[a][b];
+    };
+    const int n = (int)gx.size();
+    if (d == 0)
+    {
+        for (int q = 0; q < n; ++q) for (int r = 0; r < n; ++r)
+            add(gw[q]*gw[r], xiToAngle((j+gx[q])*h), xiToAngle((k+gx[r])*h));
+        out[0] = cellDetA(na, j, k);
+    }
+    else
+    {
+        for (int q = 0; q < n; ++q)
+        {
+            if (d == 1) add(gw[q], xiToAngle(j*h), xiToAngle((k+gx[q])*h));
+            else add(gw[q], xiToAngle((j+gx[q])*h), xiToAngle(k*h));
+        }
+        const double s = std::tan(xiToAngle((d == 1 ? j : k)*h));
+        const double t0 = std::tan(xiToAngle((d == 1 ? k : j)*h)), t1 = std::tan(xiToAngle(((d == 1 ? k : j)+1)*h));
+        out[0] = (kPi/2)/h*(t1/std::sqrt(1+s*s+t1*t1) - t0/std::sqrt(1+s*s+t0*t0));
+    }
+    const Vec3 nrm = faceNormal(na, p, d, j, k);
+    for (int a = 0; a < 3; ++a) out[10+a] = nrm.x[a];
+}
 } // namespace hc

@@ -15,25 +15,20 @@
 #include <vector>
 
 #include "../../include/heliocubed.h"
+#include "hc_common.hpp"
 #include "hc_geometry.hpp"
 #include "hc_physics.hpp"
 
-namespace
+std::string& hc::lastError()
 {
-thread_local std::string g_lastError;
-
-int fail(const std::string& msg)
-{
-    g_lastError = msg;
-    return -1;
+    thread_local std::string message;
+    return message;
 }
 
-#define HC_CHECK(call)                                                              \
-    do {                                                                            \
-        hipError_t e_ = (call);                                                     \
-        if (e_ != hipSuccess)                                                       \
-            throw std::runtime_error(std::string(#call) + ": " + hipGetErrorString(e_)); \
-    } while (0)
+namespace
+{
+using hc::fail;
+#define HC_CHECK HC_THROW
 
 constexpr int kBlock = 256;
 
@@ -248,7 +243,7 @@ k_face_flux(LevelView v, const double* __restrict__ W, const double* __restrict_
     const double* fr = v.frame[D] + 10LL*(((long long)p*fk + k)*fj + j);
     const double area = (D == 0 ? rad(v,i)[2] : rad(v,i)[3])*fr[9];
     double out[8];
-    hc::o2FaceFlux(out, l, rr, fr, area, v.gamma, v.entropy, v.roe != 0);
+    hc::o2FaceFluxMixed<double>(out, l, rr, fr, area, v.gamma, v.entropy, v.roe != 0);
     const long long cstride = (long long)fi*fj*fk;
     const long long base = (long long)p*8*cstride + ((long long)k*fj + j)*fi + i;
 #pragma unroll
@@ -325,6 +320,142 @@ k_divergence(LevelView v, const double* __restrict__ F0, const double* __restric
     }
 }
 
+// 8. Seam faces: both panels' fluxes of every physical seam face from the
+// two interior states, reconciled 1/2 (F_local + sign F_adjacent), stored for
+// both panels in SF[p][d-1][s][c][t][i]. Each physical face is handled once.
+template <typename S>
+__global__ void __launch_bounds__(kBlock)
+k_seam_flux(LevelView v, const double* __restrict__ PS, double* __restrict__ SF)
+{
+    const long long n = 24LL*v.na*v.nr;
+    const long long t = (long long)blockIdx.x*blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int i = t % v.nr;
+    long long r = t / v.nr;
+    const int tt = r % v.na; r /= v.na;
+    const int s = r % 2; r /= 2;
+    const int d = 1 + (r % 2);
+    const int p = r / 2;
+    const int* e = v.edge + 4*((p*2 + (d-1))*2 + s);
+    const int q = e[0], dq = e[1], sq = e[2];
+    const int self = (p*2 + (d-1))*2 + s, other = (q*2 + (dq-1))*2 + sq;
+    if (self > other) return;
+    const int tq = e[3] ? v.na-1-tt : tt;
+    const long long cstride = (long long)v.na*v.nr;
+    const long long bp = ((long long)self*8*v.na + tt)*v.nr + i;
+    const long long bq = ((long long)other*8*v.na + tq)*v.nr + i;
+    double sp[8], sqs[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) { sp[c] = PS[bp + c*cstride]; sqs[c] = PS[bq + c*cstride]; }
+    auto flux = [&](int pp, int dd, int ss, int tg, const double* own, const double* adj, double out[8]) {
+        const int fj = v.na + (dd == 1), fk = v.na + (dd == 2);
+        const int f = ss ? v.na : 0;
+        const int jj = dd == 1 ? f : tg, kk = dd == 1 ? tg : f;
+        const double* fr = v.frame[dd] + 10LL*(((long long)pp*fk + kk)*fj + jj);
+        const double area = rad(v,i)[3]*fr[9];
+        if (ss == 0) hc::o2FaceFluxMixed<S>(out, adj, own, fr, area, v.gamma, v.entropy, v.roe != 0);
+        else hc::o2FaceFluxMixed<S>(out, own, adj, fr, area, v.gamma, v.entropy, v.roe != 0);
+    };
+    double fp[8], fq[8];
+    flux(p, d, s, tt, sp, sqs, fp);
+    flux(q, dq, sq, tq, sqs, sp, fq);
+    const double sign = (s != sq) ? 1.0 : -1.0;
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+    {
+        SF[bp + c*cstride] = 0.5*(fp[c] + sign*fq[c]);
+        SF[bq + c*cstride] = 0.5*(fq[c] + sign*fp[c]);
+    }
+}
+
+// 9. Fused RHS: one block per tile of TI x TJ x TK cells of one panel. For
+// each direction the block evaluates every face of the tile once into LDS
+// (seam faces come reconciled from SF), then each thread accumulates the
+// divergence of its cell. K = d<JU>/dt; U += b_dt*K when U is given.
+constexpr int kTI = 32, kTJ = 4, kTK = 2;
+constexpr int kTileCells = kTI*kTJ*kTK;
+constexpr int kMax2(int a, int b) { return a > b ? a : b; }
+constexpr int kMaxFaces = kMax2(kMax2((kTI+1)*kTJ*kTK, kTI*(kTJ+1)*kTK), kTI*kTJ*(kTK+1));
+
+template <typename S>
+__global__ void __launch_bounds__(kTileCells)
+k_rhs_fused(LevelView v, const double* __restrict__ W, const double* __restrict__ SF,
+            double* __restrict__ K, double* __restrict__ U, double b_dt, int tilesI, int tilesJ, int tilesK)
+{
+    __shared__ double flux[8*kMaxFaces];
+    int b = blockIdx.x;
+    const int ti = b % tilesI; b /= tilesI;
+    const int tj = b % tilesJ; b /= tilesJ;
+    const int tk = b % tilesK;
+    const int p = b / tilesK;
+    const int i0 = ti*kTI, j0 = tj*kTJ, k0 = tk*kTK;
+    const int tid = threadIdx.x;
+    const int ci = tid % kTI, cj = (tid / kTI) % kTJ, ck = tid / (kTI*kTJ);
+    const bool active = i0+ci < v.nr && j0+cj < v.na && k0+ck < v.na;
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const double inv[3] = {-1.0/(1.0/v.nr), -1.0/(1.0/v.na), -1.0/(1.0/v.na)};
+#pragma unroll 1
+    for (int d = 0; d < 3; ++d)
+    {
+        const int ni = kTI + (d == 0), nj = kTJ + (d == 1), nk = kTK + (d == 2);
+        const int nf = ni*nj*nk;
+        for (int f = tid; f < nf; f += kTileCells)
+        {
+            const int fi = f % ni, fj = (f / ni) % nj, fk = f / (ni*nj);
+            const int i = i0+fi, j = j0+fj, k = k0+fk;
+            double out[8];
+            const bool inside = i <= v.nr - (d != 0) && j <= v.na - (d != 1) && k <= v.na - (d != 2);
+            if (inside)
+            {
+                const int fa = d == 1 ? j : k, tg = d == 1 ? k : j;
+                if (d > 0 && (fa == 0 || fa == v.na))
+                {
+                    const int s = fa == 0 ? 0 : 1;
+                    const long long base = ((((long long)(p*2+(d-1))*2+s)*8)*v.na + tg)*v.nr + i;
+                    const long long cstride = (long long)v.na*v.nr;
+#pragma unroll
+                    for (int c = 0; c < 8; ++c) out[c] = SF[base + c*cstride];
+                }
+                else
+                {
+                    const int e0 = d == 0, e1 = d == 1, e2 = d == 2;
+                    double mm[8], m[8], c[8], pp[8], l[8], rr[8];
+                    loadStage(v, W, p, i-2*e0, j-2*e1, k-2*e2, mm);
+                    loadStage(v, W, p, i-e0, j-e1, k-e2, m);
+                    loadStage(v, W, p, i, j, k, c);
+                    loadStage(v, W, p, i+e0, j+e1, k+e2, pp);
+                    hc::plmFaceStatesMixed<S>(l, rr, mm, m, c, pp, v.limiter);
+                    const int fjd = v.na + (d == 1), fkd = v.na + (d == 2);
+                    const double* fr = v.frame[d] + 10LL*(((long long)p*fkd + k)*fjd + j);
+                    const double area = (d == 0 ? rad(v,i)[2] : rad(v,i)[3])*fr[9];
+                    hc::o2FaceFluxMixed<S>(out, l, rr, fr, area, v.gamma, v.entropy, v.roe != 0);
+                }
+            }
+            else
+            {
+#pragma unroll
+                for (int c = 0; c < 8; ++c) out[c] = 0;
+            }
+#pragma unroll
+            for (int c = 0; c < 8; ++c) flux[c*kMaxFaces + f] = out[c];
+        }
+        __syncthreads();
+        const int flo = (ck*nj + cj)*ni + ci;
+        const int fhi = flo + (d == 0 ? 1 : d == 1 ? ni : ni*nj);
+#pragma unroll
+        for (int c = 0; c < 8; ++c) acc[c] += inv[d]*(flux[c*kMaxFaces + fhi] - flux[c*kMaxFaces + flo]);
+        __syncthreads();
+    }
+    if (!active) return;
+    const long long vb = validIndex(v, p, 0, i0+ci, j0+cj, k0+ck);
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+    {
+        K[vb + c*v.vcs] = acc[c];
+        if (U) U[vb + c*v.vcs] += b_dt*acc[c];
+    }
+}
+
 inline unsigned blocksFor(long long n) { return (unsigned)((n + kBlock - 1)/kBlock); }
 
 template <typename T>
@@ -387,6 +518,7 @@ struct hc_level
     double* W = nullptr;
     double* F[3] = {nullptr, nullptr, nullptr};
     double* PS = nullptr;
+    double* SF = nullptr;
     double* U0 = nullptr;
     double* K = nullptr;
 
@@ -409,6 +541,7 @@ struct hc_level
         HC_CHECK(hipMemset(W, 0, stageCount()*sizeof(double)));
         for (int d = 0; d < 3; ++d) { HC_CHECK(hipMalloc(&F[d], faceCount(d)*sizeof(double))); owned.push_back(F[d]); }
         HC_CHECK(hipMalloc(&PS, 24ull*8*dims.na*dims.nr*sizeof(double))); owned.push_back(PS);
+        HC_CHECK(hipMalloc(&SF, 24ull*8*dims.na*dims.nr*sizeof(double))); owned.push_back(SF);
         HC_CHECK(hipMalloc(&U0, validCount()*sizeof(double))); owned.push_back(U0);
         HC_CHECK(hipMalloc(&K, validCount()*sizeof(double))); owned.push_back(K);
     }
@@ -519,11 +652,33 @@ void prepareStage(hc_level& L, const double* U0, const double* K, double a_dt,
     HC_CHECK(hipGetLastError());
 }
 
+template <typename S>
+void stageRhsFused(hc_level& L, const double* W, double* K, double* U, double b_dt, hipStream_t s)
+{
+    const LevelView& v = L.view;
+    KernelProfile& P = L.profile;
+    P.run("k_panel_states", s, [&] {
+        hipLaunchKernelGGL(k_panel_states, dim3(blocksFor(24LL*v.na*v.nr)), dim3(kBlock), 0, s, v, W, L.PS); });
+    P.run("k_seam_flux", s, [&] {
+        hipLaunchKernelGGL(k_seam_flux<S>, dim3(blocksFor(24LL*v.na*v.nr)), dim3(kBlock), 0, s, v, L.PS, L.SF); });
+    const int tI = (v.nr + kTI - 1)/kTI, tJ = (v.na + kTJ - 1)/kTJ, tK = (v.na + kTK - 1)/kTK;
+    P.run("k_rhs_fused", s, [&] {
+        hipLaunchKernelGGL(k_rhs_fused<S>, dim3(6u*tI*tJ*tK), dim3(kTileCells), 0, s,
+                           v, W, L.SF, K, U, b_dt, tI, tJ, tK); });
+    HC_CHECK(hipGetLastError());
+}
+
 void stageRhs(hc_level& L, const double* W, double* K, double* U, double b_dt,
               double* F0, double* F1, double* F2, hipStream_t s)
 {
     L.ensureWorkspace();
     const LevelView& v = L.view;
+    if (!F0 && !F1 && !F2)
+    {
+        if (L.opt.precision == 1) stageRhsFused<float>(L, W, K, U, b_dt, s);
+        else stageRhsFused<double>(L, W, K, U, b_dt, s);
+        return;
+    }
     double* F[3] = {F0 ? F0 : L.F[0], F1 ? F1 : L.F[1], F2 ? F2 : L.F[2]};
     KernelProfile& P = L.profile;
     P.run("k_panel_states", s, [&] {
@@ -561,8 +716,26 @@ struct WindState
     double r0, r1, gamma;
     double center[3], width, pulse;
     bool hemispheric;
+    int nmodes = 0;
+    double kvec[16][3], evec[16][3], phase[16], amp = 0;
     // Cartesian primitive state at position X.
     void at(const double X[3], double w[8]) const
+    {
+        base(X, w);
+        if (nmodes == 0) return;
+        const double r = std::sqrt(X[0]*X[0]+X[1]*X[1]+X[2]*X[2]);
+        const double rh[3] = {X[0]/r, X[1]/r, X[2]/r};
+        double dv[3] = {0, 0, 0};
+        for (int m = 0; m < nmodes; ++m)
+        {
+            const double s = std::sin(kvec[m][0]*rh[0]+kvec[m][1]*rh[1]+kvec[m][2]*rh[2]+phase[m]);
+            const double en = evec[m][0]*rh[0]+evec[m][1]*rh[1]+evec[m][2]*rh[2];
+            for (int a = 0; a < 3; ++a) dv[a] += s*(evec[m][a]-en*rh[a]);
+        }
+        const double sq = std::sqrt(4*hc::kPi*w[0]);
+        for (int a = 0; a < 3; ++a) { w[1+a] += amp*dv[a]; w[5+a] -= amp*dv[a]*sq; }
+    }
+    void base(const double X[3], double w[8]) const
     {
         const double r = std::sqrt(X[0]*X[0]+X[1]*X[1]+X[2]*X[2]);
         const double rho = (r0/r)*(r0/r);
@@ -578,7 +751,7 @@ struct WindState
 
 extern "C" {
 
-const char* hc_last_error(void) { return g_lastError.c_str(); }
+const char* hc_last_error(void) { return hc::lastError().c_str(); }
 
 int hc_level_create(const hc_shell* shell, const hc_options* options, hc_level** out)
 {
@@ -590,6 +763,7 @@ int hc_level_create(const hc_shell* shell, const hc_options* options, hc_level**
     if (options->riemann_solver != 1 && options->riemann_solver != 2)
         return fail("hc_level_create: riemann_solver must be 1 (Rusanov) or 2 (Roe)");
     if (options->limiter < 0 || options->limiter > 3) return fail("hc_level_create: limiter must be 0..3");
+    if (options->precision != 0 && options->precision != 1) return fail("hc_level_create: precision must be 0 (fp64) or 1 (mixed)");
     try
     {
         std::unique_ptr<hc_level> L(new hc_level());
@@ -734,6 +908,100 @@ int hc_level_profile_read(hc_level* level, int slot, char* name, int name_len, d
         return 0;
     }
     catch (const std::exception& e) { return fail(std::string("hc_level_profile_read: ") + e.what()); }
+}
+
+int hc_init_native_boxes(const hc_shell* shell, const hc_options* options, int config, unsigned long long seed,
+                         const hc_box* boxes, int nbox, int quad, double* USph)
+{
+    if (!shell || !options || !boxes || !USph || nbox <= 0) return fail("hc_init_native_boxes: bad argument");
+    if (config < 0 || config > 3) return fail("hc_init_native_boxes: config must be 0..3");
+    if (quad < 1 || quad > 8) return fail("hc_init_native_boxes: quad must be 1..8");
+    const int nr = shell->n_rad, na = shell->n_ang, g = 5;
+    const hc::ShellDims dims{nr, na, g, shell->inner_radius, shell->outer_radius, shell->radial_stretch};
+    SplitMix rng{seed};
+    WindState ws;
+    ws.r0 = shell->inner_radius; ws.r1 = shell->outer_radius; ws.gamma = options->gamma;
+    const double cz = 2.0*rng.uniform()-1.0, phi = 2.0*hc::kPi*rng.uniform();
+    const double rc = 0.5*(ws.r0+ws.r1), sz = std::sqrt(std::max(0.0, 1.0-cz*cz));
+    ws.center[0] = rc*sz*std::cos(phi); ws.center[1] = rc*sz*std::sin(phi); ws.center[2] = rc*cz;
+    ws.width = 0.1*(ws.r1-ws.r0);
+    ws.pulse = config == 2 ? 0.0 : 0.2;
+    ws.hemispheric = config == 0 || config == 3;
+    if (config == 3)
+    {
+        // Alfvenic modes with angular wavenumber |k| <= 8, amplitude 0.05.
+        ws.nmodes = 16; ws.amp = 0.05/std::sqrt(16.0);
+        for (int m = 0; m < 16; ++m)
+        {
+            double kk[3], n2 = 0;
+            for (int a = 0; a < 3; ++a) { kk[a] = 2.0*rng.uniform()-1.0; n2 += kk[a]*kk[a]; }
+            const double kmag = 1.0 + 7.0*rng.uniform();
+            for (int a = 0; a < 3; ++a) ws.kvec[m][a] = kk[a]/std::sqrt(n2)*kmag;
+            double e[3], e2 = 0;
+            for (int a = 0; a < 3; ++a) { e[a] = 2.0*rng.uniform()-1.0; e2 += e[a]*e[a]; }
+            for (int a = 0; a < 3; ++a) ws.evec[m][a] = e[a]/std::sqrt(e2);
+            ws.phase[m] = 2.0*hc::kPi*rng.uniform();
+        }
+    }
+    std::vector<double> gx, gw;
+    hc::gaussLegendre01(quad, gx, gw);
+    const double h = 1.0/na, gamma = options->gamma;
+    size_t off = 0;
+    for (int n = 0; n < nbox; ++n)
+    {
+        const hc_box& b = boxes[n];
+        int lo[3], e[3];
+        for (int d = 0; d < 3; ++d) { lo[d] = b.lo[d]-g; e[d] = b.hi[d]-b.lo[d]+1+2*g; }
+        const size_t ec = (size_t)e[0]*e[1]*e[2];
+        for (int kk = 0; kk < e[2]; ++kk) for (int jj = 0; jj < e[1]; ++jj) for (int ii = 0; ii < e[0]; ++ii)
+        {
+            const int i = lo[0]+ii, j = lo[1]+jj, k = lo[2]+kk;
+            double avg[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            double noise = 1.0;
+            if (config == 3)
+            {
+                // lognormal density factor per cell, sigma 0.05, mean-preserving
+                SplitMix cr{seed ^ (0x9E3779B97F4A7C15ull*((((unsigned long long)b.block*4096+(k+8))*4096+(j+8))*65536+(i+8)))};
+                const double u1 = std::max(cr.uniform(), 1e-300), u2 = cr.uniform();
+                const double gn = std::sqrt(-2.0*std::log(u1))*std::cos(2*hc::kPi*u2);
+                noise = std::exp(0.05*gn-0.00125);
+            }
+            for (int qk = 0; qk < quad; ++qk) for (int qj = 0; qj < quad; ++qj)
+            {
+                const double al = hc::xiToAngle((j+gx[qj])*h), be = hc::xiToAngle((k+gx[qk])*h);
+                double A[3][3], Ai[3][3];
+                hc::pointBasis(b.block, al, be, A);
+                hc::invert3(A, Ai);
+                const hc::Vec3 dir = hc::panelPoint(b.block, al, be);
+                for (int qi = 0; qi < quad; ++qi)
+                {
+                    const double eta = (i+gx[qi])/double(nr);
+                    const double C = dims.stretch;
+                    const double span = C == 0.0 ? (dims.r1-dims.r0) : (dims.r1-dims.r0)/std::expm1(C);
+                    const double r = dims.r0 + span*(C == 0.0 ? eta : std::expm1(C*eta));
+                    const double X[3] = {r*dir.x[0], r*dir.x[1], r*dir.x[2]};
+                    double w[8];
+                    ws.at(X, w);
+                    w[0] *= noise;
+                    double u[8], v2 = 0, b2 = 0;
+                    u[0] = w[0];
+                    for (int a = 0; a < 3; ++a) { v2 += w[1+a]*w[1+a]; b2 += w[5+a]*w[5+a]; }
+                    u[4] = w[4]/(gamma-1.0) + 0.5*w[0]*v2 + b2/(8*hc::kPi);
+                    for (int a = 0; a < 3; ++a)
+                    {
+                        double m = 0, bb = 0;
+                        for (int c = 0; c < 3; ++c) { m += Ai[a][c]*w[0]*w[1+c]; bb += Ai[a][c]*w[5+c]; }
+                        u[1+a] = m; u[5+a] = bb;
+                    }
+                    const double wt = gw[qi]*gw[qj]*gw[qk];
+                    for (int c = 0; c < 8; ++c) avg[c] += wt*u[c];
+                }
+            }
+            for (int c = 0; c < 8; ++c) USph[off + c*ec + ((size_t)kk*e[1]+jj)*e[0]+ii] = avg[c];
+        }
+        off += 8*ec;
+    }
+    return 0;
 }
 
 double hc_geom_cell_detA(int n_ang, int j, int k) { return hc::cellDetA(n_ang, j, k); }

@@ -1,0 +1,941 @@
+// Fourth-order standard operator of the HelioCubed MHD RHS on MI355X, as a
+// batched box operator: every box of a box table is processed by the same
+// launches (box index = blockIdx.y), so a whole AMR level or a uniform shell
+// is one launch per kernel per RK stage.
+//
+// Restates (src/ = HelioCubed reference repository) the fourth-order path of
+// BoxOp_EulerCubedSphere::operator() (src/BoxOp_EulerCubedSphereImplem.H:97-402):
+// sphToPrim (src/BoxOp_Utilities.H:599-707), Limiters_4O with flattening
+// (src/BoxOp_Limiters.H:28-359), repairRoeFaceStates and primToMappedRoeFlux
+// (src/BoxOp_RoeCubedSphere.H), primToFlux (src/BoxOp_EulerCubedSphereImplem.H:735-802),
+// primToMappedFlux (:404-534), factoredRoe8WaveDissipation
+// (src/MHD_Roe8Wave.H:634-740) and the flux divergence (:336-381).
+// Face products use <ab> = a b + (1/12) sum_t D_t a D_t b with
+// D_t = (shift+1 - shift-1)/2 in the two directions transverse to the face.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/heliocubed.h"
+#include "hc_common.hpp"
+#include "hc_geometry.hpp"
+#include "hc_physics.hpp"
+
+namespace
+{
+constexpr int kG = 5;          // ghost width of the fourth-order operator
+constexpr int kB = 256;
+
+// Per-box scratch fields, in units of the box+5 cell count e0*e1*e2.
+enum Field : int
+{
+    F_WB = 0, F_MEB = 8, F_W = 9, F_ETA = 17, F_PTL = 18, F_PTR = 26, F_P0L = 34, F_P0R = 35,
+    F_MEL = 36, F_MER = 37, F_PHL = 38, F_PHR = 46, F_DPT = 54, F_FLUX = 62, F_COUNT = 86
+};
+
+struct BoxDesc
+{
+    int p, lo[3], hi[3], n[3];
+    long long uOff, rOff, sOff;
+};
+
+struct O4View
+{
+    int nr, na, naE, limiter;
+    double gamma, entropy;
+    const BoxDesc* boxes;
+    const double* cellA;     // [6][naE][naE][9]
+    const double* faceA[3];  // [6][naE][naE][9]
+    const double* fmet[3];   // [6][naE][naE][13]: detA, det(A) A, cofactor row
+    const double* radial;    // [nr+13][2]: adjDr_0 (area0), adjDr_t (areaT) for i = -6 ..
+};
+
+__device__ __forceinline__ long long angIdx(const O4View& v, int p, int j, int k)
+{
+    return ((long long)p*v.naE + (k+kG))*v.naE + (j+kG);
+}
+
+__device__ __forceinline__ double areaOf(const O4View& v, int d, int i)
+{
+    return v.radial[2*(i+6) + (d == 0 ? 0 : 1)];
+}
+
+__device__ __forceinline__ double radialPoint(const O4View& v, int d, int i)
+{
+    if (d == 0) return areaOf(v, 0, i);
+    return areaOf(v, d, i) - (1.0/24.0)*(areaOf(v, d, i+1) - 2.0*areaOf(v, d, i) + areaOf(v, d, i-1));
+}
+
+struct BoxIdx
+{
+    int lo[3], e[3];
+    long long base, ecount;
+    __device__ BoxIdx(const BoxDesc& b)
+    {
+        for (int d = 0; d < 3; ++d) { lo[d] = b.lo[d]; e[d] = b.n[d] + 2*kG; }
+        base = b.sOff;
+        ecount = (long long)e[0]*e[1]*e[2];
+    }
+    __device__ long long operator()(int f, int i, int j, int k) const
+    {
+        return base + f*ecount + ((long long)(k-lo[2]+kG)*e[1] + (j-lo[1]+kG))*e[0] + (i-lo[0]+kG);
+    }
+};
+
+// Region iteration: thread t of the box's region [lo-gl, hi+gh] (+ext along d).
+__device__ __forceinline__ bool regionPoint(const BoxDesc& b, const int L[3], const int H[3], long long t, int x[3])
+{
+    const int n0 = H[0]-L[0]+1, n1 = H[1]-L[1]+1, n2 = H[2]-L[2]+1;
+    if (t >= (long long)n0*n1*n2) return false;
+    x[0] = L[0] + (int)(t % n0);
+    x[1] = L[1] + (int)((t / n0) % n1);
+    x[2] = L[2] + (int)(t / ((long long)n0*n1));
+    return true;
+}
+
+__device__ void energiesNative(const double* A, const double w[8], double& ke, double& me, double& vb)
+{
+    double vc[3] = {0, 0, 0}, bc[3] = {0, 0, 0};
+    ke = 0; me = 0; vb = 0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+    {
+#pragma unroll
+        for (int b = 0; b < 3; ++b) { vc[a] += A[3*a+b]*w[1+b]; bc[a] += A[3*a+b]*w[5+b]; }
+        ke += vc[a]*vc[a];
+        me += bc[a]*bc[a]/8.0/hc::kPi;
+        vb += vc[a]*bc[a];
+    }
+    ke *= w[0]*.5;
+}
+
+__device__ __forceinline__ void sphToPrimPoint(double w[8], const double u[8])
+{
+    w[0] = fmax(u[0], hc::kRhoFloor);
+    w[1] = u[1]/w[0]; w[2] = u[2]/w[0]; w[3] = u[3]/w[0];
+    w[4] = u[4]; w[5] = u[5]; w[6] = u[6]; w[7] = u[7];
+}
+
+__device__ __forceinline__ bool usablePrimitive(const double w[8])
+{
+    bool f = true;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) f = f && isfinite(w[c]);
+    return f && w[0] >= hc::kRhoFloor && w[4] >= 0.0;
+}
+
+__device__ void primFlux(double f[8], const double w[8], double ke, double me, double vb, double gamma, int d)
+{
+    const int t1 = (d+1)%3, t2 = (d+2)%3;
+    const double vn = w[d+1], vt1 = w[t1+1], vt2 = w[t2+1];
+    const double bn = w[d+5], bt1 = w[t1+5], bt2 = w[t2+5];
+    f[0] = w[0]*vn;
+    f[d+1] = f[0]*vn - bn*bn/4.0/hc::kPi;
+    f[t1+1] = f[0]*vt1 - bn*bt1/4.0/hc::kPi;
+    f[t2+1] = f[0]*vt2 - bn*bt2/4.0/hc::kPi;
+    f[4] = (2.0*me + ke + w[4]*gamma/(gamma-1.0))*vn - bn*vb/4.0/hc::kPi;
+    f[5] = vn*w[5] - w[1]*bn;
+    f[6] = vn*w[6] - w[2]*bn;
+    f[7] = vn*w[7] - w[3]*bn;
+}
+
+// Extremum-preserving limiter of one component of one cell,
+// src/BoxOp_Limiters.H:28-148 (La: left state of the cell's high face,
+// Rr: right state of its low face).
+__device__ void limitComponent(double& La, double& Rr, double W, double Wa, double Wa2, double Wb, double Wb2,
+                               double d2, double d2a, double d2b, double d3, double d3b, double d3a, double d3a2)
+{
+    const double rhs = 1.0-1.0e-12;
+    const double dfm = W-Rr, dfp = La-W;
+    const double d2f = 6.0*(Rr-2.0*W+La);
+    double d2lim;
+    if (d2b >= 0.0 && d2 >= 0.0 && d2a >= 0.0 && d2f >= 0.0)
+        d2lim = fmin(fmin(fabs(d2f), 1.25*fabs(d2b)), fmin(1.25*fabs(d2), 1.25*fabs(d2a)));
+    else if (d2b < 0.0 && d2 < 0.0 && d2a < 0.0 && d2f < 0.0)
+        d2lim = -1.0*fmin(fmin(fabs(d2f), 1.25*fabs(d2b)), fmin(1.25*fabs(d2), 1.25*fabs(d2a)));
+    else d2lim = 0.0;
+    const double rhsRho = 1.0e-12*fmax(fmax(fmax(fabs(Wb2), fabs(Wb)), fmax(fabs(W), fabs(Wa))), fabs(Wa2));
+    const double rho = fabs(d2f) <= rhsRho ? 0.0 : d2lim/d2f;
+    const double d3min = fmin(fmin(d3b, d3), fmin(d3a, d3a2)), d3max = fmax(fmax(d3b, d3), fmax(d3a, d3a2));
+    if ((dfm*dfp <= 0.0) || ((W-Wb2)*(Wa2-W) <= 0.0))
+    {
+        if (rho < rhs)
+        {
+            const double rt = d3max-d3min, lt = 0.1*fmax(fabs(d3min), fabs(d3max));
+            if (lt <= rt)
+            {
+                double r2 = 2.0*fabs(dfm), l2 = fabs(dfp);
+                if (dfm*dfp < 0.0) La = W+rho*dfp;
+                else if (l2 >= r2) La = W+2.0*(1.0-rho)*dfm+rho*dfp;
+                r2 = 2.0*fabs(dfp); l2 = fabs(dfm);
+                if (dfm*dfp < 0.0) Rr = W-rho*dfm;
+                else if (l2 >= r2) Rr = W-2.0*(1.0-rho)*dfp-rho*dfm;
+            }
+        }
+    }
+    else
+    {
+        double r2 = 2.0*fabs(dfm), l2 = fabs(dfp);
+        if (l2 >= r2) La = W+2.0*dfm;
+        r2 = 2.0*fabs(dfp); l2 = fabs(dfm);
+        if (l2 >= r2) Rr = W-2.0*dfp;
+    }
+}
+
+// Flattening indicator of one direction, src/BoxOp_Limiters.H:151-188.
+__device__ double etaTilde(const double* pb2, const double* pb, const double* pa, const double* pa2,
+                           double vb1, double va1)
+{
+    const double z1 = 0.85, z0 = 0.75, delta = 0.33;
+    double si = 0;
+    const double p_a = pa[0], p_b = pb[0], p_a2 = pa2[0], p_b2 = pb2[0];
+    const double arg = fabs(p_a-p_b)/fabs(p_a2-p_b2);
+    if (arg > z1) si = 0.0;
+    if (arg < z0) si = 1.0;
+    if ((arg <= z1) && (arg >= z0)) si = 1.0-(arg-z0)/(z1-z0);
+    const double lhs1 = vb1-va1;
+    const double lhs2 = fabs(p_a-p_b)/fmin(p_a, p_b);
+    return (lhs1 > 0.0 && lhs2 > delta) ? si : 1.0;
+}
+
+// Mapped and factored Roe dissipation at a face point, src/MHD_Roe8Wave.H:634-740.
+template <typename S>
+__device__ void factoredRoe(double out[8], const double wl[8], const double wr[8], const double* Am,
+                            double radialArea, int dir, double gamma, double efix)
+{
+    double A[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) A[a][b] = Am[3*a+b];
+    const int t1 = (dir+1)%3, t2 = (dir+2)%3;
+    double frame[3][3];
+    double normal2 = 0, tangent2 = 0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+    {
+        const int j = (i+1)%3, k = (i+2)%3;
+        frame[0][i] = A[j][t1]*A[k][t2]-A[k][t1]*A[j][t2];
+        frame[1][i] = A[i][t1];
+        normal2 += frame[0][i]*frame[0][i];
+        tangent2 += frame[1][i]*frame[1][i];
+    }
+    const double nm = sqrt(normal2), tm = sqrt(tangent2);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) { frame[0][i] /= nm; frame[1][i] /= tm; }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) { const int j = (i+1)%3, k = (i+2)%3; frame[2][i] = frame[0][j]*frame[1][k]-frame[0][k]*frame[1][j]; }
+    double loc[2][8];
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+    {
+        const double* w = s == 0 ? wl : wr;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) loc[s][c] = 0;
+        loc[s][0] = fmax(w[0], hc::kRhoFloor);
+        loc[s][4] = fmax(w[4], 0.0);
+        double vc[3] = {0, 0, 0}, bc[3] = {0, 0, 0};
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) { vc[i] += A[i][j]*w[1+j]; bc[i] += A[i][j]*w[5+j]; }
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+#pragma unroll
+            for (int i = 0; i < 3; ++i) { loc[s][1+d] += frame[d][i]*vc[i]; loc[s][5+d] += frame[d][i]*bc[i]; }
+    }
+    S ds[8];
+    if (!hc::roe8WaveDissipation<S>(ds, loc[0], loc[1], gamma, efix))
+        hc::rusanovDissipation<S>(ds, loc[0], loc[1], gamma);
+    double diss[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) diss[c] = double(ds[c]);
+    const double area = radialArea*nm;
+    double mapped[8];
+    mapped[0] = area*diss[0]; mapped[4] = area*diss[4];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+    {
+        mapped[1+i] = 0; mapped[5+i] = 0;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) { mapped[1+i] += area*frame[d][i]*diss[1+d]; mapped[5+i] += area*frame[d][i]*diss[5+d]; }
+    }
+    double adj[3][3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+    {
+        const int u1 = (d+1)%3, u2 = (d+2)%3;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) { const int j = (i+1)%3, k = (i+2)%3; adj[d][i] = A[j][u1]*A[k][u2]-A[k][u1]*A[j][u2]; }
+    }
+    const double det = A[0][0]*adj[0][0]+A[1][0]*adj[0][1]+A[2][0]*adj[0][2];
+    out[0] = mapped[0]/det; out[4] = mapped[4]/det;
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+    {
+        out[1+d] = 0; out[5+d] = 0;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) { out[1+d] += adj[d][i]*mapped[1+i]/(det*det); out[5+d] += adj[d][i]*mapped[5+i]/(det*det); }
+    }
+}
+
+// ---- kernels ------------------------------------------------------------------
+// 1. Cell-average primitives WBar and their magnetic energy on box+5.
+__global__ void __launch_bounds__(kB) k_o4_wbar(O4View v, const double* __restrict__ U, double* __restrict__ S)
+{
+    const BoxDesc& b = v.boxes[blockIdx.y];
+    int L[3], H[3], x[3];
+    for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]-kG; H[d] = b.hi[d]+kG; }
+    if (!regionPoint(b, L, H, (long long)blockIdx.x*blockDim.x + threadIdx.x, x)) return;
+    const BoxIdx I(b);
+    const long long e = I.ecount;
+    const long long u = b.uOff + ((long long)(x[2]-L[2])*(b.n[1]+2*kG) + (x[1]-L[1]))*(b.n[0]+2*kG) + (x[0]-L[0]);
+    double uu[8], w[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) uu[c] = U[u + c*e];
+    sphToPrimPoint(w, uu);
+    double ke, me, vb;
+    energiesNative(v.cellA + 9*angIdx(v, b.p, x[1], x[2]), w, ke, me, vb);
+    w[4] = (w[4]-ke-me)*(v.gamma-1.0);
+    if (w[4] < 0) w[4] = 0.0;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) S[I(F_WB+c, x[0], x[1], x[2])] = w[c];
+    S[I(F_MEB, x[0], x[1], x[2])] = me;
+}
+
+// 2. Point primitives W on box+4 with the complete-state fallbacks,
+// src/BoxOp_Utilities.H:619-696.
+__global__ void __launch_bounds__(kB) k_o4_w(O4View v, const double* __restrict__ U, double* __restrict__ S)
+{
+    const BoxDesc& b = v.boxes[blockIdx.y];
+    int L[3], H[3], x[3];
+    for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]-4; H[d] = b.hi[d]+4; }
+    if (!regionPoint(b, L, H, (long long)blockIdx.x*blockDim.x + threadIdx.x, x)) return;
+    const BoxIdx I(b);
+    const int e0 = b.n[0]+2*kG, e1 = b.n[1]+2*kG;
+    const long long e = I.ecount;
+    auto ui = [&](int i, int j, int k) {
+        return b.uOff + ((long long)(k-b.lo[2]+kG)*e1 + (j-b.lo[1]+kG))*e0 + (i-b.lo[0]+kG); };
+    double up[8], uc[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+    {
+        const long long o = c*e;
+        const double lap = U[ui(x[0]+1,x[1],x[2])+o]+U[ui(x[0]-1,x[1],x[2])+o]+U[ui(x[0],x[1]+1,x[2])+o]
+                          +U[ui(x[0],x[1]-1,x[2])+o]+U[ui(x[0],x[1],x[2]+1)+o]+U[ui(x[0],x[1],x[2]-1)+o]
+                          -6.0*U[ui(x[0],x[1],x[2])+o];
+        uc[c] = U[ui(x[0],x[1],x[2])+o];
+        up[c] = uc[c]-lap/24.0;
+    }
+    if (!(isfinite(up[0]) && up[0] >= hc::kRhoFloor) && isfinite(uc[0]) && uc[0] >= hc::kRhoFloor)
+#pragma unroll
+        for (int c = 0; c < 8; ++c) up[c] = uc[c];
+    double w[8];
+    sphToPrimPoint(w, up);
+    double ke, me, vb;
+    energiesNative(v.cellA + 9*angIdx(v, b.p, x[1], x[2]), w, ke, me, vb);
+    w[4] = (w[4]-ke-me)*(v.gamma-1.0);
+    double wb[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) wb[c] = S[I(F_WB+c, x[0], x[1], x[2])];
+    if (!usablePrimitive(w) && usablePrimitive(wb))
+#pragma unroll
+        for (int c = 0; c < 8; ++c) w[c] = wb[c];
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+    {
+        const int f = F_WB+c;
+        const double lap = S[I(f,x[0]+1,x[1],x[2])]+S[I(f,x[0]-1,x[1],x[2])]+S[I(f,x[0],x[1]+1,x[2])]
+                          +S[I(f,x[0],x[1]-1,x[2])]+S[I(f,x[0],x[1],x[2]+1)]+S[I(f,x[0],x[1],x[2]-1)]
+                          -6.0*S[I(f,x[0],x[1],x[2])];
+        S[I(F_W+c, x[0], x[1], x[2])] = w[c]+lap/24.0;
+    }
+}
+
+// 3. Flattening coefficient eta on box+2: minimum over the three directions
+// of the three-cell minimum of eta_tilde, src/BoxOp_Limiters.H:304-334.
+__global__ void __launch_bounds__(kB) k_o4_eta(O4View v, double* __restrict__ S)
+{
+    const BoxDesc& b = v.boxes[blockIdx.y];
+    int L[3], H[3], x[3];
+    for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]-2; H[d] = b.hi[d]+2; }
+    if (!regionPoint(b, L, H, (long long)blockIdx.x*blockDim.x + threadIdx.x, x)) return;
+    const BoxIdx I(b);
+    double e = 0;
+    for (int dd = 0; dd < 3; ++dd)
+    {
+        double ed = 0;
+        for (int o = 0; o < 3; ++o)
+        {
+            const int oo = o == 0 ? 0 : (o == 1 ? 1 : -1);
+            int y[3] = {x[0], x[1], x[2]};
+            y[dd] += oo;
+            double pr[5], vel[5];
+            for (int q = -2; q <= 2; ++q)
+            {
+                int z[3] = {y[0], y[1], y[2]};
+                z[dd] += q;
+                pr[q+2] = S[I(F_WB+4, z[0], z[1], z[2])] + S[I(F_MEB, z[0], z[1], z[2])];
+                vel[q+2] = S[I(F_WB+1+dd, z[0], z[1], z[2])];
+            }
+            const double et = etaTilde(&pr[0], &pr[1], &pr[3], &pr[4], vel[1], vel[3]);
+            ed = o == 0 ? et : fmin(ed, et);
+        }
+        e = dd == 0 ? ed : fmin(ed, e);
+    }
+    S[I(F_ETA, x[0], x[1], x[2])] = e;
+}
+
+// Face-average state WFaceBar at face x (direction d): (WBar_{x-e} + WBar_x)/2.
+__device__ __forceinline__ void faceBar(const BoxIdx& I, const double* S, int d, int i, int j, int k, double wf[8])
+{
+    const int a = i-(d == 0), b = j-(d == 1), c = k-(d == 2);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) wf[q] = 0.5*S[I(F_WB+q, a, b, c)] + 0.5*S[I(F_WB+q, i, j, k)];
+}
+
+// 4. Limited, flattened and repaired face states; point states by transverse
+// deconvolution. Faces [lo_d, hi_d+1] x transverse [lo-2, hi+2].
+template <int D>
+__global__ void __launch_bounds__(kB) k_o4_faces(O4View v, double* __restrict__ S)
+{
+    const BoxDesc& b = v.boxes[blockIdx.y];
+    int L[3], H[3], x[3];
+    for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]-2; H[d] = b.hi[d]+2; }
+    L[D] = b.lo[D]; H[D] = b.hi[D]+1;
+    if (!regionPoint(b, L, H, (long long)blockIdx.x*blockDim.x + threadIdx.x, x)) return;
+    const BoxIdx I(b);
+    constexpr int t1 = D == 0 ? 1 : 0, t2 = D == 2 ? 1 : 2;
+    auto Wat = [&](int c, int o) {
+        int y[3] = {x[0], x[1], x[2]}; y[D] += o;
+        return S[I(F_W+c, y[0], y[1], y[2])]; };
+    double lf[8], rf[8];
+#pragma unroll 1
+    for (int c = 0; c < 8; ++c)
+    {
+        double w[8];   // W at offsets -4..+3 from the face's right cell
+#pragma unroll
+        for (int o = -4; o <= 3; ++o) w[o+4] = Wat(c, o);
+        // unlimited 4th-order face values at faces x-1, x, x+1 (index relative to x)
+        auto c2f = [&](int f) { return (7.0/12.0)*(w[f-1+4]+w[f+4]) - (1.0/12.0)*(w[f-2+4]+w[f+1+4]); };
+        double left = c2f(0), right = c2f(0);
+        if (v.limiter == 1)
+        {
+            // cell x-1 (its high face is x) and cell x (its low face is x)
+            for (int side = 0; side < 2; ++side)
+            {
+                const int cc = side == 0 ? -1 : 0;   // cell offset
+                double La = c2f(cc+1), Rr = c2f(cc);
+                auto ww = [&](int o) { return w[cc+o+4]; };
+                auto del2 = [&](int o) { return ww(o-1)-2.0*ww(o)+ww(o+1); };
+                auto del3 = [&](int o) { return del2(o)-del2(o-1); };
+                limitComponent(La, Rr, ww(0), ww(1), ww(2), ww(-1), ww(-2),
+                               del2(0), del2(1), del2(-1), del3(0), del3(-1), del3(1), del3(2));
+                int y[3] = {x[0], x[1], x[2]}; y[D] += cc;
+                const double et = S[I(F_ETA, y[0], y[1], y[2])];
+                const double wc = ww(0);
+                if (side == 0) left = et*La+(1.-et)*wc;
+                else right = et*Rr+(1.-et)*wc;
+            }
+        }
+        lf[c] = left; rf[c] = right;
+    }
+    // Roe repair with the cell-average donors, src/BoxOp_RoeCubedSphere.H:5-27,
+    // then point values by transverse deconvolution.
+    double corr[8], lapb[8];
+    {
+        double wfx[8], a[8], bq[8];
+        faceBar(I, S, D, x[0], x[1], x[2], wfx);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) lapb[q] = 0.0;
+        for (int t = 0; t < 2; ++t)
+        {
+            const int tt = t == 0 ? t1 : t2;
+            int y[3] = {x[0], x[1], x[2]}, z[3] = {x[0], x[1], x[2]};
+            y[tt] += 1; z[tt] -= 1;
+            faceBar(I, S, D, y[0], y[1], y[2], a);
+            faceBar(I, S, D, z[0], z[1], z[2], bq);
+#pragma unroll
+            for (int q = 0; q < 8; ++q) lapb[q] += a[q] - 2.0*wfx[q] + bq[q];
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) corr[q] = lapb[q]*(1.0/24.0);
+    }
+    {   // Roe is the fourth-order Riemann solver here, so the repair always applies
+        double pl[8], pr[8], cl[8], cr[8];
+        int m1[3] = {x[0], x[1], x[2]}; m1[D] -= 1;
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+        {
+            pl[q] = lf[q]-lapb[q]/24.0;
+            pr[q] = rf[q]-lapb[q]/24.0;
+            cl[q] = S[I(F_WB+q, m1[0], m1[1], m1[2])];
+            cr[q] = S[I(F_WB+q, x[0], x[1], x[2])];
+        }
+        if (!(usablePrimitive(pl) && usablePrimitive(pr)) && usablePrimitive(cl) && usablePrimitive(cr))
+#pragma unroll
+            for (int q = 0; q < 8; ++q) { lf[q] = cl[q]+corr[q]; rf[q] = cr[q]+corr[q]; }
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+    {
+        S[I(F_PTL+q, x[0], x[1], x[2])] = lf[q]-lapb[q]/24.0;
+        S[I(F_PTR+q, x[0], x[1], x[2])] = rf[q]-lapb[q]/24.0;
+    }
+    S[I(F_P0L, x[0], x[1], x[2])] = lf[4];
+    S[I(F_P0R, x[0], x[1], x[2])] = rf[4];
+}
+
+// Energies (ke, me, v.B) and the physical flux of the face-average state at
+// face (i,j,k) of direction D.
+template <int D>
+__device__ void barFlux(const O4View& v, const BoxDesc& b, const BoxIdx& I, const double* S,
+                        int i, int j, int k, double f[8], double& meBar, double& pBar)
+{
+    double wf[8];
+    faceBar(I, S, D, i, j, k, wf);
+    double ke, me, vb;
+    energiesNative(v.faceA[D] + 9*angIdx(v, b.p, j, k), wf, ke, me, vb);
+    primFlux(f, wf, ke, me, vb, v.gamma, D);
+    meBar = me;
+    pBar = wf[4];
+}
+
+// 5. Physical fluxes of both face states (point flux + transverse Laplacian of
+// the face-average flux / 24), face-average total pressure p + me, and the
+// point Roe dissipation. Faces [lo_d, hi_d+1] x transverse [lo-2, hi+2].
+template <int D, typename S_>
+__global__ void __launch_bounds__(kB) k_o4_phys(O4View v, double* __restrict__ S)
+{
+    const BoxDesc& b = v.boxes[blockIdx.y];
+    int L[3], H[3], x[3];
+    for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]-2; H[d] = b.hi[d]+2; }
+    L[D] = b.lo[D]; H[D] = b.hi[D]+1;
+    if (!regionPoint(b, L, H, (long long)blockIdx.x*blockDim.x + threadIdx.x, x)) return;
+    const BoxIdx I(b);
+    constexpr int t1 = D == 0 ? 1 : 0, t2 = D == 2 ? 1 : 2;
+    // transverse Laplacian of the face-average flux and magnetic energy
+    double fx[8], lapF[8], mex, px, lapMe = 0;
+    barFlux<D>(v, b, I, S, x[0], x[1], x[2], fx, mex, px);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) lapF[q] = 0;
+    for (int t = 0; t < 2; ++t)
+    {
+        const int tt = t == 0 ? t1 : t2;
+        int y[3] = {x[0], x[1], x[2]}, z[3] = {x[0], x[1], x[2]};
+        y[tt] += 1; z[tt] -= 1;
+        double fa[8], fb[8], ma, mb, pa, pb;
+        barFlux<D>(v, b, I, S, y[0], y[1], y[2], fa, ma, pa);
+        barFlux<D>(v, b, I, S, z[0], z[1], z[2], fb, mb, pb);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) lapF[q] += fa[q] - 2.0*fx[q] + fb[q];
+        lapMe += ma - 2.0*mex + mb;
+    }
+    const double* A = v.faceA[D] + 9*angIdx(v, b.p, x[1], x[2]);
+    for (int side = 0; side < 2; ++side)
+    {
+        double w[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) w[q] = S[I((side == 0 ? F_PTL : F_PTR)+q, x[0], x[1], x[2])];
+        double ke, me, vb, f[8];
+        energiesNative(A, w, ke, me, vb);
+        primFlux(f, w, ke, me, vb, v.gamma, D);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) S[I((side == 0 ? F_PHL : F_PHR)+q, x[0], x[1], x[2])] = f[q]+lapF[q]/24.0;
+        const int fp = side == 0 ? F_P0L : F_P0R;
+        S[I(fp, x[0], x[1], x[2])] = S[I(fp, x[0], x[1], x[2])] + (me + lapMe/24.0);
+    }
+    bool inner = true;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) if (d != D) inner = inner && x[d] >= b.lo[d]-1 && x[d] <= b.hi[d]+1;
+    if (inner)
+    {
+        double wl[8], wr[8], out[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) { wl[q] = S[I(F_PTL+q, x[0], x[1], x[2])]; wr[q] = S[I(F_PTR+q, x[0], x[1], x[2])]; }
+        factoredRoe<S_>(out, wl, wr, A, radialPoint(v, D, x[0]), D, v.gamma, v.entropy);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) S[I(F_DPT+q, x[0], x[1], x[2])] = out[q];
+    }
+}
+
+// 6. Mapped face flux on the flux faces [lo_d, hi_d+1] x [lo, hi]:
+// face products with det A / det(A) A, the radial-area tensor product, the
+// pressure force through the cofactor row, the averaged Roe dissipation, and
+// F = (F_L + F_R - D)/2.
+template <int D>
+__global__ void __launch_bounds__(kB) k_o4_flux(O4View v, double* __restrict__ S)
+{
+    const BoxDesc& b = v.boxes[blockIdx.y];
+    int L[3], H[3], x[3];
+    for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]; H[d] = b.hi[d]; }
+    H[D] = b.hi[D]+1;
+    if (!regionPoint(b, L, H, (long long)blockIdx.x*blockDim.x + threadIdx.x, x)) return;
+    const BoxIdx I(b);
+    constexpr int t1 = D == 0 ? 1 : 0, t2 = D == 2 ? 1 : 2;
+    const double* met = v.fmet[D];
+    auto M = [&](int c, const int y[3]) { return met[13*angIdx(v, b.p, y[1], y[2]) + c]; };
+    auto Dm = [&](int c, int t, const int y[3]) {
+        int a[3] = {y[0], y[1], y[2]}, z[3] = {y[0], y[1], y[2]};
+        a[t] += 1; z[t] -= 1;
+        return 0.5*M(c, a) - 0.5*M(c, z); };
+    auto Fv = [&](int f, const int y[3]) { return S[I(f, y[0], y[1], y[2])]; };
+    auto Df = [&](int f, int t, const int y[3]) {
+        int a[3] = {y[0], y[1], y[2]}, z[3] = {y[0], y[1], y[2]};
+        a[t] += 1; z[t] -= 1;
+        return 0.5*Fv(f, a) - 0.5*Fv(f, z); };
+    // <det A F> and <det(A) A F> of the physical flux of one side at point y
+    auto mapped = [&](int base, const int y[3], double out[8]) {
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+        {
+            for (int grp = 0; grp < 2; ++grp)
+            {
+                const int c0 = grp == 0 ? 1 : 5;
+                double val = 0;
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+                {
+                    val += Dm(1+3*r+c, t1, y)*Df(base+c0+c, t1, y)*(1.0/12.0);
+                    val += Dm(1+3*r+c, t2, y)*Df(base+c0+c, t2, y)*(1.0/12.0);
+                    val += M(1+3*r+c, y)*Fv(base+c0+c, y);
+                }
+                out[c0+r] = val;
+            }
+        }
+        for (int s = 0; s < 2; ++s)
+        {
+            const int c = s == 0 ? 0 : 4;
+            double val = 0;
+            val += Dm(0, t1, y)*Df(base+c, t1, y)*(1.0/12.0);
+            val += Dm(0, t2, y)*Df(base+c, t2, y)*(1.0/12.0);
+            val += M(0, y)*Fv(base+c, y);
+            out[c] = val;
+        }
+    };
+    const double adjX = areaOf(v, D, x[0]);
+    // radial-area differences in the transverse directions (only radial varies)
+    auto adjD = [&](int t) {
+        if (t != 0) return 0.5*adjX - 0.5*adjX;
+        return 0.5*areaOf(v, D, x[0]+1) - 0.5*areaOf(v, D, x[0]-1); };
+    double Fsum[8];
+    for (int side = 0; side < 2; ++side)
+    {
+        const int base = side == 0 ? F_PHL : F_PHR;
+        double mx[8], ma[2][8], mb[2][8];
+        mapped(base, x, mx);
+        for (int t = 0; t < 2; ++t)
+        {
+            const int tt = t == 0 ? t1 : t2;
+            int y[3] = {x[0], x[1], x[2]}, z[3] = {x[0], x[1], x[2]};
+            y[tt] += 1; z[tt] -= 1;
+            mapped(base, y, ma[t]);
+            mapped(base, z, mb[t]);
+        }
+        double Fs[8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+        {
+            double val = 0;
+            val += adjD(t1)*(0.5*ma[0][c] - 0.5*mb[0][c])*(1.0/12.0);
+            val += adjD(t2)*(0.5*ma[1][c] - 0.5*mb[1][c])*(1.0/12.0);
+            val += adjX*mx[c];
+            Fs[c] = val;
+        }
+        // pressure force: p0 (face average) with p0Bar = pBar + meBar
+        const int fp = side == 0 ? F_P0L : F_P0R;
+        double fdum[8], meb, pb;
+        barFlux<D>(v, b, I, S, x[0], x[1], x[2], fdum, meb, pb);
+        double p0bD[2], radPD[2];
+        for (int t = 0; t < 2; ++t)
+        {
+            const int tt = t == 0 ? t1 : t2;
+            int y[3] = {x[0], x[1], x[2]}, z[3] = {x[0], x[1], x[2]};
+            y[tt] += 1; z[tt] -= 1;
+            double mya, pya, myb, pyb;
+            barFlux<D>(v, b, I, S, y[0], y[1], y[2], fdum, mya, pya);
+            barFlux<D>(v, b, I, S, z[0], z[1], z[2], fdum, myb, pyb);
+            const double pa = pya + mya, pz = pyb + myb;
+            p0bD[t] = 0.5*pa - 0.5*pz;
+            radPD[t] = 0.5*(pa*areaOf(v, D, y[0])) - 0.5*(pz*areaOf(v, D, z[0]));
+        }
+        double tmp3 = 0;
+        tmp3 += adjD(t1)*p0bD[0]*(1.0/12.0);
+        tmp3 += adjD(t2)*p0bD[1]*(1.0/12.0);
+        tmp3 += adjX*Fv(fp, x);
+        for (int r = 0; r < 3; ++r)
+        {
+            double pf = 0;
+            pf += radPD[0]*Dm(10+r, t1, x)*(1.0/12.0);
+            pf += radPD[1]*Dm(10+r, t2, x)*(1.0/12.0);
+            pf += tmp3*M(10+r, x);
+            Fs[1+r] += pf;
+        }
+#pragma unroll
+        for (int c = 0; c < 8; ++c) Fsum[c] = side == 0 ? Fs[c] : Fsum[c] + Fs[c];
+    }
+    // averaged point dissipation and its metric products
+    double Davg[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+    {
+        double lap = 0;
+        for (int t = 0; t < 2; ++t)
+        {
+            const int tt = t == 0 ? t1 : t2;
+            int y[3] = {x[0], x[1], x[2]}, z[3] = {x[0], x[1], x[2]};
+            y[tt] += 1; z[tt] -= 1;
+            lap += Fv(F_DPT+c, y) - 2.0*Fv(F_DPT+c, x) + Fv(F_DPT+c, z);
+        }
+        Davg[c] = Fv(F_DPT+c, x) + lap/24.0;
+    }
+    double Fd[8];
+    for (int s = 0; s < 2; ++s)
+    {
+        const int c = s == 0 ? 0 : 4;
+        double val = 0;
+        val += Dm(0, t1, x)*Df(F_DPT+c, t1, x)*(1.0/12.0);
+        val += Dm(0, t2, x)*Df(F_DPT+c, t2, x)*(1.0/12.0);
+        val += M(0, x)*Davg[c];
+        Fd[c] = val;
+    }
+    for (int r = 0; r < 3; ++r)
+        for (int grp = 0; grp < 2; ++grp)
+        {
+            const int c0 = grp == 0 ? 1 : 5;
+            double val = 0;
+            for (int c = 0; c < 3; ++c)
+            {
+                val += Dm(1+3*r+c, t1, x)*Df(F_DPT+c0+c, t1, x)*(1.0/12.0);
+                val += Dm(1+3*r+c, t2, x)*Df(F_DPT+c0+c, t2, x)*(1.0/12.0);
+                val += M(1+3*r+c, x)*Davg[c0+c];
+            }
+            Fd[c0+r] = val;
+        }
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+        S[I(F_FLUX+8*D+c, x[0], x[1], x[2])] = 0.5*(Fsum[c]-Fd[c]);
+}
+
+// 7. Flux divergence accumulated into the box RHS (the caller zeroes it).
+__global__ void __launch_bounds__(kB) k_o4_div(O4View v, const double* __restrict__ S, double* __restrict__ R)
+{
+    const BoxDesc& b = v.boxes[blockIdx.y];
+    int x[3];
+    if (!regionPoint(b, b.lo, b.hi, (long long)blockIdx.x*blockDim.x + threadIdx.x, x)) return;
+    const BoxIdx I(b);
+    const long long n = (long long)b.n[0]*b.n[1]*b.n[2];
+    const long long r = b.rOff + ((long long)(x[2]-b.lo[2])*b.n[1] + (x[1]-b.lo[1]))*b.n[0] + (x[0]-b.lo[0]);
+    for (int c = 0; c < 8; ++c)
+    {
+        double acc = R[r + c*n];
+        for (int d = 0; d < 3; ++d)
+        {
+            const double dxInv = -1.0/(d == 0 ? 1.0/v.nr : 1.0/v.na);
+            int a[3] = {x[0], x[1], x[2]}; a[d] += 1;
+            double inc = 0;
+            inc += (-1.0*dxInv)*S[I(F_FLUX+8*d+c, x[0], x[1], x[2])];
+            inc += (1.0*dxInv)*S[I(F_FLUX+8*d+c, a[0], a[1], a[2])];
+            acc += inc;
+        }
+        R[r + c*n] = acc;
+    }
+}
+
+// 8. Copy the face fluxes of the valid box extruded in d out of scratch.
+__global__ void __launch_bounds__(kB) k_o4_fluxout(O4View v, int D, const double* __restrict__ S,
+                                                   double* __restrict__ F, const long long* __restrict__ fOff)
+{
+    const BoxDesc& b = v.boxes[blockIdx.y];
+    int L[3] = {b.lo[0], b.lo[1], b.lo[2]}, H[3] = {b.hi[0], b.hi[1], b.hi[2]}, x[3];
+    H[D] += 1;
+    if (!regionPoint(b, L, H, (long long)blockIdx.x*blockDim.x + threadIdx.x, x)) return;
+    const BoxIdx I(b);
+    const int m0 = b.n[0]+(D == 0), m1 = b.n[1]+(D == 1), m2 = b.n[2]+(D == 2);
+    const long long n = (long long)m0*m1*m2;
+    const long long o = fOff[blockIdx.y] + ((long long)(x[2]-b.lo[2])*m1 + (x[1]-b.lo[1]))*m0 + (x[0]-b.lo[0]);
+    for (int c = 0; c < 8; ++c) F[o + c*n] = S[I(F_FLUX+8*D+c, x[0], x[1], x[2])];
+}
+
+template <typename T>
+T* upload(const std::vector<T>& h, std::vector<void*>& owned)
+{
+    T* p = nullptr;
+    HC_THROW(hipMalloc(&p, std::max<size_t>(1, h.size())*sizeof(T)));
+    owned.push_back(p);
+    if (!h.empty()) HC_THROW(hipMemcpy(p, h.data(), h.size()*sizeof(T), hipMemcpyHostToDevice));
+    return p;
+}
+} // namespace
+
+struct hc_boxop
+{
+    hc_options opt{};
+    hc::ShellDims dims{};
+    int nbox = 0;
+    std::vector<BoxDesc> boxes;
+    O4View view{};
+    double* scratch = nullptr;
+    long long* fOff[3] = {nullptr, nullptr, nullptr};
+    std::vector<long long> fOffHost[3];
+    long long uCount = 0, rCount = 0, fCount[3] = {0, 0, 0};
+    int maxRegion = 0;
+    std::vector<void*> owned;
+    ~hc_boxop() { for (void* p : owned) (void)hipFree(p); }
+};
+
+namespace
+{
+void buildBoxop(hc_boxop& B, const hc_box* boxes, int nbox)
+{
+    const int nr = B.dims.nr, na = B.dims.na;
+    const int naE = na + 2*kG + 1;
+    O4View& v = B.view;
+    v.nr = nr; v.na = na; v.naE = naE; v.limiter = B.opt.limiter;
+    v.gamma = B.opt.gamma; v.entropy = B.opt.entropy_fix;
+    std::vector<double> cellA(9ull*6*naE*naE), faceA[3], fmet[3];
+    for (int d = 0; d < 3; ++d) { faceA[d].resize(9ull*6*naE*naE); fmet[d].resize(13ull*6*naE*naE); }
+    const double h = 1.0/na;
+    for (int p = 0; p < 6; ++p)
+        for (int kk = 0; kk < naE; ++kk)
+            for (int jj = 0; jj < naE; ++jj)
+            {
+                const int j = jj-kG, k = kk-kG;
+                const size_t x = ((size_t)p*naE+kk)*naE+jj;
+                double A[3][3];
+                hc::pointBasis(p, hc::xiToAngle((j+0.5)*h), hc::xiToAngle((k+0.5)*h), A);
+                for (int a = 0; a < 9; ++a) cellA[9*x+a] = A[a/3][a%3];
+                for (int d = 0; d < 3; ++d)
+                {
+                    const double oa = d == 1 ? 0.0 : 0.5, ob = d == 2 ? 0.0 : 0.5;
+                    hc::pointBasis(p, hc::xiToAngle((j+oa)*h), hc::xiToAngle((k+ob)*h), A);
+                    for (int a = 0; a < 9; ++a) faceA[d][9*x+a] = A[a/3][a%3];
+                    hc::faceMetricAverages(na, p, d, j, k, &fmet[d][13*x]);
+                }
+            }
+    v.cellA = upload(cellA, B.owned);
+    for (int d = 0; d < 3; ++d) { v.faceA[d] = upload(faceA[d], B.owned); v.fmet[d] = upload(fmet[d], B.owned); }
+    std::vector<double> radial(2ull*(nr+13));
+    for (int i = -6; i <= nr+6; ++i)
+    {
+        const hc::RadialCell c = hc::radialCell(B.dims, i);
+        radial[2*(i+6)] = c.area0; radial[2*(i+6)+1] = c.areaT;
+    }
+    v.radial = upload(radial, B.owned);
+    long long u = 0, r = 0, sc = 0;
+    for (int d = 0; d < 3; ++d) B.fOffHost[d].clear();
+    long long f[3] = {0, 0, 0};
+    for (int n = 0; n < nbox; ++n)
+    {
+        BoxDesc bd{};
+        bd.p = boxes[n].block;
+        for (int d = 0; d < 3; ++d) { bd.lo[d] = boxes[n].lo[d]; bd.hi[d] = boxes[n].hi[d]; bd.n[d] = bd.hi[d]-bd.lo[d]+1; }
+        const long long e = (long long)(bd.n[0]+2*kG)*(bd.n[1]+2*kG)*(bd.n[2]+2*kG);
+        bd.uOff = u; bd.rOff = r; bd.sOff = sc;
+        u += 8*e; r += 8LL*bd.n[0]*bd.n[1]*bd.n[2]; sc += F_COUNT*e;
+        for (int d = 0; d < 3; ++d)
+        {
+            B.fOffHost[d].push_back(f[d]);
+            f[d] += 8LL*(bd.n[0]+(d == 0))*(bd.n[1]+(d == 1))*(bd.n[2]+(d == 2));
+        }
+        B.maxRegion = std::max<long long>(B.maxRegion, e);
+        B.boxes.push_back(bd);
+    }
+    B.uCount = u; B.rCount = r;
+    for (int d = 0; d < 3; ++d) { B.fCount[d] = f[d]; B.fOff[d] = upload(B.fOffHost[d], B.owned); }
+    v.boxes = upload(B.boxes, B.owned);
+    HC_THROW(hipMalloc(&B.scratch, std::max<long long>(1, sc)*sizeof(double)));
+    B.owned.push_back(B.scratch);
+    B.nbox = nbox;
+}
+
+template <typename S_>
+void applyBoxop(hc_boxop& B, const double* U, double* R, double* const F[3], hipStream_t s)
+{
+    const O4View& v = B.view;
+    const dim3 grid((B.maxRegion + kB - 1)/kB, B.nbox);
+    hipLaunchKernelGGL(k_o4_wbar, grid, dim3(kB), 0, s, v, U, B.scratch);
+    hipLaunchKernelGGL(k_o4_w, grid, dim3(kB), 0, s, v, U, B.scratch);
+    hipLaunchKernelGGL(k_o4_eta, grid, dim3(kB), 0, s, v, B.scratch);
+    hipLaunchKernelGGL(k_o4_faces<0>, grid, dim3(kB), 0, s, v, B.scratch);
+    hipLaunchKernelGGL((k_o4_phys<0, S_>), grid, dim3(kB), 0, s, v, B.scratch);
+    hipLaunchKernelGGL(k_o4_flux<0>, grid, dim3(kB), 0, s, v, B.scratch);
+    hipLaunchKernelGGL(k_o4_faces<1>, grid, dim3(kB), 0, s, v, B.scratch);
+    hipLaunchKernelGGL((k_o4_phys<1, S_>), grid, dim3(kB), 0, s, v, B.scratch);
+    hipLaunchKernelGGL(k_o4_flux<1>, grid, dim3(kB), 0, s, v, B.scratch);
+    hipLaunchKernelGGL(k_o4_faces<2>, grid, dim3(kB), 0, s, v, B.scratch);
+    hipLaunchKernelGGL((k_o4_phys<2, S_>), grid, dim3(kB), 0, s, v, B.scratch);
+    hipLaunchKernelGGL(k_o4_flux<2>, grid, dim3(kB), 0, s, v, B.scratch);
+    hipLaunchKernelGGL(k_o4_div, grid, dim3(kB), 0, s, v, B.scratch, R);
+    for (int d = 0; d < 3; ++d)
+        if (F[d]) hipLaunchKernelGGL(k_o4_fluxout, grid, dim3(kB), 0, s, v, d, B.scratch, F[d], B.fOff[d]);
+    HC_THROW(hipGetLastError());
+}
+} // namespace
+
+extern "C" {
+
+int hc_boxop_create(const hc_shell* shell, const hc_options* options, const hc_box* boxes, int nbox,
+                    hc_boxop** out)
+{
+    if (!shell || !options || !boxes || !out || nbox <= 0) return hc::fail("hc_boxop_create: bad argument");
+    if (options->spatial_order != 4) return hc::fail("hc_boxop_create: spatial_order must be 4");
+    if (options->riemann_solver != 2) return hc::fail("hc_boxop_create: the fourth-order operator implements riemann_solver 2 (Roe)");
+    if (options->limiter != 0 && options->limiter != 1) return hc::fail("hc_boxop_create: limiter must be 0 or 1");
+    if (options->precision != 0 && options->precision != 1) return hc::fail("hc_boxop_create: precision must be 0 or 1");
+    for (int n = 0; n < nbox; ++n)
+    {
+        const hc_box& b = boxes[n];
+        if (b.block < 0 || b.block > 5) return hc::fail("hc_boxop_create: box block must be 0..5");
+        for (int d = 0; d < 3; ++d)
+        {
+            const int lim = d == 0 ? shell->n_rad : shell->n_ang;
+            if (b.lo[d] < 0 || b.hi[d] >= lim || b.hi[d] < b.lo[d] + 3)
+                return hc::fail("hc_boxop_create: box outside the panel or thinner than 4 cells");
+        }
+    }
+    try
+    {
+        std::unique_ptr<hc_boxop> B(new hc_boxop());
+        B->opt = *options;
+        B->dims = {shell->n_rad, shell->n_ang, kG, shell->inner_radius, shell->outer_radius, shell->radial_stretch};
+        buildBoxop(*B, boxes, nbox);
+        *out = B.release();
+        return 0;
+    }
+    catch (const std::exception& e) { return hc::fail(std::string("hc_boxop_create: ") + e.what()); }
+}
+
+void hc_boxop_destroy(hc_boxop* op) { delete op; }
+
+long long hc_boxop_count(const hc_boxop* op, int what)
+{
+    if (!op) return -1;
+    switch (what)
+    {
+    case 0: return op->uCount;
+    case 1: return op->rCount;
+    case 2: case 3: case 4: return op->fCount[what-2];
+    default: return -1;
+    }
+}
+
+int hc_boxop_apply(hc_boxop* op, const double* USph, double* rhs, double* F0, double* F1, double* F2,
+                   hc_stream stream)
+{
+    if (!op || !USph || !rhs) return hc::fail("hc_boxop_apply: null argument");
+    try
+    {
+        double* const F[3] = {F0, F1, F2};
+        if (op->opt.precision == 1) applyBoxop<float>(*op, USph, rhs, F, static_cast<hipStream_t>(stream));
+        else applyBoxop<double>(*op, USph, rhs, F, static_cast<hipStream_t>(stream));
+        return 0;
+    }
+    catch (const std::exception& e) { return hc::fail(std::string("hc_boxop_apply: ") + e.what()); }
+}
+
+} // extern "C"

@@ -63,6 +63,34 @@ HC_HD void plmFaceStates(R lo[8], R hi[8], const R mm[8], const R m[8],
     if (!(hi[0] >= R(kRhoFloor) && hi[4] >= R(0))) for (int k = 0; k < 8; ++k) hi[k] = c[k];
 }
 
+// Same reconstruction with the slope limiting done in precision S: the
+// one-sided differences are formed in FP64 and rounded to S, the limited
+// slope is added to the FP64 cell value. S = double reproduces plmFaceStates.
+template <typename S>
+HC_HD void plmFaceStatesMixed(double lo[8], double hi[8], const double mm[8], const double m[8],
+                              const double c[8], const double p[8], int limiter)
+{
+    for (int k = 0; k < 8; ++k)
+    {
+        const S a = S(m[k]-mm[k]), b = S(c[k]-m[k]), d = S(p[k]-c[k]);
+        S sl = S(0.5)*(a+b), sr = S(0.5)*(b+d);
+        if (limiter != 0)
+        {
+            sl = a*b <= S(0) ? S(0) : (a > S(0) ? fmin(a,b) : fmax(a,b));
+            sr = b*d <= S(0) ? S(0) : (b > S(0) ? fmin(b,d) : fmax(b,d));
+            if (limiter == 3)
+            {
+                if (a*b > S(0)) sl = (a > S(0) ? S(1) : S(-1))*fmin(S(0.5)*fabs(a+b), S(2)*fmin(fabs(a),fabs(b)));
+                if (b*d > S(0)) sr = (b > S(0) ? S(1) : S(-1))*fmin(S(0.5)*fabs(b+d), S(2)*fmin(fabs(b),fabs(d)));
+            }
+        }
+        lo[k] = m[k]+0.5*double(sl);
+        hi[k] = c[k]-0.5*double(sr);
+    }
+    if (!(lo[0] >= kRhoFloor && lo[4] >= 0.0)) for (int k = 0; k < 8; ++k) lo[k] = m[k];
+    if (!(hi[0] >= kRhoFloor && hi[4] >= 0.0)) for (int k = 0; k < 8; ++k) hi[k] = c[k];
+}
+
 // Only the state on one side of the face, with that side's fallback.
 template <typename R>
 HC_HD void plmLowSide(R lo[8], const R mm[8], const R m[8], const R c[8], int limiter)
@@ -101,16 +129,17 @@ HC_HD void plmHighSide(R hi[8], const R m[8], const R c[8], const R p[8], int li
 }
 
 // Local Lax-Friedrichs dissipation speed * (U_R - U_L) in the face frame,
-// src/MHD_Roe8Wave.H:604-629.
-template <typename R>
-HC_HD void rusanovDissipation(R out[8], const R l[8], const R r[8], R gamma)
+// src/MHD_Roe8Wave.H:604-629. Conserved states and their difference are
+// formed in FP64; the wave speed and the product are evaluated in S.
+template <typename S>
+HC_HD void rusanovDissipation(S out[8], const double l[8], const double r[8], double gamma)
 {
-    R cons[2][8];
-    R speed = 0;
+    double cons[2][8];
+    S speed = 0;
     for (int s = 0; s < 2; ++s)
     {
-        const R* w = s == 0 ? l : r;
-        R v2 = 0, b2 = 0;
+        const double* w = s == 0 ? l : r;
+        double v2 = 0, b2 = 0;
         cons[s][0] = w[0];
         for (int d = 0; d < 3; ++d)
         {
@@ -119,53 +148,65 @@ HC_HD void rusanovDissipation(R out[8], const R l[8], const R r[8], R gamma)
             cons[s][1+d] = w[0]*w[1+d];
             cons[s][5+d] = w[5+d];
         }
-        cons[s][4] = w[4]/(gamma-R(1))+R(0.5)*w[0]*v2+R(0.5)*R(kInv4Pi)*b2;
-        speed = fmax(speed, fabs(w[1])+sqrt((gamma*w[4]+R(kInv4Pi)*b2)/w[0]));
+        cons[s][4] = w[4]/(gamma-1.0)+0.5*w[0]*v2+0.5*kInv4Pi*b2;
+        const S wn = S(w[1]), g = S(gamma), p = S(w[4]), bb = S(b2), rho = S(w[0]);
+        speed = fmax(speed, fabs(wn)+sqrt((g*p+S(kInv4Pi)*bb)/rho));
     }
-    for (int c = 0; c < 8; ++c) out[c] = speed*(cons[1][c]-cons[0][c]);
+    for (int c = 0; c < 8; ++c) out[c] = speed*S(cons[1][c]-cons[0][c]);
 }
 
-template <typename R>
-HC_HD R entropyFixedSpeed(R lambda, R del)
+template <typename S>
+HC_HD S entropyFixedSpeed(S lambda, S del)
 {
-    const R z = fabs(lambda);
-    return z >= del ? z : R(0.5)*(z*z+del*del)/del;
+    const S z = fabs(lambda);
+    return z >= del ? z : S(0.5)*(z*z+del*del)/del;
 }
 
 // Roe 8-wave dissipation D = |A| (U_R - U_L), arithmetic averages, eight-wave
 // entropy fix and the wave-fan admissibility walk, src/MHD_Roe8Wave.H:16-600.
-// Returns false when the caller must fall back to Rusanov.
-template <typename R>
-HC_HD bool roe8WaveDissipation(R out[8], const R WL[8], const R WR[8], R gamma, R efix)
+// Returns false when the caller must fall back to Rusanov. The averages, the
+// conserved jump and the walked fan states are FP64; the eigen-system and the
+// projections are evaluated in S (S = double: all FP64).
+template <typename S>
+HC_HD bool roe8WaveDissipation(S out[8], const double WL[8], const double WR[8], double gamma_d, double efix_d)
 {
-    const R rs2 = R(0.707106781186547524400844362104849039);
-    const R smallB = R(1.0e-9);
-    if (!(WL[0] > R(0) && WR[0] > R(0) && WL[4] > R(0) && WR[4] > R(0))) return false;
-    const R hg = gamma-R(1);
-    const R rL = WL[0], rR = WR[0];
-    const R mL0 = rL*WL[1], mL1 = rL*WL[2], mL2 = rL*WL[3];
-    const R mR0 = rR*WR[1], mR1 = rR*WR[2], mR2 = rR*WR[3];
-    const R bbL = R(kInv8Pi)*(WL[5]*WL[5]+WL[6]*WL[6]+WL[7]*WL[7]);
-    const R keL = R(0.5)*(mL0*WL[1]+mL1*WL[2]+mL2*WL[3]);
-    const R EL = WL[4]/hg+keL+bbL, a2L = gamma*WL[4]/rL;
-    const R bbR = R(kInv8Pi)*(WR[5]*WR[5]+WR[6]*WR[6]+WR[7]*WR[7]);
-    const R keR = R(0.5)*(mR0*WR[1]+mR1*WR[2]+mR2*WR[3]);
-    const R ER = WR[4]/hg+keR+bbR, a2R = gamma*WR[4]/rR;
-    const R ccmi = a2L > a2R ? a2R : a2L, ccma = a2L > a2R ? a2L : a2R;
-    const R Rho = R(0.5)*(rL+rR);
-    const R U = R(0.5)*(WL[1]+WR[1]), V = R(0.5)*(WL[2]+WR[2]), W = R(0.5)*(WL[3]+WR[3]);
-    const R BX = R(0.5)*(WL[5]+WR[5]), BY = R(0.5)*(WL[6]+WR[6]), BZ = R(0.5)*(WL[7]+WR[7]);
-    const R P = R(0.5)*((WL[4]+bbL)+(WR[4]+bbR));
-    const R BX2 = BX*BX, BYZ2 = BY*BY+BZ*BZ;
-    const R BxBH = R(kInv8Pi)*(BX2+BYZ2);
-    const R kk = R(0.5)*(U*U+V*V+W*W);
-    R cc = gamma*(P-BxBH)/Rho;
-    const R bb = R(kInv4Pi)*BX2/Rho, b = sqrt(bb);
-    if (cc < ccmi) cc = ccmi;
-    if (cc > ccma) cc = ccma;
-    if (!(cc > R(0)) || !isfinite(cc)) return false;
-    const R c = sqrt(cc);
-    R hyy, hzz, aaf, aas, af, as;
+    const S rs2 = S(0.707106781186547524400844362104849039);
+    const S smallB = S(1.0e-9);
+    if (!(WL[0] > 0.0 && WR[0] > 0.0 && WL[4] > 0.0 && WR[4] > 0.0)) return false;
+    const double hgd = gamma_d-1.0;
+    const double rL = WL[0], rR = WR[0];
+    const double mL0 = rL*WL[1], mL1 = rL*WL[2], mL2 = rL*WL[3];
+    const double mR0 = rR*WR[1], mR1 = rR*WR[2], mR2 = rR*WR[3];
+    const double bbL = kInv8Pi*(WL[5]*WL[5]+WL[6]*WL[6]+WL[7]*WL[7]);
+    const double keL = 0.5*(mL0*WL[1]+mL1*WL[2]+mL2*WL[3]);
+    const double EL = WL[4]/hgd+keL+bbL, a2L = gamma_d*WL[4]/rL;
+    const double bbR = kInv8Pi*(WR[5]*WR[5]+WR[6]*WR[6]+WR[7]*WR[7]);
+    const double keR = 0.5*(mR0*WR[1]+mR1*WR[2]+mR2*WR[3]);
+    const double ER = WR[4]/hgd+keR+bbR, a2R = gamma_d*WR[4]/rR;
+    const double Rhod = 0.5*(rL+rR);
+    const double Ud = 0.5*(WL[1]+WR[1]), Vd = 0.5*(WL[2]+WR[2]), Wd = 0.5*(WL[3]+WR[3]);
+    const double BXd = 0.5*(WL[5]+WR[5]), BYd = 0.5*(WL[6]+WR[6]), BZd = 0.5*(WL[7]+WR[7]);
+    const double Pd = 0.5*((WL[4]+bbL)+(WR[4]+bbR));
+    const double BX2d = BXd*BXd, BYZ2d = BYd*BYd+BZd*BZd;
+    const double BxBHd = kInv8Pi*(BX2d+BYZ2d);
+    const double kkd = 0.5*(Ud*Ud+Vd*Vd+Wd*Wd);
+    double ccd = gamma_d*(Pd-BxBHd)/Rhod;
+    const double ccmi = a2L > a2R ? a2R : a2L, ccma = a2L > a2R ? a2L : a2R;
+    if (ccd < ccmi) ccd = ccmi;
+    if (ccd > ccma) ccd = ccma;
+    if (!(ccd > 0.0) || !isfinite(ccd)) return false;
+    // Jump in primitive-like variables (left minus right), FP64.
+    double dud[8] = {rL-rR, mL0-mR0, mL1-mR1, mL2-mR2, EL-ER, WL[5]-WR[5], WL[6]-WR[6], WL[7]-WR[7]};
+    dud[4] = -hgd*(-kkd*dud[0]+Ud*dud[1]+Vd*dud[2]+Wd*dud[3]-dud[4]+(BXd*dud[5]+BYd*dud[6]+BZd*dud[7])*kInv4Pi);
+    dud[1] = (-Ud*dud[0]+dud[1])/Rhod;
+    dud[2] = (-Vd*dud[0]+dud[2])/Rhod;
+    dud[3] = (-Wd*dud[0]+dud[3])/Rhod;
+    // Eigen-system in S.
+    const S Rho = S(Rhod), U = S(Ud), V = S(Vd), W = S(Wd), BX = S(BXd), BY = S(BYd), BZ = S(BZd);
+    const S BX2 = S(BX2d), BYZ2 = S(BYZ2d), BxBH = S(BxBHd), cc = S(ccd);
+    const S bb = S(kInv4Pi*BX2d/Rhod), b = sqrt(bb);
+    const S c = sqrt(cc);
+    S hyy, hzz, aaf, aas, af, as;
     if (BYZ2 < smallB*smallB)
     {
         hyy = rs2; hzz = rs2;
@@ -174,106 +215,117 @@ HC_HD bool roe8WaveDissipation(R out[8], const R WL[8], const R WR[8], R gamma, 
     }
     else
     {
-        const R hyz = sqrt(BYZ2);
+        const S hyz = sqrt(BYZ2);
         hyy = BY/hyz; hzz = BZ/hyz;
-        const R szb = cc+(BxBH+BxBH)/Rho;
-        const R hh = sqrt(fmax(R(0), szb*szb-R(4)*cc*bb));
-        aas = R(0.5)*(szb-hh);
-        if (aas <= R(0)) { aas = 0; as = 0; }
+        const S szb = cc+(BxBH+BxBH)/Rho;
+        const S hh = sqrt(fmax(S(0), szb*szb-S(4)*cc*bb));
+        aas = S(0.5)*(szb-hh);
+        if (aas <= S(0)) { aas = 0; as = 0; }
         else if (aas >= cc) { aas = cc; as = c; }
         else as = sqrt(aas);
-        aaf = R(0.5)*(szb+hh);
+        aaf = S(0.5)*(szb+hh);
         if (aaf <= cc) { aaf = cc; af = c; }
         else af = sqrt(aaf);
     }
-    const R lam[8] = {U, U, U+b, U-b, U+af, U-af, U+as, U-as};
-    const R del = efix*(fabs(U)+fabs(V)+fabs(W)+af);
-    const R dfs = aaf-aas;
-    R alf, als;
-    if (dfs < R(1.0e-8)) { alf = 1; als = 0; }
-    else if ((cc-aas) <= R(0)) { alf = 0; als = 1; }
-    else if ((aaf-cc) <= R(0)) { alf = 1; als = 0; }
+    (void)BX2;
+    const S lam[8] = {U, U, U+b, U-b, U+af, U-af, U+as, U-as};
+    const S del = S(efix_d)*(fabs(U)+fabs(V)+fabs(W)+af);
+    const S dfs = aaf-aas;
+    S alf, als;
+    if (dfs < S(1.0e-8)) { alf = 1; als = 0; }
+    else if ((cc-aas) <= S(0)) { alf = 0; als = 1; }
+    else if ((aaf-cc) <= S(0)) { alf = 1; als = 0; }
     else { alf = sqrt((cc-aas)/dfs); als = sqrt((aaf-cc)/dfs); }
-    const R sih = BX >= R(0) ? R(1) : R(-1);
-    R du[8] = {rL-rR, mL0-mR0, mL1-mR1, mL2-mR2, EL-ER, WL[5]-WR[5], WL[6]-WR[6], WL[7]-WR[7]};
-    du[4] = -hg*(-kk*du[0]+U*du[1]+V*du[2]+W*du[3]-du[4]+(BX*du[5]+BY*du[6]+BZ*du[7])*R(kInv4Pi));
-    du[1] = (-U*du[0]+du[1])/Rho;
-    du[2] = (-V*du[0]+du[2])/Rho;
-    du[3] = (-W*du[0]+du[3])/Rho;
-    const R sqPiRho = sqrt(R(3.14159265358979323846264338327950288)*Rho);
-    const R sq2PiRho = sqrt(R(6.28318530717958647692528676655900576)*Rho);
-    // Right eigenvectors (rows: primitive-like variables; columns: waves).
-    const R aa = alf*af, sa = als*as, rc = Rho*cc;
-    const R fy = R(2)*sqPiRho*c*hyy, fz = R(2)*sqPiRho*c*hzz;
-    const R Rm[8][8] = {
-        {R(1), 0, 0, 0, Rho*alf, Rho*alf, Rho*als, Rho*als},
-        {0, 0, 0, 0, aa, -aa, sa, -sa},
-        {0, 0, -hzz*rs2, -hzz*rs2, -sa*hyy*sih, sa*hyy*sih, aa*hyy*sih, -aa*hyy*sih},
-        {0, 0, hyy*rs2, hyy*rs2, -sa*hzz*sih, sa*hzz*sih, aa*hzz*sih, -aa*hzz*sih},
-        {0, 0, 0, 0, alf*rc, alf*rc, als*rc, als*rc},
-        {0, R(1), 0, 0, 0, 0, 0, 0},
-        {0, 0, hzz*sq2PiRho*sih, -hzz*sq2PiRho*sih, als*fy, als*fy, -alf*fy, -alf*fy},
-        {0, 0, -hyy*sq2PiRho*sih, hyy*sq2PiRho*sih, als*fz, als*fz, -alf*fz, -alf*fz}};
-    // Left eigenvectors (rows: waves).
-    const R ih = R(0.5)*sih/sq2PiRho, qy = R(0.25)*hyy/(c*sqPiRho), qz = R(0.25)*hzz/(c*sqPiRho);
-    const R Lm[8][8] = {
-        {R(1), 0, 0, 0, -R(1)/cc, 0, 0, 0},
-        {0, 0, 0, 0, 0, R(1), 0, 0},
-        {0, 0, -hzz*rs2, hyy*rs2, 0, 0, hzz*ih, -hyy*ih},
-        {0, 0, -hzz*rs2, hyy*rs2, 0, 0, -hzz*ih, hyy*ih},
-        {0, R(0.5)*aa/cc, -R(0.5)*sa*hyy*sih/cc, -R(0.5)*sa*hzz*sih/cc, R(0.5)*alf/(cc*Rho), 0, als*qy, als*qz},
-        {0, -R(0.5)*aa/cc, R(0.5)*sa*hyy*sih/cc, R(0.5)*sa*hzz*sih/cc, R(0.5)*alf/(cc*Rho), 0, als*qy, als*qz},
-        {0, R(0.5)*sa/cc, R(0.5)*aa*hyy*sih/cc, R(0.5)*aa*hzz*sih/cc, R(0.5)*als/(Rho*cc), 0, -alf*qy, -alf*qz},
-        {0, -R(0.5)*sa/cc, -R(0.5)*aa*hyy*sih/cc, -R(0.5)*aa*hzz*sih/cc, R(0.5)*als/(Rho*cc), 0, -alf*qy, -alf*qz}};
+    const S sih = BX >= S(0) ? S(1) : S(-1);
+    S du[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) du[q] = S(dud[q]);
+    const S sqPiRho = sqrt(S(3.14159265358979323846264338327950288)*Rho);
+    const S sq2PiRho = sqrt(S(6.28318530717958647692528676655900576)*Rho);
+    // Eigenvector entries (src/MHD_Roe8Wave.H:356-498) from a few scalars;
+    // rows of the left matrix and columns of the right matrix are formed on
+    // the fly so no 8x8 matrix is held in registers. Non-zero terms are summed
+    // in the same order as the dense products.
+    const S aa = alf*af, sa = als*as, rc = Rho*cc;
+    const S fy = S(2)*sqPiRho*c*hyy, fz = S(2)*sqPiRho*c*hzz;
+    const S ih = S(0.5)*sih/sq2PiRho, qy = S(0.25)*hyy/(c*sqPiRho), qz = S(0.25)*hzz/(c*sqPiRho);
+    const S hzr = -hzz*rs2, hyr = hyy*rs2;
+    const S zq = hzz*sq2PiRho*sih, yq = hyy*sq2PiRho*sih;
+    const S ayh = aa*hyy*sih, azh = aa*hzz*sih, syh = sa*hyy*sih, szh = sa*hzz*sih;
+    auto rcol = [&](int w, S col[8]) {
+        for (int q = 0; q < 8; ++q) col[q] = 0;
+        switch (w)
+        {
+        case 0: col[0] = S(1); break;
+        case 1: col[5] = S(1); break;
+        case 2: col[2] = hzr; col[3] = hyr; col[6] = zq; col[7] = -yq; break;
+        case 3: col[2] = hzr; col[3] = hyr; col[6] = -zq; col[7] = yq; break;
+        case 4: col[0] = Rho*alf; col[1] = aa; col[2] = -syh; col[3] = -szh; col[4] = alf*rc; col[6] = als*fy; col[7] = als*fz; break;
+        case 5: col[0] = Rho*alf; col[1] = -aa; col[2] = syh; col[3] = szh; col[4] = alf*rc; col[6] = als*fy; col[7] = als*fz; break;
+        case 6: col[0] = Rho*als; col[1] = sa; col[2] = ayh; col[3] = azh; col[4] = als*rc; col[6] = -alf*fy; col[7] = -alf*fz; break;
+        default: col[0] = Rho*als; col[1] = -sa; col[2] = -ayh; col[3] = -azh; col[4] = als*rc; col[6] = -alf*fy; col[7] = -alf*fz; break;
+        }
+    };
+    // Projections of the jump on the left eigenvectors.
+    S Ldu[8];
+    {
+        const S a2 = S(0.5)*aa/cc, s2 = S(0.5)*sa/cc;
+        const S sy2 = S(0.5)*sa*hyy*sih/cc, sz2 = S(0.5)*sa*hzz*sih/cc;
+        const S ay2 = S(0.5)*aa*hyy*sih/cc, az2 = S(0.5)*aa*hzz*sih/cc;
+        const S f4 = S(0.5)*alf/(cc*Rho), f6 = S(0.5)*als/(Rho*cc);
+        Ldu[0] = du[0] + (-S(1)/cc)*du[4];
+        Ldu[1] = du[5];
+        Ldu[2] = hzr*du[2] + hyr*du[3] + (hzz*ih)*du[6] + (-hyy*ih)*du[7];
+        Ldu[3] = hzr*du[2] + hyr*du[3] + (-hzz*ih)*du[6] + (hyy*ih)*du[7];
+        Ldu[4] = a2*du[1] + (-sy2)*du[2] + (-sz2)*du[3] + f4*du[4] + (als*qy)*du[6] + (als*qz)*du[7];
+        Ldu[5] = (-a2)*du[1] + sy2*du[2] + sz2*du[3] + f4*du[4] + (als*qy)*du[6] + (als*qz)*du[7];
+        Ldu[6] = s2*du[1] + ay2*du[2] + az2*du[3] + f6*du[4] + (-alf*qy)*du[6] + (-alf*qz)*du[7];
+        Ldu[7] = (-s2)*du[1] + (-ay2)*du[2] + (-az2)*du[3] + f6*du[4] + (-alf*qy)*du[6] + (-alf*qz)*du[7];
+    }
     // Walk the conservative wave fan in speed order; reject a fan whose
     // intermediate state has non-positive density or negative thermal energy.
     {
         const int order[8] = {5,3,7,0,1,6,2,4};
-        R st[8] = {rL, mL0, mL1, mL2, EL, WL[5], WL[6], WL[7]};
+        double st[8] = {rL, mL0, mL1, mL2, EL, WL[5], WL[6], WL[7]};
 #pragma unroll
         for (int wv = 0; wv < 7; ++wv)
         {
             const int k = order[wv];
-            R amp = 0;
+            const S amp = -Ldu[k];
+            S col[8];
+            rcol(k, col);
+            double dl[8];
 #pragma unroll
-            for (int q = 0; q < 8; ++q) amp -= Lm[k][q]*du[q];
-            R dl[8];
-#pragma unroll
-            for (int q = 0; q < 8; ++q) dl[q] = amp*Rm[q][k];
-            dl[4] = kk*dl[0]+Rho*(U*dl[1]+V*dl[2]+W*dl[3])+dl[4]/hg+R(kInv4Pi)*(BX*dl[5]+BY*dl[6]+BZ*dl[7]);
-            dl[1] = U*dl[0]+Rho*dl[1];
-            dl[2] = V*dl[0]+Rho*dl[2];
-            dl[3] = W*dl[0]+Rho*dl[3];
+            for (int q = 0; q < 8; ++q) dl[q] = double(amp*col[q]);
+            dl[4] = kkd*dl[0]+Rhod*(Ud*dl[1]+Vd*dl[2]+Wd*dl[3])+dl[4]/hgd+kInv4Pi*(BXd*dl[5]+BYd*dl[6]+BZd*dl[7]);
+            dl[1] = Ud*dl[0]+Rhod*dl[1];
+            dl[2] = Vd*dl[0]+Rhod*dl[2];
+            dl[3] = Wd*dl[0]+Rhod*dl[3];
 #pragma unroll
             for (int q = 0; q < 8; ++q) st[q] += dl[q];
             if (lam[order[wv+1]] > lam[k])
             {
-                if (!(st[0] > R(0)) || !isfinite(st[0])) return false;
-                const R th = st[4]-R(0.5)*(st[1]*st[1]+st[2]*st[2]+st[3]*st[3])/st[0]
-                    -R(kInv8Pi)*(st[5]*st[5]+st[6]*st[6]+st[7]*st[7]);
-                if (!(th >= R(0)) || !isfinite(th)) return false;
+                if (!(st[0] > 0.0) || !isfinite(st[0])) return false;
+                const double th = st[4]-0.5*(st[1]*st[1]+st[2]*st[2]+st[3]*st[3])/st[0]
+                    -kInv8Pi*(st[5]*st[5]+st[6]*st[6]+st[7]*st[7]);
+                if (!(th >= 0.0) || !isfinite(th)) return false;
             }
         }
     }
-    R lu[8];
+    S lu[8];
 #pragma unroll
-    for (int w = 0; w < 8; ++w)
-    {
-        R s = 0;
-#pragma unroll
-        for (int q = 0; q < 8; ++q) s += Lm[w][q]*du[q];
-        lu[w] = entropyFixedSpeed(lam[w], del)*s;
-    }
-    R ra[8];
-#pragma unroll
-    for (int r = 0; r < 8; ++r)
-    {
-        R s = 0;
-#pragma unroll
-        for (int w = 0; w < 8; ++w) s += Rm[r][w]*lu[w];
-        ra[r] = s;
-    }
-    ra[4] = ra[0]*kk+Rho*(U*ra[1]+V*ra[2]+W*ra[3])+ra[4]/hg+R(kInv4Pi)*(BX*ra[5]+BY*ra[6]+BZ*ra[7]);
+    for (int w = 0; w < 8; ++w) lu[w] = entropyFixedSpeed(lam[w], del)*Ldu[w];
+    S ra[8];
+    ra[0] = lu[0] + (Rho*alf)*lu[4] + (Rho*alf)*lu[5] + (Rho*als)*lu[6] + (Rho*als)*lu[7];
+    ra[1] = aa*lu[4] + (-aa)*lu[5] + sa*lu[6] + (-sa)*lu[7];
+    ra[2] = hzr*lu[2] + hzr*lu[3] + (-syh)*lu[4] + syh*lu[5] + ayh*lu[6] + (-ayh)*lu[7];
+    ra[3] = hyr*lu[2] + hyr*lu[3] + (-szh)*lu[4] + szh*lu[5] + azh*lu[6] + (-azh)*lu[7];
+    ra[4] = (alf*rc)*lu[4] + (alf*rc)*lu[5] + (als*rc)*lu[6] + (als*rc)*lu[7];
+    ra[5] = lu[1];
+    ra[6] = zq*lu[2] + (-zq)*lu[3] + (als*fy)*lu[4] + (als*fy)*lu[5] + (-alf*fy)*lu[6] + (-alf*fy)*lu[7];
+    ra[7] = (-yq)*lu[2] + yq*lu[3] + (als*fz)*lu[4] + (als*fz)*lu[5] + (-alf*fz)*lu[6] + (-alf*fz)*lu[7];
+    const S kk = S(kkd), hg = S(hgd);
+    ra[4] = ra[0]*kk+Rho*(U*ra[1]+V*ra[2]+W*ra[3])+ra[4]/hg+S(kInv4Pi)*(BX*ra[5]+BY*ra[6]+BZ*ra[7]);
     ra[1] = U*ra[0]+Rho*ra[1];
     ra[2] = V*ra[0]+Rho*ra[2];
     ra[3] = W*ra[0]+Rho*ra[3];
@@ -311,53 +363,63 @@ HC_HD FaceFrame makeFaceFrame(const double n[3])
     return F;
 }
 
-// Rotated MHD face flux times face area, src/MHD_SecondOrderFlux.H:34-74.
-// frame: 9 row-major entries; area = radial area * |normal|.
-template <typename R>
-HC_HD void o2FaceFlux(R out[8], const R l[8], const R r[8], const R frame[9],
-                      R area, R gamma, R entropy, bool roe)
+// Rotated MHD face flux times face area, src/MHD_SecondOrderFlux.H:34-74,
+// with the Riemann dissipation evaluated in precision S: frame
+// rotation, central flux, area and the back-rotation stay FP64. S = double
+// reproduces o2FaceFlux operation for operation.
+template <typename S>
+HC_HD void o2FaceFluxMixed(double out[8], const double l[8], const double r[8], const double frame[9],
+                           double area, double gamma, double entropy, bool roe)
 {
-    const R inv4pi = R(1.0/(4.0*3.14159265358979323846));
-    R loc[2][8], fl[2][8];
+    const double inv4pi = 1.0/(4.0*3.14159265358979323846);
+    double loc[2][8];
 #pragma unroll
     for (int s = 0; s < 2; ++s)
     {
-        const R* w = s == 0 ? l : r;
-        R* q = loc[s];
+        const double* w = s == 0 ? l : r;
+        double* q = loc[s];
         q[0] = w[0]; q[4] = w[4];
 #pragma unroll
         for (int i = 0; i < 3; ++i)
         {
-            R v = 0, bsum = 0;
+            double v = 0, bsum = 0;
 #pragma unroll
             for (int j = 0; j < 3; ++j) { v += frame[3*i+j]*w[1+j]; bsum += frame[3*i+j]*w[5+j]; }
             q[1+i] = v; q[5+i] = bsum;
         }
-        R v2 = 0, b2 = 0, vb = 0;
+    }
+    S ds[8];
+    if (!roe || !roe8WaveDissipation<S>(ds, loc[0], loc[1], gamma, entropy))
+        rusanovDissipation<S>(ds, loc[0], loc[1], gamma);
+    double num[8];
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+    {
+        const double* q = loc[s];
+        double v2 = 0, b2 = 0, vb = 0;
 #pragma unroll
         for (int i = 0; i < 3; ++i) { v2 += q[1+i]*q[1+i]; b2 += q[5+i]*q[5+i]; vb += q[1+i]*q[5+i]; }
-        const R pressure = q[4]+R(0.5)*inv4pi*b2;
-        const R energy = q[4]/(gamma-R(1))+R(0.5)*q[0]*v2+R(0.5)*inv4pi*b2;
-        fl[s][0] = q[0]*q[1];
-        fl[s][4] = (energy+pressure)*q[1]-inv4pi*q[5]*vb;
+        const double pressure = q[4]+0.5*inv4pi*b2;
+        const double energy = q[4]/(gamma-1.0)+0.5*q[0]*v2+0.5*inv4pi*b2;
+        double f[8];
+        f[0] = q[0]*q[1];
+        f[4] = (energy+pressure)*q[1]-inv4pi*q[5]*vb;
 #pragma unroll
         for (int i = 0; i < 3; ++i)
         {
-            fl[s][1+i] = q[0]*q[1]*q[1+i]-inv4pi*q[5]*q[5+i]+(i == 0 ? pressure : R(0));
-            fl[s][5+i] = q[1]*q[5+i]-q[1+i]*q[5];
+            f[1+i] = q[0]*q[1]*q[1+i]-inv4pi*q[5]*q[5+i]+(i == 0 ? pressure : 0.0);
+            f[5+i] = q[1]*q[5+i]-q[1+i]*q[5];
         }
-    }
-    R diss[8];
-    if (!roe || !roe8WaveDissipation(diss, loc[0], loc[1], gamma, entropy))
-        rusanovDissipation(diss, loc[0], loc[1], gamma);
-    R num[8];
 #pragma unroll
-    for (int c = 0; c < 8; ++c) num[c] = R(0.5)*area*(fl[0][c]+fl[1][c]-diss[c]);
+        for (int c = 0; c < 8; ++c) num[c] = s == 0 ? f[c] : num[c]+f[c];
+    }
+#pragma unroll
+    for (int c = 0; c < 8; ++c) num[c] = 0.5*area*(num[c]-double(ds[c]));
     out[0] = num[0]; out[4] = num[4];
 #pragma unroll
     for (int i = 0; i < 3; ++i)
     {
-        R v = 0, bsum = 0;
+        double v = 0, bsum = 0;
 #pragma unroll
         for (int j = 0; j < 3; ++j) { v += frame[3*j+i]*num[1+j]; bsum += frame[3*j+i]*num[5+j]; }
         out[1+i] = v; out[5+i] = bsum;

@@ -47,7 +47,11 @@ typedef struct hc_shell
 /* Operator options: the hot-path subset of ParseInputs
  * (src/Inputs_Parsing.H:71-123). spatial_order 2 selects the PLM operator
  * (src/BoxOp_SecondOrder.H:155-370); limiter 0 none, 1 or 2 minmod, 3 MC;
- * riemann_solver 1 Rusanov, 2 Roe 8-wave. */
+ * riemann_solver 1 Rusanov, 2 Roe 8-wave. precision 0 evaluates everything
+ * in FP64; precision 1 (mixed) limits slopes and evaluates the Riemann
+ * dissipation (Rusanov speed, Roe eigen-system) in FP32 while states, jumps,
+ * metric terms, central fluxes, pressure recovery, divergence and the RK
+ * accumulate stay FP64. */
 typedef struct hc_options
 {
     int spatial_order;
@@ -55,9 +59,21 @@ typedef struct hc_options
     int riemann_solver;
     double entropy_fix;
     double gamma;
+    int precision;
 } hc_options;
 
+/* A box of the shell: panel (block) 0..5 and inclusive cell range lo..hi in
+ * the panel's (radial, alpha, beta) index space, like a Proto Box of an
+ * MBDisjointBoxLayout (src/BoxOp_EulerCubedSphere.H:317-368). */
+typedef struct hc_box
+{
+    int block;
+    int lo[3];
+    int hi[3];
+} hc_box;
+
 typedef struct hc_level hc_level;
+typedef struct hc_boxop hc_boxop;
 typedef void* hc_stream; /* hipStream_t; NULL is the default stream */
 
 /* Builds the level geometry (radial lines, face frames, seam gather tables)
@@ -88,7 +104,8 @@ int hc_prepare_stage(hc_level* level, const double* U0, const double* K, double 
 
 /* Level RHS: K = d<JU>/dt of every valid cell from the stage level W, with
  * shared seam face states and seam flux reconciliation; face fluxes are
- * written when F0/F1/F2 are non-NULL. If U is non-NULL it is incremented by
+ * written when F0/F1/F2 are non-NULL (then the per-direction face kernels run;
+ * otherwise one fused tile kernel computes every face once in LDS). If U is non-NULL it is incremented by
  * b_dt*K in the same pass (the RK stage accumulate).
  * Replaces StageLevel + the panel register reflux
  * (src/BoxOp_EulerCubedSphereImplem.H:955-1012, src/MHD_AMR.H:1417-1499). */
@@ -101,6 +118,28 @@ int hc_stage_rhs(hc_level* level, const double* W, double* K, double* U, double 
 int hc_rk4_advance(hc_level* level, double* U, const double* inner_ghosts, double dt,
                    int nsteps, hc_stream stream);
 
+/* Fourth-order box operator (spatial_order 4, standard operator, limiter 0
+ * or 1, riemann_solver 2) over a table of boxes, evaluated for every box in
+ * the same launches. Replaces BoxOp_EulerCubedSphere::operator()
+ * (src/BoxOp_EulerCubedSphere.H:412-421, src/BoxOp_EulerCubedSphereImplem.H:4-402)
+ * as called per box by StageLevel (:955-1012).
+ * Arrays concatenate the boxes in table order, each box component-major
+ * with radial index fastest (Proto BoxData layout):
+ *   USph  native conserved cell averages on the box grown by ghost 5
+ *         (rho, rho v^(r,alpha,beta), E, B^(r,alpha,beta): contravariant
+ *         components in d xhat / d xi), 8*(n0+10)*(n1+10)*(n2+10) per box;
+ *   rhs   d<JU>/dt per unit xi^3 on the valid box, Cartesian vector
+ *         components, 8*n0*n1*n2 per box; ACCUMULATED into (zero it first);
+ *   F_d   face fluxes of the valid box extruded in d (optional, NULL skips).
+ * hc_boxop_count returns the element count of USph (what 0), rhs (1) and
+ * F_0..F_2 (2..4). */
+int hc_boxop_create(const hc_shell* shell, const hc_options* options, const hc_box* boxes, int nbox,
+                    hc_boxop** out);
+void hc_boxop_destroy(hc_boxop* op);
+long long hc_boxop_count(const hc_boxop* op, int what);
+int hc_boxop_apply(hc_boxop* op, const double* USph, double* rhs, double* F0, double* F1, double* F2,
+                   hc_stream stream);
+
 /* Host-side seeded initial state (test problem data, not on the hot path):
  * config 0 = radial wind rho=(r0/r)^2, v_r=1, B_r=+-(r0/r)^2 by hemisphere
  * (sign of z), p=0.1 rho (1 + 0.2 exp(-|x-x_c|^2/w^2)), w = 0.1 (r1-r0), x_c at
@@ -111,6 +150,16 @@ int hc_rk4_advance(hc_level* level, double* U, const double* inner_ghosts, doubl
  * src/BoxOp_Utilities.H:67-219) for these problems. */
 int hc_init_state(const hc_shell* shell, const hc_options* options, int config,
                   unsigned long long seed, double* JU_host, double* inner_host);
+
+/* Host-side native conserved cell averages for the fourth-order box
+ * operator (test problem data): the states of hc_init_state (config 0-2) and
+ * config 3 = config 0 plus Alfvenic velocity/field perturbations (amplitude
+ * 0.05, angular wavenumbers <= 8, dB = -dv sqrt(4 pi rho)) and a lognormal
+ * density factor exp(0.05 g - 0.00125) per cell, averaged over each ghosted
+ * cell by a quad^3 Gauss-Legendre rule in xi. Writes the hc_boxop USph layout
+ * into a HOST array. */
+int hc_init_native_boxes(const hc_shell* shell, const hc_options* options, int config, unsigned long long seed,
+                         const hc_box* boxes, int nbox, int quad, double* USph_host);
 
 /* Host-side geometry probes (no device use): cell-averaged det A of angular
  * cell (j,k), and the face-averaged cofactor row of face (d; j,k) of panel p. */

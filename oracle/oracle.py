@@ -62,6 +62,8 @@ def lib() -> ctypes.CDLL:
             "orc_cartesian_primitive": (None, [P, P, D, D]),
             "orc_ghost_donors": (I, [cfgp, I, I, I, P, P, P, P]),
             "orc_seam_edge": (None, [cfgp, I, I, I, P]),
+            "orc_o4_rhs": (I, [cfgp, I, P, P, P, P, P, P, P]),
+            "orc_face_average": (None, [cfgp, I, I, I, I, I, P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -177,3 +179,29 @@ def basis(p, alpha, beta):
     A = np.zeros(9)
     lib().orc_basis(p, alpha, beta, _p(A))
     return A.reshape(3, 3)
+
+
+def o4_rhs_boxes(n_rad, n_ang, boxes, USph, limiter=1, entropy_fix=0.3, gamma=5.0 / 3.0, r0=21.5, r1=215.0,
+                 with_fluxes=False):
+    """Fourth-order RHS (and optionally fluxes) of every box, concatenated like hc_boxop."""
+    cfg = OrcCfg(n_rad, n_ang, 5, r0, r1, 0.0, gamma, limiter, 2, entropy_fix)
+    g = 5
+    rhs_parts, flux_parts, off = [], [[], [], []], 0
+    for blk, lo, hi in boxes:
+        n = [hi[d] - lo[d] + 1 for d in range(3)]
+        size = 8 * (n[0] + 2 * g) * (n[1] + 2 * g) * (n[2] + 2 * g)
+        u = np.ascontiguousarray(USph[off:off + size])
+        off += size
+        r = np.zeros(8 * n[0] * n[1] * n[2])
+        lo_a = np.array(lo, dtype=np.int32)
+        hi_a = np.array(hi, dtype=np.int32)
+        F = [np.zeros(8 * (n[0] + (d == 0)) * (n[1] + (d == 1)) * (n[2] + (d == 2))) for d in range(3)]
+        fp = [_p(F[d]) if with_fluxes else None for d in range(3)]
+        lib().orc_o4_rhs(ctypes.byref(cfg), blk, lo_a.ctypes.data, hi_a.ctypes.data, _p(u), _p(r), *fp)
+        rhs_parts.append(r)
+        for d in range(3):
+            flux_parts[d].append(F[d])
+    rhs = np.concatenate(rhs_parts)
+    if with_fluxes:
+        return rhs, [np.concatenate(flux_parts[d]) for d in range(3)]
+    return rhs

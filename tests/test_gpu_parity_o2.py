@@ -64,9 +64,14 @@ def test_stage_rhs_matches_oracle(device, nr, na, limiter, riemann):
     torch.cuda.synchronize()
     err = rel_linf_per_var(K.cpu().numpy(), kref)
     assert err.max() <= TOL, err
+    # Face fluxes per variable, scaled by the same variable's radial-face flux
+    # magnitude: through angular faces of a radial wind the mass and energy
+    # fluxes are nearly zero, so their own maximum would amplify the 1e-15
+    # difference between the two independent face-normal computations.
     for d in range(3):
-        errf = rel_linf_per_var(F[d].cpu().numpy(), Fref[d])
-        assert errf.max() <= TOL, (d, errf)
+        diff = np.abs(F[d].cpu().numpy() - Fref[d]).max(axis=(0, 2, 3, 4))
+        scale = np.maximum(np.abs(Fref[d]).max(axis=(0, 2, 3, 4)), np.abs(Fref[0]).max(axis=(0, 2, 3, 4)))
+        assert (diff <= TOL * scale).all(), (d, diff / scale)
 
 
 @pytest.mark.gpu
@@ -118,3 +123,26 @@ def test_second_call_smaller_level(device):
         torch.cuda.synchronize()
         err = rel_linf_per_var(U.cpu().numpy(), O.rk4(ju, inner, dt, 1))
         assert err.max() <= TOL, (nr, na, err)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("riemann", [1, 2])
+def test_mixed_precision_error_vs_fp64_oracle(device, riemann):
+    """Mixed precision (FP32 slopes and Riemann dissipation) stays within
+    1e-6 relative L-infinity per variable of the FP64 oracle after one step."""
+    nr, na = 24, 16
+    shell = hc.Shell(nr, na)
+    opts = hc.Options(limiter=1, riemann_solver=riemann, precision=1)
+    ju, inner = hc.init_state(shell, opts, 0, 11)
+    O = orc.Oracle(nr, na, limiter=1, riemann=riemann)
+    dt = cfl_dt(na)
+    ref = O.rk4(ju, inner, dt, 1)
+    L = hc.Level(shell, opts)
+    U = _dev(ju)
+    L.rk4_advance(U, _dev(inner), dt, 1)
+    torch.cuda.synchronize()
+    got = U.cpu().numpy()
+    err = rel_linf_per_var(got, ref)
+    errd = rel_linf_per_var(got - ju, ref - ju)
+    print("mixed riemann", riemann, "state err", err, "increment err", errd)
+    assert err.max() <= 1e-6, err
