@@ -5,6 +5,8 @@
 #include <stdexcept>
 #include <string>
 
+#include "../../include/heliocubed.h"
+
 namespace hc
 {
 // Message of the calling thread's last failed C-ABI call.
@@ -15,6 +17,15 @@ inline int fail(const std::string& msg)
     lastError() = msg;
     return -1;
 }
+
+// Fourth-order level (defined with the fourth-order kernels): stage
+// preparation JU -> USph (ghost 5) and the box-table RHS over the level.
+struct Level4;
+Level4* level4Create(const hc_shell& shell, const hc_options& opt, int boxSize);
+void level4Destroy(Level4* L);
+void level4Prepare(Level4& L, const double* U0, const double* K, double a_dt, const double* inner, double* US,
+                   hipStream_t s);
+void level4Rhs(Level4& L, const double* US, double* K, double* U, double b_dt, hipStream_t s);
 } // namespace hc
 
 #define HC_THROW(call)                                                              \

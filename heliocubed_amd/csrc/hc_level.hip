@@ -505,6 +505,7 @@ struct KernelProfile
 struct hc_level
 {
     KernelProfile profile;
+    hc::Level4* l4 = nullptr;     // fourth-order level (spatial_order 4)
     hc::ShellDims dims{};
     hc_options opt{};
     LevelView view{};
@@ -524,6 +525,7 @@ struct hc_level
 
     ~hc_level()
     {
+        hc::level4Destroy(l4);
         for (void* p : owned) (void)hipFree(p);
     }
     template <typename T> T* own(T* p) { owned.push_back(p); return p; }
@@ -539,6 +541,12 @@ struct hc_level
         if (W) return;
         HC_CHECK(hipMalloc(&W, stageCount()*sizeof(double))); owned.push_back(W);
         HC_CHECK(hipMemset(W, 0, stageCount()*sizeof(double)));
+        if (l4)
+        {
+            HC_CHECK(hipMalloc(&U0, validCount()*sizeof(double))); owned.push_back(U0);
+            HC_CHECK(hipMalloc(&K, validCount()*sizeof(double))); owned.push_back(K);
+            return;
+        }
         for (int d = 0; d < 3; ++d) { HC_CHECK(hipMalloc(&F[d], faceCount(d)*sizeof(double))); owned.push_back(F[d]); }
         HC_CHECK(hipMalloc(&PS, 24ull*8*dims.na*dims.nr*sizeof(double))); owned.push_back(PS);
         HC_CHECK(hipMalloc(&SF, 24ull*8*dims.na*dims.nr*sizeof(double))); owned.push_back(SF);
@@ -640,6 +648,11 @@ void buildLevel(hc_level& L)
 void prepareStage(hc_level& L, const double* U0, const double* K, double a_dt,
                   const double* inner, double* W, hipStream_t s)
 {
+    if (L.l4)
+    {
+        L.profile.run("k4_prepare", s, [&] { hc::level4Prepare(*L.l4, U0, K, a_dt, inner, W, s); });
+        return;
+    }
     const LevelView& v = L.view;
     KernelProfile& P = L.profile;
     P.run("k_prepare_valid", s, [&] {
@@ -673,6 +686,12 @@ void stageRhs(hc_level& L, const double* W, double* K, double* U, double b_dt,
 {
     L.ensureWorkspace();
     const LevelView& v = L.view;
+    if (L.l4)
+    {
+        if (F0 || F1 || F2) throw std::runtime_error("face fluxes of the fourth-order level: use hc_boxop_apply");
+        L.profile.run("k4_rhs", s, [&] { hc::level4Rhs(*L.l4, W, K, U, b_dt, s); });
+        return;
+    }
     if (!F0 && !F1 && !F2)
     {
         if (L.opt.precision == 1) stageRhsFused<float>(L, W, K, U, b_dt, s);
@@ -759,17 +778,32 @@ int hc_level_create(const hc_shell* shell, const hc_options* options, hc_level**
     if (shell->n_rad < 2 || shell->n_ang < 4) return fail("hc_level_create: need n_rad >= 2 and n_ang >= 4");
     if (!(shell->outer_radius > shell->inner_radius && shell->inner_radius > 0))
         return fail("hc_level_create: need 0 < inner_radius < outer_radius");
-    if (options->spatial_order != 2) return fail("hc_level_create: only spatial_order 2 is implemented");
+    if (options->spatial_order != 2 && options->spatial_order != 4)
+        return fail("hc_level_create: spatial_order must be 2 or 4");
     if (options->riemann_solver != 1 && options->riemann_solver != 2)
         return fail("hc_level_create: riemann_solver must be 1 (Rusanov) or 2 (Roe)");
     if (options->limiter < 0 || options->limiter > 3) return fail("hc_level_create: limiter must be 0..3");
+    if (options->spatial_order == 4 && (options->riemann_solver != 2 || options->limiter > 1))
+        return fail("hc_level_create: the fourth-order operator takes riemann_solver 2 (Roe) and limiter 0 or 1");
+    if (options->spatial_order == 4 && (shell->n_ang < 8 || shell->n_rad < 4))
+        return fail("hc_level_create: the fourth-order operator needs n_ang >= 8 and n_rad >= 4");
     if (options->precision != 0 && options->precision != 1) return fail("hc_level_create: precision must be 0 (fp64) or 1 (mixed)");
     try
     {
         std::unique_ptr<hc_level> L(new hc_level());
-        L->dims = {shell->n_rad, shell->n_ang, 2, shell->inner_radius, shell->outer_radius, shell->radial_stretch};
+        const int g = options->spatial_order == 4 ? 5 : 2;
+        L->dims = {shell->n_rad, shell->n_ang, g, shell->inner_radius, shell->outer_radius, shell->radial_stretch};
         L->opt = *options;
-        buildLevel(*L);
+        if (options->spatial_order == 4)
+        {
+            LevelView& v = L->view;
+            v.nr = shell->n_rad; v.na = shell->n_ang; v.g = g;
+            v.NI = v.nr+2*g; v.NJ = v.na+2*g; v.NK = v.na+2*g;
+            v.cs = (long long)v.NI*v.NJ*v.NK; v.ps = 8*v.cs;
+            v.vcs = (long long)v.nr*v.na*v.na; v.vps = 8*v.vcs;
+            L->l4 = hc::level4Create(*shell, *options, 32);
+        }
+        else buildLevel(*L);
         *out = L.release();
         return 0;
     }

@@ -37,10 +37,23 @@ enum Field : int
     F_MEL = 36, F_MER = 37, F_PHL = 38, F_PHR = 46, F_DPT = 54, F_FLUX = 62, F_COUNT = 86
 };
 
+// One box of the table. U (ghosted native conserved state) and R (RHS) are
+// addressed through strides so a box can view either its own dense BoxData
+// or its tile of a level-wide array.
 struct BoxDesc
 {
     int p, lo[3], hi[3], n[3];
-    long long uOff, rOff, sOff;
+    long long uOff, rOff, sOff;     // element offsets of U(lo-5), R(lo), scratch
+    long long us[3], rs[3];         // (y, z, component) strides of U and R
+};
+
+// A physical seam face seen from two boxes: scratch indices of the flux
+// component 0 in each box, their component strides, and the sign relating
+// the two face orientations.
+struct SeamPair
+{
+    long long a, ea, b, eb;
+    double sign;
 };
 
 struct O4View
@@ -52,6 +65,8 @@ struct O4View
     const double* faceA[3];  // [6][naE][naE][9]
     const double* fmet[3];   // [6][naE][naE][13]: detA, det(A) A, cofactor row
     const double* radial;    // [nr+13][2]: adjDr_0 (area0), adjDr_t (areaT) for i = -6 ..
+    const SeamPair* seams;
+    int nseams;
 };
 
 __device__ __forceinline__ long long angIdx(const O4View& v, int p, int j, int k)
@@ -292,11 +307,10 @@ __global__ void __launch_bounds__(kB) k_o4_wbar(O4View v, const double* __restri
     for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]-kG; H[d] = b.hi[d]+kG; }
     if (!regionPoint(b, L, H, (long long)blockIdx.x*blockDim.x + threadIdx.x, x)) return;
     const BoxIdx I(b);
-    const long long e = I.ecount;
-    const long long u = b.uOff + ((long long)(x[2]-L[2])*(b.n[1]+2*kG) + (x[1]-L[1]))*(b.n[0]+2*kG) + (x[0]-L[0]);
+    const long long u = b.uOff + (x[2]-L[2])*b.us[1] + (x[1]-L[1])*b.us[0] + (x[0]-L[0]);
     double uu[8], w[8];
 #pragma unroll
-    for (int c = 0; c < 8; ++c) uu[c] = U[u + c*e];
+    for (int c = 0; c < 8; ++c) uu[c] = U[u + c*b.us[2]];
     sphToPrimPoint(w, uu);
     double ke, me, vb;
     energiesNative(v.cellA + 9*angIdx(v, b.p, x[1], x[2]), w, ke, me, vb);
@@ -316,15 +330,13 @@ __global__ void __launch_bounds__(kB) k_o4_w(O4View v, const double* __restrict_
     for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]-4; H[d] = b.hi[d]+4; }
     if (!regionPoint(b, L, H, (long long)blockIdx.x*blockDim.x + threadIdx.x, x)) return;
     const BoxIdx I(b);
-    const int e0 = b.n[0]+2*kG, e1 = b.n[1]+2*kG;
-    const long long e = I.ecount;
     auto ui = [&](int i, int j, int k) {
-        return b.uOff + ((long long)(k-b.lo[2]+kG)*e1 + (j-b.lo[1]+kG))*e0 + (i-b.lo[0]+kG); };
+        return b.uOff + (k-b.lo[2]+kG)*b.us[1] + (j-b.lo[1]+kG)*b.us[0] + (i-b.lo[0]+kG); };
     double up[8], uc[8];
 #pragma unroll
     for (int c = 0; c < 8; ++c)
     {
-        const long long o = c*e;
+        const long long o = c*b.us[2];
         const double lap = U[ui(x[0]+1,x[1],x[2])+o]+U[ui(x[0]-1,x[1],x[2])+o]+U[ui(x[0],x[1]+1,x[2])+o]
                           +U[ui(x[0],x[1]-1,x[2])+o]+U[ui(x[0],x[1],x[2]+1)+o]+U[ui(x[0],x[1],x[2]-1)+o]
                           -6.0*U[ui(x[0],x[1],x[2])+o];
@@ -721,18 +733,37 @@ __global__ void __launch_bounds__(kB) k_o4_flux(O4View v, double* __restrict__ S
         S[I(F_FLUX+8*D+c, x[0], x[1], x[2])] = 0.5*(Fsum[c]-Fd[c]);
 }
 
-// 7. Flux divergence accumulated into the box RHS (the caller zeroes it).
-__global__ void __launch_bounds__(kB) k_o4_div(O4View v, const double* __restrict__ S, double* __restrict__ R)
+// 6b. Seam flux reconciliation 1/2 (F_a + sign F_b) on both boxes' copies
+// of each physical seam face, src/MHD_AMR.H:1484-1496.
+__global__ void __launch_bounds__(kB) k_o4_seams(O4View v, double* __restrict__ S)
+{
+    const int t = blockIdx.x*blockDim.x + threadIdx.x;
+    if (t >= v.nseams) return;
+    const SeamPair sp = v.seams[t];
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+    {
+        const double fa = S[sp.a + c*sp.ea], fb = S[sp.b + c*sp.eb];
+        S[sp.a + c*sp.ea] = 0.5*(fa + sp.sign*fb);
+        S[sp.b + c*sp.eb] = 0.5*(fb + sp.sign*fa);
+    }
+}
+
+// 7. Flux divergence into the box RHS: accumulated (overwrite = 0, the
+// BoxOp convention; the caller zeroes it) or overwritten; optionally the RK
+// stage accumulate U += b_dt * RHS with U in the RHS layout.
+__global__ void __launch_bounds__(kB) k_o4_div(O4View v, const double* __restrict__ S, double* __restrict__ R,
+                                               int overwrite, double* __restrict__ Uacc, double b_dt)
 {
     const BoxDesc& b = v.boxes[blockIdx.y];
     int x[3];
     if (!regionPoint(b, b.lo, b.hi, (long long)blockIdx.x*blockDim.x + threadIdx.x, x)) return;
     const BoxIdx I(b);
-    const long long n = (long long)b.n[0]*b.n[1]*b.n[2];
-    const long long r = b.rOff + ((long long)(x[2]-b.lo[2])*b.n[1] + (x[1]-b.lo[1]))*b.n[0] + (x[0]-b.lo[0]);
+    const long long r = b.rOff + (x[2]-b.lo[2])*b.rs[1] + (x[1]-b.lo[1])*b.rs[0] + (x[0]-b.lo[0]);
     for (int c = 0; c < 8; ++c)
     {
-        double acc = R[r + c*n];
+        const long long n = c*b.rs[2];
+        double acc = overwrite ? 0.0 : R[r + n];
         for (int d = 0; d < 3; ++d)
         {
             const double dxInv = -1.0/(d == 0 ? 1.0/v.nr : 1.0/v.na);
@@ -742,7 +773,8 @@ __global__ void __launch_bounds__(kB) k_o4_div(O4View v, const double* __restric
             inc += (1.0*dxInv)*S[I(F_FLUX+8*d+c, a[0], a[1], a[2])];
             acc += inc;
         }
-        R[r + c*n] = acc;
+        R[r + n] = acc;
+        if (Uacc) Uacc[r + n] += b_dt*acc;
     }
 }
 
@@ -836,6 +868,8 @@ void buildBoxop(hc_boxop& B, const hc_box* boxes, int nbox)
         for (int d = 0; d < 3; ++d) { bd.lo[d] = boxes[n].lo[d]; bd.hi[d] = boxes[n].hi[d]; bd.n[d] = bd.hi[d]-bd.lo[d]+1; }
         const long long e = (long long)(bd.n[0]+2*kG)*(bd.n[1]+2*kG)*(bd.n[2]+2*kG);
         bd.uOff = u; bd.rOff = r; bd.sOff = sc;
+        bd.us[0] = bd.n[0]+2*kG; bd.us[1] = (long long)(bd.n[0]+2*kG)*(bd.n[1]+2*kG); bd.us[2] = e;
+        bd.rs[0] = bd.n[0]; bd.rs[1] = (long long)bd.n[0]*bd.n[1]; bd.rs[2] = (long long)bd.n[0]*bd.n[1]*bd.n[2];
         u += 8*e; r += 8LL*bd.n[0]*bd.n[1]*bd.n[2]; sc += F_COUNT*e;
         for (int d = 0; d < 3; ++d)
         {
@@ -848,13 +882,16 @@ void buildBoxop(hc_boxop& B, const hc_box* boxes, int nbox)
     B.uCount = u; B.rCount = r;
     for (int d = 0; d < 3; ++d) { B.fCount[d] = f[d]; B.fOff[d] = upload(B.fOffHost[d], B.owned); }
     v.boxes = upload(B.boxes, B.owned);
+    v.seams = nullptr;
+    v.nseams = 0;
     HC_THROW(hipMalloc(&B.scratch, std::max<long long>(1, sc)*sizeof(double)));
     B.owned.push_back(B.scratch);
     B.nbox = nbox;
 }
 
 template <typename S_>
-void applyBoxop(hc_boxop& B, const double* U, double* R, double* const F[3], hipStream_t s)
+void applyBoxop(hc_boxop& B, const double* U, double* R, double* const F[3], hipStream_t s,
+                int overwrite = 0, double* Uacc = nullptr, double b_dt = 0.0)
 {
     const O4View& v = B.view;
     const dim3 grid((B.maxRegion + kB - 1)/kB, B.nbox);
@@ -870,12 +907,411 @@ void applyBoxop(hc_boxop& B, const double* U, double* R, double* const F[3], hip
     hipLaunchKernelGGL(k_o4_faces<2>, grid, dim3(kB), 0, s, v, B.scratch);
     hipLaunchKernelGGL((k_o4_phys<2, S_>), grid, dim3(kB), 0, s, v, B.scratch);
     hipLaunchKernelGGL(k_o4_flux<2>, grid, dim3(kB), 0, s, v, B.scratch);
-    hipLaunchKernelGGL(k_o4_div, grid, dim3(kB), 0, s, v, B.scratch, R);
+    if (v.nseams > 0)
+        hipLaunchKernelGGL(k_o4_seams, dim3((v.nseams + kB - 1)/kB), dim3(kB), 0, s, v, B.scratch);
+    hipLaunchKernelGGL(k_o4_div, grid, dim3(kB), 0, s, v, B.scratch, R, overwrite, Uacc, b_dt);
     for (int d = 0; d < 3; ++d)
         if (F[d]) hipLaunchKernelGGL(k_o4_fluxout, grid, dim3(kB), 0, s, v, d, B.scratch, F[d], B.fOff[d]);
     HC_THROW(hipGetLastError());
 }
 } // namespace
+
+// ===== Fourth-order level: stage preparation, box table over the level ======
+// JU (valid level) -> native conserved averages USph on the ghosted level,
+// src/MHD_StateConversion.H:153-316 (state_conversion = 1), with the radial
+// boundary ghosts of src/BoxOp_PreStage.H:105-141. Pieces the reference takes
+// from Proto are restated: deriv3case (centred difference inside a block,
+// 3-point one-sided at its edges), the seam interpolation of Cartesian cell
+// averages (cubic average-matching reconstruction on a 4x4 window of the
+// panel holding the ghost cell centre, integrated over the ghost cell), and
+// the cell matrix quotient q = <A>^-1 (<U> - (1/12) sum_d D_d<A> D_d q0).
+namespace
+{
+constexpr int kGC = 6;   // angular ghost layers of the Cartesian averages
+
+struct Prep4View
+{
+    int nr, na, NE;               // NE = na + 12 (angular index -6 .. na+5)
+    long long vcs, vps;           // valid strides (component, panel)
+    long long ccs, cps, cJ;       // Cartesian buffer strides: [6][8][na+12][na+12][nr]
+    long long scs, sps, sI, sJ;   // USph strides: [6][8][na+10][na+10][nr+10]
+    double gamma;
+    const double* det;            // [NE][NE]
+    const double* Aavg;           // [6][NE][NE][9]
+    const double* Ainv;           // [6][NE][NE][9]
+    const double* dvol;           // [nr+13], i = -6 ..
+    const double* rmid;           // [nr+13]
+};
+
+__device__ __forceinline__ double det4(const Prep4View& v, int j, int k) { return v.det[(long long)(k+6)*v.NE + (j+6)]; }
+__device__ __forceinline__ const double* Aavg4(const Prep4View& v, int p, int j, int k)
+{ return v.Aavg + 9*(((long long)p*v.NE + (k+6))*v.NE + (j+6)); }
+__device__ __forceinline__ const double* Ainv4(const Prep4View& v, int p, int j, int k)
+{ return v.Ainv + 9*(((long long)p*v.NE + (k+6))*v.NE + (j+6)); }
+
+template <typename F>
+__device__ __forceinline__ double deriv3(F q, int x, int lo, int hi)
+{
+    if (x == lo) return 0.5*(-3.0*q(x)+4.0*q(x+1)-q(x+2));
+    if (x == hi) return 0.5*(3.0*q(x)-4.0*q(x-1)+q(x-2));
+    return 0.5*(q(x+1)-q(x-1));
+}
+
+// Angular quotient with the fourth-order correction, valid cells.
+__global__ void __launch_bounds__(kB) k4_angular(Prep4View v, const double* __restrict__ U0, const double* __restrict__ K,
+                                                 double a_dt, double* __restrict__ ang)
+{
+    const long long n = 6LL*8*v.na*v.na*v.nr;
+    const long long t = (long long)blockIdx.x*blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int i = t % v.nr;
+    long long r = t / v.nr;
+    const int j = r % v.na; r /= v.na;
+    const int k = r % v.na; r /= v.na;
+    const int c = r % 8;
+    const int p = r / 8;
+    const long long base = p*v.vps + c*v.vcs;
+    auto ju = [&](int jj, int kk) {
+        const long long x = base + ((long long)kk*v.na + jj)*v.nr + i;
+        return K ? U0[x] + a_dt*K[x] : U0[x]; };
+    auto q1 = [&](int jj) { return ju(jj, k)/det4(v, jj, k); };
+    auto q2 = [&](int kk) { return ju(j, kk)/det4(v, j, kk); };
+    double corr = 0;
+    corr += deriv3(q1, j, 0, v.na-1)*(0.5*(det4(v, j+1, k)-det4(v, j-1, k)));
+    corr += deriv3(q2, k, 0, v.na-1)*(0.5*(det4(v, j, k+1)-det4(v, j, k-1)));
+    ang[base + ((long long)k*v.na + j)*v.nr + i] = (ju(j, k)-corr/12.0)*(1.0/det4(v, j, k));
+}
+
+// Radial quotient with its correction, into the Cartesian buffer.
+__global__ void __launch_bounds__(kB) k4_radial(Prep4View v, const double* __restrict__ ang, double* __restrict__ cart)
+{
+    const long long n = 6LL*8*v.na*v.na*v.nr;
+    const long long t = (long long)blockIdx.x*blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int i = t % v.nr;
+    long long r = t / v.nr;
+    const int j = r % v.na; r /= v.na;
+    const int k = r % v.na; r /= v.na;
+    const int c = r % 8;
+    const int p = r / 8;
+    const double* a = ang + p*v.vps + c*v.vcs + ((long long)k*v.na + j)*v.nr;
+    auto dv = [&](int ii) { return v.dvol[ii+6]; };
+    auto rq = [&](int ii) { return a[ii]/dv(ii); };
+    const double dq = deriv3(rq, i, 0, v.nr-1);
+    const double dd = 0.5*(dv(i+1)-dv(i-1));
+    cart[p*v.cps + c*v.ccs + ((long long)(k+kGC)*v.cJ + (j+kGC))*v.nr + i] = (a[i]-dq*dd/12.0)*(1.0/dv(i));
+}
+
+// Seam ghost columns of the Cartesian averages (16 donors each).
+__global__ void __launch_bounds__(kB) k4_seam(Prep4View v, int ncol, const long long* __restrict__ dst,
+                                              const long long* __restrict__ src, const double* __restrict__ w,
+                                              double* __restrict__ cart)
+{
+    const long long t = (long long)blockIdx.x*blockDim.x + threadIdx.x;
+    if (t >= (long long)ncol*v.nr) return;
+    const int i = t % v.nr;
+    const int col = t / v.nr;
+    for (int c = 0; c < 8; ++c)
+    {
+        double s = 0;
+        for (int d = 0; d < 16; ++d) s += w[16*col+d]*cart[src[16*col+d] + c*v.ccs + i];
+        cart[dst[col] + c*v.ccs + i] = s;
+    }
+}
+
+// Native components by the fourth-order matrix quotient on the USph cells
+// i in [0, nr), j,k in [-5, na+5).
+__global__ void __launch_bounds__(kB) k4_native(Prep4View v, const double* __restrict__ cart, double* __restrict__ US)
+{
+    const int NA = v.na+10;
+    const long long n = 6LL*NA*NA*v.nr;
+    const long long t = (long long)blockIdx.x*blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int i = t % v.nr;
+    long long r = t / v.nr;
+    const int j = (int)(r % NA) - 5; r /= NA;
+    const int k = (int)(r % NA) - 5;
+    const int p = r / NA;
+    auto ca = [&](int c, int jj, int kk) {
+        return cart[p*v.cps + c*v.ccs + ((long long)(kk+kGC)*v.cJ + (jj+kGC))*v.nr + i]; };
+    auto us = [&](int c) -> double& {
+        return US[p*v.sps + c*v.scs + ((long long)(k+5)*v.sJ + (j+5))*v.sI + (i+5)]; };
+    us(0) = ca(0, j, k);
+    us(4) = ca(4, j, k);
+    for (int grp = 0; grp < 2; ++grp)
+    {
+        const int st = grp == 0 ? 1 : 5;
+        auto q0 = [&](int jj, int kk, int row) {
+            const double* Mi = Ainv4(v, p, jj, kk);
+            double val = 0;
+            for (int cc = 0; cc < 3; ++cc) val += Mi[3*row+cc]*ca(st+cc, jj, kk);
+            return val; };
+        double rhs[3];
+        for (int row = 0; row < 3; ++row)
+        {
+            double corr = 0;
+            for (int d = 1; d < 3; ++d)
+            {
+                const int dj = d == 1, dk = d == 2;
+                for (int cc = 0; cc < 3; ++cc)
+                {
+                    const double dA = 0.5*(Aavg4(v, p, j+dj, k+dk)[3*row+cc]-Aavg4(v, p, j-dj, k-dk)[3*row+cc]);
+                    const double dq = 0.5*(q0(j+dj, k+dk, cc)-q0(j-dj, k-dk, cc));
+                    corr += dA*dq;
+                }
+            }
+            rhs[row] = ca(st+row, j, k)-corr/12.0;
+        }
+        const double* Mi = Ainv4(v, p, j, k);
+        for (int row = 0; row < 3; ++row)
+        {
+            double val = 0;
+            for (int cc = 0; cc < 3; ++cc) val += Mi[3*row+cc]*rhs[cc];
+            us(st+row) = val;
+        }
+    }
+}
+
+// Radial boundary ghosts of USph: inner fixed native state, outer copy of the
+// last layer scaled by (r_last/r)^2 (r_last/r for the angular B components).
+__global__ void __launch_bounds__(kB) k4_radial_bc(Prep4View v, const double* __restrict__ inner, double* __restrict__ US)
+{
+    const int NA = v.na+10;
+    const int t = blockIdx.x*blockDim.x + threadIdx.x;
+    if (t >= 6*NA*NA) return;
+    const int jj = t % NA, kk = (t / NA) % NA, p = t / (NA*NA);
+    auto us = [&](int c, int i) -> double& {
+        return US[p*v.sps + c*v.scs + ((long long)kk*v.sJ + jj)*v.sI + (i+5)]; };
+    for (int ig = 0; ig < 5; ++ig)
+        for (int c = 0; c < 8; ++c)
+            us(c, -5+ig) = inner[((((long long)p*8+c)*NA+kk)*NA+jj)*5+ig];
+    const double rlast = v.rmid[v.nr-1+6];
+    for (int i = v.nr; i < v.nr+5; ++i)
+    {
+        const double ratio = rlast/v.rmid[i+6];
+        for (int c = 0; c < 6; ++c) us(c, i) = us(c, v.nr-1)*(ratio*ratio);
+        for (int c = 6; c < 8; ++c) us(c, i) = us(c, v.nr-1)*ratio;
+    }
+}
+} // namespace
+
+namespace hc
+{
+struct Level4
+{
+    hc::ShellDims dims{};
+    hc_options opt{};
+    Prep4View pv{};
+    hc_boxop* box = nullptr;
+    int ncol = 0;
+    long long* colDst = nullptr;
+    long long* donorSrc = nullptr;
+    double* donorW = nullptr;
+    double* ang = nullptr;
+    double* cart = nullptr;
+    std::vector<void*> owned;
+    ~Level4()
+    {
+        delete box;
+        for (void* p : owned) (void)hipFree(p);
+    }
+};
+
+// Seam donors of a ghost column of the Cartesian averages: 16 cells of the
+// panel holding the ghost centre and their weights.
+struct AvgDonor { int q, j, k; double w; };
+static std::vector<AvgDonor> averageDonors(int na, int p, int j, int k)
+{
+    static std::vector<double> gx, gw;
+    if (gx.empty()) gaussLegendre01(6, gx, gw);
+    const double h = 1.0/na;
+    const int q = closestPanel(panelPoint(p, xiToAngle((j+0.5)*h), xiToAngle((k+0.5)*h)));
+    auto coords = [&](double a, double b, double& u, double& v) {
+        const Vec3 L = toPanelLocal(q, panelPoint(p, a, b));
+        u = (std::atan2(L.x[1], L.x[0])+kPi/4)/(h*kPi/2);
+        v = (std::atan2(L.x[2], L.x[0])+kPi/4)/(h*kPi/2);
+    };
+    double uc, vc;
+    coords(xiToAngle((j+0.5)*h), xiToAngle((k+0.5)*h), uc, vc);
+    const int bu = std::max(0, std::min(na-4, (int)std::floor(uc)-1));
+    const int bv = std::max(0, std::min(na-4, (int)std::floor(vc)-1));
+    // average-matching cubic basis on 4 cells: derivative of the Lagrange
+    // interpolant of the primitive function on the 5 cell edges
+    auto basis = [](int b, double u, int base) {
+        double val = 0;
+        for (int s = b+1; s <= 4; ++s)
+            for (int m = 0; m <= 4; ++m)
+            {
+                if (m == s) continue;
+                double prod = 1.0/double(s-m);
+                for (int n = 0; n <= 4; ++n)
+                    if (n != s && n != m) prod *= (u-(base+n))/double(s-n);
+                val += prod;
+            }
+        return val;
+    };
+    double w[4][4] = {};
+    for (int a = 0; a < 6; ++a)
+        for (int b = 0; b < 6; ++b)
+        {
+            double u, v;
+            coords(xiToAngle((j+gx[a])*h), xiToAngle((k+gx[b])*h), u, v);
+            const double wt = gw[a]*gw[b];
+            double bu_[4], bv_[4];
+            for (int m = 0; m < 4; ++m) { bu_[m] = basis(m, u, bu); bv_[m] = basis(m, v, bv); }
+            for (int m = 0; m < 4; ++m) for (int n = 0; n < 4; ++n) w[n][m] += wt*bu_[m]*bv_[n];
+        }
+    std::vector<AvgDonor> out;
+    for (int n = 0; n < 4; ++n) for (int m = 0; m < 4; ++m) out.push_back({q, bu+m, bv+n, w[n][m]});
+    return out;
+}
+
+Level4* level4Create(const hc_shell& shell, const hc_options& opt, int boxSize)
+{
+    std::unique_ptr<Level4> L(new Level4());
+    L->opt = opt;
+    L->dims = {shell.n_rad, shell.n_ang, kG, shell.inner_radius, shell.outer_radius, shell.radial_stretch};
+    const int nr = shell.n_rad, na = shell.n_ang;
+    Prep4View& v = L->pv;
+    v.nr = nr; v.na = na; v.NE = na+12;
+    v.vcs = (long long)nr*na*na; v.vps = 8*v.vcs;
+    v.cJ = na+2*kGC; v.ccs = v.cJ*v.cJ*nr; v.cps = 8*v.ccs;
+    v.sI = nr+10; v.sJ = na+10; v.scs = v.sI*v.sJ*(na+10); v.sps = 8*v.scs;
+    v.gamma = opt.gamma;
+    std::vector<double> det((size_t)v.NE*v.NE), Aavg(9ull*6*v.NE*v.NE), Ainv(9ull*6*v.NE*v.NE);
+    std::vector<double> gx, gw;
+    gaussLegendre01(10, gx, gw);
+    const double h = 1.0/na;
+    for (int k = -6; k < na+6; ++k) for (int j = -6; j < na+6; ++j) det[(size_t)(k+6)*v.NE+(j+6)] = cellDetA(na, j, k);
+    for (int p = 0; p < 6; ++p) for (int k = -6; k < na+6; ++k) for (int j = -6; j < na+6; ++j)
+    {
+        double M[3][3] = {}, Mi[3][3];
+        for (int a = 0; a < (int)gx.size(); ++a) for (int b = 0; b < (int)gx.size(); ++b)
+        {
+            double A[3][3];
+            pointBasis(p, xiToAngle((j+gx[a])*h), xiToAngle((k+gx[b])*h), A);
+            for (int r = 0; r < 3; ++r) for (int c = 0; c < 3; ++c) M[r][c] += gw[a]*gw[b]*A[r][c];
+        }
+        invert3(M, Mi);
+        const size_t x = 9*(((size_t)p*v.NE+(k+6))*v.NE+(j+6));
+        for (int r = 0; r < 3; ++r) for (int c = 0; c < 3; ++c) { Aavg[x+3*r+c] = M[r][c]; Ainv[x+3*r+c] = Mi[r][c]; }
+    }
+    std::vector<double> dvol(nr+13), rmid(nr+13);
+    for (int i = -6; i <= nr+6; ++i) { const RadialCell c = radialCell(L->dims, i); dvol[i+6] = c.dvol; rmid[i+6] = c.rmid; }
+    v.det = upload(det, L->owned); v.Aavg = upload(Aavg, L->owned); v.Ainv = upload(Ainv, L->owned);
+    v.dvol = upload(dvol, L->owned); v.rmid = upload(rmid, L->owned);
+    std::vector<long long> colDst, donorSrc;
+    std::vector<double> donorW;
+    for (int p = 0; p < 6; ++p) for (int k = -kGC; k < na+kGC; ++k) for (int j = -kGC; j < na+kGC; ++j)
+    {
+        if (j >= 0 && j < na && k >= 0 && k < na) continue;
+        colDst.push_back(p*v.cps + ((long long)(k+kGC)*v.cJ + (j+kGC))*nr);
+        for (const auto& d : averageDonors(na, p, j, k))
+        {
+            donorSrc.push_back(d.q*v.cps + ((long long)(d.k+kGC)*v.cJ + (d.j+kGC))*nr);
+            donorW.push_back(d.w);
+        }
+    }
+    L->ncol = (int)colDst.size();
+    L->colDst = upload(colDst, L->owned);
+    L->donorSrc = upload(donorSrc, L->owned);
+    L->donorW = upload(donorW, L->owned);
+    HC_THROW(hipMalloc(&L->ang, 6ull*8*na*na*nr*sizeof(double))); L->owned.push_back(L->ang);
+    HC_THROW(hipMalloc(&L->cart, 6ull*v.cps*sizeof(double))); L->owned.push_back(L->cart);
+    HC_THROW(hipMemset(L->cart, 0, 6ull*v.cps*sizeof(double)));
+    // box table over the level: tiles of boxSize^3 viewing the level arrays
+    std::vector<hc_box> boxes;
+    const int bs[3] = {std::min(boxSize, nr), std::min(boxSize, na), std::min(boxSize, na)};
+    for (int p = 0; p < 6; ++p)
+        for (int k = 0; k < na; k += bs[2]) for (int j = 0; j < na; j += bs[1]) for (int i = 0; i < nr; i += bs[0])
+        {
+            hc_box b{};
+            b.block = p;
+            b.lo[0] = i; b.lo[1] = j; b.lo[2] = k;
+            b.hi[0] = std::min(i+bs[0], nr)-1; b.hi[1] = std::min(j+bs[1], na)-1; b.hi[2] = std::min(k+bs[2], na)-1;
+            boxes.push_back(b);
+        }
+    std::unique_ptr<hc_boxop> B(new hc_boxop());
+    B->opt = opt;
+    B->dims = L->dims;
+    buildBoxop(*B, boxes.data(), (int)boxes.size());
+    // re-point every box at its tile of the level arrays
+    for (auto& bd : B->boxes)
+    {
+        bd.uOff = bd.p*v.sps + ((long long)bd.lo[2]*v.sJ + bd.lo[1])*v.sI + bd.lo[0];
+        bd.us[0] = v.sI; bd.us[1] = v.sI*v.sJ; bd.us[2] = v.scs;
+        bd.rOff = bd.p*v.vps + ((long long)bd.lo[2]*na + bd.lo[1])*nr + bd.lo[0];
+        bd.rs[0] = nr; bd.rs[1] = (long long)nr*na; bd.rs[2] = v.vcs;
+    }
+    HC_THROW(hipMemcpy(const_cast<BoxDesc*>(B->view.boxes), B->boxes.data(), B->boxes.size()*sizeof(BoxDesc),
+                       hipMemcpyHostToDevice));
+    // seam pairs between the boxes' face fluxes
+    auto boxOf = [&](int p, int i, int j, int k) {
+        for (size_t n = 0; n < B->boxes.size(); ++n)
+        {
+            const BoxDesc& b = B->boxes[n];
+            if (b.p == p && i >= b.lo[0] && i <= b.hi[0] && j >= b.lo[1] && j <= b.hi[1] && k >= b.lo[2] && k <= b.hi[2])
+                return (int)n;
+        }
+        return -1;
+    };
+    auto faceIndex = [&](int n, int d, int i, int jf, int kf, long long& ec) {
+        const BoxDesc& b = B->boxes[n];
+        const long long e0 = b.n[0]+2*kG, e1 = b.n[1]+2*kG, e2 = b.n[2]+2*kG;
+        ec = e0*e1*e2;
+        return b.sOff + (F_FLUX+8*d)*ec + ((long long)(kf-b.lo[2]+kG)*e1 + (jf-b.lo[1]+kG))*e0 + (i-b.lo[0]+kG);
+    };
+    std::vector<SeamPair> pairs;
+    for (int p = 0; p < 6; ++p) for (int d = 1; d < 3; ++d) for (int s = 0; s < 2; ++s)
+    {
+        const SeamEdge e = seamEdge(na, p, d, s);
+        if ((p*2+(d-1))*2+s > (e.q*2+(e.dq-1))*2+e.sq) continue;
+        const int f = s ? na : 0, fq = e.sq ? na : 0;
+        for (int t = 0; t < na; ++t) for (int i = 0; i < nr; ++i)
+        {
+            const int tq = e.rev ? na-1-t : t;
+            const int jf = d == 1 ? f : t, kf = d == 1 ? t : f;
+            const int jq = e.dq == 1 ? fq : tq, kq = e.dq == 1 ? tq : fq;
+            const int na1 = na-1;
+            const int na_ = boxOf(p, i, d == 1 ? (s ? na1 : 0) : t, d == 1 ? t : (s ? na1 : 0));
+            const int nb = boxOf(e.q, i, e.dq == 1 ? (e.sq ? na1 : 0) : tq, e.dq == 1 ? tq : (e.sq ? na1 : 0));
+            SeamPair sp;
+            sp.a = faceIndex(na_, d, i, jf, kf, sp.ea);
+            sp.b = faceIndex(nb, e.dq, i, jq, kq, sp.eb);
+            sp.sign = s != e.sq ? 1.0 : -1.0;
+            pairs.push_back(sp);
+        }
+    }
+    B->view.seams = upload(pairs, B->owned);
+    B->view.nseams = (int)pairs.size();
+    L->box = B.release();
+    return L.release();
+}
+
+void level4Destroy(Level4* L) { delete L; }
+
+void level4Prepare(Level4& L, const double* U0, const double* K, double a_dt, const double* inner, double* US,
+                   hipStream_t s)
+{
+    const Prep4View& v = L.pv;
+    const long long nv = 6LL*8*v.na*v.na*v.nr;
+    hipLaunchKernelGGL(k4_angular, dim3((nv+kB-1)/kB), dim3(kB), 0, s, v, U0, K, a_dt, L.ang);
+    hipLaunchKernelGGL(k4_radial, dim3((nv+kB-1)/kB), dim3(kB), 0, s, v, L.ang, L.cart);
+    hipLaunchKernelGGL(k4_seam, dim3(((long long)L.ncol*v.nr+kB-1)/kB), dim3(kB), 0, s,
+                       v, L.ncol, L.colDst, L.donorSrc, L.donorW, L.cart);
+    const long long nn = 6LL*(v.na+10)*(v.na+10)*v.nr;
+    hipLaunchKernelGGL(k4_native, dim3((nn+kB-1)/kB), dim3(kB), 0, s, v, L.cart, US);
+    hipLaunchKernelGGL(k4_radial_bc, dim3((6*(v.na+10)*(v.na+10)+kB-1)/kB), dim3(kB), 0, s, v, inner, US);
+    HC_THROW(hipGetLastError());
+}
+
+void level4Rhs(Level4& L, const double* US, double* K, double* U, double b_dt, hipStream_t s)
+{
+    double* const F[3] = {nullptr, nullptr, nullptr};
+    if (L.opt.precision == 1) applyBoxop<float>(*L.box, US, K, F, s, 1, U, b_dt);
+    else applyBoxop<double>(*L.box, US, K, F, s, 1, U, b_dt);
+}
+} // namespace hc
 
 extern "C" {
 

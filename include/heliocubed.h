@@ -46,8 +46,13 @@ typedef struct hc_shell
 
 /* Operator options: the hot-path subset of ParseInputs
  * (src/Inputs_Parsing.H:71-123). spatial_order 2 selects the PLM operator
- * (src/BoxOp_SecondOrder.H:155-370); limiter 0 none, 1 or 2 minmod, 3 MC;
- * riemann_solver 1 Rusanov, 2 Roe 8-wave. precision 0 evaluates everything
+ * (src/BoxOp_SecondOrder.H:155-370; limiter 0 none, 1 or 2 minmod, 3 MC;
+ * riemann_solver 1 Rusanov, 2 Roe 8-wave); spatial_order 4 the standard
+ * fourth-order operator (src/BoxOp_EulerCubedSphereImplem.H:97-402; limiter
+ * 0 none or 1 Limiters_4O with flattening; riemann_solver 2) with
+ * state_conversion 1 (src/MHD_StateConversion.H:153-316). A fourth-order
+ * level uses ghost 5; its stage array W holds native conserved averages and
+ * its inner ghosts are native conserved averages. precision 0 evaluates everything
  * in FP64; precision 1 (mixed) limits slopes and evaluates the Riemann
  * dissipation (Rusanov speed, Roe eigen-system) in FP32 while states, jumps,
  * metric terms, central fluxes, pressure recovery, divergence and the RK
