@@ -23,14 +23,19 @@ HBM_PEAK_GBS = 8000.0
 # the kernels that can dominate (unit = face for the face-flux kernels, cell
 # for the others).
 WORKLOADS = {
-    "shell-6x128x64x64-o2-minmod-roe": dict(n_rad=128, n_ang=64, limiter=1, riemann=2),
-    "shell-6x32x32x32-o2-minmod-rusanov": dict(n_rad=32, n_ang=32, limiter=1, riemann=1),
+    # 6 panels x 128 radial x 64x64 angular, fourth-order standard operator +
+    # Roe 8-wave, radial wind with density noise and Alfvenic perturbations
+    "shell-6x128x64x64-o4-roe": dict(n_rad=128, n_ang=64, limiter=1, riemann=2, order=4, config=3),
+    "shell-6x128x64x64-o2-minmod-roe": dict(n_rad=128, n_ang=64, limiter=1, riemann=2, order=2, config=0),
+    "shell-6x32x32x32-o2-minmod-rusanov": dict(n_rad=32, n_ang=32, limiter=1, riemann=1, order=2, config=0),
 }
 # Algorithmic HBM bytes per unit for each kernel (8 FP64 components = 64 B):
 #   face flux: read the 8-component cell state once + write the 8-component face flux;
 #   divergence: read 3 face fluxes + write K + read-modify-write U;
 #   prepare: read U0 and K, write W.
 ALGO_BYTES = {
+    # fourth-order stage: read U_stage (as JU and U0+K), read-modify-write U, write the next stage
+    "k4_rhs": 320, "k4_prepare": 320,
     # fused RHS: read the 8-component stage state, write K, read-modify-write U
     "k_rhs_fused": 64 + 64 + 128,
     "k_face_flux_0": 128, "k_face_flux_1": 128, "k_face_flux_2": 128,
@@ -54,12 +59,20 @@ def cpu_baseline(wl, ju, inner, dt):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc  # the oracle is the timed CPU baseline only
 
+    cells = 6 * wl["n_rad"] * wl["n_ang"] ** 2
+    if wl["order"] == 4:
+        O = orc.Oracle4(wl["n_rad"], wl["n_ang"], limiter=wl["limiter"])
+        t0 = time.perf_counter()
+        O.rhs(O.prepare(ju, inner))
+        t = time.perf_counter() - t0
+        return {"value": cells / t, "unit": "cell-updates/s per RK stage", "cores": threads, "kind": "port",
+                "sample": f"1 RK stage (prepare + RHS) of the same shell on the CPU FP64 oracle, "
+                          f"{threads} OpenMP threads, {t:.2f} s"}
     O = orc.Oracle(wl["n_rad"], wl["n_ang"], limiter=wl["limiter"], riemann=wl["riemann"])
     O.rk4(ju, inner, dt, 1)  # first call builds the OpenMP pool and caches
     t0 = time.perf_counter()
     O.rk4(ju, inner, dt, 1)
     t = time.perf_counter() - t0
-    cells = 6 * wl["n_rad"] * wl["n_ang"] ** 2
     return {"value": cells * 4 / t, "unit": "cell-updates/s per RK stage", "cores": threads,
             "kind": "port",
             "sample": f"1 RK4 step of the same shell on the CPU FP64 oracle, {threads} OpenMP threads, {t:.2f} s"}
@@ -70,7 +83,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="shell-6x128x64x64-o2-minmod-roe", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="shell-6x128x64x64-o4-roe", choices=sorted(WORKLOADS))
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -85,9 +98,12 @@ def main():
 
     wl = WORKLOADS[args.workload]
     shell = hc.Shell(wl["n_rad"], wl["n_ang"])
-    opts = hc.Options(limiter=wl["limiter"], riemann_solver=wl["riemann"],
+    opts = hc.Options(spatial_order=wl["order"], limiter=wl["limiter"], riemann_solver=wl["riemann"],
                       precision=1 if args.precision == "mixed" else 0)
-    ju, inner = hc.init_state(shell, opts, 0, 20261203)
+    if wl["order"] == 4:
+        ju, inner = hc.init_level4(shell, opts, wl["config"], 20261203, 2)
+    else:
+        ju, inner = hc.init_state(shell, opts, wl["config"], 20261203)
     dt = 0.25 * (shell.inner_radius * np.pi / 2 / shell.n_ang) / 1.6
     dev = torch.device("cuda", 0)
     U = torch.from_numpy(ju).to(dev)
@@ -126,10 +142,12 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64" if args.precision == "fp64" else "f64 state, f32 slopes and Riemann dissipation",
-        "data": "synthetic seeded radial-wind shell with a Gaussian pressure pulse",
+        "data": ("synthetic seeded radial-wind shell with a Gaussian pressure pulse"
+                 + (", lognormal density noise and Alfvenic perturbations" if wl["config"] == 3 else "")),
         "config": {"workload": args.workload, "cells": cells, "panels": 6, "n_rad": wl["n_rad"],
-                   "n_ang": wl["n_ang"], "limiter": wl["limiter"], "riemann_solver": wl["riemann"],
-                   "precision": args.precision},
+                   "n_ang": wl["n_ang"], "spatial_order": wl["order"], "limiter": wl["limiter"],
+                   "riemann_solver": wl["riemann"], "precision": args.precision,
+                   "boxes": "32^3 box table" if wl["order"] == 4 else "panel arrays"},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "avg_launch_us": avg_s * 1e6},

@@ -76,6 +76,7 @@ _SIGNATURES = [
     ("hc_boxop_apply", _I, [_P, _P, _P, _P, _P, _P, _P]),
     ("hc_init_native_boxes", _I, [ctypes.POINTER(HcShell), ctypes.POINTER(HcOptions), _I, ctypes.c_ulonglong,
                                   ctypes.POINTER(HcBox), _I, _I, _P]),
+    ("hc_init_level4", _I, [ctypes.POINTER(HcShell), ctypes.POINTER(HcOptions), _I, ctypes.c_ulonglong, _I, _P, _P]),
     ("hc_level_profile", _I, [_P, _I]),
     ("hc_level_profile_read", _I, [_P, _I, ctypes.c_char_p, _I, ctypes.POINTER(_D), ctypes.POINTER(ctypes.c_longlong)]),
     ("hc_last_error", ctypes.c_char_p, []),
@@ -152,6 +153,20 @@ def init_state(shell: Shell, options: Options, config: int = 0, seed: int = 2026
     sh, op = shell.c(), options.c()
     _check(lib().hc_init_state(ctypes.byref(sh), ctypes.byref(op), config, seed,
                                ju.ctypes.data, inner.ctypes.data))
+    return ju, inner
+
+
+def init_level4(shell: Shell, options: Options, config: int = 3, seed: int = 20261203, quad: int = 3):
+    """<J U> of the valid level and native inner-ghost averages of a fourth-order level (numpy)."""
+    import numpy as np
+
+    g = 5
+    na, nr = shell.n_ang, shell.n_rad
+    ju = np.zeros((6, 8, na, na, nr))
+    inner = np.zeros((6, 8, na + 2 * g, na + 2 * g, g))
+    sh, op = shell.c(), options.c()
+    _check(lib().hc_init_level4(ctypes.byref(sh), ctypes.byref(op), config, seed, quad,
+                                ju.ctypes.data, inner.ctypes.data))
     return ju, inner
 
 

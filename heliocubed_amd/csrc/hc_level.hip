@@ -785,8 +785,9 @@ int hc_level_create(const hc_shell* shell, const hc_options* options, hc_level**
     if (options->limiter < 0 || options->limiter > 3) return fail("hc_level_create: limiter must be 0..3");
     if (options->spatial_order == 4 && (options->riemann_solver != 2 || options->limiter > 1))
         return fail("hc_level_create: the fourth-order operator takes riemann_solver 2 (Roe) and limiter 0 or 1");
-    if (options->spatial_order == 4 && (shell->n_ang < 8 || shell->n_rad < 4))
-        return fail("hc_level_create: the fourth-order operator needs n_ang >= 8 and n_rad >= 4");
+    if (options->spatial_order == 4 && (shell->n_ang < 16 || shell->n_rad < 4))
+        return fail("hc_level_create: the fourth-order level needs n_ang >= 16 (five ghost layers must stay "
+                    "inside the panel's horizon) and n_rad >= 4");
     if (options->precision != 0 && options->precision != 1) return fail("hc_level_create: precision must be 0 (fp64) or 1 (mixed)");
     try
     {
@@ -1034,6 +1035,110 @@ int hc_init_native_boxes(const hc_shell* shell, const hc_options* options, int c
             for (int c = 0; c < 8; ++c) USph[off + c*ec + ((size_t)kk*e[1]+jj)*e[0]+ii] = avg[c];
         }
         off += 8*ec;
+    }
+    return 0;
+}
+
+int hc_init_level4(const hc_shell* shell, const hc_options* options, int config, unsigned long long seed,
+                   int quad, double* JU, double* inner)
+{
+    if (!shell || !options || !JU || !inner) return fail("hc_init_level4: null argument");
+    if (config < 0 || config > 3) return fail("hc_init_level4: config must be 0..3");
+    if (quad < 1 || quad > 8) return fail("hc_init_level4: quad must be 1..8");
+    const int nr = shell->n_rad, na = shell->n_ang, g = 5;
+    // inner radial ghosts: native averages of the analytic state
+    std::vector<hc_box> boxes;
+    for (int p = 0; p < 6; ++p) { hc_box b{}; b.block = p; b.lo[0] = -g; b.hi[0] = -1;
+        b.lo[1] = 0; b.lo[2] = 0; b.hi[1] = na-1; b.hi[2] = na-1; boxes.push_back(b); }
+    // hc_init_native_boxes writes each box grown by 5: take the ghosted rows
+    // i in [-5, -1] of the angular range [-5, na+5) from a thin box
+    const int e0 = g + 2*g, e1 = na + 2*g;
+    std::vector<double> tmp((size_t)6*8*e0*e1*e1);
+    const int rc = hc_init_native_boxes(shell, options, config, seed, boxes.data(), 6, quad, tmp.data());
+    if (rc != 0) return rc;
+    for (int p = 0; p < 6; ++p) for (int c = 0; c < 8; ++c) for (int kk = 0; kk < e1; ++kk) for (int jj = 0; jj < e1; ++jj)
+        for (int ig = 0; ig < g; ++ig)
+            inner[((((size_t)p*8+c)*e1+kk)*e1+jj)*g+ig] =
+                tmp[(size_t)p*8*e0*e1*e1 + (((size_t)c*e1+kk)*e1+jj)*e0 + (ig+g)];
+    // <J U>: J-weighted averages with Cartesian vector components
+    const hc::ShellDims dims{nr, na, g, shell->inner_radius, shell->outer_radius, shell->radial_stretch};
+    SplitMix rng{seed};
+    WindState ws;
+    ws.r0 = shell->inner_radius; ws.r1 = shell->outer_radius; ws.gamma = options->gamma;
+    const double cz = 2.0*rng.uniform()-1.0, phi = 2.0*hc::kPi*rng.uniform();
+    const double rcn = 0.5*(ws.r0+ws.r1), sz = std::sqrt(std::max(0.0, 1.0-cz*cz));
+    ws.center[0] = rcn*sz*std::cos(phi); ws.center[1] = rcn*sz*std::sin(phi); ws.center[2] = rcn*cz;
+    ws.width = 0.1*(ws.r1-ws.r0);
+    ws.pulse = config == 2 ? 0.0 : 0.2;
+    ws.hemispheric = config == 0 || config == 3;
+    if (config == 3)
+    {
+        ws.nmodes = 16; ws.amp = 0.05/std::sqrt(16.0);
+        for (int m = 0; m < 16; ++m)
+        {
+            double kk[3], n2 = 0;
+            for (int a = 0; a < 3; ++a) { kk[a] = 2.0*rng.uniform()-1.0; n2 += kk[a]*kk[a]; }
+            const double kmag = 1.0 + 7.0*rng.uniform();
+            for (int a = 0; a < 3; ++a) ws.kvec[m][a] = kk[a]/std::sqrt(n2)*kmag;
+            double e[3], e2 = 0;
+            for (int a = 0; a < 3; ++a) { e[a] = 2.0*rng.uniform()-1.0; e2 += e[a]*e[a]; }
+            for (int a = 0; a < 3; ++a) ws.evec[m][a] = e[a]/std::sqrt(e2);
+            ws.phase[m] = 2.0*hc::kPi*rng.uniform();
+        }
+    }
+    std::vector<double> gx, gw;
+    hc::gaussLegendre01(quad, gx, gw);
+    const double h = 1.0/na, gamma = options->gamma;
+    const double C = dims.stretch;
+    const double span = C == 0.0 ? (dims.r1-dims.r0) : (dims.r1-dims.r0)/std::expm1(C);
+    for (int p = 0; p < 6; ++p) for (int k = 0; k < na; ++k) for (int j = 0; j < na; ++j)
+    {
+        // per-column angular samples
+        std::vector<double> dirs(3*quad*quad), dets(quad*quad);
+        for (int qk = 0; qk < quad; ++qk) for (int qj = 0; qj < quad; ++qj)
+        {
+            const double al = hc::xiToAngle((j+gx[qj])*h), be = hc::xiToAngle((k+gx[qk])*h);
+            double A[3][3];
+            hc::pointBasis(p, al, be, A);
+            dets[qk*quad+qj] = A[0][0]*(A[1][1]*A[2][2]-A[1][2]*A[2][1]) - A[0][1]*(A[1][0]*A[2][2]-A[1][2]*A[2][0])
+                             + A[0][2]*(A[1][0]*A[2][1]-A[1][1]*A[2][0]);
+            const hc::Vec3 d = hc::panelPoint(p, al, be);
+            for (int a = 0; a < 3; ++a) dirs[3*(qk*quad+qj)+a] = d.x[a];
+        }
+        for (int i = 0; i < nr; ++i)
+        {
+            double noise = 1.0;
+            if (config == 3)
+            {
+                SplitMix cr{seed ^ (0x9E3779B97F4A7C15ull*((((unsigned long long)p*4096+(k+8))*4096+(j+8))*65536+(i+8)))};
+                const double u1 = std::max(cr.uniform(), 1e-300), u2 = cr.uniform();
+                const double gn = std::sqrt(-2.0*std::log(u1))*std::cos(2*hc::kPi*u2);
+                noise = std::exp(0.05*gn-0.00125);
+            }
+            double avg[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            for (int qi = 0; qi < quad; ++qi)
+            {
+                const double eta = (i+gx[qi])/double(nr);
+                const double r = dims.r0 + span*(C == 0.0 ? eta : std::expm1(C*eta));
+                const double drdxi = C == 0.0 ? span : span*C*std::exp(C*eta);
+                for (int qa = 0; qa < quad*quad; ++qa)
+                {
+                    const double* d = &dirs[3*qa];
+                    const double X[3] = {r*d[0], r*d[1], r*d[2]};
+                    double w[8];
+                    ws.at(X, w);
+                    w[0] *= noise;
+                    double v2 = 0, b2 = 0;
+                    for (int a = 0; a < 3; ++a) { v2 += w[1+a]*w[1+a]; b2 += w[5+a]*w[5+a]; }
+                    const double J = r*r*drdxi*dets[qa];
+                    const double wt = gw[qi]*gw[qa % quad]*gw[qa / quad]*J;
+                    avg[0] += wt*w[0];
+                    for (int a = 0; a < 3; ++a) { avg[1+a] += wt*w[0]*w[1+a]; avg[5+a] += wt*w[5+a]; }
+                    avg[4] += wt*(w[4]/(gamma-1.0) + 0.5*w[0]*v2 + b2/(8*hc::kPi));
+                }
+            }
+            for (int c = 0; c < 8; ++c) JU[((((size_t)p*8+c)*na+k)*na+j)*nr+i] = avg[c];
+        }
     }
     return 0;
 }

@@ -1002,19 +1002,21 @@ __global__ void __launch_bounds__(kB) k4_radial(Prep4View v, const double* __res
     cart[p*v.cps + c*v.ccs + ((long long)(k+kGC)*v.cJ + (j+kGC))*v.nr + i] = (a[i]-dq*dd/12.0)*(1.0/dv(i));
 }
 
-// Seam ghost columns of the Cartesian averages (16 donors each).
+// Seam ghost columns of the Cartesian averages: weighted sums of donor
+// columns (valid cells of the panels the ghost cell overlaps).
 __global__ void __launch_bounds__(kB) k4_seam(Prep4View v, int ncol, const long long* __restrict__ dst,
-                                              const long long* __restrict__ src, const double* __restrict__ w,
-                                              double* __restrict__ cart)
+                                              const int* __restrict__ start, const long long* __restrict__ src,
+                                              const double* __restrict__ w, double* __restrict__ cart)
 {
     const long long t = (long long)blockIdx.x*blockDim.x + threadIdx.x;
     if (t >= (long long)ncol*v.nr) return;
     const int i = t % v.nr;
     const int col = t / v.nr;
+    const int d0 = start[col], d1 = start[col+1];
     for (int c = 0; c < 8; ++c)
     {
         double s = 0;
-        for (int d = 0; d < 16; ++d) s += w[16*col+d]*cart[src[16*col+d] + c*v.ccs + i];
+        for (int d = d0; d < d1; ++d) s += w[d]*cart[src[d] + c*v.ccs + i];
         cart[dst[col] + c*v.ccs + i] = s;
     }
 }
@@ -1105,6 +1107,7 @@ struct Level4
     hc_boxop* box = nullptr;
     int ncol = 0;
     long long* colDst = nullptr;
+    int* colStart = nullptr;
     long long* donorSrc = nullptr;
     double* donorW = nullptr;
     double* ang = nullptr;
@@ -1117,26 +1120,18 @@ struct Level4
     }
 };
 
-// Seam donors of a ghost column of the Cartesian averages: 16 cells of the
-// panel holding the ghost centre and their weights.
+// Seam donors of a ghost column of the Cartesian averages. The ghost cell is
+// integrated by a 6x6 Gauss-Legendre rule in its own panel's xi; each
+// quadrature point takes the cubic average-matching reconstruction (the
+// derivative of the quartic interpolant of the primitive function on five
+// cell edges) of the panel holding that point, on the 4x4 window around it
+// clamped inside the panel. Donors of all points are merged.
 struct AvgDonor { int q, j, k; double w; };
 static std::vector<AvgDonor> averageDonors(int na, int p, int j, int k)
 {
     static std::vector<double> gx, gw;
     if (gx.empty()) gaussLegendre01(6, gx, gw);
     const double h = 1.0/na;
-    const int q = closestPanel(panelPoint(p, xiToAngle((j+0.5)*h), xiToAngle((k+0.5)*h)));
-    auto coords = [&](double a, double b, double& u, double& v) {
-        const Vec3 L = toPanelLocal(q, panelPoint(p, a, b));
-        u = (std::atan2(L.x[1], L.x[0])+kPi/4)/(h*kPi/2);
-        v = (std::atan2(L.x[2], L.x[0])+kPi/4)/(h*kPi/2);
-    };
-    double uc, vc;
-    coords(xiToAngle((j+0.5)*h), xiToAngle((k+0.5)*h), uc, vc);
-    const int bu = std::max(0, std::min(na-4, (int)std::floor(uc)-1));
-    const int bv = std::max(0, std::min(na-4, (int)std::floor(vc)-1));
-    // average-matching cubic basis on 4 cells: derivative of the Lagrange
-    // interpolant of the primitive function on the 5 cell edges
     auto basis = [](int b, double u, int base) {
         double val = 0;
         for (int s = b+1; s <= 4; ++s)
@@ -1150,19 +1145,30 @@ static std::vector<AvgDonor> averageDonors(int na, int p, int j, int k)
             }
         return val;
     };
-    double w[4][4] = {};
+    std::vector<AvgDonor> out;
     for (int a = 0; a < 6; ++a)
         for (int b = 0; b < 6; ++b)
         {
-            double u, v;
-            coords(xiToAngle((j+gx[a])*h), xiToAngle((k+gx[b])*h), u, v);
+            const Vec3 X = panelPoint(p, xiToAngle((j+gx[a])*h), xiToAngle((k+gx[b])*h));
+            const int q = closestPanel(X);
+            const Vec3 L = toPanelLocal(q, X);
+            const double u = (std::atan2(L.x[1], L.x[0])+kPi/4)/(h*kPi/2);
+            const double v = (std::atan2(L.x[2], L.x[0])+kPi/4)/(h*kPi/2);
+            const int bu = std::max(0, std::min(na-4, (int)std::floor(u)-1));
+            const int bv = std::max(0, std::min(na-4, (int)std::floor(v)-1));
             const double wt = gw[a]*gw[b];
-            double bu_[4], bv_[4];
-            for (int m = 0; m < 4; ++m) { bu_[m] = basis(m, u, bu); bv_[m] = basis(m, v, bv); }
-            for (int m = 0; m < 4; ++m) for (int n = 0; n < 4; ++n) w[n][m] += wt*bu_[m]*bv_[n];
+            double fu[4], fv[4];
+            for (int m = 0; m < 4; ++m) { fu[m] = basis(m, u, bu); fv[m] = basis(m, v, bv); }
+            for (int n = 0; n < 4; ++n)
+                for (int m = 0; m < 4; ++m)
+                {
+                    const double w = wt*fu[m]*fv[n];
+                    bool found = false;
+                    for (auto& d : out)
+                        if (d.q == q && d.j == bu+m && d.k == bv+n) { d.w += w; found = true; break; }
+                    if (!found) out.push_back({q, bu+m, bv+n, w});
+                }
         }
-    std::vector<AvgDonor> out;
-    for (int n = 0; n < 4; ++n) for (int m = 0; m < 4; ++m) out.push_back({q, bu+m, bv+n, w[n][m]});
     return out;
 }
 
@@ -1201,6 +1207,7 @@ Level4* level4Create(const hc_shell& shell, const hc_options& opt, int boxSize)
     v.det = upload(det, L->owned); v.Aavg = upload(Aavg, L->owned); v.Ainv = upload(Ainv, L->owned);
     v.dvol = upload(dvol, L->owned); v.rmid = upload(rmid, L->owned);
     std::vector<long long> colDst, donorSrc;
+    std::vector<int> colStart{0};
     std::vector<double> donorW;
     for (int p = 0; p < 6; ++p) for (int k = -kGC; k < na+kGC; ++k) for (int j = -kGC; j < na+kGC; ++j)
     {
@@ -1211,9 +1218,11 @@ Level4* level4Create(const hc_shell& shell, const hc_options& opt, int boxSize)
             donorSrc.push_back(d.q*v.cps + ((long long)(d.k+kGC)*v.cJ + (d.j+kGC))*nr);
             donorW.push_back(d.w);
         }
+        colStart.push_back((int)donorW.size());
     }
     L->ncol = (int)colDst.size();
     L->colDst = upload(colDst, L->owned);
+    L->colStart = upload(colStart, L->owned);
     L->donorSrc = upload(donorSrc, L->owned);
     L->donorW = upload(donorW, L->owned);
     HC_THROW(hipMalloc(&L->ang, 6ull*8*na*na*nr*sizeof(double))); L->owned.push_back(L->ang);
@@ -1298,7 +1307,7 @@ void level4Prepare(Level4& L, const double* U0, const double* K, double a_dt, co
     hipLaunchKernelGGL(k4_angular, dim3((nv+kB-1)/kB), dim3(kB), 0, s, v, U0, K, a_dt, L.ang);
     hipLaunchKernelGGL(k4_radial, dim3((nv+kB-1)/kB), dim3(kB), 0, s, v, L.ang, L.cart);
     hipLaunchKernelGGL(k4_seam, dim3(((long long)L.ncol*v.nr+kB-1)/kB), dim3(kB), 0, s,
-                       v, L.ncol, L.colDst, L.donorSrc, L.donorW, L.cart);
+                       v, L.ncol, L.colDst, L.colStart, L.donorSrc, L.donorW, L.cart);
     const long long nn = 6LL*(v.na+10)*(v.na+10)*v.nr;
     hipLaunchKernelGGL(k4_native, dim3((nn+kB-1)/kB), dim3(kB), 0, s, v, L.cart, US);
     hipLaunchKernelGGL(k4_radial_bc, dim3((6*(v.na+10)*(v.na+10)+kB-1)/kB), dim3(kB), 0, s, v, inner, US);

@@ -166,6 +166,14 @@ int hc_init_state(const hc_shell* shell, const hc_options* options, int config,
 int hc_init_native_boxes(const hc_shell* shell, const hc_options* options, int config, unsigned long long seed,
                          const hc_box* boxes, int nbox, int quad, double* USph_host);
 
+/* Host-side initial data of a fourth-order level (HOST arrays): J-weighted
+ * cell averages <J U> of the valid level (Cartesian vector components,
+ * quad^3 Gauss-Legendre in xi with the point Jacobian r^2 dr/dxi det A) and
+ * the native conserved averages of the inner radial ghosts
+ * [6][8][na+10][na+10][5], for the states of hc_init_native_boxes. */
+int hc_init_level4(const hc_shell* shell, const hc_options* options, int config, unsigned long long seed,
+                   int quad, double* JU_host, double* inner_host);
+
 /* Host-side geometry probes (no device use): cell-averaged det A of angular
  * cell (j,k), and the face-averaged cofactor row of face (d; j,k) of panel p. */
 double hc_geom_cell_detA(int n_ang, int j, int k);

@@ -64,6 +64,12 @@ def lib() -> ctypes.CDLL:
             "orc_seam_edge": (None, [cfgp, I, I, I, P]),
             "orc_o4_rhs": (I, [cfgp, I, P, P, P, P, P, P, P]),
             "orc_face_average": (None, [cfgp, I, I, I, I, I, P]),
+            "orc4_create": (P, [cfgp]),
+            "orc4_destroy": (None, [P]),
+            "orc4_prepare": (I, [P, P, P, P]),
+            "orc4_rhs": (I, [P, P, P]),
+            "orc4_rk4": (I, [P, P, P, D, I]),
+            "orc4_ghost_donors": (I, [cfgp, I, I, I, P, P, P, P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -205,3 +211,36 @@ def o4_rhs_boxes(n_rad, n_ang, boxes, USph, limiter=1, entropy_fix=0.3, gamma=5.
     if with_fluxes:
         return rhs, [np.concatenate(flux_parts[d]) for d in range(3)]
     return rhs
+
+
+class Oracle4:
+    """Fourth-order level (standard operator + Roe, state_conversion 1) on the CPU."""
+
+    def __init__(self, n_rad, n_ang, limiter=1, entropy_fix=0.3, gamma=5.0 / 3.0, r0=21.5, r1=215.0):
+        self.cfg = OrcCfg(n_rad, n_ang, 5, r0, r1, 0.0, gamma, limiter, 2, entropy_fix)
+        self.nr, self.na = n_rad, n_ang
+        self._h = ctypes.c_void_p(lib().orc4_create(ctypes.byref(self.cfg)))
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().orc4_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    @property
+    def stage_shape(self):
+        return (6, 8, self.na + 10, self.na + 10, self.nr + 10)
+
+    def prepare(self, ju, inner):
+        W = np.zeros(self.stage_shape)
+        lib().orc4_prepare(self._h, _p(np.ascontiguousarray(ju)), _p(np.ascontiguousarray(inner)), _p(W))
+        return W
+
+    def rhs(self, W):
+        k = np.zeros((6, 8, self.na, self.na, self.nr))
+        lib().orc4_rhs(self._h, _p(np.ascontiguousarray(W)), _p(k))
+        return k
+
+    def rk4(self, ju, inner, dt, nsteps=1):
+        u = np.ascontiguousarray(ju.copy())
+        lib().orc4_rk4(self._h, _p(u), _p(np.ascontiguousarray(inner)), dt, nsteps)
+        return u

@@ -87,3 +87,25 @@ def test_o4_rhs_converges_at_fourth_order():
     rate = np.log2(e0[[0, 1, 4]] / e1[[0, 1, 4]])
     print("O4 convergence rates (rho, mom_r, E):", rate)
     assert (rate > 3.3).all(), rate
+
+
+def test_o4_prepare_converges_at_fourth_order():
+    """JU (J-weighted averages) -> native averages: the error against the exact
+    native averages (quadrature of the analytic state) falls by ~16 per
+    refinement on valid cells and on the seam ghost cells."""
+    errs = []
+    for na in (16, 32):
+        nr = na
+        shell = hc.Shell(nr, na, 21.5, 60.0)
+        opts = hc.Options(spatial_order=4, limiter=1, riemann_solver=2)
+        ju, inner = hc.init_level4(shell, opts, 1, 3, 5)
+        W = orc.Oracle4(nr, na, r1=60.0).prepare(ju, inner)
+        boxes = [(p, (0, 0, 0), (nr - 1, na - 1, na - 1)) for p in range(6)]
+        ex = hc.init_native_boxes(shell, opts, boxes, config=1, seed=3, quad=5).reshape(W.shape)
+        g = 5
+        sl = np.s_[:, :, g - 2:g + na + 2, g - 2:g + na + 2, g:g + nr]
+        d = np.abs(W[sl] - ex[sl])
+        errs.append(np.array([d[:, c].max() for c in (0, 1, 4)]))
+    rate = np.log2(errs[0] / errs[1])
+    print("prepare convergence rates (rho, mom_r, E):", rate)
+    assert (rate > 3.4).all(), rate
