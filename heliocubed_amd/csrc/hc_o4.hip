@@ -34,7 +34,14 @@ constexpr int kB = 256;
 enum Field : int
 {
     F_WB = 0, F_MEB = 8, F_W = 9, F_ETA = 17, F_PTL = 18, F_PTR = 26, F_P0L = 34, F_P0R = 35,
-    F_MEL = 36, F_MER = 37, F_PHL = 38, F_PHR = 46, F_DPT = 54, F_FLUX = 62, F_COUNT = 86
+    F_MEL = 36, F_MER = 37, F_PHL = 38, F_PHR = 46, F_DPT = 54, F_FLUX = 62,
+    F_FB = 86, F_MEF = 94, F_COUNT = 95,
+    // mapped products of the two sides' point fluxes; reuse the face state
+    // slots, which are dead once k_o4_roe has run
+    F_MPL = F_PTL, F_MPR = F_PTR,
+    // face-average total pressure pBar + meBar of the faces of the direction in
+    // progress; reuses the cell meBar slot, which is dead once k_o4_eta has run
+    F_P0B = F_MEB
 };
 
 // One box of the table. U (ghosted native conserved state) and R (RHS) are
@@ -517,10 +524,30 @@ __device__ void barFlux(const O4View& v, const BoxDesc& b, const BoxIdx& I, cons
     pBar = wf[4];
 }
 
-// 5. Physical fluxes of both face states (point flux + transverse Laplacian of
-// the face-average flux / 24), face-average total pressure p + me, and the
-// point Roe dissipation. Faces [lo_d, hi_d+1] x transverse [lo-2, hi+2].
-template <int D, typename S_>
+// 5a. Physical flux of the face-average state WFaceBar (F_FB), its magnetic
+// energy (F_MEF) and total pressure pBar + meBar (F_P0B) on the faces
+// [lo_d, hi_d+1] x transverse [lo-3, hi+3].
+template <int D>
+__global__ void __launch_bounds__(kB) k_o4_bar(O4View v, double* __restrict__ S)
+{
+    const BoxDesc& b = v.boxes[blockIdx.y];
+    int L[3], H[3], x[3];
+    for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]-3; H[d] = b.hi[d]+3; }
+    L[D] = b.lo[D]; H[D] = b.hi[D]+1;
+    if (!regionPoint(b, L, H, (long long)blockIdx.x*blockDim.x + threadIdx.x, x)) return;
+    const BoxIdx I(b);
+    double f[8], me, p;
+    barFlux<D>(v, b, I, S, x[0], x[1], x[2], f, me, p);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) S[I(F_FB+q, x[0], x[1], x[2])] = f[q];
+    S[I(F_MEF, x[0], x[1], x[2])] = me;
+    S[I(F_P0B, x[0], x[1], x[2])] = p + me;
+}
+
+// 5b. Physical fluxes of both face states (point flux + transverse Laplacian
+// of the face-average flux / 24) and the face-average total pressure p + me.
+// Faces [lo_d, hi_d+1] x transverse [lo-2, hi+2].
+template <int D>
 __global__ void __launch_bounds__(kB) k_o4_phys(O4View v, double* __restrict__ S)
 {
     const BoxDesc& b = v.boxes[blockIdx.y];
@@ -531,23 +558,23 @@ __global__ void __launch_bounds__(kB) k_o4_phys(O4View v, double* __restrict__ S
     const BoxIdx I(b);
     constexpr int t1 = D == 0 ? 1 : 0, t2 = D == 2 ? 1 : 2;
     // transverse Laplacian of the face-average flux and magnetic energy
-    double fx[8], lapF[8], mex, px, lapMe = 0;
-    barFlux<D>(v, b, I, S, x[0], x[1], x[2], fx, mex, px);
+    double lapF[8], lapMe = 0;
+    const double mex = S[I(F_MEF, x[0], x[1], x[2])];
 #pragma unroll
     for (int q = 0; q < 8; ++q) lapF[q] = 0;
+#pragma unroll
     for (int t = 0; t < 2; ++t)
     {
         const int tt = t == 0 ? t1 : t2;
         int y[3] = {x[0], x[1], x[2]}, z[3] = {x[0], x[1], x[2]};
         y[tt] += 1; z[tt] -= 1;
-        double fa[8], fb[8], ma, mb, pa, pb;
-        barFlux<D>(v, b, I, S, y[0], y[1], y[2], fa, ma, pa);
-        barFlux<D>(v, b, I, S, z[0], z[1], z[2], fb, mb, pb);
 #pragma unroll
-        for (int q = 0; q < 8; ++q) lapF[q] += fa[q] - 2.0*fx[q] + fb[q];
-        lapMe += ma - 2.0*mex + mb;
+        for (int q = 0; q < 8; ++q)
+            lapF[q] += S[I(F_FB+q, y[0], y[1], y[2])] - 2.0*S[I(F_FB+q, x[0], x[1], x[2])] + S[I(F_FB+q, z[0], z[1], z[2])];
+        lapMe += S[I(F_MEF, y[0], y[1], y[2])] - 2.0*mex + S[I(F_MEF, z[0], z[1], z[2])];
     }
     const double* A = v.faceA[D] + 9*angIdx(v, b.p, x[1], x[2]);
+#pragma unroll 1
     for (int side = 0; side < 2; ++side)
     {
         double w[8];
@@ -561,50 +588,63 @@ __global__ void __launch_bounds__(kB) k_o4_phys(O4View v, double* __restrict__ S
         const int fp = side == 0 ? F_P0L : F_P0R;
         S[I(fp, x[0], x[1], x[2])] = S[I(fp, x[0], x[1], x[2])] + (me + lapMe/24.0);
     }
-    bool inner = true;
-#pragma unroll
-    for (int d = 0; d < 3; ++d) if (d != D) inner = inner && x[d] >= b.lo[d]-1 && x[d] <= b.hi[d]+1;
-    if (inner)
-    {
-        double wl[8], wr[8], out[8];
-#pragma unroll
-        for (int q = 0; q < 8; ++q) { wl[q] = S[I(F_PTL+q, x[0], x[1], x[2])]; wr[q] = S[I(F_PTR+q, x[0], x[1], x[2])]; }
-        factoredRoe<S_>(out, wl, wr, A, radialPoint(v, D, x[0]), D, v.gamma, v.entropy);
-#pragma unroll
-        for (int q = 0; q < 8; ++q) S[I(F_DPT+q, x[0], x[1], x[2])] = out[q];
-    }
 }
 
-// 6. Mapped face flux on the flux faces [lo_d, hi_d+1] x [lo, hi]:
-// face products with det A / det(A) A, the radial-area tensor product, the
-// pressure force through the cofactor row, the averaged Roe dissipation, and
-// F = (F_L + F_R - D)/2.
-template <int D>
-__global__ void __launch_bounds__(kB) k_o4_flux(O4View v, double* __restrict__ S)
+// 5c. Point Roe dissipation of the face states on the faces [lo_d, hi_d+1] x
+// transverse [lo-1, hi+1].
+template <int D, typename S_>
+__global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(2))) k_o4_roe(O4View v, double* __restrict__ S)
 {
     const BoxDesc& b = v.boxes[blockIdx.y];
     int L[3], H[3], x[3];
-    for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]; H[d] = b.hi[d]; }
-    H[D] = b.hi[D]+1;
+    for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]-1; H[d] = b.hi[d]+1; }
+    L[D] = b.lo[D]; H[D] = b.hi[D]+1;
     if (!regionPoint(b, L, H, (long long)blockIdx.x*blockDim.x + threadIdx.x, x)) return;
     const BoxIdx I(b);
-    constexpr int t1 = D == 0 ? 1 : 0, t2 = D == 2 ? 1 : 2;
-    const double* met = v.fmet[D];
-    auto M = [&](int c, const int y[3]) { return met[13*angIdx(v, b.p, y[1], y[2]) + c]; };
-    auto Dm = [&](int c, int t, const int y[3]) {
+    const double* A = v.faceA[D] + 9*angIdx(v, b.p, x[1], x[2]);
+    double wl[8], wr[8], out[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) { wl[q] = S[I(F_PTL+q, x[0], x[1], x[2])]; wr[q] = S[I(F_PTR+q, x[0], x[1], x[2])]; }
+    factoredRoe<S_>(out, wl, wr, A, radialPoint(v, D, x[0]), D, v.gamma, v.entropy);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) S[I(F_DPT+q, x[0], x[1], x[2])] = out[q];
+}
+
+// Face-product helpers of direction D at one box: metric values M, centred
+// transverse differences Dm/Df of metrics and scratch fields, and the mapped
+// products <det A F> (scalars) and <det(A) A F> (vectors) of a point flux.
+template <int D>
+struct FaceOps
+{
+    static constexpr int t1 = D == 0 ? 1 : 0, t2 = D == 2 ? 1 : 2;
+    const O4View& v;
+    const BoxDesc& b;
+    const BoxIdx& I;
+    const double* S;
+    const double* met;
+    __device__ FaceOps(const O4View& v_, const BoxDesc& b_, const BoxIdx& I_, const double* S_)
+        : v(v_), b(b_), I(I_), S(S_), met(v_.fmet[D]) {}
+    __device__ double M(int c, const int y[3]) const { return met[13*angIdx(v, b.p, y[1], y[2]) + c]; }
+    __device__ double Dm(int c, int t, const int y[3]) const
+    {
         int a[3] = {y[0], y[1], y[2]}, z[3] = {y[0], y[1], y[2]};
         a[t] += 1; z[t] -= 1;
-        return 0.5*M(c, a) - 0.5*M(c, z); };
-    auto Fv = [&](int f, const int y[3]) { return S[I(f, y[0], y[1], y[2])]; };
-    auto Df = [&](int f, int t, const int y[3]) {
+        return 0.5*M(c, a) - 0.5*M(c, z);
+    }
+    __device__ double Fv(int f, const int y[3]) const { return S[I(f, y[0], y[1], y[2])]; }
+    __device__ double Df(int f, int t, const int y[3]) const
+    {
         int a[3] = {y[0], y[1], y[2]}, z[3] = {y[0], y[1], y[2]};
         a[t] += 1; z[t] -= 1;
-        return 0.5*Fv(f, a) - 0.5*Fv(f, z); };
-    // <det A F> and <det(A) A F> of the physical flux of one side at point y
-    auto mapped = [&](int base, const int y[3], double out[8]) {
+        return 0.5*Fv(f, a) - 0.5*Fv(f, z);
+    }
+    // <det A F> and <det(A) A F> of the point flux in fields base.. at y
+    __device__ void mapped(int base, const int y[3], double out[8]) const
+    {
 #pragma unroll
         for (int r = 0; r < 3; ++r)
         {
+#pragma unroll
             for (int grp = 0; grp < 2; ++grp)
             {
                 const int c0 = grp == 0 ? 1 : 5;
@@ -619,6 +659,7 @@ __global__ void __launch_bounds__(kB) k_o4_flux(O4View v, double* __restrict__ S
                 out[c0+r] = val;
             }
         }
+#pragma unroll
         for (int s = 0; s < 2; ++s)
         {
             const int c = s == 0 ? 0 : 4;
@@ -628,56 +669,88 @@ __global__ void __launch_bounds__(kB) k_o4_flux(O4View v, double* __restrict__ S
             val += M(0, y)*Fv(base+c, y);
             out[c] = val;
         }
-    };
+    }
+};
+
+// 6a. Mapped products of both sides' physical point fluxes into the face
+// state slots (dead once k_o4_roe has run): faces [lo_d, hi_d+1] x [lo, hi],
+// grown by one in the radial index when it is transverse (d = 1, 2), where
+// the radial-area tensor product needs them.
+template <int D>
+__global__ void __launch_bounds__(kB) k_o4_map(O4View v, double* __restrict__ S)
+{
+    const BoxDesc& b = v.boxes[blockIdx.y];
+    int L[3], H[3], x[3];
+    for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]; H[d] = b.hi[d]; }
+    H[D] = b.hi[D]+1;
+    if (D != 0) { L[0] -= 1; H[0] += 1; }
+    if (!regionPoint(b, L, H, (long long)blockIdx.x*blockDim.x + threadIdx.x, x)) return;
+    const BoxIdx I(b);
+    const FaceOps<D> op(v, b, I, S);
+    double out[8];
+    op.mapped(F_PHL, x, out);
+#pragma unroll
+    for (int c = 0; c < 8; ++c) S[I(F_MPL+c, x[0], x[1], x[2])] = out[c];
+    op.mapped(F_PHR, x, out);
+#pragma unroll
+    for (int c = 0; c < 8; ++c) S[I(F_MPR+c, x[0], x[1], x[2])] = out[c];
+}
+
+// 6b. Mapped face flux on the flux faces [lo_d, hi_d+1] x [lo, hi]: the
+// radial-area tensor product of the mapped products, the pressure force
+// through the cofactor row, the averaged Roe dissipation, and
+// F = (F_L + F_R - D)/2.
+template <int D>
+__global__ void __launch_bounds__(kB) k_o4_flux(O4View v, double* __restrict__ S)
+{
+    const BoxDesc& b = v.boxes[blockIdx.y];
+    int L[3], H[3], x[3];
+    for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]; H[d] = b.hi[d]; }
+    H[D] = b.hi[D]+1;
+    if (!regionPoint(b, L, H, (long long)blockIdx.x*blockDim.x + threadIdx.x, x)) return;
+    const BoxIdx I(b);
+    constexpr int t1 = D == 0 ? 1 : 0, t2 = D == 2 ? 1 : 2;
+    const FaceOps<D> op(v, b, I, S);
+    auto M = [&](int c, const int y[3]) { return op.M(c, y); };
+    auto Dm = [&](int c, int t, const int y[3]) { return op.Dm(c, t, y); };
+    auto Fv = [&](int f, const int y[3]) { return op.Fv(f, y); };
+    auto Df = [&](int f, int t, const int y[3]) { return op.Df(f, t, y); };
     const double adjX = areaOf(v, D, x[0]);
-    // radial-area differences in the transverse directions (only radial varies)
-    auto adjD = [&](int t) {
-        if (t != 0) return 0.5*adjX - 0.5*adjX;
-        return 0.5*areaOf(v, D, x[0]+1) - 0.5*areaOf(v, D, x[0]-1); };
+    // The radial area varies only with the radial index, so its transverse
+    // difference is exactly zero unless the transverse direction is radial:
+    // t2 is never radial and t1 is radial for D = 1, 2. The zero terms of the
+    // product rule add +-0 and are left out (bit-identical for finite data).
+    const double adjD1 = D == 0 ? 0.0 : 0.5*areaOf(v, D, x[0]+1) - 0.5*areaOf(v, D, x[0]-1);
+    // face-average total pressure at the transverse neighbours
+    double p0bD[2], radPD[2];
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+    {
+        const int tt = t == 0 ? t1 : t2;
+        int y[3] = {x[0], x[1], x[2]}, z[3] = {x[0], x[1], x[2]};
+        y[tt] += 1; z[tt] -= 1;
+        const double pa = Fv(F_P0B, y), pz = Fv(F_P0B, z);
+        p0bD[t] = 0.5*pa - 0.5*pz;
+        radPD[t] = 0.5*(pa*areaOf(v, D, y[0])) - 0.5*(pz*areaOf(v, D, z[0]));
+    }
     double Fsum[8];
     for (int side = 0; side < 2; ++side)
     {
-        const int base = side == 0 ? F_PHL : F_PHR;
-        double mx[8], ma[2][8], mb[2][8];
-        mapped(base, x, mx);
-        for (int t = 0; t < 2; ++t)
-        {
-            const int tt = t == 0 ? t1 : t2;
-            int y[3] = {x[0], x[1], x[2]}, z[3] = {x[0], x[1], x[2]};
-            y[tt] += 1; z[tt] -= 1;
-            mapped(base, y, ma[t]);
-            mapped(base, z, mb[t]);
-        }
+        const int base = side == 0 ? F_MPL : F_MPR;
+        const int y[3] = {x[0]+1, x[1], x[2]}, z[3] = {x[0]-1, x[1], x[2]};
         double Fs[8];
 #pragma unroll
         for (int c = 0; c < 8; ++c)
         {
             double val = 0;
-            val += adjD(t1)*(0.5*ma[0][c] - 0.5*mb[0][c])*(1.0/12.0);
-            val += adjD(t2)*(0.5*ma[1][c] - 0.5*mb[1][c])*(1.0/12.0);
-            val += adjX*mx[c];
+            if (D != 0) val += adjD1*(0.5*Fv(base+c, y) - 0.5*Fv(base+c, z))*(1.0/12.0);
+            val += adjX*Fv(base+c, x);
             Fs[c] = val;
         }
         // pressure force: p0 (face average) with p0Bar = pBar + meBar
         const int fp = side == 0 ? F_P0L : F_P0R;
-        double fdum[8], meb, pb;
-        barFlux<D>(v, b, I, S, x[0], x[1], x[2], fdum, meb, pb);
-        double p0bD[2], radPD[2];
-        for (int t = 0; t < 2; ++t)
-        {
-            const int tt = t == 0 ? t1 : t2;
-            int y[3] = {x[0], x[1], x[2]}, z[3] = {x[0], x[1], x[2]};
-            y[tt] += 1; z[tt] -= 1;
-            double mya, pya, myb, pyb;
-            barFlux<D>(v, b, I, S, y[0], y[1], y[2], fdum, mya, pya);
-            barFlux<D>(v, b, I, S, z[0], z[1], z[2], fdum, myb, pyb);
-            const double pa = pya + mya, pz = pyb + myb;
-            p0bD[t] = 0.5*pa - 0.5*pz;
-            radPD[t] = 0.5*(pa*areaOf(v, D, y[0])) - 0.5*(pz*areaOf(v, D, z[0]));
-        }
         double tmp3 = 0;
-        tmp3 += adjD(t1)*p0bD[0]*(1.0/12.0);
-        tmp3 += adjD(t2)*p0bD[1]*(1.0/12.0);
+        if (D != 0) tmp3 += adjD1*p0bD[0]*(1.0/12.0);
         tmp3 += adjX*Fv(fp, x);
         for (int r = 0; r < 3; ++r)
         {
@@ -889,6 +962,17 @@ void buildBoxop(hc_boxop& B, const hc_box* boxes, int nbox)
     B.nbox = nbox;
 }
 
+template <int D, typename S_>
+void directionPass(const O4View& v, dim3 grid, hipStream_t s, double* scratch)
+{
+    hipLaunchKernelGGL(k_o4_faces<D>, grid, dim3(kB), 0, s, v, scratch);
+    hipLaunchKernelGGL(k_o4_bar<D>, grid, dim3(kB), 0, s, v, scratch);
+    hipLaunchKernelGGL(k_o4_phys<D>, grid, dim3(kB), 0, s, v, scratch);
+    hipLaunchKernelGGL((k_o4_roe<D, S_>), grid, dim3(kB), 0, s, v, scratch);
+    hipLaunchKernelGGL(k_o4_map<D>, grid, dim3(kB), 0, s, v, scratch);
+    hipLaunchKernelGGL(k_o4_flux<D>, grid, dim3(kB), 0, s, v, scratch);
+}
+
 template <typename S_>
 void applyBoxop(hc_boxop& B, const double* U, double* R, double* const F[3], hipStream_t s,
                 int overwrite = 0, double* Uacc = nullptr, double b_dt = 0.0)
@@ -898,15 +982,9 @@ void applyBoxop(hc_boxop& B, const double* U, double* R, double* const F[3], hip
     hipLaunchKernelGGL(k_o4_wbar, grid, dim3(kB), 0, s, v, U, B.scratch);
     hipLaunchKernelGGL(k_o4_w, grid, dim3(kB), 0, s, v, U, B.scratch);
     hipLaunchKernelGGL(k_o4_eta, grid, dim3(kB), 0, s, v, B.scratch);
-    hipLaunchKernelGGL(k_o4_faces<0>, grid, dim3(kB), 0, s, v, B.scratch);
-    hipLaunchKernelGGL((k_o4_phys<0, S_>), grid, dim3(kB), 0, s, v, B.scratch);
-    hipLaunchKernelGGL(k_o4_flux<0>, grid, dim3(kB), 0, s, v, B.scratch);
-    hipLaunchKernelGGL(k_o4_faces<1>, grid, dim3(kB), 0, s, v, B.scratch);
-    hipLaunchKernelGGL((k_o4_phys<1, S_>), grid, dim3(kB), 0, s, v, B.scratch);
-    hipLaunchKernelGGL(k_o4_flux<1>, grid, dim3(kB), 0, s, v, B.scratch);
-    hipLaunchKernelGGL(k_o4_faces<2>, grid, dim3(kB), 0, s, v, B.scratch);
-    hipLaunchKernelGGL((k_o4_phys<2, S_>), grid, dim3(kB), 0, s, v, B.scratch);
-    hipLaunchKernelGGL(k_o4_flux<2>, grid, dim3(kB), 0, s, v, B.scratch);
+    directionPass<0, S_>(v, grid, s, B.scratch);
+    directionPass<1, S_>(v, grid, s, B.scratch);
+    directionPass<2, S_>(v, grid, s, B.scratch);
     if (v.nseams > 0)
         hipLaunchKernelGGL(k_o4_seams, dim3((v.nseams + kB - 1)/kB), dim3(kB), 0, s, v, B.scratch);
     hipLaunchKernelGGL(k_o4_div, grid, dim3(kB), 0, s, v, B.scratch, R, overwrite, Uacc, b_dt);
