@@ -43,6 +43,9 @@ class HcOptions(ctypes.Structure):
         ("entropy_fix", ctypes.c_double),
         ("gamma", ctypes.c_double),
         ("precision", ctypes.c_int),
+        ("spatial_operator", ctypes.c_int),
+        ("berta_eta_d", ctypes.c_double),
+        ("berta_fallback", ctypes.c_int),
     ]
 
 
@@ -135,11 +138,14 @@ class Options:
     riemann_solver: int = 1
     entropy_fix: float = 0.3
     gamma: float = 5.0 / 3.0
-    precision: int = 0  # 0 fp64, 1 mixed (fp32 slopes and Riemann dissipation)
+    precision: int = 0  # 0 fp64, 1 mixed (fp32 slopes / reconstruction weights and Riemann dissipation)
+    spatial_operator: int = 1  # fourth order: 1 standard, 2 Berta WENOZ
+    berta_eta_d: float = 0.02  # Berta Jameson detector threshold
+    berta_fallback: int = 0  # Berta fallback: 0 WENO3, 1 PLM
 
     def c(self) -> HcOptions:
         return HcOptions(self.spatial_order, self.limiter, self.riemann_solver, self.entropy_fix, self.gamma,
-                         self.precision)
+                         self.precision, self.spatial_operator, self.berta_eta_d, self.berta_fallback)
 
 
 def init_state(shell: Shell, options: Options, config: int = 0, seed: int = 20261203):

@@ -18,6 +18,20 @@ inline int fail(const std::string& msg)
     return -1;
 }
 
+// Fourth-order operator selection: returns an error text or nullptr.
+inline const char* checkOperatorOptions(const hc_options& o)
+{
+    if (o.spatial_operator != 0 && o.spatial_operator != 1 && o.spatial_operator != 2)
+        return "spatial_operator must be 1 (standard) or 2 (Berta WENOZ)";
+    if (o.spatial_operator == 2)
+    {
+        if (o.spatial_order != 4) return "the Berta operator is fourth order (spatial_order 4)";
+        if (!(o.berta_eta_d > 0.0)) return "berta_eta_d must be positive";
+        if (o.berta_fallback != 0 && o.berta_fallback != 1) return "berta_fallback must be 0 (WENO3) or 1 (PLM)";
+    }
+    return nullptr;
+}
+
 // Fourth-order level (defined with the fourth-order kernels): stage
 // preparation JU -> USph (ghost 5) and the box-table RHS over the level.
 struct Level4;

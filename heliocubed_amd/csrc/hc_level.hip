@@ -737,11 +737,22 @@ struct WindState
     bool hemispheric;
     int nmodes = 0;
     double kvec[16][3], evec[16][3], phase[16], amp = 0;
+    bool cone = false;             // low-beta cone: p = 1e-4 |B|^2/2 inside it
+    double coneAxis[3] = {0, 0, 1}, coneCos = 1.0;
     // Cartesian primitive state at position X.
     void at(const double X[3], double w[8]) const
     {
         base(X, w);
-        if (nmodes == 0) return;
+        if (nmodes != 0) modes(X, w);
+        if (cone)
+        {
+            const double r = std::sqrt(X[0]*X[0]+X[1]*X[1]+X[2]*X[2]);
+            const double c = (X[0]*coneAxis[0]+X[1]*coneAxis[1]+X[2]*coneAxis[2])/r;
+            if (c > coneCos) w[4] = 1.0e-4*0.5*(w[5]*w[5]+w[6]*w[6]+w[7]*w[7]);
+        }
+    }
+    void modes(const double X[3], double w[8]) const
+    {
         const double r = std::sqrt(X[0]*X[0]+X[1]*X[1]+X[2]*X[2]);
         const double rh[3] = {X[0]/r, X[1]/r, X[2]/r};
         double dv[3] = {0, 0, 0};
@@ -766,6 +777,48 @@ struct WindState
         for (int a = 0; a < 3; ++a) { w[1+a] = X[a]/r; w[5+a] = sign*rho*X[a]/r; }
     }
 };
+
+// Seeded test states of the fourth-order paths: config 0 radial wind with a
+// Gaussian pressure pulse and hemispheric B_r, 1 monopole B_r, 2 monopole
+// without the pulse, 3 = 0 plus Alfvenic modes (|k| <= 8, amplitude 0.05) and
+// (applied by the callers) lognormal density noise, 4 = 3 plus a low-beta cone
+// (p = 1e-4 |B|^2/2 within 15 degrees of a seeded axis).
+WindState makeWind(const hc_shell& shell, const hc_options& options, int config, unsigned long long seed)
+{
+    SplitMix rng{seed};
+    WindState ws;
+    ws.r0 = shell.inner_radius; ws.r1 = shell.outer_radius; ws.gamma = options.gamma;
+    const double cz = 2.0*rng.uniform()-1.0, phi = 2.0*hc::kPi*rng.uniform();
+    const double rc = 0.5*(ws.r0+ws.r1), sz = std::sqrt(std::max(0.0, 1.0-cz*cz));
+    ws.center[0] = rc*sz*std::cos(phi); ws.center[1] = rc*sz*std::sin(phi); ws.center[2] = rc*cz;
+    ws.width = 0.1*(ws.r1-ws.r0);
+    ws.pulse = config == 2 ? 0.0 : 0.2;
+    ws.hemispheric = config == 0 || config >= 3;
+    if (config >= 3)
+    {
+        ws.nmodes = 16; ws.amp = 0.05/std::sqrt(16.0);
+        for (int m = 0; m < 16; ++m)
+        {
+            double kk[3], n2 = 0;
+            for (int a = 0; a < 3; ++a) { kk[a] = 2.0*rng.uniform()-1.0; n2 += kk[a]*kk[a]; }
+            const double kmag = 1.0 + 7.0*rng.uniform();
+            for (int a = 0; a < 3; ++a) ws.kvec[m][a] = kk[a]/std::sqrt(n2)*kmag;
+            double e[3], e2 = 0;
+            for (int a = 0; a < 3; ++a) { e[a] = 2.0*rng.uniform()-1.0; e2 += e[a]*e[a]; }
+            for (int a = 0; a < 3; ++a) ws.evec[m][a] = e[a]/std::sqrt(e2);
+            ws.phase[m] = 2.0*hc::kPi*rng.uniform();
+        }
+    }
+    if (config == 4)
+    {
+        const double az = 2.0*rng.uniform()-1.0, aphi = 2.0*hc::kPi*rng.uniform();
+        const double as = std::sqrt(std::max(0.0, 1.0-az*az));
+        ws.cone = true;
+        ws.coneAxis[0] = as*std::cos(aphi); ws.coneAxis[1] = as*std::sin(aphi); ws.coneAxis[2] = az;
+        ws.coneCos = std::cos(15.0*hc::kPi/180.0);
+    }
+    return ws;
+}
 } // namespace
 
 extern "C" {
@@ -789,6 +842,7 @@ int hc_level_create(const hc_shell* shell, const hc_options* options, hc_level**
         return fail("hc_level_create: the fourth-order level needs n_ang >= 16 (five ghost layers must stay "
                     "inside the panel's horizon) and n_rad >= 4");
     if (options->precision != 0 && options->precision != 1) return fail("hc_level_create: precision must be 0 (fp64) or 1 (mixed)");
+    if (const char* e = hc::checkOperatorOptions(*options)) return fail(std::string("hc_level_create: ") + e);
     try
     {
         std::unique_ptr<hc_level> L(new hc_level());
@@ -949,35 +1003,11 @@ int hc_init_native_boxes(const hc_shell* shell, const hc_options* options, int c
                          const hc_box* boxes, int nbox, int quad, double* USph)
 {
     if (!shell || !options || !boxes || !USph || nbox <= 0) return fail("hc_init_native_boxes: bad argument");
-    if (config < 0 || config > 3) return fail("hc_init_native_boxes: config must be 0..3");
+    if (config < 0 || config > 4) return fail("hc_init_native_boxes: config must be 0..4");
     if (quad < 1 || quad > 8) return fail("hc_init_native_boxes: quad must be 1..8");
     const int nr = shell->n_rad, na = shell->n_ang, g = 5;
     const hc::ShellDims dims{nr, na, g, shell->inner_radius, shell->outer_radius, shell->radial_stretch};
-    SplitMix rng{seed};
-    WindState ws;
-    ws.r0 = shell->inner_radius; ws.r1 = shell->outer_radius; ws.gamma = options->gamma;
-    const double cz = 2.0*rng.uniform()-1.0, phi = 2.0*hc::kPi*rng.uniform();
-    const double rc = 0.5*(ws.r0+ws.r1), sz = std::sqrt(std::max(0.0, 1.0-cz*cz));
-    ws.center[0] = rc*sz*std::cos(phi); ws.center[1] = rc*sz*std::sin(phi); ws.center[2] = rc*cz;
-    ws.width = 0.1*(ws.r1-ws.r0);
-    ws.pulse = config == 2 ? 0.0 : 0.2;
-    ws.hemispheric = config == 0 || config == 3;
-    if (config == 3)
-    {
-        // Alfvenic modes with angular wavenumber |k| <= 8, amplitude 0.05.
-        ws.nmodes = 16; ws.amp = 0.05/std::sqrt(16.0);
-        for (int m = 0; m < 16; ++m)
-        {
-            double kk[3], n2 = 0;
-            for (int a = 0; a < 3; ++a) { kk[a] = 2.0*rng.uniform()-1.0; n2 += kk[a]*kk[a]; }
-            const double kmag = 1.0 + 7.0*rng.uniform();
-            for (int a = 0; a < 3; ++a) ws.kvec[m][a] = kk[a]/std::sqrt(n2)*kmag;
-            double e[3], e2 = 0;
-            for (int a = 0; a < 3; ++a) { e[a] = 2.0*rng.uniform()-1.0; e2 += e[a]*e[a]; }
-            for (int a = 0; a < 3; ++a) ws.evec[m][a] = e[a]/std::sqrt(e2);
-            ws.phase[m] = 2.0*hc::kPi*rng.uniform();
-        }
-    }
+    const WindState ws = makeWind(*shell, *options, config, seed);
     std::vector<double> gx, gw;
     hc::gaussLegendre01(quad, gx, gw);
     const double h = 1.0/na, gamma = options->gamma;
@@ -993,7 +1023,7 @@ int hc_init_native_boxes(const hc_shell* shell, const hc_options* options, int c
             const int i = lo[0]+ii, j = lo[1]+jj, k = lo[2]+kk;
             double avg[8] = {0, 0, 0, 0, 0, 0, 0, 0};
             double noise = 1.0;
-            if (config == 3)
+            if (config >= 3)
             {
                 // lognormal density factor per cell, sigma 0.05, mean-preserving
                 SplitMix cr{seed ^ (0x9E3779B97F4A7C15ull*((((unsigned long long)b.block*4096+(k+8))*4096+(j+8))*65536+(i+8)))};
@@ -1043,7 +1073,7 @@ int hc_init_level4(const hc_shell* shell, const hc_options* options, int config,
                    int quad, double* JU, double* inner)
 {
     if (!shell || !options || !JU || !inner) return fail("hc_init_level4: null argument");
-    if (config < 0 || config > 3) return fail("hc_init_level4: config must be 0..3");
+    if (config < 0 || config > 4) return fail("hc_init_level4: config must be 0..4");
     if (quad < 1 || quad > 8) return fail("hc_init_level4: quad must be 1..8");
     const int nr = shell->n_rad, na = shell->n_ang, g = 5;
     // inner radial ghosts: native averages of the analytic state
@@ -1062,30 +1092,7 @@ int hc_init_level4(const hc_shell* shell, const hc_options* options, int config,
                 tmp[(size_t)p*8*e0*e1*e1 + (((size_t)c*e1+kk)*e1+jj)*e0 + (ig+g)];
     // <J U>: J-weighted averages with Cartesian vector components
     const hc::ShellDims dims{nr, na, g, shell->inner_radius, shell->outer_radius, shell->radial_stretch};
-    SplitMix rng{seed};
-    WindState ws;
-    ws.r0 = shell->inner_radius; ws.r1 = shell->outer_radius; ws.gamma = options->gamma;
-    const double cz = 2.0*rng.uniform()-1.0, phi = 2.0*hc::kPi*rng.uniform();
-    const double rcn = 0.5*(ws.r0+ws.r1), sz = std::sqrt(std::max(0.0, 1.0-cz*cz));
-    ws.center[0] = rcn*sz*std::cos(phi); ws.center[1] = rcn*sz*std::sin(phi); ws.center[2] = rcn*cz;
-    ws.width = 0.1*(ws.r1-ws.r0);
-    ws.pulse = config == 2 ? 0.0 : 0.2;
-    ws.hemispheric = config == 0 || config == 3;
-    if (config == 3)
-    {
-        ws.nmodes = 16; ws.amp = 0.05/std::sqrt(16.0);
-        for (int m = 0; m < 16; ++m)
-        {
-            double kk[3], n2 = 0;
-            for (int a = 0; a < 3; ++a) { kk[a] = 2.0*rng.uniform()-1.0; n2 += kk[a]*kk[a]; }
-            const double kmag = 1.0 + 7.0*rng.uniform();
-            for (int a = 0; a < 3; ++a) ws.kvec[m][a] = kk[a]/std::sqrt(n2)*kmag;
-            double e[3], e2 = 0;
-            for (int a = 0; a < 3; ++a) { e[a] = 2.0*rng.uniform()-1.0; e2 += e[a]*e[a]; }
-            for (int a = 0; a < 3; ++a) ws.evec[m][a] = e[a]/std::sqrt(e2);
-            ws.phase[m] = 2.0*hc::kPi*rng.uniform();
-        }
-    }
+    const WindState ws = makeWind(*shell, *options, config, seed);
     std::vector<double> gx, gw;
     hc::gaussLegendre01(quad, gx, gw);
     const double h = 1.0/na, gamma = options->gamma;
@@ -1108,7 +1115,7 @@ int hc_init_level4(const hc_shell* shell, const hc_options* options, int config,
         for (int i = 0; i < nr; ++i)
         {
             double noise = 1.0;
-            if (config == 3)
+            if (config >= 3)
             {
                 SplitMix cr{seed ^ (0x9E3779B97F4A7C15ull*((((unsigned long long)p*4096+(k+8))*4096+(j+8))*65536+(i+8)))};
                 const double u1 = std::max(cr.uniform(), 1e-300), u2 = cr.uniform();

@@ -67,6 +67,8 @@ struct O4View
 {
     int nr, na, naE, limiter;
     double gamma, entropy;
+    int op, fallback;        // 1 standard operator, 2 Berta WENOZ; Berta fallback 0 WENO3, 1 PLM
+    double etaD;             // Berta Jameson detector threshold
     const BoxDesc* boxes;
     const double* cellA;     // [6][naE][naE][9]
     const double* faceA[3];  // [6][naE][naE][9]
@@ -902,6 +904,9 @@ void buildBoxop(hc_boxop& B, const hc_box* boxes, int nbox)
     O4View& v = B.view;
     v.nr = nr; v.na = na; v.naE = naE; v.limiter = B.opt.limiter;
     v.gamma = B.opt.gamma; v.entropy = B.opt.entropy_fix;
+    v.op = B.opt.spatial_operator == 2 ? 2 : 1;
+    v.fallback = B.opt.berta_fallback;
+    v.etaD = B.opt.berta_eta_d;
     std::vector<double> cellA(9ull*6*naE*naE), faceA[3], fmet[3];
     for (int d = 0; d < 3; ++d) { faceA[d].resize(9ull*6*naE*naE); fmet[d].resize(13ull*6*naE*naE); }
     const double h = 1.0/na;
@@ -962,6 +967,8 @@ void buildBoxop(hc_boxop& B, const hc_box* boxes, int nbox)
     B.nbox = nbox;
 }
 
+#include "hc_berta.hpp"
+
 template <int D, typename S_>
 void directionPass(const O4View& v, dim3 grid, hipStream_t s, double* scratch)
 {
@@ -979,12 +986,16 @@ void applyBoxop(hc_boxop& B, const double* U, double* R, double* const F[3], hip
 {
     const O4View& v = B.view;
     const dim3 grid((B.maxRegion + kB - 1)/kB, B.nbox);
-    hipLaunchKernelGGL(k_o4_wbar, grid, dim3(kB), 0, s, v, U, B.scratch);
-    hipLaunchKernelGGL(k_o4_w, grid, dim3(kB), 0, s, v, U, B.scratch);
-    hipLaunchKernelGGL(k_o4_eta, grid, dim3(kB), 0, s, v, B.scratch);
-    directionPass<0, S_>(v, grid, s, B.scratch);
-    directionPass<1, S_>(v, grid, s, B.scratch);
-    directionPass<2, S_>(v, grid, s, B.scratch);
+    if (v.op == 2) bertaFluxes<S_>(v, grid, s, U, B.scratch);
+    else
+    {
+        hipLaunchKernelGGL(k_o4_wbar, grid, dim3(kB), 0, s, v, U, B.scratch);
+        hipLaunchKernelGGL(k_o4_w, grid, dim3(kB), 0, s, v, U, B.scratch);
+        hipLaunchKernelGGL(k_o4_eta, grid, dim3(kB), 0, s, v, B.scratch);
+        directionPass<0, S_>(v, grid, s, B.scratch);
+        directionPass<1, S_>(v, grid, s, B.scratch);
+        directionPass<2, S_>(v, grid, s, B.scratch);
+    }
     if (v.nseams > 0)
         hipLaunchKernelGGL(k_o4_seams, dim3((v.nseams + kB - 1)/kB), dim3(kB), 0, s, v, B.scratch);
     hipLaunchKernelGGL(k_o4_div, grid, dim3(kB), 0, s, v, B.scratch, R, overwrite, Uacc, b_dt);
@@ -1410,6 +1421,7 @@ int hc_boxop_create(const hc_shell* shell, const hc_options* options, const hc_b
     if (options->riemann_solver != 2) return hc::fail("hc_boxop_create: the fourth-order operator implements riemann_solver 2 (Roe)");
     if (options->limiter != 0 && options->limiter != 1) return hc::fail("hc_boxop_create: limiter must be 0 or 1");
     if (options->precision != 0 && options->precision != 1) return hc::fail("hc_boxop_create: precision must be 0 or 1");
+    if (const char* e = hc::checkOperatorOptions(*options)) return hc::fail(std::string("hc_boxop_create: ") + e);
     for (int n = 0; n < nbox; ++n)
     {
         const hc_box& b = boxes[n];

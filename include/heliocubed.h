@@ -45,16 +45,21 @@ typedef struct hc_shell
 } hc_shell;
 
 /* Operator options: the hot-path subset of ParseInputs
- * (src/Inputs_Parsing.H:71-123). spatial_order 2 selects the PLM operator
+ * (src/Inputs_Parsing.H:71-123, names kept: spatial_operator, Berta_eta_d,
+ * Berta_fallback_type). spatial_order 2 selects the PLM operator
  * (src/BoxOp_SecondOrder.H:155-370; limiter 0 none, 1 or 2 minmod, 3 MC;
  * riemann_solver 1 Rusanov, 2 Roe 8-wave); spatial_order 4 the standard
  * fourth-order operator (src/BoxOp_EulerCubedSphereImplem.H:97-402; limiter
  * 0 none or 1 Limiters_4O with flattening; riemann_solver 2) with
- * state_conversion 1 (src/MHD_StateConversion.H:153-316). A fourth-order
+ * state_conversion 1 (src/MHD_StateConversion.H:153-316); spatial_operator 2
+ * replaces the standard operator by the Berta WENOZ operator
+ * (src/BoxOp_BertaCubedSphereImplem.H:1737-2076: Jameson detector with
+ * threshold berta_eta_d, WENOZ with WENO3 or PLM fallback, Roe). A fourth-order
  * level uses ghost 5; its stage array W holds native conserved averages and
  * its inner ghosts are native conserved averages. precision 0 evaluates everything
- * in FP64; precision 1 (mixed) limits slopes and evaluates the Riemann
- * dissipation (Rusanov speed, Roe eigen-system) in FP32 while states, jumps,
+ * in FP64; precision 1 (mixed) limits slopes (O2), evaluates the Berta
+ * reconstruction weights and the Riemann dissipation (Rusanov speed, Roe
+ * eigen-system) in FP32 while states, jumps,
  * metric terms, central fluxes, pressure recovery, divergence and the RK
  * accumulate stay FP64. */
 typedef struct hc_options
@@ -65,6 +70,9 @@ typedef struct hc_options
     double entropy_fix;
     double gamma;
     int precision;
+    int spatial_operator;   /* 0 or 1: standard operator, 2: Berta WENOZ (fourth order) */
+    double berta_eta_d;     /* Berta Jameson detector threshold (reference default 0.02) */
+    int berta_fallback;     /* Berta fallback reconstruction: 0 WENO3, 1 PLM */
 } hc_options;
 
 /* A box of the shell: panel (block) 0..5 and inclusive cell range lo..hi in
@@ -157,10 +165,12 @@ int hc_init_state(const hc_shell* shell, const hc_options* options, int config,
                   unsigned long long seed, double* JU_host, double* inner_host);
 
 /* Host-side native conserved cell averages for the fourth-order box
- * operator (test problem data): the states of hc_init_state (config 0-2) and
+ * operator (test problem data): the states of hc_init_state (config 0-2),
  * config 3 = config 0 plus Alfvenic velocity/field perturbations (amplitude
  * 0.05, angular wavenumbers <= 8, dB = -dv sqrt(4 pi rho)) and a lognormal
- * density factor exp(0.05 g - 0.00125) per cell, averaged over each ghosted
+ * density factor exp(0.05 g - 0.00125) per cell, and config 4 = config 3 plus
+ * a low-beta cone (p = 1e-4 |B|^2/2 within 15 degrees of a seeded axis),
+ * averaged over each ghosted
  * cell by a quad^3 Gauss-Legendre rule in xi. Writes the hc_boxop USph layout
  * into a HOST array. */
 int hc_init_native_boxes(const hc_shell* shell, const hc_options* options, int config, unsigned long long seed,

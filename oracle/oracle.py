@@ -28,6 +28,9 @@ class OrcCfg(ctypes.Structure):
         ("limiter", ctypes.c_int),
         ("riemann", ctypes.c_int),
         ("entropy_fix", ctypes.c_double),
+        ("spatial_operator", ctypes.c_int),
+        ("berta_eta_d", ctypes.c_double),
+        ("berta_fallback", ctypes.c_int),
     ]
 
 
@@ -70,6 +73,10 @@ def lib() -> ctypes.CDLL:
             "orc4_rhs": (I, [P, P, P]),
             "orc4_rk4": (I, [P, P, P, D, I]),
             "orc4_ghost_donors": (I, [cfgp, I, I, I, P, P, P, P]),
+            "orc_wenoz": (None, [P, P]),
+            "orc_berta_theta": (None, [cfgp, I, P, P, P, P]),
+            "orc_weno3": (None, [P, P, D]),
+            "orc_plm_berta": (None, [P, P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -181,6 +188,33 @@ def face_flux(left, right, normal, area, gamma=5.0 / 3.0, entropy_fix=0.3, roe=F
     return out
 
 
+def berta_theta(n_rad, n_ang, p, lo, hi, U, eta_d=0.02, gamma=5.0 / 3.0, r0=21.5, r1=215.0):
+    """Berta Jameson flag on the box grown by 5 ([n2+10][n1+10][n0+10])."""
+    cfg = OrcCfg(n_rad, n_ang, 5, r0, r1, 0.0, gamma, 1, 2, 0.3, 2, eta_d, 0)
+    n = [hi[d] - lo[d] + 11 for d in range(3)]
+    th = np.zeros((n[2], n[1], n[0]))
+    lo_a = np.array(lo, dtype=np.int32)
+    hi_a = np.array(hi, dtype=np.int32)
+    lib().orc_berta_theta(ctypes.byref(cfg), p, lo_a.ctypes.data, hi_a.ctypes.data,
+                          _p(np.ascontiguousarray(U, dtype=np.float64)), _p(th))
+    return th
+
+
+def berta_face(values, method="wenoz", dx2=0.0):
+    """(left, right) point values of one component at the face between the
+    third and fourth of six cells (WENOZ) or the second and third of four
+    (WENO3, PLM)."""
+    out = np.zeros(2)
+    v = np.ascontiguousarray(values, dtype=np.float64)
+    if method == "wenoz":
+        lib().orc_wenoz(_p(out), _p(v))
+    elif method == "weno3":
+        lib().orc_weno3(_p(out), _p(v), dx2)
+    else:
+        lib().orc_plm_berta(_p(out), _p(v))
+    return out[0], out[1]
+
+
 def basis(p, alpha, beta):
     A = np.zeros(9)
     lib().orc_basis(p, alpha, beta, _p(A))
@@ -188,9 +222,12 @@ def basis(p, alpha, beta):
 
 
 def o4_rhs_boxes(n_rad, n_ang, boxes, USph, limiter=1, entropy_fix=0.3, gamma=5.0 / 3.0, r0=21.5, r1=215.0,
-                 with_fluxes=False):
-    """Fourth-order RHS (and optionally fluxes) of every box, concatenated like hc_boxop."""
-    cfg = OrcCfg(n_rad, n_ang, 5, r0, r1, 0.0, gamma, limiter, 2, entropy_fix)
+                 with_fluxes=False, spatial_operator=1, eta_d=0.02, fallback=0):
+    """Fourth-order RHS (and optionally fluxes) of every box, concatenated like hc_boxop.
+
+    spatial_operator 1 is the standard operator, 2 the Berta WENOZ operator
+    (Jameson detector threshold eta_d, fallback 0 WENO3 / 1 PLM)."""
+    cfg = OrcCfg(n_rad, n_ang, 5, r0, r1, 0.0, gamma, limiter, 2, entropy_fix, spatial_operator, eta_d, fallback)
     g = 5
     rhs_parts, flux_parts, off = [], [[], [], []], 0
     for blk, lo, hi in boxes:
@@ -214,10 +251,12 @@ def o4_rhs_boxes(n_rad, n_ang, boxes, USph, limiter=1, entropy_fix=0.3, gamma=5.
 
 
 class Oracle4:
-    """Fourth-order level (standard operator + Roe, state_conversion 1) on the CPU."""
+    """Fourth-order level (standard or Berta operator + Roe, state_conversion 1) on the CPU."""
 
-    def __init__(self, n_rad, n_ang, limiter=1, entropy_fix=0.3, gamma=5.0 / 3.0, r0=21.5, r1=215.0):
-        self.cfg = OrcCfg(n_rad, n_ang, 5, r0, r1, 0.0, gamma, limiter, 2, entropy_fix)
+    def __init__(self, n_rad, n_ang, limiter=1, entropy_fix=0.3, gamma=5.0 / 3.0, r0=21.5, r1=215.0,
+                 spatial_operator=1, eta_d=0.02, fallback=0):
+        self.cfg = OrcCfg(n_rad, n_ang, 5, r0, r1, 0.0, gamma, limiter, 2, entropy_fix, spatial_operator, eta_d,
+                          fallback)
         self.nr, self.na = n_rad, n_ang
         self._h = ctypes.c_void_p(lib().orc4_create(ctypes.byref(self.cfg)))
 

@@ -40,6 +40,15 @@ def test_create_rejects_bad_options_without_device():
     sh = hc.Shell(8, 2).c()
     op = hc.Options().c()
     assert hc.lib().hc_level_create(ctypes.byref(sh), ctypes.byref(op), ctypes.byref(h)) != 0
+    # Berta operator options: order 4 only, positive detector threshold, fallback 0/1
+    sh = hc.Shell(16, 16).c()
+    for bad in (dict(spatial_order=2, spatial_operator=2), dict(spatial_operator=3),
+                dict(spatial_order=4, riemann_solver=2, spatial_operator=2, berta_eta_d=0.0),
+                dict(spatial_order=4, riemann_solver=2, spatial_operator=2, berta_fallback=2)):
+        op = hc.Options(**bad).c()
+        rc = hc.lib().hc_level_create(ctypes.byref(sh), ctypes.byref(op), ctypes.byref(h))
+        assert rc != 0, bad
+        assert b"berta" in hc.lib().hc_last_error().lower() or b"operator" in hc.lib().hc_last_error()
 
 
 def test_init_state_is_seeded_and_physical():

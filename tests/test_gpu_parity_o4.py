@@ -11,21 +11,9 @@ import torch
 
 import heliocubed_amd as hc
 import oracle as orc
-from helpers import rel_linf_per_var
+from helpers import per_var as _per_var, rel_linf_per_var
 
 TOL = 1e-12
-
-
-def _per_var(flat, boxes, interior=True):
-    """Reshape concatenated box data (8 comps per box) to (8, cells)."""
-    out, off = [[] for _ in range(8)], 0
-    for _, lo, hi in boxes:
-        n = int(np.prod([hi[d] - lo[d] + 1 for d in range(3)]))
-        blk = flat[off:off + 8 * n].reshape(8, n)
-        for c in range(8):
-            out[c].append(blk[c])
-        off += 8 * n
-    return np.array([np.concatenate(o) for o in out])
 
 
 CASES = [
