@@ -14,7 +14,8 @@
 //     Roe repair with the cell-average donors (:1817-1844)
 //     point fluxes of both sides summed, theta-aware face averages (:1246-1266, :1904-1946)
 //     BertaPrimPointToMappedFlux (:1334-1534) with the separable-metric shortcut (:1-98)
-//     primToMappedRoeFlux with theta-aware dissipation average (src/BoxOp_RoeCubedSphere.H:29-145)
+//     primToMappedRoeFlux with theta-aware dissipation average (src/BoxOp_RoeCubedSphere.H:29-145),
+//     or BertaPrimPointToMappedRusanovFlux (:1541-1730) for riemann_solver 1
 //     F = (Sum - D)/2 (:2004-2026)
 // Max/min follow the reference's std::max / std::min ((a < b) comparisons).
 
@@ -265,7 +266,7 @@ __global__ void __launch_bounds__(kB) k_b_faces(O4View v, double* __restrict__ S
     }
     l[0] = refMax(l[0], hc::kRhoFloor); r[0] = refMax(r[0], hc::kRhoFloor);
     l[4] = refMax(l[4], 1.0e-12); r[4] = refMax(r[4], 1.0e-12);
-    if (!(usablePrimitive(l) && usablePrimitive(r)))
+    if (v.roe && !(usablePrimitive(l) && usablePrimitive(r)))
     {
         double cl[8], cr[8];
 #pragma unroll
@@ -292,6 +293,21 @@ __global__ void __launch_bounds__(kB) k_b_faces(O4View v, double* __restrict__ S
     }
     S[I(F_THF, x[0], x[1], x[2])] = thF;
     S[I(F_PTOT, x[0], x[1], x[2])] = (l[4] + meL) + (r[4] + meR);
+    if (!v.roe)
+    {
+        // Rusanov: pointwise conserved jump R - L times the radial point area
+        // and the speed of the point states (:1589-1613, :1653-1725)
+        const double rp = radialPoint(v, D, x[0]);
+        S[I(F_DPT, x[0], x[1], x[2])] = rp*r[0] - rp*l[0];
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+        {
+            S[I(F_DPT+1+a, x[0], x[1], x[2])] = rp*r[0]*r[1+a] - rp*l[0]*l[1+a];
+            S[I(F_DPT+5+a, x[0], x[1], x[2])] = rp*r[5+a] - rp*l[5+a];
+        }
+        S[I(F_DPT+4, x[0], x[1], x[2])] = rp*(keR + meR + r[4]/(v.gamma-1.0)) - rp*(keL + meL + l[4]/(v.gamma-1.0));
+        S[I(F_SPD, x[0], x[1], x[2])] = rusanovSpeedO4(l, r, A, v.gamma, D, true);
+    }
 }
 
 // 5. Theta-aware face averages of the summed point flux and total pressure
@@ -455,6 +471,16 @@ __global__ void __launch_bounds__(kB) k_b_flux(O4View v, double* __restrict__ S)
         Davg[c] = smooth ? Fv(F_DPT+c, x) + lap/24.0 : Fv(F_DPT+c, x);
     }
     double Fd[8];
+    if (!v.roe)
+    {
+        // Rusanov: the averaged jump mapped by the face metric averages, times the speed
+        double dAA[9];
+#pragma unroll
+        for (int a = 0; a < 9; ++a) dAA[a] = op.M(1+a, x);
+        rusanovMapped(Fd, Davg, op.M(0, x), dAA, Fv(F_SPD, x));
+    }
+    else
+    {
 #pragma unroll
     for (int s = 0; s < 2; ++s)
     {
@@ -481,6 +507,7 @@ __global__ void __launch_bounds__(kB) k_b_flux(O4View v, double* __restrict__ S)
             }
             Fd[c0+r] = val;
         }
+    }
 #pragma unroll
     for (int c = 0; c < 8; ++c)
         S[I(F_FLUX+8*D+c, x[0], x[1], x[2])] = 0.5*(Fsum[c]-Fd[c]);
@@ -491,7 +518,7 @@ void bertaDirectionPass(const O4View& v, dim3 grid, hipStream_t s, double* scrat
 {
     hipLaunchKernelGGL((k_b_faces<D, S_>), grid, dim3(kB), 0, s, v, scratch);
     hipLaunchKernelGGL(k_b_avg<D>, grid, dim3(kB), 0, s, v, scratch);
-    hipLaunchKernelGGL((k_o4_roe<D, S_>), grid, dim3(kB), 0, s, v, scratch);
+    if (v.roe) hipLaunchKernelGGL((k_o4_roe<D, S_>), grid, dim3(kB), 0, s, v, scratch);
     hipLaunchKernelGGL(k_b_map<D>, grid, dim3(kB), 0, s, v, scratch);
     hipLaunchKernelGGL(k_b_flux<D>, grid, dim3(kB), 0, s, v, scratch);
 }

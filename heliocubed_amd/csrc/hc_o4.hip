@@ -39,6 +39,9 @@ enum Field : int
     // mapped products of the two sides' point fluxes; reuse the face state
     // slots, which are dead once k_o4_roe has run
     F_MPL = F_PTL, F_MPR = F_PTR,
+    // Rusanov (riemann_solver 1): the speed of a face, and the conserved point
+    // values of both sides (the dissipation and face-average flux slots)
+    F_SPD = F_MER, F_QL = F_DPT, F_QR = F_FB,
     // face-average total pressure pBar + meBar of the faces of the direction in
     // progress; reuses the cell meBar slot, which is dead once k_o4_eta has run
     F_P0B = F_MEB
@@ -68,6 +71,7 @@ struct O4View
     int nr, na, naE, limiter;
     double gamma, entropy;
     int op, fallback;        // 1 standard operator, 2 Berta WENOZ; Berta fallback 0 WENO3, 1 PLM
+    int roe;                 // 1 Roe 8-wave dissipation, 0 Rusanov
     double etaD;             // Berta Jameson detector threshold
     const BoxDesc* boxes;
     const double* cellA;     // [6][naE][naE][9]
@@ -307,6 +311,74 @@ __device__ void factoredRoe(double out[8], const double wl[8], const double wr[8
     }
 }
 
+// Rusanov speed |w_d| + c_f |A e_d| of a face from its two states,
+// primToMappedRusanovFlux (src/BoxOp_EulerCubedSphereImplem.H:692-731); the
+// Berta variant (src/BoxOp_BertaCubedSphereImplem.H:1653-1725) clamps the
+// radicands at 0 and groups 4 pi rho. B_n is taken along the first column of
+// A, as the reference does.
+__device__ double rusanovSpeedO4(const double L[8], const double R[8], const double* Am, double gamma, int d,
+                                 bool berta)
+{
+    double rho = 0.5*(L[0]+R[0]);
+    const double w_d = 0.5*(L[1+d]+R[1+d]);
+    const double bs[3] = {0.5*(L[5]+R[5]), 0.5*(L[6]+R[6]), 0.5*(L[7]+R[7])};
+    double p = 0.5*(L[4]+R[4]);
+    if (p < 0.0) p = 0.0;
+    rho = rho < hc::kRhoFloor ? hc::kRhoFloor : rho;
+    double bc[3] = {0.0, 0.0, 0.0}, Bn = 0.0, Bm = 0.0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+    {
+#pragma unroll
+        for (int b = 0; b < 3; ++b) bc[a] += Am[3*a+b]*bs[b];
+        Bn += Am[3*a]*bc[a];
+        Bm += bc[a]*bc[a];
+    }
+    Bm = sqrt(Bm);
+    const double ce = sqrt(gamma*p/rho);
+    double af;
+    if (berta)
+    {
+        const double aa = ce*ce + Bm*Bm/(4.0*hc::kPi*rho);
+        const double bb = fabs(Bn)*ce/sqrt(hc::kPi*rho);
+        const double s1 = aa + bb, s2 = aa - bb;
+        af = 0.5*(sqrt(0.0 < s1 ? s1 : 0.0) + sqrt(0.0 < s2 ? s2 : 0.0));
+    }
+    else
+    {
+        const double a = ce*ce + Bm*Bm/4.0/hc::kPi/rho;
+        const double b = fabs(Bn)*ce/sqrt(hc::kPi*rho);
+        af = 0.5*(sqrt(a + b) + sqrt(a - b));
+    }
+    const double Ad = sqrt(Am[d]*Am[d] + Am[3+d]*Am[3+d] + Am[6+d]*Am[6+d]);
+    return fabs(w_d) + af*Ad;
+}
+
+// Mapped Rusanov dissipation of an averaged conserved jump: det(A) A times the
+// vector parts (plain matrix-vector product), det A times the scalars, times
+// the speed.
+__device__ __forceinline__ void rusanovMapped(double out[8], const double diff[8], double detA,
+                                              const double* detAA, double speed)
+{
+    out[0] = detA*diff[0];
+    out[4] = detA*diff[4];
+#pragma unroll
+    for (int g = 0; g < 2; ++g)
+    {
+        const int c0 = g == 0 ? 1 : 5;
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+        {
+            double acc = 0.0;
+#pragma unroll
+            for (int j = 0; j < 3; ++j) acc += detAA[3*r+j]*diff[c0+j];
+            out[c0+r] = acc;
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 8; ++c) out[c] *= speed;
+}
+
 // ---- kernels ------------------------------------------------------------------
 // 1. Cell-average primitives WBar and their magnetic energy on box+5.
 __global__ void __launch_bounds__(kB) k_o4_wbar(O4View v, const double* __restrict__ U, double* __restrict__ S)
@@ -486,7 +558,8 @@ __global__ void __launch_bounds__(kB) k_o4_faces(O4View v, double* __restrict__ 
 #pragma unroll
         for (int q = 0; q < 8; ++q) corr[q] = lapb[q]*(1.0/24.0);
     }
-    {   // Roe is the fourth-order Riemann solver here, so the repair always applies
+    if (v.roe)
+    {   // the Roe face-state repair (riemann_solver 2 only)
         double pl[8], pr[8], cl[8], cr[8];
         int m1[3] = {x[0], x[1], x[2]}; m1[D] -= 1;
 #pragma unroll
@@ -509,6 +582,9 @@ __global__ void __launch_bounds__(kB) k_o4_faces(O4View v, double* __restrict__ 
     }
     S[I(F_P0L, x[0], x[1], x[2])] = lf[4];
     S[I(F_P0R, x[0], x[1], x[2])] = rf[4];
+    if (!v.roe)
+        S[I(F_SPD, x[0], x[1], x[2])] = rusanovSpeedO4(lf, rf, v.faceA[D] + 9*angIdx(v, b.p, x[1], x[2]),
+                                                       v.gamma, D, false);
 }
 
 // Energies (ke, me, v.B) and the physical flux of the face-average state at
@@ -610,6 +686,41 @@ __global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(2))) k_
     factoredRoe<S_>(out, wl, wr, A, radialPoint(v, D, x[0]), D, v.gamma, v.entropy);
 #pragma unroll
     for (int q = 0; q < 8; ++q) S[I(F_DPT+q, x[0], x[1], x[2])] = out[q];
+}
+
+// 5c'. Rusanov (riemann_solver 1): conserved point values of both face
+// states times the radial point area, primToMappedRusanovFlux
+// (src/BoxOp_EulerCubedSphereImplem.H:586-640). Faces [lo_d, hi_d+1] x
+// transverse [lo-1, hi+1].
+template <int D>
+__global__ void __launch_bounds__(kB) k_o4_rusq(O4View v, double* __restrict__ S)
+{
+    const BoxDesc& b = v.boxes[blockIdx.y];
+    int L[3], H[3], x[3];
+    for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]-1; H[d] = b.hi[d]+1; }
+    L[D] = b.lo[D]; H[D] = b.hi[D]+1;
+    if (!regionPoint(b, L, H, (long long)blockIdx.x*blockDim.x + threadIdx.x, x)) return;
+    const BoxIdx I(b);
+    const double* A = v.faceA[D] + 9*angIdx(v, b.p, x[1], x[2]);
+    const double rp = radialPoint(v, D, x[0]);
+#pragma unroll 1
+    for (int side = 0; side < 2; ++side)
+    {
+        double w[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) w[q] = S[I((side == 0 ? F_PTL : F_PTR)+q, x[0], x[1], x[2])];
+        double ke, me, vb;
+        energiesNative(A, w, ke, me, vb);
+        const int f = side == 0 ? F_QL : F_QR;
+        S[I(f, x[0], x[1], x[2])] = rp*w[0];
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+        {
+            S[I(f+1+a, x[0], x[1], x[2])] = rp*w[0]*w[1+a];
+            S[I(f+5+a, x[0], x[1], x[2])] = rp*w[5+a];
+        }
+        S[I(f+4, x[0], x[1], x[2])] = rp*(ke + me + w[4]/(v.gamma-1.0));
+    }
 }
 
 // Face-product helpers of direction D at one box: metric values M, centred
@@ -765,6 +876,40 @@ __global__ void __launch_bounds__(kB) k_o4_flux(O4View v, double* __restrict__ S
 #pragma unroll
         for (int c = 0; c < 8; ++c) Fsum[c] = side == 0 ? Fs[c] : Fsum[c] + Fs[c];
     }
+    double Fd[8];
+    if (!v.roe)
+    {
+        // Rusanov: face-averaged conserved values of both sides, their
+        // difference mapped by the face metric averages, times the speed
+        double diff[8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+        {
+            double a2[2];
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+            {
+                const int f = (s == 0 ? F_QL : F_QR) + c;
+                double lap = 0;
+#pragma unroll
+                for (int t = 0; t < 2; ++t)
+                {
+                    const int tt = t == 0 ? t1 : t2;
+                    int y[3] = {x[0], x[1], x[2]}, z[3] = {x[0], x[1], x[2]};
+                    y[tt] += 1; z[tt] -= 1;
+                    lap += Fv(f, y) - 2.0*Fv(f, x) + Fv(f, z);
+                }
+                a2[s] = Fv(f, x) + lap/24.0;
+            }
+            diff[c] = a2[1] - a2[0];
+        }
+        double dAA[9];
+#pragma unroll
+        for (int a = 0; a < 9; ++a) dAA[a] = M(1+a, x);
+        rusanovMapped(Fd, diff, M(0, x), dAA, Fv(F_SPD, x));
+    }
+    else
+    {
     // averaged point dissipation and its metric products
     double Davg[8];
 #pragma unroll
@@ -780,7 +925,6 @@ __global__ void __launch_bounds__(kB) k_o4_flux(O4View v, double* __restrict__ S
         }
         Davg[c] = Fv(F_DPT+c, x) + lap/24.0;
     }
-    double Fd[8];
     for (int s = 0; s < 2; ++s)
     {
         const int c = s == 0 ? 0 : 4;
@@ -803,6 +947,7 @@ __global__ void __launch_bounds__(kB) k_o4_flux(O4View v, double* __restrict__ S
             }
             Fd[c0+r] = val;
         }
+    }
 #pragma unroll
     for (int c = 0; c < 8; ++c)
         S[I(F_FLUX+8*D+c, x[0], x[1], x[2])] = 0.5*(Fsum[c]-Fd[c]);
@@ -905,6 +1050,7 @@ void buildBoxop(hc_boxop& B, const hc_box* boxes, int nbox)
     v.nr = nr; v.na = na; v.naE = naE; v.limiter = B.opt.limiter;
     v.gamma = B.opt.gamma; v.entropy = B.opt.entropy_fix;
     v.op = B.opt.spatial_operator == 2 ? 2 : 1;
+    v.roe = B.opt.riemann_solver == 2;
     v.fallback = B.opt.berta_fallback;
     v.etaD = B.opt.berta_eta_d;
     std::vector<double> cellA(9ull*6*naE*naE), faceA[3], fmet[3];
@@ -975,7 +1121,8 @@ void directionPass(const O4View& v, dim3 grid, hipStream_t s, double* scratch)
     hipLaunchKernelGGL(k_o4_faces<D>, grid, dim3(kB), 0, s, v, scratch);
     hipLaunchKernelGGL(k_o4_bar<D>, grid, dim3(kB), 0, s, v, scratch);
     hipLaunchKernelGGL(k_o4_phys<D>, grid, dim3(kB), 0, s, v, scratch);
-    hipLaunchKernelGGL((k_o4_roe<D, S_>), grid, dim3(kB), 0, s, v, scratch);
+    if (v.roe) hipLaunchKernelGGL((k_o4_roe<D, S_>), grid, dim3(kB), 0, s, v, scratch);
+    else hipLaunchKernelGGL(k_o4_rusq<D>, grid, dim3(kB), 0, s, v, scratch);
     hipLaunchKernelGGL(k_o4_map<D>, grid, dim3(kB), 0, s, v, scratch);
     hipLaunchKernelGGL(k_o4_flux<D>, grid, dim3(kB), 0, s, v, scratch);
 }
@@ -1418,7 +1565,8 @@ int hc_boxop_create(const hc_shell* shell, const hc_options* options, const hc_b
 {
     if (!shell || !options || !boxes || !out || nbox <= 0) return hc::fail("hc_boxop_create: bad argument");
     if (options->spatial_order != 4) return hc::fail("hc_boxop_create: spatial_order must be 4");
-    if (options->riemann_solver != 2) return hc::fail("hc_boxop_create: the fourth-order operator implements riemann_solver 2 (Roe)");
+    if (options->riemann_solver != 1 && options->riemann_solver != 2)
+        return hc::fail("hc_boxop_create: riemann_solver must be 1 (Rusanov) or 2 (Roe)");
     if (options->limiter != 0 && options->limiter != 1) return hc::fail("hc_boxop_create: limiter must be 0 or 1");
     if (options->precision != 0 && options->precision != 1) return hc::fail("hc_boxop_create: precision must be 0 or 1");
     if (const char* e = hc::checkOperatorOptions(*options)) return hc::fail(std::string("hc_boxop_create: ") + e);

@@ -90,3 +90,22 @@ def test_o4_mixed_precision_error(device):
     errd = rel_linf_per_var(got - ju, ref - ju)
     print("o4 mixed state err", err, "increment err", errd)
     assert err.max() <= 1e-6, err
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("operator", [1, 2])
+def test_o4_rusanov_level_rhs_matches_oracle(device, operator):
+    nr, na = 16, 16
+    shell = hc.Shell(nr, na, 21.5, R1)
+    opts = hc.Options(spatial_order=4, limiter=1, riemann_solver=1, spatial_operator=operator)
+    ju, inner = hc.init_level4(shell, opts, 3, 3, 3)
+    O = orc.Oracle4(nr, na, limiter=1, r1=R1, spatial_operator=operator, riemann=1)
+    W = O.prepare(ju, inner)
+    ref = O.rhs(W)
+    L = hc.Level(shell, opts)
+    K = torch.zeros(L.valid_shape, dtype=torch.float64, device=device)
+    L.stage_rhs(_dev(W), K)
+    torch.cuda.synchronize()
+    err = rel_linf_per_var(K.cpu().numpy(), ref)
+    print("o4 rusanov level rhs err", operator, err)
+    assert err.max() <= TOL, err
