@@ -74,6 +74,8 @@ _SIGNATURES = [
     ("hc_geom_ghost_donors", _I, [_I, _I, _I, _I, _P, _P, _P, _P]),
     ("hc_boxop_create", _I, [ctypes.POINTER(HcShell), ctypes.POINTER(HcOptions), ctypes.POINTER(HcBox), _I,
                              ctypes.POINTER(_P)]),
+    ("hc_boxop_create_levels", _I, [ctypes.POINTER(HcShell), _I, ctypes.POINTER(HcOptions), ctypes.POINTER(HcBox),
+                                    ctypes.POINTER(ctypes.c_int), _I, ctypes.POINTER(_P)]),
     ("hc_boxop_destroy", None, [_P]),
     ("hc_boxop_count", ctypes.c_longlong, [_P, _I]),
     ("hc_boxop_apply", _I, [_P, _P, _P, _P, _P, _P, _P]),
@@ -194,8 +196,8 @@ class Level:
         self.device = torch.device("cuda", torch.cuda.current_device())
 
     def __del__(self):
-        if getattr(self, "_handle", None) and self._handle.value:
-            lib().hc_level_destroy(self._handle)
+        if getattr(self, "_handle", None) and self._handle.value and _lib is not None:
+            _lib.hc_level_destroy(self._handle)
             self._handle = ctypes.c_void_p()
 
     def ghost(self) -> int:
@@ -320,17 +322,24 @@ class BoxOp:
     accumulated into, as the reference's a_Rhs (the caller zeroes it).
     """
 
-    def __init__(self, shell: Shell, options: Options, boxes):
-        self.shell, self.options, self.boxes = shell, options, list(boxes)
+    def __init__(self, shell, options: Options, boxes, levels=None):
+        """shell: a Shell, or a list of Shells (one per refinement level) with
+        levels[n] the level of box n (hc_boxop_create_levels)."""
+        self.options, self.boxes = options, list(boxes)
+        self.shells = list(shell) if isinstance(shell, (list, tuple)) else [shell]
+        self.shell = self.shells[0]
+        self.levels = list(levels) if levels is not None else [0] * len(self.boxes)
         self._handle = ctypes.c_void_p()
-        sh, op = shell.c(), options.c()
+        op = options.c()
         tab = box_table(self.boxes)
-        _check(lib().hc_boxop_create(ctypes.byref(sh), ctypes.byref(op), tab, len(self.boxes),
-                                     ctypes.byref(self._handle)))
+        shells = (HcShell * len(self.shells))(*[s.c() for s in self.shells])
+        lev = (ctypes.c_int * len(self.boxes))(*self.levels)
+        _check(lib().hc_boxop_create_levels(shells, len(self.shells), ctypes.byref(op), tab, lev, len(self.boxes),
+                                            ctypes.byref(self._handle)))
 
     def __del__(self):
-        if getattr(self, "_handle", None) and self._handle.value:
-            lib().hc_boxop_destroy(self._handle)
+        if getattr(self, "_handle", None) and self._handle.value and _lib is not None:
+            _lib.hc_boxop_destroy(self._handle)
             self._handle = ctypes.c_void_p()
 
     def count(self, what: int) -> int:

@@ -66,38 +66,43 @@ __device__ double wenoNormalize(double (&v)[N])
     return scale;
 }
 
-// One side of f_wenoz_pointwise_faceF (:782-878) on normalized values, in S.
+// One side of f_wenoz_pointwise_faceF (:782-878) on normalized values. The
+// candidate values q_k and the undivided differences are formed in FP64; the
+// smoothness indicators and nonlinear weights in S (S = double reproduces
+// the reference operation for operation); the weighted sum in FP64.
 template <typename S>
-__device__ __forceinline__ S wenozSide(S m2, S m1, S c, S p1, S p2)
+__device__ __forceinline__ double wenozSide(double m2, double m1, double c, double p1, double p2)
 {
     const S eps = S(1.0e-40), d0 = S(1.0/16.0), d1 = S(5.0/8.0), d2 = S(5.0/16.0);
-    const S q0 = (S(3.0)*m2 - S(10.0)*m1 + S(15.0)*c)/S(8.0);
-    const S q1 = (-m1 + S(6.0)*c + S(3.0)*p1)/S(8.0);
-    const S q2 = (S(3.0)*c + S(6.0)*p1 - p2)/S(8.0);
-    const S b0a = m2 - S(2.0)*m1 + c, b0b = m2 - S(4.0)*m1 + S(3.0)*c;
-    const S b1a = m1 - S(2.0)*c + p1, b1b = m1 - p1;
-    const S b2a = c - S(2.0)*p1 + p2, b2b = S(3.0)*c - S(4.0)*p1 + p2;
+    const double q0 = (3.0*m2 - 10.0*m1 + 15.0*c)/8.0;
+    const double q1 = (-m1 + 6.0*c + 3.0*p1)/8.0;
+    const double q2 = (3.0*c + 6.0*p1 - p2)/8.0;
+    const S b0a = S(m2 - 2.0*m1 + c), b0b = S(m2 - 4.0*m1 + 3.0*c);
+    const S b1a = S(m1 - 2.0*c + p1), b1b = S(m1 - p1);
+    const S b2a = S(c - 2.0*p1 + p2), b2b = S(3.0*c - 4.0*p1 + p2);
     const S beta0 = (S(13.0)/S(12.0))*b0a*b0a + S(0.25)*b0b*b0b;
     const S beta1 = (S(13.0)/S(12.0))*b1a*b1a + S(0.25)*b1b*b1b;
     const S beta2 = (S(13.0)/S(12.0))*b2a*b2a + S(0.25)*b2b*b2b;
     const S tau5 = fabs(beta0 - beta2);
-    const S a0 = d0*(S(1.0) + tau5/(beta0 + eps));
-    const S a1 = d1*(S(1.0) + tau5/(beta1 + eps));
-    const S a2 = d2*(S(1.0) + tau5/(beta2 + eps));
-    const S asum = a0 + a1 + a2;
+    const double a0 = double(d0*(S(1.0) + tau5/(beta0 + eps)));
+    const double a1 = double(d1*(S(1.0) + tau5/(beta1 + eps)));
+    const double a2 = double(d2*(S(1.0) + tau5/(beta2 + eps)));
+    const double asum = a0 + a1 + a2;
     return (a0*q0 + a1*q1 + a2*q2)/asum;
 }
 
-// One side of f_weno3_pointwise_faceF (:881-946), eps = dx^2.
+// One side of f_weno3_pointwise_faceF (:881-946), eps = dx^2; differences and
+// candidates in FP64, indicators and weights in S.
 template <typename S>
-__device__ __forceinline__ S weno3Side(S m1, S c, S p1, S eps)
+__device__ __forceinline__ double weno3Side(double m1, double c, double p1, double eps_d)
 {
-    const S d0 = S(3.0/4.0), d1 = S(1.0/4.0);
-    const S Dm = c - m1, Dp = p1 - c;
-    const S beta0 = Dp*Dp, beta1 = Dm*Dm, tau = (Dp - Dm)*(Dp - Dm);
-    const S a0 = d0*(S(1.0) + tau/(beta0 + eps));
-    const S a1 = d1*(S(1.0) + tau/(beta1 + eps));
-    const S q0 = S(0.5)*(p1 + c), q1 = S(0.5)*(-m1 + S(3.0)*c);
+    const S d0 = S(3.0/4.0), d1 = S(1.0/4.0), eps = S(eps_d);
+    const double Dm = c - m1, Dp = p1 - c;
+    const S Dms = S(Dm), Dps = S(Dp), Dd = S(Dp - Dm);
+    const S beta0 = Dps*Dps, beta1 = Dms*Dms, tau = Dd*Dd;
+    const double a0 = double(d0*(S(1.0) + tau/(beta0 + eps)));
+    const double a1 = double(d1*(S(1.0) + tau/(beta1 + eps)));
+    const double q0 = 0.5*(p1 + c), q1 = 0.5*(-m1 + 3.0*c);
     return (a0*q0 + a1*q1)/(a0 + a1);
 }
 
@@ -115,10 +120,10 @@ __device__ __forceinline__ S minmod3(S a, S b, S c)
 // the largest of the three, the 2-norm over directions, 1 below eta_d.
 __global__ void __launch_bounds__(kB) k_b_thraw(O4View v, double* __restrict__ S)
 {
-    const BoxDesc& b = v.boxes[blockIdx.y];
+    const BoxDesc& b = blockBoxDesc(v);
     int L[3], H[3], x[3];
     for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]-4; H[d] = b.hi[d]+4; }
-    if (!regionPoint(b, L, H, (long long)blockIdx.x*blockDim.x + threadIdx.x, x)) return;
+    if (!regionPoint(b, L, H, boxThread(b), x)) return;
     const BoxIdx I(b);
     auto eta1 = [](double qm, double q0, double qp) {
         const double num = fabs(qp - 2.0*q0 + qm);
@@ -147,10 +152,10 @@ __global__ void __launch_bounds__(kB) k_b_thraw(O4View v, double* __restrict__ S
 // on box+3 (f_berta_dilate_thetaF, :410-434), 0 in the two outer layers (:500-503).
 __global__ void __launch_bounds__(kB) k_b_theta(O4View v, double* __restrict__ S)
 {
-    const BoxDesc& b = v.boxes[blockIdx.y];
+    const BoxDesc& b = blockBoxDesc(v);
     int L[3], H[3], x[3];
     for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]-kG; H[d] = b.hi[d]+kG; }
-    if (!regionPoint(b, L, H, (long long)blockIdx.x*blockDim.x + threadIdx.x, x)) return;
+    if (!regionPoint(b, L, H, boxThread(b), x)) return;
     const BoxIdx I(b);
     bool inner = true;
 #pragma unroll
@@ -170,10 +175,10 @@ __global__ void __launch_bounds__(kB) k_b_theta(O4View v, double* __restrict__ S
 // with the pressure clipped at 0 (:696-741, :212-275).
 __global__ void __launch_bounds__(kB) k_b_wp(O4View v, const double* __restrict__ U, double* __restrict__ S)
 {
-    const BoxDesc& b = v.boxes[blockIdx.y];
+    const BoxDesc& b = blockBoxDesc(v);
     int L[3], H[3], x[3];
     for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]-4; H[d] = b.hi[d]+4; }
-    if (!regionPoint(b, L, H, (long long)blockIdx.x*blockDim.x + threadIdx.x, x)) return;
+    if (!regionPoint(b, L, H, boxThread(b), x)) return;
     const BoxIdx I(b);
     auto ui = [&](int i, int j, int k) {
         return b.uOff + (k-b.lo[2]+kG)*b.us[1] + (j-b.lo[1]+kG)*b.us[0] + (i-b.lo[0]+kG); };
@@ -192,7 +197,7 @@ __global__ void __launch_bounds__(kB) k_b_wp(O4View v, const double* __restrict_
     double w[8];
     sphToPrimPoint(w, up);
     double ke, me, vb;
-    energiesNative(v.cellA + 9*angIdx(v, b.p, x[1], x[2]), w, ke, me, vb);
+    energiesNative(v.cellA + 9*angIdx(b, x[1], x[2]), w, ke, me, vb);
     w[4] = (w[4]-ke-me)*(v.gamma-1.0);
     if (w[4] < 0.0) w[4] = 0.0;
 #pragma unroll
@@ -207,11 +212,11 @@ __global__ void __launch_bounds__(kB) k_b_wp(O4View v, const double* __restrict_
 template <int D, typename S_>
 __global__ void __launch_bounds__(kB) k_b_faces(O4View v, double* __restrict__ S)
 {
-    const BoxDesc& b = v.boxes[blockIdx.y];
+    const BoxDesc& b = blockBoxDesc(v);
     int L[3], H[3], x[3];
     for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]-2; H[d] = b.hi[d]+2; }
     L[D] = b.lo[D]; H[D] = b.hi[D]+1;
-    if (!regionPoint(b, L, H, (long long)blockIdx.x*blockDim.x + threadIdx.x, x)) return;
+    if (!regionPoint(b, L, H, boxThread(b), x)) return;
     const BoxIdx I(b);
     auto at = [&](int f, int o) {
         int y[3] = {x[0], x[1], x[2]}; y[D] += o;
@@ -220,7 +225,7 @@ __global__ void __launch_bounds__(kB) k_b_faces(O4View v, double* __restrict__ S
     double tr = at(F_THETA, -3);
     tr = refMin(tr, at(F_THETA, -2)); tr = refMin(tr, at(F_THETA, -1)); tr = refMin(tr, at(F_THETA, 0));
     tr = refMin(tr, at(F_THETA, 1)); tr = refMin(tr, at(F_THETA, 2));
-    const double dx = D == 0 ? 1.0/v.nr : 1.0/v.na;
+    const double dx = D == 0 ? 1.0/b.nr : 1.0/b.na;
     const double dx2 = dx*dx;
     double l[8], r[8];
 #pragma unroll 1
@@ -235,8 +240,8 @@ __global__ void __launch_bounds__(kB) k_b_faces(O4View v, double* __restrict__ S
             if (scale == 0.0) { l[c] = 0.0; r[c] = 0.0; }
             else
             {
-                l[c] = scale*double(wenozSide<S_>(S_(s6[0]), S_(s6[1]), S_(s6[2]), S_(s6[3]), S_(s6[4])));
-                r[c] = scale*double(wenozSide<S_>(S_(s6[5]), S_(s6[4]), S_(s6[3]), S_(s6[2]), S_(s6[1])));
+                l[c] = scale*wenozSide<S_>(s6[0], s6[1], s6[2], s6[3], s6[4]);
+                r[c] = scale*wenozSide<S_>(s6[5], s6[4], s6[3], s6[2], s6[1]);
             }
         }
         else
@@ -250,8 +255,8 @@ __global__ void __launch_bounds__(kB) k_b_faces(O4View v, double* __restrict__ S
                 if (scale == 0.0) { l[c] = 0.0; r[c] = 0.0; }
                 else
                 {
-                    l[c] = scale*double(weno3Side<S_>(S_(s4[0]), S_(s4[1]), S_(s4[2]), S_(dx2)));
-                    r[c] = scale*double(weno3Side<S_>(S_(s4[3]), S_(s4[2]), S_(s4[1]), S_(dx2)));
+                    l[c] = scale*weno3Side<S_>(s4[0], s4[1], s4[2], dx2);
+                    r[c] = scale*weno3Side<S_>(s4[3], s4[2], s4[1], dx2);
                 }
             }
             else
@@ -278,7 +283,7 @@ __global__ void __launch_bounds__(kB) k_b_faces(O4View v, double* __restrict__ S
             thF = 0.0;
         }
     }
-    const double* A = v.faceA[D] + 9*angIdx(v, b.p, x[1], x[2]);
+    const double* A = v.faceA[D] + 9*angIdx(b, x[1], x[2]);
     double keL, meL, vbL, keR, meR, vbR, fl[8], fr[8];
     energiesNative(A, l, keL, meL, vbL);
     energiesNative(A, r, keR, meR, vbR);
@@ -297,7 +302,7 @@ __global__ void __launch_bounds__(kB) k_b_faces(O4View v, double* __restrict__ S
     {
         // Rusanov: pointwise conserved jump R - L times the radial point area
         // and the speed of the point states (:1589-1613, :1653-1725)
-        const double rp = radialPoint(v, D, x[0]);
+        const double rp = radialPoint(v, b, D, x[0]);
         S[I(F_DPT, x[0], x[1], x[2])] = rp*r[0] - rp*l[0];
 #pragma unroll
         for (int a = 0; a < 3; ++a)
@@ -315,11 +320,11 @@ __global__ void __launch_bounds__(kB) k_b_faces(O4View v, double* __restrict__ S
 template <int D>
 __global__ void __launch_bounds__(kB) k_b_avg(O4View v, double* __restrict__ S)
 {
-    const BoxDesc& b = v.boxes[blockIdx.y];
+    const BoxDesc& b = blockBoxDesc(v);
     int L[3], H[3], x[3];
     for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]-1; H[d] = b.hi[d]+1; }
     L[D] = b.lo[D]; H[D] = b.hi[D]+1;
-    if (!regionPoint(b, L, H, (long long)blockIdx.x*blockDim.x + threadIdx.x, x)) return;
+    if (!regionPoint(b, L, H, boxThread(b), x)) return;
     const BoxIdx I(b);
     constexpr int t1 = D == 0 ? 1 : 0, t2 = D == 2 ? 1 : 2;
     const bool smooth = S[I(F_THF, x[0], x[1], x[2])] > 0.5;
@@ -348,12 +353,12 @@ __global__ void __launch_bounds__(kB) k_b_avg(O4View v, double* __restrict__ S)
 template <int D>
 __global__ void __launch_bounds__(kB) k_b_map(O4View v, double* __restrict__ S)
 {
-    const BoxDesc& b = v.boxes[blockIdx.y];
+    const BoxDesc& b = blockBoxDesc(v);
     int L[3], H[3], x[3];
     for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]; H[d] = b.hi[d]; }
     H[D] = b.hi[D]+1;
     if (D != 0) { L[0] -= 1; H[0] += 1; }
-    if (!regionPoint(b, L, H, (long long)blockIdx.x*blockDim.x + threadIdx.x, x)) return;
+    if (!regionPoint(b, L, H, boxThread(b), x)) return;
     const BoxIdx I(b);
     constexpr int t1 = D == 0 ? 1 : 0, t2 = D == 2 ? 1 : 2;
     const FaceOps<D> op(v, b, I, S);
@@ -398,17 +403,17 @@ __global__ void __launch_bounds__(kB) k_b_map(O4View v, double* __restrict__ S)
 template <int D>
 __global__ void __launch_bounds__(kB) k_b_flux(O4View v, double* __restrict__ S)
 {
-    const BoxDesc& b = v.boxes[blockIdx.y];
+    const BoxDesc& b = blockBoxDesc(v);
     int L[3], H[3], x[3];
     for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]; H[d] = b.hi[d]; }
     H[D] = b.hi[D]+1;
-    if (!regionPoint(b, L, H, (long long)blockIdx.x*blockDim.x + threadIdx.x, x)) return;
+    if (!regionPoint(b, L, H, boxThread(b), x)) return;
     const BoxIdx I(b);
     constexpr int t1 = D == 0 ? 1 : 0, t2 = D == 2 ? 1 : 2;
     const FaceOps<D> op(v, b, I, S);
     auto Fv = [&](int f, const int y[3]) { return op.Fv(f, y); };
-    const double adjX = areaOf(v, D, x[0]);
-    auto dAdj = [&](int i) { return 0.5*areaOf(v, D, i+1) - 0.5*areaOf(v, D, i-1); };
+    const double adjX = areaOf(v, b, D, x[0]);
+    auto dAdj = [&](int i) { return 0.5*areaOf(v, b, D, i+1) - 0.5*areaOf(v, b, D, i-1); };
     const int rp[3] = {x[0]+1, x[1], x[2]}, rm[3] = {x[0]-1, x[1], x[2]};
     double Fsum[8];
 #pragma unroll
@@ -432,7 +437,7 @@ __global__ void __launch_bounds__(kB) k_b_flux(O4View v, double* __restrict__ S)
             const double dv = 0.5*Fv(F_PAVG, a) - 0.5*Fv(F_PAVG, z);
             val += dv*dAdj(y[0])*(1.0/12.0);
         }
-        val += Fv(F_PAVG, y)*areaOf(v, D, y[0])*1.0;
+        val += Fv(F_PAVG, y)*areaOf(v, b, D, y[0])*1.0;
         return val; };
     const double t0v = tmp3(x);
     double dT[2] = {0.0, 0.0};

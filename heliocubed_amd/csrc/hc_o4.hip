@@ -1,7 +1,8 @@
-// Fourth-order standard operator of the HelioCubed MHD RHS on MI355X, as a
-// batched box operator: every box of a box table is processed by the same
-// launches (box index = blockIdx.y), so a whole AMR level or a uniform shell
-// is one launch per kernel per RK stage.
+// Fourth-order operators of the HelioCubed MHD RHS on MI355X, as a batched
+// box operator: every box of a box table (uneven sizes, any mix of refinement
+// levels) is processed by the same launches, each box owning a run of blocks
+// of a flattened grid, so a whole AMR hierarchy or a uniform shell is one
+// launch per kernel per RK stage.
 //
 // Restates (src/ = HelioCubed reference repository) the fourth-order path of
 // BoxOp_EulerCubedSphere::operator() (src/BoxOp_EulerCubedSphereImplem.H:97-402):
@@ -55,6 +56,10 @@ struct BoxDesc
     int p, lo[3], hi[3], n[3];
     long long uOff, rOff, sOff;     // element offsets of U(lo-5), R(lo), scratch
     long long us[3], rs[3];         // (y, z, component) strides of U and R
+    int nr, na, naE;                // cells per panel of the box's level; angular table width
+    int radOff;                     // offset of the level's radial table
+    long long angOff;               // entry offset of the panel's angular tables at the box's level
+    long long block0;               // first block of the box in the flattened launch
 };
 
 // A physical seam face seen from two boxes: scratch indices of the flux
@@ -77,25 +82,34 @@ struct O4View
     const double* cellA;     // [6][naE][naE][9]
     const double* faceA[3];  // [6][naE][naE][9]
     const double* fmet[3];   // [6][naE][naE][13]: detA, det(A) A, cofactor row
-    const double* radial;    // [nr+13][2]: adjDr_0 (area0), adjDr_t (areaT) for i = -6 ..
+    const double* radial;    // per level [nr+13][2]: adjDr_0 (area0), adjDr_t (areaT) for i = -6 ..
+    const int* blockBox;     // box of each block of the flattened launch
     const SeamPair* seams;
     int nseams;
 };
 
-__device__ __forceinline__ long long angIdx(const O4View& v, int p, int j, int k)
+// Entry of the angular tables (cellA, faceA, fmet) for cell / face (j, k) of the box's panel and level.
+__device__ __forceinline__ long long angIdx(const BoxDesc& b, int j, int k)
 {
-    return ((long long)p*v.naE + (k+kG))*v.naE + (j+kG);
+    return b.angOff + (long long)(k+kG)*b.naE + (j+kG);
 }
 
-__device__ __forceinline__ double areaOf(const O4View& v, int d, int i)
+__device__ __forceinline__ double areaOf(const O4View& v, const BoxDesc& b, int d, int i)
 {
-    return v.radial[2*(i+6) + (d == 0 ? 0 : 1)];
+    return v.radial[b.radOff + 2*(i+6) + (d == 0 ? 0 : 1)];
 }
 
-__device__ __forceinline__ double radialPoint(const O4View& v, int d, int i)
+__device__ __forceinline__ double radialPoint(const O4View& v, const BoxDesc& b, int d, int i)
 {
-    if (d == 0) return areaOf(v, 0, i);
-    return areaOf(v, d, i) - (1.0/24.0)*(areaOf(v, d, i+1) - 2.0*areaOf(v, d, i) + areaOf(v, d, i-1));
+    if (d == 0) return areaOf(v, b, 0, i);
+    return areaOf(v, b, d, i) - (1.0/24.0)*(areaOf(v, b, d, i+1) - 2.0*areaOf(v, b, d, i) + areaOf(v, b, d, i-1));
+}
+
+// Box of this block and the thread's index in the box's launch region.
+__device__ __forceinline__ const BoxDesc& blockBoxDesc(const O4View& v) { return v.boxes[v.blockBox[blockIdx.x]]; }
+__device__ __forceinline__ long long boxThread(const BoxDesc& b)
+{
+    return (long long)(blockIdx.x - b.block0)*blockDim.x + threadIdx.x;
 }
 
 struct BoxIdx
@@ -383,10 +397,10 @@ __device__ __forceinline__ void rusanovMapped(double out[8], const double diff[8
 // 1. Cell-average primitives WBar and their magnetic energy on box+5.
 __global__ void __launch_bounds__(kB) k_o4_wbar(O4View v, const double* __restrict__ U, double* __restrict__ S)
 {
-    const BoxDesc& b = v.boxes[blockIdx.y];
+    const BoxDesc& b = blockBoxDesc(v);
     int L[3], H[3], x[3];
     for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]-kG; H[d] = b.hi[d]+kG; }
-    if (!regionPoint(b, L, H, (long long)blockIdx.x*blockDim.x + threadIdx.x, x)) return;
+    if (!regionPoint(b, L, H, boxThread(b), x)) return;
     const BoxIdx I(b);
     const long long u = b.uOff + (x[2]-L[2])*b.us[1] + (x[1]-L[1])*b.us[0] + (x[0]-L[0]);
     double uu[8], w[8];
@@ -394,7 +408,7 @@ __global__ void __launch_bounds__(kB) k_o4_wbar(O4View v, const double* __restri
     for (int c = 0; c < 8; ++c) uu[c] = U[u + c*b.us[2]];
     sphToPrimPoint(w, uu);
     double ke, me, vb;
-    energiesNative(v.cellA + 9*angIdx(v, b.p, x[1], x[2]), w, ke, me, vb);
+    energiesNative(v.cellA + 9*angIdx(b, x[1], x[2]), w, ke, me, vb);
     w[4] = (w[4]-ke-me)*(v.gamma-1.0);
     if (w[4] < 0) w[4] = 0.0;
 #pragma unroll
@@ -406,10 +420,10 @@ __global__ void __launch_bounds__(kB) k_o4_wbar(O4View v, const double* __restri
 // src/BoxOp_Utilities.H:619-696.
 __global__ void __launch_bounds__(kB) k_o4_w(O4View v, const double* __restrict__ U, double* __restrict__ S)
 {
-    const BoxDesc& b = v.boxes[blockIdx.y];
+    const BoxDesc& b = blockBoxDesc(v);
     int L[3], H[3], x[3];
     for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]-4; H[d] = b.hi[d]+4; }
-    if (!regionPoint(b, L, H, (long long)blockIdx.x*blockDim.x + threadIdx.x, x)) return;
+    if (!regionPoint(b, L, H, boxThread(b), x)) return;
     const BoxIdx I(b);
     auto ui = [&](int i, int j, int k) {
         return b.uOff + (k-b.lo[2]+kG)*b.us[1] + (j-b.lo[1]+kG)*b.us[0] + (i-b.lo[0]+kG); };
@@ -430,7 +444,7 @@ __global__ void __launch_bounds__(kB) k_o4_w(O4View v, const double* __restrict_
     double w[8];
     sphToPrimPoint(w, up);
     double ke, me, vb;
-    energiesNative(v.cellA + 9*angIdx(v, b.p, x[1], x[2]), w, ke, me, vb);
+    energiesNative(v.cellA + 9*angIdx(b, x[1], x[2]), w, ke, me, vb);
     w[4] = (w[4]-ke-me)*(v.gamma-1.0);
     double wb[8];
 #pragma unroll
@@ -453,10 +467,10 @@ __global__ void __launch_bounds__(kB) k_o4_w(O4View v, const double* __restrict_
 // of the three-cell minimum of eta_tilde, src/BoxOp_Limiters.H:304-334.
 __global__ void __launch_bounds__(kB) k_o4_eta(O4View v, double* __restrict__ S)
 {
-    const BoxDesc& b = v.boxes[blockIdx.y];
+    const BoxDesc& b = blockBoxDesc(v);
     int L[3], H[3], x[3];
     for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]-2; H[d] = b.hi[d]+2; }
-    if (!regionPoint(b, L, H, (long long)blockIdx.x*blockDim.x + threadIdx.x, x)) return;
+    if (!regionPoint(b, L, H, boxThread(b), x)) return;
     const BoxIdx I(b);
     double e = 0;
     for (int dd = 0; dd < 3; ++dd)
@@ -496,11 +510,11 @@ __device__ __forceinline__ void faceBar(const BoxIdx& I, const double* S, int d,
 template <int D>
 __global__ void __launch_bounds__(kB) k_o4_faces(O4View v, double* __restrict__ S)
 {
-    const BoxDesc& b = v.boxes[blockIdx.y];
+    const BoxDesc& b = blockBoxDesc(v);
     int L[3], H[3], x[3];
     for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]-2; H[d] = b.hi[d]+2; }
     L[D] = b.lo[D]; H[D] = b.hi[D]+1;
-    if (!regionPoint(b, L, H, (long long)blockIdx.x*blockDim.x + threadIdx.x, x)) return;
+    if (!regionPoint(b, L, H, boxThread(b), x)) return;
     const BoxIdx I(b);
     constexpr int t1 = D == 0 ? 1 : 0, t2 = D == 2 ? 1 : 2;
     auto Wat = [&](int c, int o) {
@@ -583,7 +597,7 @@ __global__ void __launch_bounds__(kB) k_o4_faces(O4View v, double* __restrict__ 
     S[I(F_P0L, x[0], x[1], x[2])] = lf[4];
     S[I(F_P0R, x[0], x[1], x[2])] = rf[4];
     if (!v.roe)
-        S[I(F_SPD, x[0], x[1], x[2])] = rusanovSpeedO4(lf, rf, v.faceA[D] + 9*angIdx(v, b.p, x[1], x[2]),
+        S[I(F_SPD, x[0], x[1], x[2])] = rusanovSpeedO4(lf, rf, v.faceA[D] + 9*angIdx(b, x[1], x[2]),
                                                        v.gamma, D, false);
 }
 
@@ -596,7 +610,7 @@ __device__ void barFlux(const O4View& v, const BoxDesc& b, const BoxIdx& I, cons
     double wf[8];
     faceBar(I, S, D, i, j, k, wf);
     double ke, me, vb;
-    energiesNative(v.faceA[D] + 9*angIdx(v, b.p, j, k), wf, ke, me, vb);
+    energiesNative(v.faceA[D] + 9*angIdx(b, j, k), wf, ke, me, vb);
     primFlux(f, wf, ke, me, vb, v.gamma, D);
     meBar = me;
     pBar = wf[4];
@@ -608,11 +622,11 @@ __device__ void barFlux(const O4View& v, const BoxDesc& b, const BoxIdx& I, cons
 template <int D>
 __global__ void __launch_bounds__(kB) k_o4_bar(O4View v, double* __restrict__ S)
 {
-    const BoxDesc& b = v.boxes[blockIdx.y];
+    const BoxDesc& b = blockBoxDesc(v);
     int L[3], H[3], x[3];
     for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]-3; H[d] = b.hi[d]+3; }
     L[D] = b.lo[D]; H[D] = b.hi[D]+1;
-    if (!regionPoint(b, L, H, (long long)blockIdx.x*blockDim.x + threadIdx.x, x)) return;
+    if (!regionPoint(b, L, H, boxThread(b), x)) return;
     const BoxIdx I(b);
     double f[8], me, p;
     barFlux<D>(v, b, I, S, x[0], x[1], x[2], f, me, p);
@@ -628,11 +642,11 @@ __global__ void __launch_bounds__(kB) k_o4_bar(O4View v, double* __restrict__ S)
 template <int D>
 __global__ void __launch_bounds__(kB) k_o4_phys(O4View v, double* __restrict__ S)
 {
-    const BoxDesc& b = v.boxes[blockIdx.y];
+    const BoxDesc& b = blockBoxDesc(v);
     int L[3], H[3], x[3];
     for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]-2; H[d] = b.hi[d]+2; }
     L[D] = b.lo[D]; H[D] = b.hi[D]+1;
-    if (!regionPoint(b, L, H, (long long)blockIdx.x*blockDim.x + threadIdx.x, x)) return;
+    if (!regionPoint(b, L, H, boxThread(b), x)) return;
     const BoxIdx I(b);
     constexpr int t1 = D == 0 ? 1 : 0, t2 = D == 2 ? 1 : 2;
     // transverse Laplacian of the face-average flux and magnetic energy
@@ -651,7 +665,7 @@ __global__ void __launch_bounds__(kB) k_o4_phys(O4View v, double* __restrict__ S
             lapF[q] += S[I(F_FB+q, y[0], y[1], y[2])] - 2.0*S[I(F_FB+q, x[0], x[1], x[2])] + S[I(F_FB+q, z[0], z[1], z[2])];
         lapMe += S[I(F_MEF, y[0], y[1], y[2])] - 2.0*mex + S[I(F_MEF, z[0], z[1], z[2])];
     }
-    const double* A = v.faceA[D] + 9*angIdx(v, b.p, x[1], x[2]);
+    const double* A = v.faceA[D] + 9*angIdx(b, x[1], x[2]);
 #pragma unroll 1
     for (int side = 0; side < 2; ++side)
     {
@@ -673,17 +687,17 @@ __global__ void __launch_bounds__(kB) k_o4_phys(O4View v, double* __restrict__ S
 template <int D, typename S_>
 __global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(2))) k_o4_roe(O4View v, double* __restrict__ S)
 {
-    const BoxDesc& b = v.boxes[blockIdx.y];
+    const BoxDesc& b = blockBoxDesc(v);
     int L[3], H[3], x[3];
     for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]-1; H[d] = b.hi[d]+1; }
     L[D] = b.lo[D]; H[D] = b.hi[D]+1;
-    if (!regionPoint(b, L, H, (long long)blockIdx.x*blockDim.x + threadIdx.x, x)) return;
+    if (!regionPoint(b, L, H, boxThread(b), x)) return;
     const BoxIdx I(b);
-    const double* A = v.faceA[D] + 9*angIdx(v, b.p, x[1], x[2]);
+    const double* A = v.faceA[D] + 9*angIdx(b, x[1], x[2]);
     double wl[8], wr[8], out[8];
 #pragma unroll
     for (int q = 0; q < 8; ++q) { wl[q] = S[I(F_PTL+q, x[0], x[1], x[2])]; wr[q] = S[I(F_PTR+q, x[0], x[1], x[2])]; }
-    factoredRoe<S_>(out, wl, wr, A, radialPoint(v, D, x[0]), D, v.gamma, v.entropy);
+    factoredRoe<S_>(out, wl, wr, A, radialPoint(v, b, D, x[0]), D, v.gamma, v.entropy);
 #pragma unroll
     for (int q = 0; q < 8; ++q) S[I(F_DPT+q, x[0], x[1], x[2])] = out[q];
 }
@@ -695,14 +709,14 @@ __global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(2))) k_
 template <int D>
 __global__ void __launch_bounds__(kB) k_o4_rusq(O4View v, double* __restrict__ S)
 {
-    const BoxDesc& b = v.boxes[blockIdx.y];
+    const BoxDesc& b = blockBoxDesc(v);
     int L[3], H[3], x[3];
     for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]-1; H[d] = b.hi[d]+1; }
     L[D] = b.lo[D]; H[D] = b.hi[D]+1;
-    if (!regionPoint(b, L, H, (long long)blockIdx.x*blockDim.x + threadIdx.x, x)) return;
+    if (!regionPoint(b, L, H, boxThread(b), x)) return;
     const BoxIdx I(b);
-    const double* A = v.faceA[D] + 9*angIdx(v, b.p, x[1], x[2]);
-    const double rp = radialPoint(v, D, x[0]);
+    const double* A = v.faceA[D] + 9*angIdx(b, x[1], x[2]);
+    const double rp = radialPoint(v, b, D, x[0]);
 #pragma unroll 1
     for (int side = 0; side < 2; ++side)
     {
@@ -737,7 +751,7 @@ struct FaceOps
     const double* met;
     __device__ FaceOps(const O4View& v_, const BoxDesc& b_, const BoxIdx& I_, const double* S_)
         : v(v_), b(b_), I(I_), S(S_), met(v_.fmet[D]) {}
-    __device__ double M(int c, const int y[3]) const { return met[13*angIdx(v, b.p, y[1], y[2]) + c]; }
+    __device__ double M(int c, const int y[3]) const { return met[13*angIdx(b, y[1], y[2]) + c]; }
     __device__ double Dm(int c, int t, const int y[3]) const
     {
         int a[3] = {y[0], y[1], y[2]}, z[3] = {y[0], y[1], y[2]};
@@ -792,12 +806,12 @@ struct FaceOps
 template <int D>
 __global__ void __launch_bounds__(kB) k_o4_map(O4View v, double* __restrict__ S)
 {
-    const BoxDesc& b = v.boxes[blockIdx.y];
+    const BoxDesc& b = blockBoxDesc(v);
     int L[3], H[3], x[3];
     for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]; H[d] = b.hi[d]; }
     H[D] = b.hi[D]+1;
     if (D != 0) { L[0] -= 1; H[0] += 1; }
-    if (!regionPoint(b, L, H, (long long)blockIdx.x*blockDim.x + threadIdx.x, x)) return;
+    if (!regionPoint(b, L, H, boxThread(b), x)) return;
     const BoxIdx I(b);
     const FaceOps<D> op(v, b, I, S);
     double out[8];
@@ -816,11 +830,11 @@ __global__ void __launch_bounds__(kB) k_o4_map(O4View v, double* __restrict__ S)
 template <int D>
 __global__ void __launch_bounds__(kB) k_o4_flux(O4View v, double* __restrict__ S)
 {
-    const BoxDesc& b = v.boxes[blockIdx.y];
+    const BoxDesc& b = blockBoxDesc(v);
     int L[3], H[3], x[3];
     for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]; H[d] = b.hi[d]; }
     H[D] = b.hi[D]+1;
-    if (!regionPoint(b, L, H, (long long)blockIdx.x*blockDim.x + threadIdx.x, x)) return;
+    if (!regionPoint(b, L, H, boxThread(b), x)) return;
     const BoxIdx I(b);
     constexpr int t1 = D == 0 ? 1 : 0, t2 = D == 2 ? 1 : 2;
     const FaceOps<D> op(v, b, I, S);
@@ -828,12 +842,12 @@ __global__ void __launch_bounds__(kB) k_o4_flux(O4View v, double* __restrict__ S
     auto Dm = [&](int c, int t, const int y[3]) { return op.Dm(c, t, y); };
     auto Fv = [&](int f, const int y[3]) { return op.Fv(f, y); };
     auto Df = [&](int f, int t, const int y[3]) { return op.Df(f, t, y); };
-    const double adjX = areaOf(v, D, x[0]);
+    const double adjX = areaOf(v, b, D, x[0]);
     // The radial area varies only with the radial index, so its transverse
     // difference is exactly zero unless the transverse direction is radial:
     // t2 is never radial and t1 is radial for D = 1, 2. The zero terms of the
     // product rule add +-0 and are left out (bit-identical for finite data).
-    const double adjD1 = D == 0 ? 0.0 : 0.5*areaOf(v, D, x[0]+1) - 0.5*areaOf(v, D, x[0]-1);
+    const double adjD1 = D == 0 ? 0.0 : 0.5*areaOf(v, b, D, x[0]+1) - 0.5*areaOf(v, b, D, x[0]-1);
     // face-average total pressure at the transverse neighbours
     double p0bD[2], radPD[2];
 #pragma unroll
@@ -844,7 +858,7 @@ __global__ void __launch_bounds__(kB) k_o4_flux(O4View v, double* __restrict__ S
         y[tt] += 1; z[tt] -= 1;
         const double pa = Fv(F_P0B, y), pz = Fv(F_P0B, z);
         p0bD[t] = 0.5*pa - 0.5*pz;
-        radPD[t] = 0.5*(pa*areaOf(v, D, y[0])) - 0.5*(pz*areaOf(v, D, z[0]));
+        radPD[t] = 0.5*(pa*areaOf(v, b, D, y[0])) - 0.5*(pz*areaOf(v, b, D, z[0]));
     }
     double Fsum[8];
     for (int side = 0; side < 2; ++side)
@@ -975,9 +989,9 @@ __global__ void __launch_bounds__(kB) k_o4_seams(O4View v, double* __restrict__ 
 __global__ void __launch_bounds__(kB) k_o4_div(O4View v, const double* __restrict__ S, double* __restrict__ R,
                                                int overwrite, double* __restrict__ Uacc, double b_dt)
 {
-    const BoxDesc& b = v.boxes[blockIdx.y];
+    const BoxDesc& b = blockBoxDesc(v);
     int x[3];
-    if (!regionPoint(b, b.lo, b.hi, (long long)blockIdx.x*blockDim.x + threadIdx.x, x)) return;
+    if (!regionPoint(b, b.lo, b.hi, boxThread(b), x)) return;
     const BoxIdx I(b);
     const long long r = b.rOff + (x[2]-b.lo[2])*b.rs[1] + (x[1]-b.lo[1])*b.rs[0] + (x[0]-b.lo[0]);
     for (int c = 0; c < 8; ++c)
@@ -986,7 +1000,7 @@ __global__ void __launch_bounds__(kB) k_o4_div(O4View v, const double* __restric
         double acc = overwrite ? 0.0 : R[r + n];
         for (int d = 0; d < 3; ++d)
         {
-            const double dxInv = -1.0/(d == 0 ? 1.0/v.nr : 1.0/v.na);
+            const double dxInv = -1.0/(d == 0 ? 1.0/b.nr : 1.0/b.na);
             int a[3] = {x[0], x[1], x[2]}; a[d] += 1;
             double inc = 0;
             inc += (-1.0*dxInv)*S[I(F_FLUX+8*d+c, x[0], x[1], x[2])];
@@ -1002,14 +1016,14 @@ __global__ void __launch_bounds__(kB) k_o4_div(O4View v, const double* __restric
 __global__ void __launch_bounds__(kB) k_o4_fluxout(O4View v, int D, const double* __restrict__ S,
                                                    double* __restrict__ F, const long long* __restrict__ fOff)
 {
-    const BoxDesc& b = v.boxes[blockIdx.y];
+    const BoxDesc& b = blockBoxDesc(v);
     int L[3] = {b.lo[0], b.lo[1], b.lo[2]}, H[3] = {b.hi[0], b.hi[1], b.hi[2]}, x[3];
     H[D] += 1;
-    if (!regionPoint(b, L, H, (long long)blockIdx.x*blockDim.x + threadIdx.x, x)) return;
+    if (!regionPoint(b, L, H, boxThread(b), x)) return;
     const BoxIdx I(b);
     const int m0 = b.n[0]+(D == 0), m1 = b.n[1]+(D == 1), m2 = b.n[2]+(D == 2);
     const long long n = (long long)m0*m1*m2;
-    const long long o = fOff[blockIdx.y] + ((long long)(x[2]-b.lo[2])*m1 + (x[1]-b.lo[1]))*m0 + (x[0]-b.lo[0]);
+    const long long o = fOff[v.blockBox[blockIdx.x]] + ((long long)(x[2]-b.lo[2])*m1 + (x[1]-b.lo[1]))*m0 + (x[0]-b.lo[0]);
     for (int c = 0; c < 8; ++c) F[o + c*n] = S[I(F_FLUX+8*D+c, x[0], x[1], x[2])];
 }
 
@@ -1027,8 +1041,10 @@ T* upload(const std::vector<T>& h, std::vector<void*>& owned)
 struct hc_boxop
 {
     hc_options opt{};
-    hc::ShellDims dims{};
+    hc::ShellDims dims{};                 // level 0 (the only level of a uniform table)
+    std::vector<hc::ShellDims> levels;    // every level of the table
     int nbox = 0;
+    long long totalBlocks = 0;            // blocks of the flattened launch over all boxes
     std::vector<BoxDesc> boxes;
     O4View view{};
     double* scratch = nullptr;
@@ -1042,54 +1058,72 @@ struct hc_boxop
 
 namespace
 {
-void buildBoxop(hc_boxop& B, const hc_box* boxes, int nbox)
+// Box table over one or more levels: per-level angular and radial tables
+// (concatenated), one descriptor per box with its level's sizes and table
+// offsets, scratch offsets, and a flattened launch in which box b owns
+// ceil((n+10)^3 / kB) consecutive blocks (uneven boxes waste no blocks).
+void buildBoxop(hc_boxop& B, const hc_box* boxes, const int* boxLevel, int nbox)
 {
-    const int nr = B.dims.nr, na = B.dims.na;
-    const int naE = na + 2*kG + 1;
+    if (B.levels.empty()) B.levels.push_back(B.dims);
     O4View& v = B.view;
-    v.nr = nr; v.na = na; v.naE = naE; v.limiter = B.opt.limiter;
+    v.nr = B.levels[0].nr; v.na = B.levels[0].na; v.naE = v.na + 2*kG + 1; v.limiter = B.opt.limiter;
     v.gamma = B.opt.gamma; v.entropy = B.opt.entropy_fix;
     v.op = B.opt.spatial_operator == 2 ? 2 : 1;
     v.roe = B.opt.riemann_solver == 2;
     v.fallback = B.opt.berta_fallback;
     v.etaD = B.opt.berta_eta_d;
-    std::vector<double> cellA(9ull*6*naE*naE), faceA[3], fmet[3];
-    for (int d = 0; d < 3; ++d) { faceA[d].resize(9ull*6*naE*naE); fmet[d].resize(13ull*6*naE*naE); }
-    const double h = 1.0/na;
-    for (int p = 0; p < 6; ++p)
-        for (int kk = 0; kk < naE; ++kk)
-            for (int jj = 0; jj < naE; ++jj)
-            {
-                const int j = jj-kG, k = kk-kG;
-                const size_t x = ((size_t)p*naE+kk)*naE+jj;
-                double A[3][3];
-                hc::pointBasis(p, hc::xiToAngle((j+0.5)*h), hc::xiToAngle((k+0.5)*h), A);
-                for (int a = 0; a < 9; ++a) cellA[9*x+a] = A[a/3][a%3];
-                for (int d = 0; d < 3; ++d)
+    std::vector<double> cellA, faceA[3], fmet[3], radial;
+    std::vector<long long> levAng;      // entry offset of each level's angular tables
+    std::vector<int> levRad;            // offset of each level's radial table
+    for (const hc::ShellDims& L : B.levels)
+    {
+        const int na = L.na, nr = L.nr, naE = na + 2*kG + 1;
+        const size_t base = cellA.size()/9;
+        levAng.push_back((long long)base);
+        cellA.resize(9*(base + 6ull*naE*naE));
+        for (int d = 0; d < 3; ++d) { faceA[d].resize(9*(base + 6ull*naE*naE)); fmet[d].resize(13*(base + 6ull*naE*naE)); }
+        const double h = 1.0/na;
+        for (int p = 0; p < 6; ++p)
+            for (int kk = 0; kk < naE; ++kk)
+                for (int jj = 0; jj < naE; ++jj)
                 {
-                    const double oa = d == 1 ? 0.0 : 0.5, ob = d == 2 ? 0.0 : 0.5;
-                    hc::pointBasis(p, hc::xiToAngle((j+oa)*h), hc::xiToAngle((k+ob)*h), A);
-                    for (int a = 0; a < 9; ++a) faceA[d][9*x+a] = A[a/3][a%3];
-                    hc::faceMetricAverages(na, p, d, j, k, &fmet[d][13*x]);
+                    const int j = jj-kG, k = kk-kG;
+                    const size_t x = base + ((size_t)p*naE+kk)*naE+jj;
+                    double A[3][3];
+                    hc::pointBasis(p, hc::xiToAngle((j+0.5)*h), hc::xiToAngle((k+0.5)*h), A);
+                    for (int a = 0; a < 9; ++a) cellA[9*x+a] = A[a/3][a%3];
+                    for (int d = 0; d < 3; ++d)
+                    {
+                        const double oa = d == 1 ? 0.0 : 0.5, ob = d == 2 ? 0.0 : 0.5;
+                        hc::pointBasis(p, hc::xiToAngle((j+oa)*h), hc::xiToAngle((k+ob)*h), A);
+                        for (int a = 0; a < 9; ++a) faceA[d][9*x+a] = A[a/3][a%3];
+                        hc::faceMetricAverages(na, p, d, j, k, &fmet[d][13*x]);
+                    }
                 }
-            }
+        levRad.push_back((int)radial.size());
+        for (int i = -6; i <= nr+6; ++i)
+        {
+            const hc::RadialCell c = hc::radialCell(L, i);
+            radial.push_back(c.area0); radial.push_back(c.areaT);
+        }
+    }
     v.cellA = upload(cellA, B.owned);
     for (int d = 0; d < 3; ++d) { v.faceA[d] = upload(faceA[d], B.owned); v.fmet[d] = upload(fmet[d], B.owned); }
-    std::vector<double> radial(2ull*(nr+13));
-    for (int i = -6; i <= nr+6; ++i)
-    {
-        const hc::RadialCell c = hc::radialCell(B.dims, i);
-        radial[2*(i+6)] = c.area0; radial[2*(i+6)+1] = c.areaT;
-    }
     v.radial = upload(radial, B.owned);
-    long long u = 0, r = 0, sc = 0;
+    long long u = 0, r = 0, sc = 0, blk = 0;
     for (int d = 0; d < 3; ++d) B.fOffHost[d].clear();
     long long f[3] = {0, 0, 0};
+    std::vector<int> blockBox;
     for (int n = 0; n < nbox; ++n)
     {
         BoxDesc bd{};
+        const int lev = boxLevel ? boxLevel[n] : 0;
+        const hc::ShellDims& L = B.levels[lev];
         bd.p = boxes[n].block;
         for (int d = 0; d < 3; ++d) { bd.lo[d] = boxes[n].lo[d]; bd.hi[d] = boxes[n].hi[d]; bd.n[d] = bd.hi[d]-bd.lo[d]+1; }
+        bd.nr = L.nr; bd.na = L.na; bd.naE = L.na + 2*kG + 1;
+        bd.radOff = levRad[lev];
+        bd.angOff = levAng[lev] + (long long)bd.p*bd.naE*bd.naE;
         const long long e = (long long)(bd.n[0]+2*kG)*(bd.n[1]+2*kG)*(bd.n[2]+2*kG);
         bd.uOff = u; bd.rOff = r; bd.sOff = sc;
         bd.us[0] = bd.n[0]+2*kG; bd.us[1] = (long long)(bd.n[0]+2*kG)*(bd.n[1]+2*kG); bd.us[2] = e;
@@ -1100,12 +1134,19 @@ void buildBoxop(hc_boxop& B, const hc_box* boxes, int nbox)
             B.fOffHost[d].push_back(f[d]);
             f[d] += 8LL*(bd.n[0]+(d == 0))*(bd.n[1]+(d == 1))*(bd.n[2]+(d == 2));
         }
+        bd.block0 = blk;
+        const long long nb = (e + kB - 1)/kB;
+        for (long long q = 0; q < nb; ++q) blockBox.push_back(n);
+        blk += nb;
         B.maxRegion = std::max<long long>(B.maxRegion, e);
         B.boxes.push_back(bd);
     }
+    if (blk > 0x7fffffffLL) throw std::runtime_error("box table too large for one launch");
+    B.totalBlocks = blk;
     B.uCount = u; B.rCount = r;
     for (int d = 0; d < 3; ++d) { B.fCount[d] = f[d]; B.fOff[d] = upload(B.fOffHost[d], B.owned); }
     v.boxes = upload(B.boxes, B.owned);
+    v.blockBox = upload(blockBox, B.owned);
     v.seams = nullptr;
     v.nseams = 0;
     HC_THROW(hipMalloc(&B.scratch, std::max<long long>(1, sc)*sizeof(double)));
@@ -1132,7 +1173,7 @@ void applyBoxop(hc_boxop& B, const double* U, double* R, double* const F[3], hip
                 int overwrite = 0, double* Uacc = nullptr, double b_dt = 0.0)
 {
     const O4View& v = B.view;
-    const dim3 grid((B.maxRegion + kB - 1)/kB, B.nbox);
+    const dim3 grid((unsigned)B.totalBlocks);
     if (v.op == 2) bertaFluxes<S_>(v, grid, s, U, B.scratch);
     else
     {
@@ -1479,7 +1520,7 @@ Level4* level4Create(const hc_shell& shell, const hc_options& opt, int boxSize)
     std::unique_ptr<hc_boxop> B(new hc_boxop());
     B->opt = opt;
     B->dims = L->dims;
-    buildBoxop(*B, boxes.data(), (int)boxes.size());
+    buildBoxop(*B, boxes.data(), nullptr, (int)boxes.size());
     // re-point every box at its tile of the level arrays
     for (auto& bd : B->boxes)
     {
@@ -1563,20 +1604,35 @@ extern "C" {
 int hc_boxop_create(const hc_shell* shell, const hc_options* options, const hc_box* boxes, int nbox,
                     hc_boxop** out)
 {
-    if (!shell || !options || !boxes || !out || nbox <= 0) return hc::fail("hc_boxop_create: bad argument");
+    return hc_boxop_create_levels(shell, 1, options, boxes, nullptr, nbox, out);
+}
+
+int hc_boxop_create_levels(const hc_shell* shells, int nlevels, const hc_options* options, const hc_box* boxes,
+                           const int* box_levels, int nbox, hc_boxop** out)
+{
+    if (!shells || nlevels <= 0 || !options || !boxes || !out || nbox <= 0)
+        return hc::fail("hc_boxop_create: bad argument");
     if (options->spatial_order != 4) return hc::fail("hc_boxop_create: spatial_order must be 4");
     if (options->riemann_solver != 1 && options->riemann_solver != 2)
         return hc::fail("hc_boxop_create: riemann_solver must be 1 (Rusanov) or 2 (Roe)");
     if (options->limiter != 0 && options->limiter != 1) return hc::fail("hc_boxop_create: limiter must be 0 or 1");
     if (options->precision != 0 && options->precision != 1) return hc::fail("hc_boxop_create: precision must be 0 or 1");
     if (const char* e = hc::checkOperatorOptions(*options)) return hc::fail(std::string("hc_boxop_create: ") + e);
+    for (int l = 0; l < nlevels; ++l)
+    {
+        const hc_shell& sh = shells[l];
+        if (sh.n_rad < 4 || sh.n_ang < 4 || !(sh.outer_radius > sh.inner_radius && sh.inner_radius > 0))
+            return hc::fail("hc_boxop_create: bad level shell (need n_rad, n_ang >= 4 and 0 < r0 < r1)");
+    }
     for (int n = 0; n < nbox; ++n)
     {
         const hc_box& b = boxes[n];
+        const int lev = box_levels ? box_levels[n] : 0;
+        if (lev < 0 || lev >= nlevels) return hc::fail("hc_boxop_create: box level out of range");
         if (b.block < 0 || b.block > 5) return hc::fail("hc_boxop_create: box block must be 0..5");
         for (int d = 0; d < 3; ++d)
         {
-            const int lim = d == 0 ? shell->n_rad : shell->n_ang;
+            const int lim = d == 0 ? shells[lev].n_rad : shells[lev].n_ang;
             if (b.lo[d] < 0 || b.hi[d] >= lim || b.hi[d] < b.lo[d] + 3)
                 return hc::fail("hc_boxop_create: box outside the panel or thinner than 4 cells");
         }
@@ -1585,8 +1641,11 @@ int hc_boxop_create(const hc_shell* shell, const hc_options* options, const hc_b
     {
         std::unique_ptr<hc_boxop> B(new hc_boxop());
         B->opt = *options;
-        B->dims = {shell->n_rad, shell->n_ang, kG, shell->inner_radius, shell->outer_radius, shell->radial_stretch};
-        buildBoxop(*B, boxes, nbox);
+        for (int l = 0; l < nlevels; ++l)
+            B->levels.push_back({shells[l].n_rad, shells[l].n_ang, kG, shells[l].inner_radius, shells[l].outer_radius,
+                                 shells[l].radial_stretch});
+        B->dims = B->levels[0];
+        buildBoxop(*B, boxes, box_levels, nbox);
         *out = B.release();
         return 0;
     }

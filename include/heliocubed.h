@@ -148,6 +148,14 @@ int hc_rk4_advance(hc_level* level, double* U, const double* inner_ghosts, doubl
  * F_0..F_2 (2..4). */
 int hc_boxop_create(const hc_shell* shell, const hc_options* options, const hc_box* boxes, int nbox,
                     hc_boxop** out);
+/* Box table over several refinement levels of an AMR hierarchy (the batched
+ * form of the AMR stage evaluation, src/MHD_AMR.H:1417-1499, which calls the
+ * per-box operator level by level): shells[l] gives the cell counts of level
+ * l (same radii and stretch), box_levels[n] the level of box n (lo/hi in
+ * that level's index space). Boxes of uneven sizes and levels are evaluated
+ * in the same launches; arrays are laid out as for hc_boxop_create. */
+int hc_boxop_create_levels(const hc_shell* shells, int nlevels, const hc_options* options, const hc_box* boxes,
+                           const int* box_levels, int nbox, hc_boxop** out);
 void hc_boxop_destroy(hc_boxop* op);
 long long hc_boxop_count(const hc_boxop* op, int what);
 int hc_boxop_apply(hc_boxop* op, const double* USph, double* rhs, double* F0, double* F1, double* F2,

@@ -65,3 +65,16 @@ def test_init_state_is_seeded_and_physical():
     for ig, i in enumerate((-2, -1)):
         r = r0 + (i + 0.5) * dr
         assert np.allclose(ia[:, 0, :, :, ig], (r0 / r) ** 2, rtol=1e-14)
+
+
+def test_multilevel_box_table_validation_without_device():
+    h = ctypes.c_void_p()
+    shells = (hc.HcShell * 2)(hc.Shell(32, 16).c(), hc.Shell(64, 32).c())
+    op = hc.Options(spatial_order=4, limiter=1, riemann_solver=2).c()
+    tab = hc.box_table([(0, (40, 0, 0), (47, 7, 7))])
+    lev = (ctypes.c_int * 1)(0)          # radial 40..47 is outside level 0 (32 cells)
+    assert hc.lib().hc_boxop_create_levels(shells, 2, ctypes.byref(op), tab, lev, 1, ctypes.byref(h)) != 0
+    assert b"outside" in hc.lib().hc_last_error()
+    lev = (ctypes.c_int * 1)(2)          # no level 2
+    assert hc.lib().hc_boxop_create_levels(shells, 2, ctypes.byref(op), tab, lev, 1, ctypes.byref(h)) != 0
+    assert b"level" in hc.lib().hc_last_error()
