@@ -212,16 +212,25 @@ def main():
     L.profile(False)
     if not torch.isfinite(U).all():
         raise RuntimeError("non-finite state after the timed steps")
+    # the same steps replayed from the captured hipGraph of one RK4 step
+    # (per-kernel events are off in this mode), outside the timed region above
+    L.rk4_advance(U, G, dt, 1)
+    torch.cuda.synchronize()
+    tg = time.perf_counter()
+    L.rk4_advance(U, G, dt, args.steps)
+    torch.cuda.synchronize()
+    graph_ms = (time.perf_counter() - tg) / args.steps * 1e3
 
     mixed_err = None
     if U_init is not None:
         # the same steps in FP64 from the same state: per-variable relative
-        # L-infinity error of the mixed-precision state (outside the timed region)
+        # L-infinity error of the mixed-precision state after every step taken
+        # above (outside the timed regions)
         import dataclasses
         L64 = hc.Level(shell, dataclasses.replace(opts, precision=0))
         U64 = U_init.clone()
-        L64.rk4_advance(U64, G, dt, args.warmup)
-        L64.rk4_advance(U64, G, dt, args.steps)
+        # warmup + timed + one graph capture step + timed graph replays
+        L64.rk4_advance(U64, G, dt, args.warmup + 2 * args.steps + 1)
         torch.cuda.synchronize()
         d = (U - U64).abs().amax(dim=(0, 2, 3, 4))
         ref = U64.abs().amax(dim=(0, 2, 3, 4))
@@ -261,6 +270,7 @@ def main():
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "avg_launch_us": avg_s * 1e6},
         "kernels_ms_per_step": {k: v[0] / args.steps for k, v in prof.items()},
+        "graph_ms_per_step": graph_ms,
     }
     if mixed_err is not None:
         line["mixed_vs_fp64_rel_linf"] = mixed_err

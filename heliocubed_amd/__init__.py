@@ -68,6 +68,7 @@ _SIGNATURES = [
     ("hc_prepare_stage", _I, [_P, _P, _P, _D, _P, _P, _P]),
     ("hc_stage_rhs", _I, [_P, _P, _P, _P, _D, _P, _P, _P, _P]),
     ("hc_rk4_advance", _I, [_P, _P, _P, _D, _I, _P]),
+    ("hc_level_use_graph", _I, [_P, _I]),
     ("hc_init_state", _I, [ctypes.POINTER(HcShell), ctypes.POINTER(HcOptions), _I, ctypes.c_ulonglong, _P, _P]),
     ("hc_geom_cell_detA", _D, [_I, _I, _I]),
     ("hc_geom_face_normal", None, [_I, _I, _I, _I, _I, _P]),
@@ -271,9 +272,13 @@ class Level:
             slot += 1
 
     def rk4_advance(self, U, inner, dt: float, nsteps: int = 1):
-        """MBLevelRK4::advance: nsteps classical RK4 steps in place on U."""
+        """MBLevelRK4::advance: nsteps classical RK4 steps in place on U
+        (replayed from a captured hipGraph of one step unless use_graph(False))."""
         self._require(U, self.valid_shape, "U")
         _check(lib().hc_rk4_advance(self._handle, _ptr(U), _ptr(inner), dt, nsteps, self._stream()))
+
+    def use_graph(self, enable: bool = True):
+        _check(lib().hc_level_use_graph(self._handle, int(bool(enable))))
 
 
 def box_table(boxes):

@@ -68,6 +68,15 @@ __device__ __forceinline__ void loadStage(const LevelView& v, const double* W, i
 }
 
 // 1. Valid cells: stage state, conversion to Cartesian primitives.
+// Plain copy (the RK4 stage base state), a kernel so the step graph holds no
+// memory-copy or memset nodes.
+__global__ void __launch_bounds__(kBlock) k_copy(const double* __restrict__ src, double* __restrict__ dst,
+                                                 long long n)
+{
+    const long long t = (long long)blockIdx.x*blockDim.x + threadIdx.x;
+    if (t < n) dst[t] = src[t];
+}
+
 __global__ void __launch_bounds__(kBlock)
 k_prepare_valid(LevelView v, const double* __restrict__ U0, const double* __restrict__ K,
                 double a_dt, double* __restrict__ W)
@@ -522,9 +531,28 @@ struct hc_level
     double* SF = nullptr;
     double* U0 = nullptr;
     double* K = nullptr;
+    // hipGraph of one RK4 step (state copy + 4 x (prepare, RHS)), captured on a
+    // private stream and replayed on the caller's; re-captured when the state,
+    // inner-ghost pointers or dt change.
+    bool useGraph = true;
+    hipStream_t capStream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graphExec = nullptr;
+    const double* gU = nullptr;
+    const double* gInner = nullptr;
+    double gDt = 0.0;
+
+    void dropGraph()
+    {
+        if (graphExec) (void)hipGraphExecDestroy(graphExec);
+        if (graph) (void)hipGraphDestroy(graph);
+        graphExec = nullptr; graph = nullptr; gU = nullptr; gInner = nullptr;
+    }
 
     ~hc_level()
     {
+        dropGraph();
+        if (capStream) (void)hipStreamDestroy(capStream);
         hc::level4Destroy(l4);
         for (void* p : owned) (void)hipFree(p);
     }
@@ -901,18 +929,50 @@ int hc_rk4_advance(hc_level* level, double* U, const double* inner, double dt, i
         hipStream_t s = static_cast<hipStream_t>(stream);
         const double a[4] = {0.0, 0.5, 0.5, 1.0};
         const double b[4] = {1.0/6, 1.0/3, 1.0/3, 1.0/6};
-        for (int n = 0; n < nsteps; ++n)
-        {
-            HC_CHECK(hipMemcpyAsync(L.U0, U, L.validCount()*sizeof(double), hipMemcpyDeviceToDevice, s));
+        const long long nv = (long long)L.validCount();
+        auto step = [&](hipStream_t q) {
+            hipLaunchKernelGGL(k_copy, dim3((unsigned)((nv+kBlock-1)/kBlock)), dim3(kBlock), 0, q, U, L.U0, nv);
             for (int st = 0; st < 4; ++st)
             {
-                prepareStage(L, L.U0, st ? L.K : nullptr, dt*a[st], inner, L.W, s);
-                stageRhs(L, L.W, L.K, U, dt*b[st], nullptr, nullptr, nullptr, s);
+                prepareStage(L, L.U0, st ? L.K : nullptr, dt*a[st], inner, L.W, q);
+                stageRhs(L, L.W, L.K, U, dt*b[st], nullptr, nullptr, nullptr, q);
             }
+        };
+        if (!L.useGraph || L.profile.enabled || nsteps == 0)
+        {
+            for (int n = 0; n < nsteps; ++n) step(s);
+            HC_CHECK(hipGetLastError());
+            return 0;
         }
+        if (!L.graphExec || L.gU != U || L.gInner != inner || L.gDt != dt)
+        {
+            L.dropGraph();
+            if (!L.capStream) HC_CHECK(hipStreamCreateWithFlags(&L.capStream, hipStreamNonBlocking));
+            // order the capture stream after the caller's pending work on s
+            hipEvent_t ready;
+            HC_CHECK(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
+            HC_CHECK(hipEventRecord(ready, s));
+            HC_CHECK(hipStreamWaitEvent(L.capStream, ready, 0));
+            HC_CHECK(hipStreamSynchronize(L.capStream));
+            (void)hipEventDestroy(ready);
+            HC_CHECK(hipStreamBeginCapture(L.capStream, hipStreamCaptureModeThreadLocal));
+            step(L.capStream);
+            HC_CHECK(hipStreamEndCapture(L.capStream, &L.graph));
+            HC_CHECK(hipGraphInstantiate(&L.graphExec, L.graph, nullptr, nullptr, 0));
+            L.gU = U; L.gInner = inner; L.gDt = dt;
+        }
+        for (int n = 0; n < nsteps; ++n) HC_CHECK(hipGraphLaunch(L.graphExec, s));
         return 0;
     }
     catch (const std::exception& e) { return fail(std::string("hc_rk4_advance: ") + e.what()); }
+}
+
+int hc_level_use_graph(hc_level* level, int enable)
+{
+    if (!level) return fail("hc_level_use_graph: null level");
+    level->useGraph = enable != 0;
+    if (!level->useGraph) level->dropGraph();
+    return 0;
 }
 
 int hc_init_state(const hc_shell* shell, const hc_options* options, int config,

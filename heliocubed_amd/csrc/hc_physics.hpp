@@ -5,6 +5,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cmath>
+#include <type_traits>
 
 #define HC_HD __host__ __device__ __forceinline__
 
@@ -171,7 +172,6 @@ template <typename S>
 HC_HD bool roe8WaveDissipation(S out[8], const double WL[8], const double WR[8], double gamma_d, double efix_d)
 {
     const S rs2 = S(0.707106781186547524400844362104849039);
-    const S smallB = S(1.0e-9);
     if (!(WL[0] > 0.0 && WR[0] > 0.0 && WL[4] > 0.0 && WR[4] > 0.0)) return false;
     const double hgd = gamma_d-1.0;
     const double rL = WL[0], rR = WR[0];
@@ -195,47 +195,65 @@ HC_HD bool roe8WaveDissipation(S out[8], const double WL[8], const double WR[8],
     if (ccd < ccmi) ccd = ccmi;
     if (ccd > ccma) ccd = ccma;
     if (!(ccd > 0.0) || !isfinite(ccd)) return false;
+    if constexpr (!std::is_same<S, double>::value)
+    {
+        // Below plasma beta 1/2 (of either side or of the average state) the
+        // FP32 eigen-system loses digits to cancellation in the fast/slow
+        // recombination: evaluate such faces in FP64.
+        if (ccd*Rhod < 0.5*gamma_d*BxBHd || WL[4] < 0.5*bbL || WR[4] < 0.5*bbR)
+        {
+            double o[8];
+            const bool ok = roe8WaveDissipation<double>(o, WL, WR, gamma_d, efix_d);
+#pragma unroll
+            for (int q = 0; q < 8; ++q) out[q] = S(o[q]);
+            return ok;
+        }
+    }
     // Jump in primitive-like variables (left minus right), FP64.
     double dud[8] = {rL-rR, mL0-mR0, mL1-mR1, mL2-mR2, EL-ER, WL[5]-WR[5], WL[6]-WR[6], WL[7]-WR[7]};
     dud[4] = -hgd*(-kkd*dud[0]+Ud*dud[1]+Vd*dud[2]+Wd*dud[3]-dud[4]+(BXd*dud[5]+BYd*dud[6]+BZd*dud[7])*kInv4Pi);
     dud[1] = (-Ud*dud[0]+dud[1])/Rhod;
     dud[2] = (-Vd*dud[0]+dud[2])/Rhod;
     dud[3] = (-Wd*dud[0]+dud[3])/Rhod;
-    // Eigen-system in S.
-    const S Rho = S(Rhod), U = S(Ud), V = S(Vd), W = S(Wd), BX = S(BXd), BY = S(BYd), BZ = S(BZd);
-    const S BX2 = S(BX2d), BYZ2 = S(BYZ2d), BxBH = S(BxBHd), cc = S(ccd);
-    const S bb = S(kInv4Pi*BX2d/Rhod), b = sqrt(bb);
-    const S c = sqrt(cc);
-    S hyy, hzz, aaf, aas, af, as;
-    if (BYZ2 < smallB*smallB)
+    // Wave speeds and eigenvector coefficients: FP64 (the slow speed and the
+    // fast/slow mixing factors cancel catastrophically at low beta); the
+    // eigenvector entries, projections and products below are in S.
+    const double rs2d = 0.707106781186547524400844362104849039, smallBd = 1.0e-9;
+    const double bbd = kInv4Pi*BX2d/Rhod, bd = sqrt(bbd);
+    const double cd = sqrt(ccd);
+    double hyyd, hzzd, aafd, aasd, afd, asd;
+    if (BYZ2d < smallBd*smallBd)
     {
-        hyy = rs2; hzz = rs2;
-        if (bb > cc) { aaf = bb; aas = cc; af = b; as = c; }
-        else { aaf = cc; aas = bb; af = c; as = b; }
+        hyyd = rs2d; hzzd = rs2d;
+        if (bbd > ccd) { aafd = bbd; aasd = ccd; afd = bd; asd = cd; }
+        else { aafd = ccd; aasd = bbd; afd = cd; asd = bd; }
     }
     else
     {
-        const S hyz = sqrt(BYZ2);
-        hyy = BY/hyz; hzz = BZ/hyz;
-        const S szb = cc+(BxBH+BxBH)/Rho;
-        const S hh = sqrt(fmax(S(0), szb*szb-S(4)*cc*bb));
-        aas = S(0.5)*(szb-hh);
-        if (aas <= S(0)) { aas = 0; as = 0; }
-        else if (aas >= cc) { aas = cc; as = c; }
-        else as = sqrt(aas);
-        aaf = S(0.5)*(szb+hh);
-        if (aaf <= cc) { aaf = cc; af = c; }
-        else af = sqrt(aaf);
+        const double hyz = sqrt(BYZ2d);
+        hyyd = BYd/hyz; hzzd = BZd/hyz;
+        const double szb = ccd+(BxBHd+BxBHd)/Rhod;
+        const double hh = sqrt(fmax(0.0, szb*szb-4.0*ccd*bbd));
+        aasd = 0.5*(szb-hh);
+        if (aasd <= 0.0) { aasd = 0; asd = 0; }
+        else if (aasd >= ccd) { aasd = ccd; asd = cd; }
+        else asd = sqrt(aasd);
+        aafd = 0.5*(szb+hh);
+        if (aafd <= ccd) { aafd = ccd; afd = cd; }
+        else afd = sqrt(aafd);
     }
-    (void)BX2;
-    const S lam[8] = {U, U, U+b, U-b, U+af, U-af, U+as, U-as};
-    const S del = S(efix_d)*(fabs(U)+fabs(V)+fabs(W)+af);
-    const S dfs = aaf-aas;
-    S alf, als;
-    if (dfs < S(1.0e-8)) { alf = 1; als = 0; }
-    else if ((cc-aas) <= S(0)) { alf = 0; als = 1; }
-    else if ((aaf-cc) <= S(0)) { alf = 1; als = 0; }
-    else { alf = sqrt((cc-aas)/dfs); als = sqrt((aaf-cc)/dfs); }
+    const double deld = efix_d*(fabs(Ud)+fabs(Vd)+fabs(Wd)+afd);
+    const double dfsd = aafd-aasd;
+    double alfd, alsd;
+    if (dfsd < 1.0e-8) { alfd = 1; alsd = 0; }
+    else if ((ccd-aasd) <= 0.0) { alfd = 0; alsd = 1; }
+    else if ((aafd-ccd) <= 0.0) { alfd = 1; alsd = 0; }
+    else { alfd = sqrt((ccd-aasd)/dfsd); alsd = sqrt((aafd-ccd)/dfsd); }
+    const S Rho = S(Rhod), BX = S(BXd);
+    const S cc = S(ccd), c = S(cd), hyy = S(hyyd), hzz = S(hzzd), af = S(afd), as = S(asd);
+    const S alf = S(alfd), als = S(alsd);
+    const S lam[8] = {S(Ud), S(Ud), S(Ud+bd), S(Ud-bd), S(Ud+afd), S(Ud-afd), S(Ud+asd), S(Ud-asd)};
+    const S del = S(deld);
     const S sih = BX >= S(0) ? S(1) : S(-1);
     S du[8];
 #pragma unroll
@@ -324,14 +342,18 @@ HC_HD bool roe8WaveDissipation(S out[8], const double WL[8], const double WR[8],
     ra[5] = lu[1];
     ra[6] = zq*lu[2] + (-zq)*lu[3] + (als*fy)*lu[4] + (als*fy)*lu[5] + (-alf*fy)*lu[6] + (-alf*fy)*lu[7];
     ra[7] = (-yq)*lu[2] + yq*lu[3] + (als*fz)*lu[4] + (als*fz)*lu[5] + (-alf*fz)*lu[6] + (-alf*fz)*lu[7];
-    const S kk = S(kkd), hg = S(hgd);
-    ra[4] = ra[0]*kk+Rho*(U*ra[1]+V*ra[2]+W*ra[3])+ra[4]/hg+S(kInv4Pi)*(BX*ra[5]+BY*ra[6]+BZ*ra[7]);
-    ra[1] = U*ra[0]+Rho*ra[1];
-    ra[2] = V*ra[0]+Rho*ra[2];
-    ra[3] = W*ra[0]+Rho*ra[3];
+    // Back to conserved variables in FP64: the energy row sums kinetic,
+    // thermal and magnetic parts that cancel at low beta.
+    double rd[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) rd[q] = double(ra[q]);
+    rd[4] = rd[0]*kkd+Rhod*(Ud*rd[1]+Vd*rd[2]+Wd*rd[3])+rd[4]/hgd+kInv4Pi*(BXd*rd[5]+BYd*rd[6]+BZd*rd[7]);
+    rd[1] = Ud*rd[0]+Rhod*rd[1];
+    rd[2] = Vd*rd[0]+Rhod*rd[2];
+    rd[3] = Wd*rd[0]+Rhod*rd[3];
     bool finite = true;
 #pragma unroll
-    for (int q = 0; q < 8; ++q) { out[q] = -ra[q]; finite = finite && isfinite(out[q]); }
+    for (int q = 0; q < 8; ++q) { out[q] = S(-rd[q]); finite = finite && isfinite(rd[q]); }
     return finite;
 }
 

@@ -131,6 +131,13 @@ int hc_stage_rhs(hc_level* level, const double* W, double* K, double* U, double 
 int hc_rk4_advance(hc_level* level, double* U, const double* inner_ghosts, double dt,
                    int nsteps, hc_stream stream);
 
+/* hc_rk4_advance captures one RK4 step (state copy and the four stage
+ * preparations and RHS evaluations) into a hipGraph on its first call and
+ * replays it; the graph is re-captured when U, inner_ghosts or dt change and
+ * bypassed while per-kernel profiling is on. enable = 0 launches every kernel
+ * individually. No reference counterpart (Proto launches per box). */
+int hc_level_use_graph(hc_level* level, int enable);
+
 /* Fourth-order box operator (spatial_order 4, standard operator, limiter 0
  * or 1, riemann_solver 2) over a table of boxes, evaluated for every box in
  * the same launches. Replaces BoxOp_EulerCubedSphere::operator()

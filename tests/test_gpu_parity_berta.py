@@ -113,3 +113,24 @@ def test_berta_mixed_precision_error(device):
     errd = rel_linf_per_var(got - ju, ref - ju)
     print("berta mixed state err", err, "increment err", errd)
     assert err.max() <= 1e-6, err
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("operator", [1, 2])
+def test_mixed_precision_low_beta_cone(device, operator):
+    """Low-beta cone (p = 1e-4 |B|^2/2): mixed precision evaluates such faces'
+    Roe eigen-system in FP64, so the state stays within 1e-6 of the FP64 oracle."""
+    nr, na = 16, 16
+    shell = hc.Shell(nr, na, 21.5, R1)
+    opts = hc.Options(spatial_order=4, limiter=1, riemann_solver=2, precision=1, spatial_operator=operator)
+    ju, inner = hc.init_level4(shell, opts, 4, 5, 3)
+    O = orc.Oracle4(nr, na, r1=R1, spatial_operator=operator)
+    dt = 0.2 * (21.5 * np.pi / 2 / na) / 1.6
+    ref = O.rk4(ju, inner, dt, 1)
+    L = hc.Level(shell, opts)
+    U = _dev(ju)
+    L.rk4_advance(U, _dev(inner), dt, 1)
+    torch.cuda.synchronize()
+    err = rel_linf_per_var(U.cpu().numpy(), ref)
+    print("mixed low-beta state err", operator, err)
+    assert err.max() <= 1e-6, err
