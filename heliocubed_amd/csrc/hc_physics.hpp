@@ -9,6 +9,11 @@
 
 #define HC_HD __host__ __device__ __forceinline__
 
+// Plasma beta below which mixed precision evaluates the Roe eigen-system in FP64.
+#ifndef HC_MIXED_FP64_BETA
+#define HC_MIXED_FP64_BETA 0.5
+#endif
+
 namespace hc
 {
 constexpr double kRhoFloor = 1.67262192e-24/100.;      // RHO00 = c_MP/100
@@ -200,7 +205,8 @@ HC_HD bool roe8WaveDissipation(S out[8], const double WL[8], const double WR[8],
         // Below plasma beta 1/2 (of either side or of the average state) the
         // FP32 eigen-system loses digits to cancellation in the fast/slow
         // recombination: evaluate such faces in FP64.
-        if (ccd*Rhod < 0.5*gamma_d*BxBHd || WL[4] < 0.5*bbL || WR[4] < 0.5*bbR)
+        constexpr double betaFp64 = HC_MIXED_FP64_BETA;
+        if (ccd*Rhod < betaFp64*gamma_d*BxBHd || WL[4] < betaFp64*bbL || WR[4] < betaFp64*bbR)
         {
             double o[8];
             const bool ok = roe8WaveDissipation<double>(o, WL, WR, gamma_d, efix_d);
