@@ -123,7 +123,7 @@ __global__ void __launch_bounds__(kB) k_b_thraw(O4View v, double* __restrict__ S
     const BoxDesc& b = blockBoxDesc(v);
     int L[3], H[3], x[3];
     for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]-4; H[d] = b.hi[d]+4; }
-    if (!regionPoint(b, L, H, boxThread(b), x)) return;
+    if (!regionPoint(b, L, H, boxThread(v, b), x)) return;
     const BoxIdx I(b);
     auto eta1 = [](double qm, double q0, double qp) {
         const double num = fabs(qp - 2.0*q0 + qm);
@@ -155,7 +155,7 @@ __global__ void __launch_bounds__(kB) k_b_theta(O4View v, double* __restrict__ S
     const BoxDesc& b = blockBoxDesc(v);
     int L[3], H[3], x[3];
     for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]-kG; H[d] = b.hi[d]+kG; }
-    if (!regionPoint(b, L, H, boxThread(b), x)) return;
+    if (!regionPoint(b, L, H, boxThread(v, b), x)) return;
     const BoxIdx I(b);
     bool inner = true;
 #pragma unroll
@@ -178,7 +178,7 @@ __global__ void __launch_bounds__(kB) k_b_wp(O4View v, const double* __restrict_
     const BoxDesc& b = blockBoxDesc(v);
     int L[3], H[3], x[3];
     for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]-4; H[d] = b.hi[d]+4; }
-    if (!regionPoint(b, L, H, boxThread(b), x)) return;
+    if (!regionPoint(b, L, H, boxThread(v, b), x)) return;
     const BoxIdx I(b);
     auto ui = [&](int i, int j, int k) {
         return b.uOff + (k-b.lo[2]+kG)*b.us[1] + (j-b.lo[1]+kG)*b.us[0] + (i-b.lo[0]+kG); };
@@ -216,7 +216,7 @@ __global__ void __launch_bounds__(kB) k_b_faces(O4View v, double* __restrict__ S
     int L[3], H[3], x[3];
     for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]-2; H[d] = b.hi[d]+2; }
     L[D] = b.lo[D]; H[D] = b.hi[D]+1;
-    if (!regionPoint(b, L, H, boxThread(b), x)) return;
+    if (!regionPoint(b, L, H, boxThread(v, b), x)) return;
     const BoxIdx I(b);
     auto at = [&](int f, int o) {
         int y[3] = {x[0], x[1], x[2]}; y[D] += o;
@@ -324,7 +324,7 @@ __global__ void __launch_bounds__(kB) k_b_avg(O4View v, double* __restrict__ S)
     int L[3], H[3], x[3];
     for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]-1; H[d] = b.hi[d]+1; }
     L[D] = b.lo[D]; H[D] = b.hi[D]+1;
-    if (!regionPoint(b, L, H, boxThread(b), x)) return;
+    if (!regionPoint(b, L, H, boxThread(v, b), x)) return;
     const BoxIdx I(b);
     constexpr int t1 = D == 0 ? 1 : 0, t2 = D == 2 ? 1 : 2;
     const bool smooth = S[I(F_THF, x[0], x[1], x[2])] > 0.5;
@@ -358,7 +358,7 @@ __global__ void __launch_bounds__(kB) k_b_map(O4View v, double* __restrict__ S)
     for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]; H[d] = b.hi[d]; }
     H[D] = b.hi[D]+1;
     if (D != 0) { L[0] -= 1; H[0] += 1; }
-    if (!regionPoint(b, L, H, boxThread(b), x)) return;
+    if (!regionPoint(b, L, H, boxThread(v, b), x)) return;
     const BoxIdx I(b);
     constexpr int t1 = D == 0 ? 1 : 0, t2 = D == 2 ? 1 : 2;
     const FaceOps<D> op(v, b, I, S);
@@ -407,7 +407,7 @@ __global__ void __launch_bounds__(kB) k_b_flux(O4View v, double* __restrict__ S)
     int L[3], H[3], x[3];
     for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]; H[d] = b.hi[d]; }
     H[D] = b.hi[D]+1;
-    if (!regionPoint(b, L, H, boxThread(b), x)) return;
+    if (!regionPoint(b, L, H, boxThread(v, b), x)) return;
     const BoxIdx I(b);
     constexpr int t1 = D == 0 ? 1 : 0, t2 = D == 2 ? 1 : 2;
     const FaceOps<D> op(v, b, I, S);

@@ -84,6 +84,8 @@ struct O4View
     const double* fmet[3];   // [6][naE][naE][13]: detA, det(A) A, cofactor row
     const double* radial;    // per level [nr+13][2]: adjDr_0 (area0), adjDr_t (areaT) for i = -6 ..
     const int* blockBox;     // box of each block of the flattened launch
+    int blocksPerBox;        // > 0 when every box owns the same number of blocks (no table lookup)
+    unsigned nblocks;        // blocks of the flattened launch
     const SeamPair* seams;
     int nseams;
 };
@@ -106,10 +108,26 @@ __device__ __forceinline__ double radialPoint(const O4View& v, const BoxDesc& b,
 }
 
 // Box of this block and the thread's index in the box's launch region.
-__device__ __forceinline__ const BoxDesc& blockBoxDesc(const O4View& v) { return v.boxes[v.blockBox[blockIdx.x]]; }
-__device__ __forceinline__ long long boxThread(const BoxDesc& b)
+// XCD-aware block order: the hardware deals consecutive workgroups round-robin
+// to the 8 XCDs, each with its own L2. Logical block L of the flattened launch
+// is run by hardware block (L - start(x)) * 8 + x of XCD x, where XCD x owns
+// the contiguous logical range starting at start(x); neighbouring tiles, which
+// share stencil data, then run on the same XCD and hit in its L2.
+constexpr unsigned kXcd = 8;
+__device__ __forceinline__ unsigned logicalBlock(const O4View& v)
 {
-    return (long long)(blockIdx.x - b.block0)*blockDim.x + threadIdx.x;
+    const unsigned b = blockIdx.x, x = b % kXcd, q = v.nblocks / kXcd, r = v.nblocks % kXcd;
+    return x*q + (x < r ? x : r) + b / kXcd;
+}
+__device__ __forceinline__ int blockBoxIndex(const O4View& v)
+{
+    const unsigned L = logicalBlock(v);
+    return v.blocksPerBox > 0 ? (int)(L / (unsigned)v.blocksPerBox) : v.blockBox[L];
+}
+__device__ __forceinline__ const BoxDesc& blockBoxDesc(const O4View& v) { return v.boxes[blockBoxIndex(v)]; }
+__device__ __forceinline__ long long boxThread(const O4View& v, const BoxDesc& b)
+{
+    return (long long)(logicalBlock(v) - b.block0)*blockDim.x + threadIdx.x;
 }
 
 struct BoxIdx
@@ -400,7 +418,7 @@ __global__ void __launch_bounds__(kB) k_o4_wbar(O4View v, const double* __restri
     const BoxDesc& b = blockBoxDesc(v);
     int L[3], H[3], x[3];
     for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]-kG; H[d] = b.hi[d]+kG; }
-    if (!regionPoint(b, L, H, boxThread(b), x)) return;
+    if (!regionPoint(b, L, H, boxThread(v, b), x)) return;
     const BoxIdx I(b);
     const long long u = b.uOff + (x[2]-L[2])*b.us[1] + (x[1]-L[1])*b.us[0] + (x[0]-L[0]);
     double uu[8], w[8];
@@ -423,7 +441,7 @@ __global__ void __launch_bounds__(kB) k_o4_w(O4View v, const double* __restrict_
     const BoxDesc& b = blockBoxDesc(v);
     int L[3], H[3], x[3];
     for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]-4; H[d] = b.hi[d]+4; }
-    if (!regionPoint(b, L, H, boxThread(b), x)) return;
+    if (!regionPoint(b, L, H, boxThread(v, b), x)) return;
     const BoxIdx I(b);
     auto ui = [&](int i, int j, int k) {
         return b.uOff + (k-b.lo[2]+kG)*b.us[1] + (j-b.lo[1]+kG)*b.us[0] + (i-b.lo[0]+kG); };
@@ -470,7 +488,7 @@ __global__ void __launch_bounds__(kB) k_o4_eta(O4View v, double* __restrict__ S)
     const BoxDesc& b = blockBoxDesc(v);
     int L[3], H[3], x[3];
     for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]-2; H[d] = b.hi[d]+2; }
-    if (!regionPoint(b, L, H, boxThread(b), x)) return;
+    if (!regionPoint(b, L, H, boxThread(v, b), x)) return;
     const BoxIdx I(b);
     double e = 0;
     for (int dd = 0; dd < 3; ++dd)
@@ -514,7 +532,7 @@ __global__ void __launch_bounds__(kB) k_o4_faces(O4View v, double* __restrict__ 
     int L[3], H[3], x[3];
     for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]-2; H[d] = b.hi[d]+2; }
     L[D] = b.lo[D]; H[D] = b.hi[D]+1;
-    if (!regionPoint(b, L, H, boxThread(b), x)) return;
+    if (!regionPoint(b, L, H, boxThread(v, b), x)) return;
     const BoxIdx I(b);
     constexpr int t1 = D == 0 ? 1 : 0, t2 = D == 2 ? 1 : 2;
     auto Wat = [&](int c, int o) {
@@ -626,7 +644,7 @@ __global__ void __launch_bounds__(kB) k_o4_bar(O4View v, double* __restrict__ S)
     int L[3], H[3], x[3];
     for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]-3; H[d] = b.hi[d]+3; }
     L[D] = b.lo[D]; H[D] = b.hi[D]+1;
-    if (!regionPoint(b, L, H, boxThread(b), x)) return;
+    if (!regionPoint(b, L, H, boxThread(v, b), x)) return;
     const BoxIdx I(b);
     double f[8], me, p;
     barFlux<D>(v, b, I, S, x[0], x[1], x[2], f, me, p);
@@ -646,7 +664,7 @@ __global__ void __launch_bounds__(kB) k_o4_phys(O4View v, double* __restrict__ S
     int L[3], H[3], x[3];
     for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]-2; H[d] = b.hi[d]+2; }
     L[D] = b.lo[D]; H[D] = b.hi[D]+1;
-    if (!regionPoint(b, L, H, boxThread(b), x)) return;
+    if (!regionPoint(b, L, H, boxThread(v, b), x)) return;
     const BoxIdx I(b);
     constexpr int t1 = D == 0 ? 1 : 0, t2 = D == 2 ? 1 : 2;
     // transverse Laplacian of the face-average flux and magnetic energy
@@ -691,7 +709,7 @@ __global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(2))) k_
     int L[3], H[3], x[3];
     for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]-1; H[d] = b.hi[d]+1; }
     L[D] = b.lo[D]; H[D] = b.hi[D]+1;
-    if (!regionPoint(b, L, H, boxThread(b), x)) return;
+    if (!regionPoint(b, L, H, boxThread(v, b), x)) return;
     const BoxIdx I(b);
     const double* A = v.faceA[D] + 9*angIdx(b, x[1], x[2]);
     double wl[8], wr[8], out[8];
@@ -713,7 +731,7 @@ __global__ void __launch_bounds__(kB) k_o4_rusq(O4View v, double* __restrict__ S
     int L[3], H[3], x[3];
     for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]-1; H[d] = b.hi[d]+1; }
     L[D] = b.lo[D]; H[D] = b.hi[D]+1;
-    if (!regionPoint(b, L, H, boxThread(b), x)) return;
+    if (!regionPoint(b, L, H, boxThread(v, b), x)) return;
     const BoxIdx I(b);
     const double* A = v.faceA[D] + 9*angIdx(b, x[1], x[2]);
     const double rp = radialPoint(v, b, D, x[0]);
@@ -811,7 +829,7 @@ __global__ void __launch_bounds__(kB) k_o4_map(O4View v, double* __restrict__ S)
     for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]; H[d] = b.hi[d]; }
     H[D] = b.hi[D]+1;
     if (D != 0) { L[0] -= 1; H[0] += 1; }
-    if (!regionPoint(b, L, H, boxThread(b), x)) return;
+    if (!regionPoint(b, L, H, boxThread(v, b), x)) return;
     const BoxIdx I(b);
     const FaceOps<D> op(v, b, I, S);
     double out[8];
@@ -834,7 +852,7 @@ __global__ void __launch_bounds__(kB) k_o4_flux(O4View v, double* __restrict__ S
     int L[3], H[3], x[3];
     for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]; H[d] = b.hi[d]; }
     H[D] = b.hi[D]+1;
-    if (!regionPoint(b, L, H, boxThread(b), x)) return;
+    if (!regionPoint(b, L, H, boxThread(v, b), x)) return;
     const BoxIdx I(b);
     constexpr int t1 = D == 0 ? 1 : 0, t2 = D == 2 ? 1 : 2;
     const FaceOps<D> op(v, b, I, S);
@@ -991,7 +1009,7 @@ __global__ void __launch_bounds__(kB) k_o4_div(O4View v, const double* __restric
 {
     const BoxDesc& b = blockBoxDesc(v);
     int x[3];
-    if (!regionPoint(b, b.lo, b.hi, boxThread(b), x)) return;
+    if (!regionPoint(b, b.lo, b.hi, boxThread(v, b), x)) return;
     const BoxIdx I(b);
     const long long r = b.rOff + (x[2]-b.lo[2])*b.rs[1] + (x[1]-b.lo[1])*b.rs[0] + (x[0]-b.lo[0]);
     for (int c = 0; c < 8; ++c)
@@ -1019,11 +1037,11 @@ __global__ void __launch_bounds__(kB) k_o4_fluxout(O4View v, int D, const double
     const BoxDesc& b = blockBoxDesc(v);
     int L[3] = {b.lo[0], b.lo[1], b.lo[2]}, H[3] = {b.hi[0], b.hi[1], b.hi[2]}, x[3];
     H[D] += 1;
-    if (!regionPoint(b, L, H, boxThread(b), x)) return;
+    if (!regionPoint(b, L, H, boxThread(v, b), x)) return;
     const BoxIdx I(b);
     const int m0 = b.n[0]+(D == 0), m1 = b.n[1]+(D == 1), m2 = b.n[2]+(D == 2);
     const long long n = (long long)m0*m1*m2;
-    const long long o = fOff[v.blockBox[blockIdx.x]] + ((long long)(x[2]-b.lo[2])*m1 + (x[1]-b.lo[1]))*m0 + (x[0]-b.lo[0]);
+    const long long o = fOff[blockBoxIndex(v)] + ((long long)(x[2]-b.lo[2])*m1 + (x[1]-b.lo[1]))*m0 + (x[0]-b.lo[0]);
     for (int c = 0; c < 8; ++c) F[o + c*n] = S[I(F_FLUX+8*D+c, x[0], x[1], x[2])];
 }
 
@@ -1142,7 +1160,12 @@ void buildBoxop(hc_boxop& B, const hc_box* boxes, const int* boxLevel, int nbox)
         B.boxes.push_back(bd);
     }
     if (blk > 0x7fffffffLL) throw std::runtime_error("box table too large for one launch");
+    // equal block counts for every box: the box of a block is blockIdx.x / blocksPerBox
+    v.blocksPerBox = (nbox > 0 && blk % nbox == 0) ? (int)(blk/nbox) : 0;
+    for (int n = 0; n < nbox && v.blocksPerBox > 0; ++n)
+        if (B.boxes[n].block0 != (long long)n*v.blocksPerBox) v.blocksPerBox = 0;
     B.totalBlocks = blk;
+    v.nblocks = (unsigned)blk;
     B.uCount = u; B.rCount = r;
     for (int d = 0; d < 3; ++d) { B.fCount[d] = f[d]; B.fOff[d] = upload(B.fOffHost[d], B.owned); }
     v.boxes = upload(B.boxes, B.owned);
