@@ -74,6 +74,22 @@ ALGO_BYTES = {
 }
 
 
+def pmc_summary(kernel, workload):
+    """HBM traffic (bytes) and FP64 work (flop) per launch of `kernel` on
+    `workload` from the committed rocprofv3 PMC summary (FETCH_SIZE and
+    WRITE_SIZE scaled by the copy-kernel calibration, FP64 VALU instruction
+    counts), or (None, None) when no summary exists for the pair."""
+    import re
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_pmc.txt")
+    if not os.path.exists(path):
+        return None, None, None
+    txt = open(path).read()
+    t = re.search(r"HBM traffic per stage: ([0-9.]+) GB", txt)
+    f = re.search(r"FP64 work per stage: ([0-9.]+) GFLOP", txt)
+    return (float(t.group(1)) * 1e9 if t else None), (float(f.group(1)) * 1e9 if f else None), \
+        os.path.relpath(path, ROOT)
+
+
 def units_per_launch(name, nr, na):
     if name == "k_face_flux_0":
         return 6 * (nr + 1) * na * na
@@ -244,6 +260,7 @@ def main():
     avg_s = ms / launches / 1e3
     algo = ALGO_BYTES[dominant] * units_per_launch(dominant, wl["n_rad"], wl["n_ang"])
     achieved = algo / avg_s / 1e9
+    traffic, flop, pmc_path = pmc_summary(dominant, args.workload) if args.precision == "fp64" else (None, None, None)
     line = {
         "metric": f"cell-updates/s per RK stage ({args.precision})",
         "value": value,
@@ -267,11 +284,17 @@ def main():
                    "limiter": wl["limiter"], "riemann_solver": wl["riemann"], "precision": args.precision,
                    "boxes": "32^3 box table" if wl["order"] == 4 else "panel arrays"},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "avg_launch_us": avg_s * 1e6},
         "kernels_ms_per_step": {k: v[0] / args.steps for k, v in prof.items()},
         "graph_ms_per_step": graph_ms,
     }
+    if traffic is not None:
+        line["roofline"]["traffic_source"] = pmc_path
+        line["roofline"]["traffic_hbm_gbs"] = traffic / avg_s / 1e9
+    if flop is not None:
+        line["valu_fp64"] = {"achieved": flop / avg_s / 1e12, "peak": 78.6, "unit": "TFLOP/s",
+                             "frac": flop / avg_s / 1e12 / 78.6, "flop_per_launch": flop, "source": pmc_path}
     if mixed_err is not None:
         line["mixed_vs_fp64_rel_linf"] = mixed_err
     if not args.no_cpu_baseline:

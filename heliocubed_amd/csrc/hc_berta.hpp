@@ -125,12 +125,13 @@ __global__ void __launch_bounds__(kB) k_b_thraw(O4View v, double* __restrict__ S
     for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]-4; H[d] = b.hi[d]+4; }
     if (!regionPoint(b, L, H, boxThread(v, b), x)) return;
     const BoxIdx I(b);
+    double* const P = S + b.sOff;
     auto eta1 = [](double qm, double q0, double qp) {
         const double num = fabs(qp - 2.0*q0 + qm);
         const double den = fabs(qp) + 2.0*fabs(q0) + fabs(qm) + 1.0e-40;
         return num/den; };
     auto magp = [&](int i, int j, int k) {
-        const double bx = S[I(F_WA+5, i, j, k)], by = S[I(F_WA+6, i, j, k)], bz = S[I(F_WA+7, i, j, k)];
+        const double bx = P[I(F_WA+5, i, j, k)], by = P[I(F_WA+6, i, j, k)], bz = P[I(F_WA+7, i, j, k)];
         return 0.5*(bx*bx + by*by + bz*bz); };
     double et[3];
 #pragma unroll
@@ -138,14 +139,14 @@ __global__ void __launch_bounds__(kB) k_b_thraw(O4View v, double* __restrict__ S
     {
         const int m[3] = {x[0]-(dd == 0), x[1]-(dd == 1), x[2]-(dd == 2)};
         const int p[3] = {x[0]+(dd == 0), x[1]+(dd == 1), x[2]+(dd == 2)};
-        const double er = eta1(S[I(F_WA, m[0], m[1], m[2])], S[I(F_WA, x[0], x[1], x[2])], S[I(F_WA, p[0], p[1], p[2])]);
-        const double ep = eta1(S[I(F_WA+4, m[0], m[1], m[2])], S[I(F_WA+4, x[0], x[1], x[2])],
-                               S[I(F_WA+4, p[0], p[1], p[2])]);
+        const double er = eta1(P[I(F_WA, m[0], m[1], m[2])], P[I(F_WA, x[0], x[1], x[2])], P[I(F_WA, p[0], p[1], p[2])]);
+        const double ep = eta1(P[I(F_WA+4, m[0], m[1], m[2])], P[I(F_WA+4, x[0], x[1], x[2])],
+                               P[I(F_WA+4, p[0], p[1], p[2])]);
         const double em = eta1(magp(m[0], m[1], m[2]), magp(x[0], x[1], x[2]), magp(p[0], p[1], p[2]));
         et[dd] = refMax(er, refMax(ep, em));
     }
     const double eta = sqrt(et[0]*et[0] + et[1]*et[1] + et[2]*et[2]);
-    S[I(F_THR, x[0], x[1], x[2])] = eta < v.etaD ? 1.0 : 0.0;
+    P[I(F_THR, x[0], x[1], x[2])] = eta < v.etaD ? 1.0 : 0.0;
 }
 
 // 2. Dilated flag on box+5: the minimum over the cell and its six neighbours
@@ -157,18 +158,19 @@ __global__ void __launch_bounds__(kB) k_b_theta(O4View v, double* __restrict__ S
     for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]-kG; H[d] = b.hi[d]+kG; }
     if (!regionPoint(b, L, H, boxThread(v, b), x)) return;
     const BoxIdx I(b);
+    double* const P = S + b.sOff;
     bool inner = true;
 #pragma unroll
     for (int d = 0; d < 3; ++d) inner = inner && x[d] >= b.lo[d]-3 && x[d] <= b.hi[d]+3;
     double t = 0.0;
     if (inner)
     {
-        t = S[I(F_THR, x[0], x[1], x[2])];
-        t = refMin(t, S[I(F_THR, x[0]-1, x[1], x[2])]); t = refMin(t, S[I(F_THR, x[0]+1, x[1], x[2])]);
-        t = refMin(t, S[I(F_THR, x[0], x[1]-1, x[2])]); t = refMin(t, S[I(F_THR, x[0], x[1]+1, x[2])]);
-        t = refMin(t, S[I(F_THR, x[0], x[1], x[2]-1)]); t = refMin(t, S[I(F_THR, x[0], x[1], x[2]+1)]);
+        t = P[I(F_THR, x[0], x[1], x[2])];
+        t = refMin(t, P[I(F_THR, x[0]-1, x[1], x[2])]); t = refMin(t, P[I(F_THR, x[0]+1, x[1], x[2])]);
+        t = refMin(t, P[I(F_THR, x[0], x[1]-1, x[2])]); t = refMin(t, P[I(F_THR, x[0], x[1]+1, x[2])]);
+        t = refMin(t, P[I(F_THR, x[0], x[1], x[2]-1)]); t = refMin(t, P[I(F_THR, x[0], x[1], x[2]+1)]);
     }
-    S[I(F_THETA, x[0], x[1], x[2])] = t;
+    P[I(F_THETA, x[0], x[1], x[2])] = t;
 }
 
 // 3. Point primitives on box+4: UPoint = <U> - theta Lap<U>/24, V(UPoint)
@@ -180,9 +182,10 @@ __global__ void __launch_bounds__(kB) k_b_wp(O4View v, const double* __restrict_
     for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]-4; H[d] = b.hi[d]+4; }
     if (!regionPoint(b, L, H, boxThread(v, b), x)) return;
     const BoxIdx I(b);
+    double* const P = S + b.sOff;
     auto ui = [&](int i, int j, int k) {
         return b.uOff + (k-b.lo[2]+kG)*b.us[1] + (j-b.lo[1]+kG)*b.us[0] + (i-b.lo[0]+kG); };
-    const bool smooth = S[I(F_THETA, x[0], x[1], x[2])] > 0.5;
+    const bool smooth = P[I(F_THETA, x[0], x[1], x[2])] > 0.5;
     double up[8];
 #pragma unroll
     for (int c = 0; c < 8; ++c)
@@ -201,7 +204,7 @@ __global__ void __launch_bounds__(kB) k_b_wp(O4View v, const double* __restrict_
     w[4] = (w[4]-ke-me)*(v.gamma-1.0);
     if (w[4] < 0.0) w[4] = 0.0;
 #pragma unroll
-    for (int c = 0; c < 8; ++c) S[I(F_WPT+c, x[0], x[1], x[2])] = w[c];
+    for (int c = 0; c < 8; ++c) P[I(F_WPT+c, x[0], x[1], x[2])] = w[c];
 }
 
 // 4. Face points of direction D on the faces [lo_d, hi_d+1] x transverse
@@ -218,9 +221,10 @@ __global__ void __launch_bounds__(kB) k_b_faces(O4View v, double* __restrict__ S
     L[D] = b.lo[D]; H[D] = b.hi[D]+1;
     if (!regionPoint(b, L, H, boxThread(v, b), x)) return;
     const BoxIdx I(b);
+    double* const P = S + b.sOff;
     auto at = [&](int f, int o) {
         int y[3] = {x[0], x[1], x[2]}; y[D] += o;
-        return S[I(f, y[0], y[1], y[2])]; };
+        return P[I(f, y[0], y[1], y[2])]; };
     double thF = refMin(at(F_THETA, -1), at(F_THETA, 0));
     double tr = at(F_THETA, -3);
     tr = refMin(tr, at(F_THETA, -2)); tr = refMin(tr, at(F_THETA, -1)); tr = refMin(tr, at(F_THETA, 0));
@@ -292,26 +296,26 @@ __global__ void __launch_bounds__(kB) k_b_faces(O4View v, double* __restrict__ S
 #pragma unroll
     for (int c = 0; c < 8; ++c)
     {
-        S[I(F_BWL+c, x[0], x[1], x[2])] = l[c];
-        S[I(F_BWR+c, x[0], x[1], x[2])] = r[c];
-        S[I(F_FPS+c, x[0], x[1], x[2])] = fl[c] + fr[c];
+        P[I(F_BWL+c, x[0], x[1], x[2])] = l[c];
+        P[I(F_BWR+c, x[0], x[1], x[2])] = r[c];
+        P[I(F_FPS+c, x[0], x[1], x[2])] = fl[c] + fr[c];
     }
-    S[I(F_THF, x[0], x[1], x[2])] = thF;
-    S[I(F_PTOT, x[0], x[1], x[2])] = (l[4] + meL) + (r[4] + meR);
+    P[I(F_THF, x[0], x[1], x[2])] = thF;
+    P[I(F_PTOT, x[0], x[1], x[2])] = (l[4] + meL) + (r[4] + meR);
     if (!v.roe)
     {
         // Rusanov: pointwise conserved jump R - L times the radial point area
         // and the speed of the point states (:1589-1613, :1653-1725)
         const double rp = radialPoint(v, b, D, x[0]);
-        S[I(F_DPT, x[0], x[1], x[2])] = rp*r[0] - rp*l[0];
+        P[I(F_DPT, x[0], x[1], x[2])] = rp*r[0] - rp*l[0];
 #pragma unroll
         for (int a = 0; a < 3; ++a)
         {
-            S[I(F_DPT+1+a, x[0], x[1], x[2])] = rp*r[0]*r[1+a] - rp*l[0]*l[1+a];
-            S[I(F_DPT+5+a, x[0], x[1], x[2])] = rp*r[5+a] - rp*l[5+a];
+            P[I(F_DPT+1+a, x[0], x[1], x[2])] = rp*r[0]*r[1+a] - rp*l[0]*l[1+a];
+            P[I(F_DPT+5+a, x[0], x[1], x[2])] = rp*r[5+a] - rp*l[5+a];
         }
-        S[I(F_DPT+4, x[0], x[1], x[2])] = rp*(keR + meR + r[4]/(v.gamma-1.0)) - rp*(keL + meL + l[4]/(v.gamma-1.0));
-        S[I(F_SPD, x[0], x[1], x[2])] = rusanovSpeedO4(l, r, A, v.gamma, D, true);
+        P[I(F_DPT+4, x[0], x[1], x[2])] = rp*(keR + meR + r[4]/(v.gamma-1.0)) - rp*(keL + meL + l[4]/(v.gamma-1.0));
+        P[I(F_SPD, x[0], x[1], x[2])] = rusanovSpeedO4(l, r, A, v.gamma, D, true);
     }
 }
 
@@ -326,10 +330,11 @@ __global__ void __launch_bounds__(kB) k_b_avg(O4View v, double* __restrict__ S)
     L[D] = b.lo[D]; H[D] = b.hi[D]+1;
     if (!regionPoint(b, L, H, boxThread(v, b), x)) return;
     const BoxIdx I(b);
+    double* const P = S + b.sOff;
     constexpr int t1 = D == 0 ? 1 : 0, t2 = D == 2 ? 1 : 2;
-    const bool smooth = S[I(F_THF, x[0], x[1], x[2])] > 0.5;
+    const bool smooth = P[I(F_THF, x[0], x[1], x[2])] > 0.5;
     auto avg = [&](int f) {
-        const double c0 = S[I(f, x[0], x[1], x[2])];
+        const double c0 = P[I(f, x[0], x[1], x[2])];
         double lap = 0;
 #pragma unroll
         for (int t = 0; t < 2; ++t)
@@ -337,12 +342,12 @@ __global__ void __launch_bounds__(kB) k_b_avg(O4View v, double* __restrict__ S)
             const int tt = t == 0 ? t1 : t2;
             int y[3] = {x[0], x[1], x[2]}, z[3] = {x[0], x[1], x[2]};
             y[tt] += 1; z[tt] -= 1;
-            lap += S[I(f, y[0], y[1], y[2])] - 2.0*c0 + S[I(f, z[0], z[1], z[2])];
+            lap += P[I(f, y[0], y[1], y[2])] - 2.0*c0 + P[I(f, z[0], z[1], z[2])];
         }
         return smooth ? c0 + lap/24.0 : c0; };
 #pragma unroll
-    for (int c = 0; c < 8; ++c) S[I(F_PFA+c, x[0], x[1], x[2])] = avg(F_FPS+c);
-    S[I(F_PAVG, x[0], x[1], x[2])] = avg(F_PTOT);
+    for (int c = 0; c < 8; ++c) P[I(F_PFA+c, x[0], x[1], x[2])] = avg(F_FPS+c);
+    P[I(F_PAVG, x[0], x[1], x[2])] = avg(F_PTOT);
 }
 
 // 6. Mapped products of the averaged flux (bertaMapHostBox, :35-98): per
@@ -360,8 +365,9 @@ __global__ void __launch_bounds__(kB) k_b_map(O4View v, double* __restrict__ S)
     if (D != 0) { L[0] -= 1; H[0] += 1; }
     if (!regionPoint(b, L, H, boxThread(v, b), x)) return;
     const BoxIdx I(b);
+    double* const P = S + b.sOff;
     constexpr int t1 = D == 0 ? 1 : 0, t2 = D == 2 ? 1 : 2;
-    const FaceOps<D> op(v, b, I, S);
+    const FaceOps<D> op(v, b, I, P);
 #pragma unroll
     for (int s = 0; s < 2; ++s)
     {
@@ -372,7 +378,7 @@ __global__ void __launch_bounds__(kB) k_b_map(O4View v, double* __restrict__ S)
         term += op.M(0, x)*op.Fv(F_PFA+c, x)*1.0;
         double sum = 0;
         sum += term;
-        S[I(F_BMP+c, x[0], x[1], x[2])] = sum;
+        P[I(F_BMP+c, x[0], x[1], x[2])] = sum;
     }
 #pragma unroll
     for (int grp = 0; grp < 2; ++grp)
@@ -391,7 +397,7 @@ __global__ void __launch_bounds__(kB) k_b_map(O4View v, double* __restrict__ S)
                 term += op.M(1+3*r+col, x)*op.Fv(F_PFA+c0+col, x)*1.0;
                 sum += term;
             }
-            S[I(F_BMP+c0+r, x[0], x[1], x[2])] = sum;
+            P[I(F_BMP+c0+r, x[0], x[1], x[2])] = sum;
         }
     }
 }
@@ -409,8 +415,9 @@ __global__ void __launch_bounds__(kB) k_b_flux(O4View v, double* __restrict__ S)
     H[D] = b.hi[D]+1;
     if (!regionPoint(b, L, H, boxThread(v, b), x)) return;
     const BoxIdx I(b);
+    double* const P = S + b.sOff;
     constexpr int t1 = D == 0 ? 1 : 0, t2 = D == 2 ? 1 : 2;
-    const FaceOps<D> op(v, b, I, S);
+    const FaceOps<D> op(v, b, I, P);
     auto Fv = [&](int f, const int y[3]) { return op.Fv(f, y); };
     const double adjX = areaOf(v, b, D, x[0]);
     auto dAdj = [&](int i) { return 0.5*areaOf(v, b, D, i+1) - 0.5*areaOf(v, b, D, i-1); };
@@ -515,7 +522,7 @@ __global__ void __launch_bounds__(kB) k_b_flux(O4View v, double* __restrict__ S)
     }
 #pragma unroll
     for (int c = 0; c < 8; ++c)
-        S[I(F_FLUX+8*D+c, x[0], x[1], x[2])] = 0.5*(Fsum[c]-Fd[c]);
+        P[I(F_FLUX+8*D+c, x[0], x[1], x[2])] = 0.5*(Fsum[c]-Fd[c]);
 }
 
 template <int D, typename S_>

@@ -130,19 +130,18 @@ __device__ __forceinline__ long long boxThread(const O4View& v, const BoxDesc& b
     return (long long)(logicalBlock(v) - b.block0)*blockDim.x + threadIdx.x;
 }
 
+// Offset (in doubles, 32-bit) of field f at cell (i,j,k) inside a box's
+// scratch block, which starts at S + b.sOff; buildBoxop checks that every box's
+// fields fit in 32-bit offsets.
 struct BoxIdx
 {
-    int lo[3], e[3];
-    long long base, ecount;
+    int o0, o1, o2, e0, e1, ecount;   // o_d = lo_d - 5: the box grown by the ghost width
     __device__ BoxIdx(const BoxDesc& b)
+        : o0(b.lo[0]-kG), o1(b.lo[1]-kG), o2(b.lo[2]-kG), e0(b.n[0]+2*kG), e1(b.n[1]+2*kG),
+          ecount((b.n[0]+2*kG)*(b.n[1]+2*kG)*(b.n[2]+2*kG)) {}
+    __device__ int operator()(int f, int i, int j, int k) const
     {
-        for (int d = 0; d < 3; ++d) { lo[d] = b.lo[d]; e[d] = b.n[d] + 2*kG; }
-        base = b.sOff;
-        ecount = (long long)e[0]*e[1]*e[2];
-    }
-    __device__ long long operator()(int f, int i, int j, int k) const
-    {
-        return base + f*ecount + ((long long)(k-lo[2]+kG)*e[1] + (j-lo[1]+kG))*e[0] + (i-lo[0]+kG);
+        return f*ecount + ((k-o2)*e1 + (j-o1))*e0 + (i-o0);
     }
 };
 
@@ -420,6 +419,7 @@ __global__ void __launch_bounds__(kB) k_o4_wbar(O4View v, const double* __restri
     for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]-kG; H[d] = b.hi[d]+kG; }
     if (!regionPoint(b, L, H, boxThread(v, b), x)) return;
     const BoxIdx I(b);
+    double* const P = S + b.sOff;
     const long long u = b.uOff + (x[2]-L[2])*b.us[1] + (x[1]-L[1])*b.us[0] + (x[0]-L[0]);
     double uu[8], w[8];
 #pragma unroll
@@ -430,8 +430,8 @@ __global__ void __launch_bounds__(kB) k_o4_wbar(O4View v, const double* __restri
     w[4] = (w[4]-ke-me)*(v.gamma-1.0);
     if (w[4] < 0) w[4] = 0.0;
 #pragma unroll
-    for (int c = 0; c < 8; ++c) S[I(F_WB+c, x[0], x[1], x[2])] = w[c];
-    S[I(F_MEB, x[0], x[1], x[2])] = me;
+    for (int c = 0; c < 8; ++c) P[I(F_WB+c, x[0], x[1], x[2])] = w[c];
+    P[I(F_MEB, x[0], x[1], x[2])] = me;
 }
 
 // 2. Point primitives W on box+4 with the complete-state fallbacks,
@@ -443,6 +443,7 @@ __global__ void __launch_bounds__(kB) k_o4_w(O4View v, const double* __restrict_
     for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]-4; H[d] = b.hi[d]+4; }
     if (!regionPoint(b, L, H, boxThread(v, b), x)) return;
     const BoxIdx I(b);
+    double* const P = S + b.sOff;
     auto ui = [&](int i, int j, int k) {
         return b.uOff + (k-b.lo[2]+kG)*b.us[1] + (j-b.lo[1]+kG)*b.us[0] + (i-b.lo[0]+kG); };
     double up[8], uc[8];
@@ -466,7 +467,7 @@ __global__ void __launch_bounds__(kB) k_o4_w(O4View v, const double* __restrict_
     w[4] = (w[4]-ke-me)*(v.gamma-1.0);
     double wb[8];
 #pragma unroll
-    for (int c = 0; c < 8; ++c) wb[c] = S[I(F_WB+c, x[0], x[1], x[2])];
+    for (int c = 0; c < 8; ++c) wb[c] = P[I(F_WB+c, x[0], x[1], x[2])];
     if (!usablePrimitive(w) && usablePrimitive(wb))
 #pragma unroll
         for (int c = 0; c < 8; ++c) w[c] = wb[c];
@@ -474,10 +475,10 @@ __global__ void __launch_bounds__(kB) k_o4_w(O4View v, const double* __restrict_
     for (int c = 0; c < 8; ++c)
     {
         const int f = F_WB+c;
-        const double lap = S[I(f,x[0]+1,x[1],x[2])]+S[I(f,x[0]-1,x[1],x[2])]+S[I(f,x[0],x[1]+1,x[2])]
-                          +S[I(f,x[0],x[1]-1,x[2])]+S[I(f,x[0],x[1],x[2]+1)]+S[I(f,x[0],x[1],x[2]-1)]
-                          -6.0*S[I(f,x[0],x[1],x[2])];
-        S[I(F_W+c, x[0], x[1], x[2])] = w[c]+lap/24.0;
+        const double lap = P[I(f,x[0]+1,x[1],x[2])]+P[I(f,x[0]-1,x[1],x[2])]+P[I(f,x[0],x[1]+1,x[2])]
+                          +P[I(f,x[0],x[1]-1,x[2])]+P[I(f,x[0],x[1],x[2]+1)]+P[I(f,x[0],x[1],x[2]-1)]
+                          -6.0*P[I(f,x[0],x[1],x[2])];
+        P[I(F_W+c, x[0], x[1], x[2])] = w[c]+lap/24.0;
     }
 }
 
@@ -490,6 +491,7 @@ __global__ void __launch_bounds__(kB) k_o4_eta(O4View v, double* __restrict__ S)
     for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]-2; H[d] = b.hi[d]+2; }
     if (!regionPoint(b, L, H, boxThread(v, b), x)) return;
     const BoxIdx I(b);
+    double* const P = S + b.sOff;
     double e = 0;
     for (int dd = 0; dd < 3; ++dd)
     {
@@ -504,23 +506,23 @@ __global__ void __launch_bounds__(kB) k_o4_eta(O4View v, double* __restrict__ S)
             {
                 int z[3] = {y[0], y[1], y[2]};
                 z[dd] += q;
-                pr[q+2] = S[I(F_WB+4, z[0], z[1], z[2])] + S[I(F_MEB, z[0], z[1], z[2])];
-                vel[q+2] = S[I(F_WB+1+dd, z[0], z[1], z[2])];
+                pr[q+2] = P[I(F_WB+4, z[0], z[1], z[2])] + P[I(F_MEB, z[0], z[1], z[2])];
+                vel[q+2] = P[I(F_WB+1+dd, z[0], z[1], z[2])];
             }
             const double et = etaTilde(&pr[0], &pr[1], &pr[3], &pr[4], vel[1], vel[3]);
             ed = o == 0 ? et : fmin(ed, et);
         }
         e = dd == 0 ? ed : fmin(ed, e);
     }
-    S[I(F_ETA, x[0], x[1], x[2])] = e;
+    P[I(F_ETA, x[0], x[1], x[2])] = e;
 }
 
 // Face-average state WFaceBar at face x (direction d): (WBar_{x-e} + WBar_x)/2.
-__device__ __forceinline__ void faceBar(const BoxIdx& I, const double* S, int d, int i, int j, int k, double wf[8])
+__device__ __forceinline__ void faceBar(const BoxIdx& I, const double* P, int d, int i, int j, int k, double wf[8])
 {
     const int a = i-(d == 0), b = j-(d == 1), c = k-(d == 2);
 #pragma unroll
-    for (int q = 0; q < 8; ++q) wf[q] = 0.5*S[I(F_WB+q, a, b, c)] + 0.5*S[I(F_WB+q, i, j, k)];
+    for (int q = 0; q < 8; ++q) wf[q] = 0.5*P[I(F_WB+q, a, b, c)] + 0.5*P[I(F_WB+q, i, j, k)];
 }
 
 // 4. Limited, flattened and repaired face states; point states by transverse
@@ -534,10 +536,11 @@ __global__ void __launch_bounds__(kB) k_o4_faces(O4View v, double* __restrict__ 
     L[D] = b.lo[D]; H[D] = b.hi[D]+1;
     if (!regionPoint(b, L, H, boxThread(v, b), x)) return;
     const BoxIdx I(b);
+    double* const P = S + b.sOff;
     constexpr int t1 = D == 0 ? 1 : 0, t2 = D == 2 ? 1 : 2;
     auto Wat = [&](int c, int o) {
         int y[3] = {x[0], x[1], x[2]}; y[D] += o;
-        return S[I(F_W+c, y[0], y[1], y[2])]; };
+        return P[I(F_W+c, y[0], y[1], y[2])]; };
     double lf[8], rf[8];
 #pragma unroll 1
     for (int c = 0; c < 8; ++c)
@@ -561,7 +564,7 @@ __global__ void __launch_bounds__(kB) k_o4_faces(O4View v, double* __restrict__ 
                 limitComponent(La, Rr, ww(0), ww(1), ww(2), ww(-1), ww(-2),
                                del2(0), del2(1), del2(-1), del3(0), del3(-1), del3(1), del3(2));
                 int y[3] = {x[0], x[1], x[2]}; y[D] += cc;
-                const double et = S[I(F_ETA, y[0], y[1], y[2])];
+                const double et = P[I(F_ETA, y[0], y[1], y[2])];
                 const double wc = ww(0);
                 if (side == 0) left = et*La+(1.-et)*wc;
                 else right = et*Rr+(1.-et)*wc;
@@ -574,7 +577,7 @@ __global__ void __launch_bounds__(kB) k_o4_faces(O4View v, double* __restrict__ 
     double corr[8], lapb[8];
     {
         double wfx[8], a[8], bq[8];
-        faceBar(I, S, D, x[0], x[1], x[2], wfx);
+        faceBar(I, P, D, x[0], x[1], x[2], wfx);
 #pragma unroll
         for (int q = 0; q < 8; ++q) lapb[q] = 0.0;
         for (int t = 0; t < 2; ++t)
@@ -582,8 +585,8 @@ __global__ void __launch_bounds__(kB) k_o4_faces(O4View v, double* __restrict__ 
             const int tt = t == 0 ? t1 : t2;
             int y[3] = {x[0], x[1], x[2]}, z[3] = {x[0], x[1], x[2]};
             y[tt] += 1; z[tt] -= 1;
-            faceBar(I, S, D, y[0], y[1], y[2], a);
-            faceBar(I, S, D, z[0], z[1], z[2], bq);
+            faceBar(I, P, D, y[0], y[1], y[2], a);
+            faceBar(I, P, D, z[0], z[1], z[2], bq);
 #pragma unroll
             for (int q = 0; q < 8; ++q) lapb[q] += a[q] - 2.0*wfx[q] + bq[q];
         }
@@ -599,8 +602,8 @@ __global__ void __launch_bounds__(kB) k_o4_faces(O4View v, double* __restrict__ 
         {
             pl[q] = lf[q]-lapb[q]/24.0;
             pr[q] = rf[q]-lapb[q]/24.0;
-            cl[q] = S[I(F_WB+q, m1[0], m1[1], m1[2])];
-            cr[q] = S[I(F_WB+q, x[0], x[1], x[2])];
+            cl[q] = P[I(F_WB+q, m1[0], m1[1], m1[2])];
+            cr[q] = P[I(F_WB+q, x[0], x[1], x[2])];
         }
         if (!(usablePrimitive(pl) && usablePrimitive(pr)) && usablePrimitive(cl) && usablePrimitive(cr))
 #pragma unroll
@@ -609,24 +612,24 @@ __global__ void __launch_bounds__(kB) k_o4_faces(O4View v, double* __restrict__ 
 #pragma unroll
     for (int q = 0; q < 8; ++q)
     {
-        S[I(F_PTL+q, x[0], x[1], x[2])] = lf[q]-lapb[q]/24.0;
-        S[I(F_PTR+q, x[0], x[1], x[2])] = rf[q]-lapb[q]/24.0;
+        P[I(F_PTL+q, x[0], x[1], x[2])] = lf[q]-lapb[q]/24.0;
+        P[I(F_PTR+q, x[0], x[1], x[2])] = rf[q]-lapb[q]/24.0;
     }
-    S[I(F_P0L, x[0], x[1], x[2])] = lf[4];
-    S[I(F_P0R, x[0], x[1], x[2])] = rf[4];
+    P[I(F_P0L, x[0], x[1], x[2])] = lf[4];
+    P[I(F_P0R, x[0], x[1], x[2])] = rf[4];
     if (!v.roe)
-        S[I(F_SPD, x[0], x[1], x[2])] = rusanovSpeedO4(lf, rf, v.faceA[D] + 9*angIdx(b, x[1], x[2]),
+        P[I(F_SPD, x[0], x[1], x[2])] = rusanovSpeedO4(lf, rf, v.faceA[D] + 9*angIdx(b, x[1], x[2]),
                                                        v.gamma, D, false);
 }
 
 // Energies (ke, me, v.B) and the physical flux of the face-average state at
 // face (i,j,k) of direction D.
 template <int D>
-__device__ void barFlux(const O4View& v, const BoxDesc& b, const BoxIdx& I, const double* S,
+__device__ void barFlux(const O4View& v, const BoxDesc& b, const BoxIdx& I, const double* P,
                         int i, int j, int k, double f[8], double& meBar, double& pBar)
 {
     double wf[8];
-    faceBar(I, S, D, i, j, k, wf);
+    faceBar(I, P, D, i, j, k, wf);
     double ke, me, vb;
     energiesNative(v.faceA[D] + 9*angIdx(b, j, k), wf, ke, me, vb);
     primFlux(f, wf, ke, me, vb, v.gamma, D);
@@ -646,12 +649,13 @@ __global__ void __launch_bounds__(kB) k_o4_bar(O4View v, double* __restrict__ S)
     L[D] = b.lo[D]; H[D] = b.hi[D]+1;
     if (!regionPoint(b, L, H, boxThread(v, b), x)) return;
     const BoxIdx I(b);
+    double* const P = S + b.sOff;
     double f[8], me, p;
-    barFlux<D>(v, b, I, S, x[0], x[1], x[2], f, me, p);
+    barFlux<D>(v, b, I, P, x[0], x[1], x[2], f, me, p);
 #pragma unroll
-    for (int q = 0; q < 8; ++q) S[I(F_FB+q, x[0], x[1], x[2])] = f[q];
-    S[I(F_MEF, x[0], x[1], x[2])] = me;
-    S[I(F_P0B, x[0], x[1], x[2])] = p + me;
+    for (int q = 0; q < 8; ++q) P[I(F_FB+q, x[0], x[1], x[2])] = f[q];
+    P[I(F_MEF, x[0], x[1], x[2])] = me;
+    P[I(F_P0B, x[0], x[1], x[2])] = p + me;
 }
 
 // 5b. Physical fluxes of both face states (point flux + transverse Laplacian
@@ -666,10 +670,11 @@ __global__ void __launch_bounds__(kB) k_o4_phys(O4View v, double* __restrict__ S
     L[D] = b.lo[D]; H[D] = b.hi[D]+1;
     if (!regionPoint(b, L, H, boxThread(v, b), x)) return;
     const BoxIdx I(b);
+    double* const P = S + b.sOff;
     constexpr int t1 = D == 0 ? 1 : 0, t2 = D == 2 ? 1 : 2;
     // transverse Laplacian of the face-average flux and magnetic energy
     double lapF[8], lapMe = 0;
-    const double mex = S[I(F_MEF, x[0], x[1], x[2])];
+    const double mex = P[I(F_MEF, x[0], x[1], x[2])];
 #pragma unroll
     for (int q = 0; q < 8; ++q) lapF[q] = 0;
 #pragma unroll
@@ -680,8 +685,8 @@ __global__ void __launch_bounds__(kB) k_o4_phys(O4View v, double* __restrict__ S
         y[tt] += 1; z[tt] -= 1;
 #pragma unroll
         for (int q = 0; q < 8; ++q)
-            lapF[q] += S[I(F_FB+q, y[0], y[1], y[2])] - 2.0*S[I(F_FB+q, x[0], x[1], x[2])] + S[I(F_FB+q, z[0], z[1], z[2])];
-        lapMe += S[I(F_MEF, y[0], y[1], y[2])] - 2.0*mex + S[I(F_MEF, z[0], z[1], z[2])];
+            lapF[q] += P[I(F_FB+q, y[0], y[1], y[2])] - 2.0*P[I(F_FB+q, x[0], x[1], x[2])] + P[I(F_FB+q, z[0], z[1], z[2])];
+        lapMe += P[I(F_MEF, y[0], y[1], y[2])] - 2.0*mex + P[I(F_MEF, z[0], z[1], z[2])];
     }
     const double* A = v.faceA[D] + 9*angIdx(b, x[1], x[2]);
 #pragma unroll 1
@@ -689,14 +694,14 @@ __global__ void __launch_bounds__(kB) k_o4_phys(O4View v, double* __restrict__ S
     {
         double w[8];
 #pragma unroll
-        for (int q = 0; q < 8; ++q) w[q] = S[I((side == 0 ? F_PTL : F_PTR)+q, x[0], x[1], x[2])];
+        for (int q = 0; q < 8; ++q) w[q] = P[I((side == 0 ? F_PTL : F_PTR)+q, x[0], x[1], x[2])];
         double ke, me, vb, f[8];
         energiesNative(A, w, ke, me, vb);
         primFlux(f, w, ke, me, vb, v.gamma, D);
 #pragma unroll
-        for (int q = 0; q < 8; ++q) S[I((side == 0 ? F_PHL : F_PHR)+q, x[0], x[1], x[2])] = f[q]+lapF[q]/24.0;
+        for (int q = 0; q < 8; ++q) P[I((side == 0 ? F_PHL : F_PHR)+q, x[0], x[1], x[2])] = f[q]+lapF[q]/24.0;
         const int fp = side == 0 ? F_P0L : F_P0R;
-        S[I(fp, x[0], x[1], x[2])] = S[I(fp, x[0], x[1], x[2])] + (me + lapMe/24.0);
+        P[I(fp, x[0], x[1], x[2])] = P[I(fp, x[0], x[1], x[2])] + (me + lapMe/24.0);
     }
 }
 
@@ -711,13 +716,14 @@ __global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(2))) k_
     L[D] = b.lo[D]; H[D] = b.hi[D]+1;
     if (!regionPoint(b, L, H, boxThread(v, b), x)) return;
     const BoxIdx I(b);
+    double* const P = S + b.sOff;
     const double* A = v.faceA[D] + 9*angIdx(b, x[1], x[2]);
     double wl[8], wr[8], out[8];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) { wl[q] = S[I(F_PTL+q, x[0], x[1], x[2])]; wr[q] = S[I(F_PTR+q, x[0], x[1], x[2])]; }
+    for (int q = 0; q < 8; ++q) { wl[q] = P[I(F_PTL+q, x[0], x[1], x[2])]; wr[q] = P[I(F_PTR+q, x[0], x[1], x[2])]; }
     factoredRoe<S_>(out, wl, wr, A, radialPoint(v, b, D, x[0]), D, v.gamma, v.entropy);
 #pragma unroll
-    for (int q = 0; q < 8; ++q) S[I(F_DPT+q, x[0], x[1], x[2])] = out[q];
+    for (int q = 0; q < 8; ++q) P[I(F_DPT+q, x[0], x[1], x[2])] = out[q];
 }
 
 // 5c'. Rusanov (riemann_solver 1): conserved point values of both face
@@ -733,6 +739,7 @@ __global__ void __launch_bounds__(kB) k_o4_rusq(O4View v, double* __restrict__ S
     L[D] = b.lo[D]; H[D] = b.hi[D]+1;
     if (!regionPoint(b, L, H, boxThread(v, b), x)) return;
     const BoxIdx I(b);
+    double* const P = S + b.sOff;
     const double* A = v.faceA[D] + 9*angIdx(b, x[1], x[2]);
     const double rp = radialPoint(v, b, D, x[0]);
 #pragma unroll 1
@@ -740,18 +747,18 @@ __global__ void __launch_bounds__(kB) k_o4_rusq(O4View v, double* __restrict__ S
     {
         double w[8];
 #pragma unroll
-        for (int q = 0; q < 8; ++q) w[q] = S[I((side == 0 ? F_PTL : F_PTR)+q, x[0], x[1], x[2])];
+        for (int q = 0; q < 8; ++q) w[q] = P[I((side == 0 ? F_PTL : F_PTR)+q, x[0], x[1], x[2])];
         double ke, me, vb;
         energiesNative(A, w, ke, me, vb);
         const int f = side == 0 ? F_QL : F_QR;
-        S[I(f, x[0], x[1], x[2])] = rp*w[0];
+        P[I(f, x[0], x[1], x[2])] = rp*w[0];
 #pragma unroll
         for (int a = 0; a < 3; ++a)
         {
-            S[I(f+1+a, x[0], x[1], x[2])] = rp*w[0]*w[1+a];
-            S[I(f+5+a, x[0], x[1], x[2])] = rp*w[5+a];
+            P[I(f+1+a, x[0], x[1], x[2])] = rp*w[0]*w[1+a];
+            P[I(f+5+a, x[0], x[1], x[2])] = rp*w[5+a];
         }
-        S[I(f+4, x[0], x[1], x[2])] = rp*(ke + me + w[4]/(v.gamma-1.0));
+        P[I(f+4, x[0], x[1], x[2])] = rp*(ke + me + w[4]/(v.gamma-1.0));
     }
 }
 
@@ -765,10 +772,10 @@ struct FaceOps
     const O4View& v;
     const BoxDesc& b;
     const BoxIdx& I;
-    const double* S;
+    const double* P;          // the box's scratch block
     const double* met;
-    __device__ FaceOps(const O4View& v_, const BoxDesc& b_, const BoxIdx& I_, const double* S_)
-        : v(v_), b(b_), I(I_), S(S_), met(v_.fmet[D]) {}
+    __device__ FaceOps(const O4View& v_, const BoxDesc& b_, const BoxIdx& I_, const double* P_)
+        : v(v_), b(b_), I(I_), P(P_), met(v_.fmet[D]) {}
     __device__ double M(int c, const int y[3]) const { return met[13*angIdx(b, y[1], y[2]) + c]; }
     __device__ double Dm(int c, int t, const int y[3]) const
     {
@@ -776,7 +783,7 @@ struct FaceOps
         a[t] += 1; z[t] -= 1;
         return 0.5*M(c, a) - 0.5*M(c, z);
     }
-    __device__ double Fv(int f, const int y[3]) const { return S[I(f, y[0], y[1], y[2])]; }
+    __device__ double Fv(int f, const int y[3]) const { return P[I(f, y[0], y[1], y[2])]; }
     __device__ double Df(int f, int t, const int y[3]) const
     {
         int a[3] = {y[0], y[1], y[2]}, z[3] = {y[0], y[1], y[2]};
@@ -831,14 +838,15 @@ __global__ void __launch_bounds__(kB) k_o4_map(O4View v, double* __restrict__ S)
     if (D != 0) { L[0] -= 1; H[0] += 1; }
     if (!regionPoint(b, L, H, boxThread(v, b), x)) return;
     const BoxIdx I(b);
-    const FaceOps<D> op(v, b, I, S);
+    double* const P = S + b.sOff;
+    const FaceOps<D> op(v, b, I, P);
     double out[8];
     op.mapped(F_PHL, x, out);
 #pragma unroll
-    for (int c = 0; c < 8; ++c) S[I(F_MPL+c, x[0], x[1], x[2])] = out[c];
+    for (int c = 0; c < 8; ++c) P[I(F_MPL+c, x[0], x[1], x[2])] = out[c];
     op.mapped(F_PHR, x, out);
 #pragma unroll
-    for (int c = 0; c < 8; ++c) S[I(F_MPR+c, x[0], x[1], x[2])] = out[c];
+    for (int c = 0; c < 8; ++c) P[I(F_MPR+c, x[0], x[1], x[2])] = out[c];
 }
 
 // 6b. Mapped face flux on the flux faces [lo_d, hi_d+1] x [lo, hi]: the
@@ -854,8 +862,9 @@ __global__ void __launch_bounds__(kB) k_o4_flux(O4View v, double* __restrict__ S
     H[D] = b.hi[D]+1;
     if (!regionPoint(b, L, H, boxThread(v, b), x)) return;
     const BoxIdx I(b);
+    double* const P = S + b.sOff;
     constexpr int t1 = D == 0 ? 1 : 0, t2 = D == 2 ? 1 : 2;
-    const FaceOps<D> op(v, b, I, S);
+    const FaceOps<D> op(v, b, I, P);
     auto M = [&](int c, const int y[3]) { return op.M(c, y); };
     auto Dm = [&](int c, int t, const int y[3]) { return op.Dm(c, t, y); };
     auto Fv = [&](int f, const int y[3]) { return op.Fv(f, y); };
@@ -982,7 +991,7 @@ __global__ void __launch_bounds__(kB) k_o4_flux(O4View v, double* __restrict__ S
     }
 #pragma unroll
     for (int c = 0; c < 8; ++c)
-        S[I(F_FLUX+8*D+c, x[0], x[1], x[2])] = 0.5*(Fsum[c]-Fd[c]);
+        P[I(F_FLUX+8*D+c, x[0], x[1], x[2])] = 0.5*(Fsum[c]-Fd[c]);
 }
 
 // 6b. Seam flux reconciliation 1/2 (F_a + sign F_b) on both boxes' copies
@@ -1011,6 +1020,7 @@ __global__ void __launch_bounds__(kB) k_o4_div(O4View v, const double* __restric
     int x[3];
     if (!regionPoint(b, b.lo, b.hi, boxThread(v, b), x)) return;
     const BoxIdx I(b);
+    const double* const P = S + b.sOff;
     const long long r = b.rOff + (x[2]-b.lo[2])*b.rs[1] + (x[1]-b.lo[1])*b.rs[0] + (x[0]-b.lo[0]);
     for (int c = 0; c < 8; ++c)
     {
@@ -1021,8 +1031,8 @@ __global__ void __launch_bounds__(kB) k_o4_div(O4View v, const double* __restric
             const double dxInv = -1.0/(d == 0 ? 1.0/b.nr : 1.0/b.na);
             int a[3] = {x[0], x[1], x[2]}; a[d] += 1;
             double inc = 0;
-            inc += (-1.0*dxInv)*S[I(F_FLUX+8*d+c, x[0], x[1], x[2])];
-            inc += (1.0*dxInv)*S[I(F_FLUX+8*d+c, a[0], a[1], a[2])];
+            inc += (-1.0*dxInv)*P[I(F_FLUX+8*d+c, x[0], x[1], x[2])];
+            inc += (1.0*dxInv)*P[I(F_FLUX+8*d+c, a[0], a[1], a[2])];
             acc += inc;
         }
         R[r + n] = acc;
@@ -1039,10 +1049,11 @@ __global__ void __launch_bounds__(kB) k_o4_fluxout(O4View v, int D, const double
     H[D] += 1;
     if (!regionPoint(b, L, H, boxThread(v, b), x)) return;
     const BoxIdx I(b);
+    const double* const P = S + b.sOff;
     const int m0 = b.n[0]+(D == 0), m1 = b.n[1]+(D == 1), m2 = b.n[2]+(D == 2);
     const long long n = (long long)m0*m1*m2;
     const long long o = fOff[blockBoxIndex(v)] + ((long long)(x[2]-b.lo[2])*m1 + (x[1]-b.lo[1]))*m0 + (x[0]-b.lo[0]);
-    for (int c = 0; c < 8; ++c) F[o + c*n] = S[I(F_FLUX+8*D+c, x[0], x[1], x[2])];
+    for (int c = 0; c < 8; ++c) F[o + c*n] = P[I(F_FLUX+8*D+c, x[0], x[1], x[2])];
 }
 
 template <typename T>
@@ -1143,6 +1154,7 @@ void buildBoxop(hc_boxop& B, const hc_box* boxes, const int* boxLevel, int nbox)
         bd.radOff = levRad[lev];
         bd.angOff = levAng[lev] + (long long)bd.p*bd.naE*bd.naE;
         const long long e = (long long)(bd.n[0]+2*kG)*(bd.n[1]+2*kG)*(bd.n[2]+2*kG);
+        if (F_COUNT*e > 0x7fffffffLL) throw std::runtime_error("box too large: scratch offsets exceed 32 bits");
         bd.uOff = u; bd.rOff = r; bd.sOff = sc;
         bd.us[0] = bd.n[0]+2*kG; bd.us[1] = (long long)(bd.n[0]+2*kG)*(bd.n[1]+2*kG); bd.us[2] = e;
         bd.rs[0] = bd.n[0]; bd.rs[1] = (long long)bd.n[0]*bd.n[1]; bd.rs[2] = (long long)bd.n[0]*bd.n[1]*bd.n[2];
