@@ -236,6 +236,13 @@ def main():
     L.rk4_advance(U, G, dt, args.steps)
     torch.cuda.synchronize()
     graph_ms = (time.perf_counter() - tg) / args.steps * 1e3
+    # level CFL time step (the reference's dtCFL, taken once per step by the
+    # driver), timed on its own with the host read-back of dt
+    L.dt_cfl(U)
+    tc = time.perf_counter()
+    for _ in range(10):
+        dt_cfl = L.dt_cfl(U)
+    cfl_ms = (time.perf_counter() - tc) / 10 * 1e3
 
     mixed_err = None
     if U_init is not None:
@@ -288,6 +295,7 @@ def main():
                      "avg_launch_us": avg_s * 1e6},
         "kernels_ms_per_step": {k: v[0] / args.steps for k, v in prof.items()},
         "graph_ms_per_step": graph_ms,
+        "dt_cfl": {"value": dt_cfl, "ms": cfl_ms},
     }
     if traffic is not None:
         line["roofline"]["traffic_source"] = pmc_path

@@ -69,6 +69,7 @@ _SIGNATURES = [
     ("hc_stage_rhs", _I, [_P, _P, _P, _P, _D, _P, _P, _P, _P]),
     ("hc_rk4_advance", _I, [_P, _P, _P, _D, _I, _P]),
     ("hc_level_use_graph", _I, [_P, _I]),
+    ("hc_level_dt_cfl", _I, [_P, _P, _D, _P, _P]),
     ("hc_init_state", _I, [ctypes.POINTER(HcShell), ctypes.POINTER(HcOptions), _I, ctypes.c_ulonglong, _P, _P]),
     ("hc_geom_cell_detA", _D, [_I, _I, _I]),
     ("hc_geom_face_normal", None, [_I, _I, _I, _I, _I, _P]),
@@ -279,6 +280,15 @@ class Level:
 
     def use_graph(self, enable: bool = True):
         _check(lib().hc_level_use_graph(self._handle, int(bool(enable))))
+
+    def dt_cfl(self, U, cone_boundary_density: float = 0.0) -> float:
+        """1 / max CFL speed sum of the valid state U (BoxOp dtCFL); the caller
+        applies its CFL number."""
+        self._require(U, self.valid_shape, "U")
+        out = ctypes.c_double()
+        _check(lib().hc_level_dt_cfl(self._handle, _ptr(U), float(cone_boundary_density), ctypes.byref(out),
+                                     self._stream()))
+        return out.value
 
 
 def box_table(boxes):

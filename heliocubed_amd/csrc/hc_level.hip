@@ -515,6 +515,7 @@ struct hc_level
 {
     KernelProfile profile;
     hc::Level4* l4 = nullptr;     // fourth-order level (spatial_order 4)
+    hc::CflTable* cfl = nullptr;  // CFL geometry, built on the first time-step query
     hc::ShellDims dims{};
     hc_options opt{};
     LevelView view{};
@@ -554,6 +555,7 @@ struct hc_level
         dropGraph();
         if (capStream) (void)hipStreamDestroy(capStream);
         hc::level4Destroy(l4);
+        hc::cflDestroy(cfl);
         for (void* p : owned) (void)hipFree(p);
     }
     template <typename T> T* own(T* p) { owned.push_back(p); return p; }
@@ -973,6 +975,25 @@ int hc_level_use_graph(hc_level* level, int enable)
     level->useGraph = enable != 0;
     if (!level->useGraph) level->dropGraph();
     return 0;
+}
+
+int hc_level_dt_cfl(hc_level* level, const double* U, double cone_boundary_density, double* dt, hc_stream stream)
+{
+    if (!level || !U || !dt) return fail("hc_level_dt_cfl: null argument");
+    if (!(cone_boundary_density >= 0.0)) return fail("hc_level_dt_cfl: cone_boundary_density must be >= 0");
+    try
+    {
+        hc_level& L = *level;
+        if (!L.cfl) L.cfl = hc::cflCreate(L.dims, L.opt.gamma);
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        // the fourth-order dtCFL takes no cone bound (src/BoxOp_EulerCubedSphereImplem.H:886-887)
+        const double cone = L.opt.spatial_order == 2 ? cone_boundary_density : 0.0;
+        const double* r = hc::cflLaunch(*L.cfl, U, cone, s);
+        HC_CHECK(hipMemcpyAsync(dt, r, sizeof(double), hipMemcpyDeviceToHost, s));
+        HC_CHECK(hipStreamSynchronize(s));
+        return 0;
+    }
+    catch (const std::exception& e) { return fail(std::string("hc_level_dt_cfl: ") + e.what()); }
 }
 
 int hc_init_state(const hc_shell* shell, const hc_options* options, int config,

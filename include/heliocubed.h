@@ -138,6 +138,21 @@ int hc_rk4_advance(hc_level* level, double* U, const double* inner_ghosts, doubl
  * individually. No reference counterpart (Proto launches per box). */
 int hc_level_use_graph(hc_level* level, int enable);
 
+/* CFL time step of the level state U (valid layout): 1 / max over the valid
+ * cells of sum_d (sqrt(gamma p/rho + |B_d|^2/(4 pi rho)) + |v_d|) / width_d,
+ * each cell converted to native primitives at second order. The caller
+ * multiplies *dt by its CFL number, as exec/main.cpp:345-346,375-379 does.
+ * Replaces BoxOp_EulerCubedSphere::dtCFL with dtInv and f_speedmax
+ * (src/BoxOp_EulerCubedSphereImplem.H:878-954,1013-1131,
+ * src/BoxOp_EulerCubedSphere.H:127-194) at spatial_order 4 and
+ * dtCFLSecondOrderHost (src/BoxOp_SecondOrderCFL.H:99-210) at spatial_order 2.
+ * cone_boundary_density > 0 bounds the density under the Alfven term at
+ * spatial_order 2 (ignored at 4, like the reference); viscosity terms and the
+ * constant-field test background are not part of this path. Synchronizes
+ * the stream; dt is a host pointer. */
+int hc_level_dt_cfl(hc_level* level, const double* U, double cone_boundary_density, double* dt,
+                    hc_stream stream);
+
 /* Fourth-order box operator (spatial_order 4, standard operator, limiter 0
  * or 1, riemann_solver 2) over a table of boxes, evaluated for every box in
  * the same launches. Replaces BoxOp_EulerCubedSphere::operator()

@@ -77,6 +77,7 @@ def lib() -> ctypes.CDLL:
             "orc_berta_theta": (None, [cfgp, I, P, P, P, P]),
             "orc_weno3": (None, [P, P, D]),
             "orc_plm_berta": (None, [P, P]),
+            "orc_dt_cfl": (D, [cfgp, I, P, D]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -213,6 +214,16 @@ def berta_face(values, method="wenoz", dx2=0.0):
     else:
         lib().orc_plm_berta(_p(out), _p(v))
     return out[0], out[1]
+
+
+def dt_cfl(n_rad, n_ang, JU, order=4, cone_density=0.0, gamma=5.0 / 3.0, r0=21.5, r1=215.0):
+    """1 / max over valid cells of the CFL speed sum of a level state JU
+    [6][8][na][na][nr] (dtCFLSecondOrderHost at order 2, dtCFL at order 4);
+    the caller multiplies by its CFL number."""
+    cfg = OrcCfg(n_rad, n_ang, 2 if order == 2 else 5, r0, r1, 0.0, gamma, 1, 2, 0.3, 1, 0.02, 0)
+    ju = np.ascontiguousarray(JU, dtype=np.float64)
+    assert ju.shape == (6, 8, n_ang, n_ang, n_rad), ju.shape
+    return lib().orc_dt_cfl(ctypes.byref(cfg), order, _p(ju), cone_density)
 
 
 def basis(p, alpha, beta):
