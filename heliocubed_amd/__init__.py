@@ -70,6 +70,7 @@ _SIGNATURES = [
     ("hc_rk4_advance", _I, [_P, _P, _P, _D, _I, _P]),
     ("hc_level_use_graph", _I, [_P, _I]),
     ("hc_level_dt_cfl", _I, [_P, _P, _D, _P, _P]),
+    ("hc_level_valid_state", _I, [_P, _P, _P, _P, _P]),
     ("hc_init_state", _I, [ctypes.POINTER(HcShell), ctypes.POINTER(HcOptions), _I, ctypes.c_ulonglong, _P, _P]),
     ("hc_geom_cell_detA", _D, [_I, _I, _I]),
     ("hc_geom_face_normal", None, [_I, _I, _I, _I, _I, _P]),
@@ -289,6 +290,16 @@ class Level:
         _check(lib().hc_level_dt_cfl(self._handle, _ptr(U), float(cone_boundary_density), ctypes.byref(out),
                                      self._stream()))
         return out.value
+
+    def valid_state(self, U):
+        """(valid, first_invalid): every value finite and every density > 0
+        (BoxOp validConservedState); first_invalid is the flat index of the
+        first offending value in the valid layout, or -1."""
+        self._require(U, self.valid_shape, "U")
+        ok, first = ctypes.c_int(), ctypes.c_longlong()
+        _check(lib().hc_level_valid_state(self._handle, _ptr(U), ctypes.byref(ok), ctypes.byref(first),
+                                          self._stream()))
+        return bool(ok.value), first.value
 
 
 def box_table(boxes):

@@ -41,13 +41,15 @@ void level4Prepare(Level4& L, const double* U0, const double* K, double a_dt, co
                    hipStream_t s);
 void level4Rhs(Level4& L, const double* US, double* K, double* U, double b_dt, hipStream_t s);
 
-// CFL time step of a level (defined with its kernel): geometry tables, and
-// the reduction returning a device pointer to 1 / max speed sum.
+// Level reductions (defined with their kernels): the CFL geometry tables and
+// reduction buffers; the CFL reduction returns a device pointer to 1 / max
+// speed sum, the state check one to the first bad flat index (or -1).
 struct ShellDims;
 struct CflTable;
 CflTable* cflCreate(const ShellDims& dims, double gamma);
 void cflDestroy(CflTable* T);
 const double* cflLaunch(CflTable& T, const double* JU, double coneDensity, hipStream_t s);
+const long long* stateCheckLaunch(CflTable& T, const double* U, hipStream_t s);
 } // namespace hc
 
 #define HC_THROW(call)                                                              \

@@ -996,6 +996,25 @@ int hc_level_dt_cfl(hc_level* level, const double* U, double cone_boundary_densi
     catch (const std::exception& e) { return fail(std::string("hc_level_dt_cfl: ") + e.what()); }
 }
 
+int hc_level_valid_state(hc_level* level, const double* U, int* valid, long long* first_invalid, hc_stream stream)
+{
+    if (!level || !U || !valid) return fail("hc_level_valid_state: null argument");
+    try
+    {
+        hc_level& L = *level;
+        if (!L.cfl) L.cfl = hc::cflCreate(L.dims, L.opt.gamma);
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        const long long* r = hc::stateCheckLaunch(*L.cfl, U, s);
+        long long first = -1;
+        HC_CHECK(hipMemcpyAsync(&first, r, sizeof(long long), hipMemcpyDeviceToHost, s));
+        HC_CHECK(hipStreamSynchronize(s));
+        *valid = first < 0 ? 1 : 0;
+        if (first_invalid) *first_invalid = first;
+        return 0;
+    }
+    catch (const std::exception& e) { return fail(std::string("hc_level_valid_state: ") + e.what()); }
+}
+
 int hc_init_state(const hc_shell* shell, const hc_options* options, int config,
                   unsigned long long seed, double* JU, double* inner)
 {

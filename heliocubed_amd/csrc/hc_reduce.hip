@@ -1,3 +1,5 @@
+// Level reductions: the CFL time step and the conserved-state check.
+//
 // CFL time step of a level (replaces dtCFLSecondOrderHost,
 // src/BoxOp_SecondOrderCFL.H:99-210, and the fourth-order dtCFL + dtInv +
 // f_speedmax, src/BoxOp_EulerCubedSphereImplem.H:878-954,1013-1131,
@@ -135,6 +137,52 @@ __global__ void __launch_bounds__(kCflBlock) k_cfl_cells(CflView v, const double
     if (threadIdx.x == 0) partial[blockIdx.x] = m;
 }
 
+// Conserved-state check (validConservedState, src/BoxOp_EulerCubedSphereImplem.H:813-876):
+// a value is bad when it is not finite, or when it is a density that is not
+// > 0. Each block keeps the smallest flat index of a bad value.
+__device__ inline long long indexMin(long long m)
+{
+    for (int off = 32; off > 0; off >>= 1)
+    {
+        const long long o = __shfl_xor(m, off, 64);
+        m = o < m ? o : m;
+    }
+    __shared__ long long part[kCflBlock/64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) part[wave] = m;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int w = 1; w < kCflBlock/64; ++w) m = part[w] < m ? part[w] : m;
+    return m;
+}
+
+constexpr long long kNoIndex = 0x7fffffffffffffffLL;
+
+__global__ void __launch_bounds__(kCflBlock) k_state_check(const double* __restrict__ U, long long n,
+                                                           long long compStride, long long* __restrict__ partial)
+{
+    const long long stride = (long long)gridDim.x*blockDim.x;
+    long long first = kNoIndex;
+    for (long long x = (long long)blockIdx.x*blockDim.x + threadIdx.x; x < n; x += stride)
+    {
+        const double value = U[x];
+        const bool density = (x / compStride) % 8 == 0;
+        const bool bad = !isfinite(value) || (density && !(value > 0.0));
+        if (bad && x < first) first = x;
+    }
+    const long long m = indexMin(first);
+    if (threadIdx.x == 0) partial[blockIdx.x] = m;
+}
+
+__global__ void __launch_bounds__(kCflBlock) k_state_final(const long long* __restrict__ partial, int n,
+                                                           long long* __restrict__ result)
+{
+    long long m = kNoIndex;
+    for (int b = threadIdx.x; b < n; b += blockDim.x) m = partial[b] < m ? partial[b] : m;
+    m = indexMin(m);
+    if (threadIdx.x == 0) result[0] = m == kNoIndex ? -1 : m;
+}
+
 __global__ void __launch_bounds__(kCflBlock) k_cfl_final(const double* __restrict__ partial, int n,
                                                          double* __restrict__ result)
 {
@@ -152,16 +200,21 @@ struct CflTable
     double* ang = nullptr;
     double* rad = nullptr;
     double* partial = nullptr;   // block maxima, then the result at [kCflMaxBlocks]
+    long long* firstBad = nullptr; // block minima of the state check, result at [kCflMaxBlocks]
+    bool ready = false;            // ang and rad hold the tables
     ~CflTable()
     {
         if (ang) (void)hipFree(ang);
         if (rad) (void)hipFree(rad);
         if (partial) (void)hipFree(partial);
+        if (firstBad) (void)hipFree(firstBad);
     }
 };
 
-CflTable* cflCreate(const ShellDims& dims, double gamma)
+// Column and radial tables of the CFL arithmetic, built on the first call.
+void cflGeometry(CflTable& T)
 {
+    const ShellDims& dims = T.dims;
     const int na = dims.na, nr = dims.nr;
     const double h = 1.0/na, dxi0 = 1.0/nr;
     std::vector<double> ang((size_t)6*na*na*kAng), rad((size_t)nr*kRad);
@@ -203,16 +256,22 @@ CflTable* cflCreate(const ShellDims& dims, double gamma)
         rad[(size_t)i*kRad + 2] = ((rc.rhi - rc.rlo)/dxi0)*dxi0;
         rad[(size_t)i*kRad + 3] = rc.areaT;
     }
+    if (!T.ang) HC_THROW(hipMalloc(&T.ang, ang.size()*sizeof(double)));
+    if (!T.rad) HC_THROW(hipMalloc(&T.rad, rad.size()*sizeof(double)));
+    HC_THROW(hipMemcpy(T.ang, ang.data(), ang.size()*sizeof(double), hipMemcpyHostToDevice));
+    HC_THROW(hipMemcpy(T.rad, rad.data(), rad.size()*sizeof(double), hipMemcpyHostToDevice));
+    T.ready = true;
+}
+
+CflTable* cflCreate(const ShellDims& dims, double gamma)
+{
     CflTable* T = new CflTable;
     T->dims = dims;
     T->gamma = gamma;
     try
     {
-        HC_THROW(hipMalloc(&T->ang, ang.size()*sizeof(double)));
-        HC_THROW(hipMalloc(&T->rad, rad.size()*sizeof(double)));
         HC_THROW(hipMalloc(&T->partial, (kCflMaxBlocks + 1)*sizeof(double)));
-        HC_THROW(hipMemcpy(T->ang, ang.data(), ang.size()*sizeof(double), hipMemcpyHostToDevice));
-        HC_THROW(hipMemcpy(T->rad, rad.data(), rad.size()*sizeof(double), hipMemcpyHostToDevice));
+        HC_THROW(hipMalloc(&T->firstBad, (kCflMaxBlocks + 1)*sizeof(long long)));
     }
     catch (...)
     {
@@ -228,6 +287,7 @@ void cflDestroy(CflTable* T) { delete T; }
 // returned by the call, valid once s has reached it.
 const double* cflLaunch(CflTable& T, const double* JU, double coneDensity, hipStream_t s)
 {
+    if (!T.ready) cflGeometry(T);
     CflView v;
     v.nr = T.dims.nr;
     v.na = T.dims.na;
@@ -244,5 +304,19 @@ const double* cflLaunch(CflTable& T, const double* JU, double coneDensity, hipSt
     hipLaunchKernelGGL(k_cfl_final, dim3(1), dim3(kCflBlock), 0, s, T.partial, nb, T.partial + kCflMaxBlocks);
     HC_THROW(hipGetLastError());
     return T.partial + kCflMaxBlocks;
+}
+
+// Enqueues the conserved-state check of the valid level U; the device result
+// is the flat index of the first bad value, or -1.
+const long long* stateCheckLaunch(CflTable& T, const double* U, hipStream_t s)
+{
+    const long long compStride = (long long)T.dims.na*T.dims.na*T.dims.nr;
+    const long long n = 6LL*8*compStride;
+    const int nb = (int)std::min<long long>((n + kCflBlock - 1)/kCflBlock, kCflMaxBlocks);
+    hipLaunchKernelGGL(k_state_check, dim3(nb), dim3(kCflBlock), 0, s, U, n, compStride, T.firstBad);
+    HC_THROW(hipGetLastError());
+    hipLaunchKernelGGL(k_state_final, dim3(1), dim3(kCflBlock), 0, s, T.firstBad, nb, T.firstBad + kCflMaxBlocks);
+    HC_THROW(hipGetLastError());
+    return T.firstBad + kCflMaxBlocks;
 }
 } // namespace hc

@@ -153,6 +153,17 @@ int hc_level_use_graph(hc_level* level, int enable);
 int hc_level_dt_cfl(hc_level* level, const double* U, double cone_boundary_density, double* dt,
                     hc_stream stream);
 
+/* Conserved-state check of the level state U (valid layout): *valid = 1 when
+ * every value is finite and every density is > 0, else 0, with
+ * *first_invalid (may be NULL) the smallest flat index of an offending value
+ * in the valid layout (-1 when valid). Replaces
+ * BoxOp_EulerCubedSphere::validConservedState
+ * (src/BoxOp_EulerCubedSphereImplem.H:813-876), which the drivers call after
+ * every step (exec/main.cpp:445, src/MHD_AMR.H:422-434). Synchronizes the
+ * stream. */
+int hc_level_valid_state(hc_level* level, const double* U, int* valid, long long* first_invalid,
+                         hc_stream stream);
+
 /* Fourth-order box operator (spatial_order 4, standard operator, limiter 0
  * or 1, riemann_solver 2) over a table of boxes, evaluated for every box in
  * the same launches. Replaces BoxOp_EulerCubedSphere::operator()

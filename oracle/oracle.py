@@ -78,6 +78,7 @@ def lib() -> ctypes.CDLL:
             "orc_weno3": (None, [P, P, D]),
             "orc_plm_berta": (None, [P, P]),
             "orc_dt_cfl": (D, [cfgp, I, P, D]),
+            "orc_first_invalid": (ctypes.c_longlong, [P, I, I]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -224,6 +225,14 @@ def dt_cfl(n_rad, n_ang, JU, order=4, cone_density=0.0, gamma=5.0 / 3.0, r0=21.5
     ju = np.ascontiguousarray(JU, dtype=np.float64)
     assert ju.shape == (6, 8, n_ang, n_ang, n_rad), ju.shape
     return lib().orc_dt_cfl(ctypes.byref(cfg), order, _p(ju), cone_density)
+
+
+def first_invalid(JU):
+    """Flat index of the first non-finite value or non-positive density of a
+    valid level state [6][8][na][na][nr] (validConservedState), or -1."""
+    ju = np.ascontiguousarray(JU, dtype=np.float64)
+    na, nr = ju.shape[2], ju.shape[4]
+    return int(lib().orc_first_invalid(_p(ju), nr, na))
 
 
 def basis(p, alpha, beta):
