@@ -70,6 +70,9 @@ _SIGNATURES = [
     ("hc_rk4_advance", _I, [_P, _P, _P, _D, _I, _P]),
     ("hc_level_use_graph", _I, [_P, _I]),
     ("hc_level_dt_cfl", _I, [_P, _P, _D, _P, _P]),
+    ("hc_level_set_tile", _I, [_P, _I]),
+    ("hc_level_tile", _I, [_P]),
+    ("hc_level_autotune", _I, [_P, _P, _P, _P, _P, _P]),
     ("hc_level_valid_state", _I, [_P, _P, _P, _P, _P]),
     ("hc_init_state", _I, [ctypes.POINTER(HcShell), ctypes.POINTER(HcOptions), _I, ctypes.c_ulonglong, _P, _P]),
     ("hc_geom_cell_detA", _D, [_I, _I, _I]),
@@ -281,6 +284,23 @@ class Level:
 
     def use_graph(self, enable: bool = True):
         _check(lib().hc_level_use_graph(self._handle, int(bool(enable))))
+
+    def set_tile(self, tile: int):
+        """Edge of the fourth-order level's box-table tiles (results unchanged)."""
+        _check(lib().hc_level_set_tile(self._handle, int(tile)))
+
+    @property
+    def tile(self) -> int:
+        return lib().hc_level_tile(self._handle)
+
+    def autotune(self, U, inner):
+        """Times one stage RHS of U for tiles 32, 64 and the whole panel and
+        keeps the fastest; returns (tile, ms per stage). U is not modified."""
+        self._require(U, self.valid_shape, "U")
+        t, ms = ctypes.c_int(), ctypes.c_double()
+        _check(lib().hc_level_autotune(self._handle, _ptr(U), _ptr(inner), ctypes.byref(t), ctypes.byref(ms),
+                                       self._stream()))
+        return t.value, ms.value
 
     def dt_cfl(self, U, cone_boundary_density: float = 0.0) -> float:
         """1 / max CFL speed sum of the valid state U (BoxOp dtCFL); the caller

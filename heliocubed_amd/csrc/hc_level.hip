@@ -12,6 +12,7 @@
 #include <new>
 #include <stdexcept>
 #include <string>
+#include <algorithm>
 #include <vector>
 
 #include "../../include/heliocubed.h"
@@ -769,6 +770,9 @@ struct WindState
     double kvec[16][3], evec[16][3], phase[16], amp = 0;
     bool cone = false;             // low-beta cone: p = 1e-4 |B|^2/2 inside it
     double coneAxis[3] = {0, 0, 1}, coneCos = 1.0;
+    bool sectors = false;          // B_r sign by quadrant of the azimuth (4 sectors)
+    bool blob = false;             // magnetized blob: rho x4, v_r = 2, toroidal B = 3 |B_r|
+    double blobCenter[3] = {0, 0, 0}, blobRadius = 0;
     // Cartesian primitive state at position X.
     void at(const double X[3], double w[8]) const
     {
@@ -780,6 +784,32 @@ struct WindState
             const double c = (X[0]*coneAxis[0]+X[1]*coneAxis[1]+X[2]*coneAxis[2])/r;
             if (c > coneCos) w[4] = 1.0e-4*0.5*(w[5]*w[5]+w[6]*w[6]+w[7]*w[7]);
         }
+        if (blob) blobState(X, w);
+    }
+    // Blob of radius R around C moving radially: density x4, v = 2 r_hat, and a
+    // toroidal field about the blob's radial axis n = C/|C| of magnitude
+    // 3 |B_r| sin(pi d/R) s/d (d distance to C, s distance to the axis), zero
+    // on the axis and at the blob's surface.
+    void blobState(const double X[3], double w[8]) const
+    {
+        double rel[3], d2 = 0, cn = 0;
+        for (int a = 0; a < 3; ++a) { rel[a] = X[a]-blobCenter[a]; d2 += rel[a]*rel[a]; cn += blobCenter[a]*blobCenter[a]; }
+        const double d = std::sqrt(d2);
+        if (!(d < blobRadius)) return;
+        const double r = std::sqrt(X[0]*X[0]+X[1]*X[1]+X[2]*X[2]);
+        const double br = std::fabs((w[5]*X[0]+w[6]*X[1]+w[7]*X[2])/r);
+        w[0] *= 4.0;
+        for (int a = 0; a < 3; ++a) w[1+a] = 2.0*X[a]/r;
+        cn = std::sqrt(cn);
+        const double n[3] = {blobCenter[0]/cn, blobCenter[1]/cn, blobCenter[2]/cn};
+        const double along = rel[0]*n[0]+rel[1]*n[1]+rel[2]*n[2];
+        double sv[3], s2 = 0;
+        for (int a = 0; a < 3; ++a) { sv[a] = rel[a]-along*n[a]; s2 += sv[a]*sv[a]; }
+        const double sd = std::sqrt(s2);
+        if (!(sd > 1e-12*blobRadius)) return;
+        const double e[3] = {(n[1]*sv[2]-n[2]*sv[1])/sd, (n[2]*sv[0]-n[0]*sv[2])/sd, (n[0]*sv[1]-n[1]*sv[0])/sd};
+        const double mag = 3.0*br*std::sin(hc::kPi*d/blobRadius)*(sd/d);
+        for (int a = 0; a < 3; ++a) w[5+a] += mag*e[a];
     }
     void modes(const double X[3], double w[8]) const
     {
@@ -799,7 +829,13 @@ struct WindState
     {
         const double r = std::sqrt(X[0]*X[0]+X[1]*X[1]+X[2]*X[2]);
         const double rho = (r0/r)*(r0/r);
-        const double sign = (!hemispheric || X[2] > 0) ? 1.0 : -1.0;
+        double sign = (!hemispheric || X[2] > 0) ? 1.0 : -1.0;
+        if (sectors)
+        {
+            const double phi = std::atan2(X[1], X[0]);                     // (-pi, pi]
+            const int q = (int)std::floor((phi + hc::kPi)/(0.5*hc::kPi)) & 3;
+            sign = (q & 1) ? -1.0 : 1.0;
+        }
         double d2 = 0;
         for (int a = 0; a < 3; ++a) d2 += (X[a]-center[a])*(X[a]-center[a]);
         const double p = 0.1*rho*(1.0 + pulse*std::exp(-d2/(width*width)));
@@ -812,7 +848,10 @@ struct WindState
 // Gaussian pressure pulse and hemispheric B_r, 1 monopole B_r, 2 monopole
 // without the pulse, 3 = 0 plus Alfvenic modes (|k| <= 8, amplitude 0.05) and
 // (applied by the callers) lognormal density noise, 4 = 3 plus a low-beta cone
-// (p = 1e-4 |B|^2/2 within 15 degrees of a seeded axis).
+// (p = 1e-4 |B|^2/2 within 15 degrees of a seeded axis), 5 the radial wind
+// without pulse, modes or noise, with B_r signed by four azimuthal sectors and
+// a magnetized blob (radius 0.05 (r1 - r0) at a seeded point of the middle
+// radius: density x4, v_r = 2, toroidal B of 3 |B_r|).
 WindState makeWind(const hc_shell& shell, const hc_options& options, int config, unsigned long long seed)
 {
     SplitMix rng{seed};
@@ -822,9 +861,16 @@ WindState makeWind(const hc_shell& shell, const hc_options& options, int config,
     const double rc = 0.5*(ws.r0+ws.r1), sz = std::sqrt(std::max(0.0, 1.0-cz*cz));
     ws.center[0] = rc*sz*std::cos(phi); ws.center[1] = rc*sz*std::sin(phi); ws.center[2] = rc*cz;
     ws.width = 0.1*(ws.r1-ws.r0);
-    ws.pulse = config == 2 ? 0.0 : 0.2;
-    ws.hemispheric = config == 0 || config >= 3;
-    if (config >= 3)
+    ws.pulse = (config == 2 || config == 5) ? 0.0 : 0.2;
+    ws.hemispheric = config == 0 || config == 3 || config == 4;
+    if (config == 5)
+    {
+        ws.sectors = true;
+        ws.blob = true;
+        ws.blobRadius = 0.05*(ws.r1-ws.r0);
+        for (int a = 0; a < 3; ++a) ws.blobCenter[a] = ws.center[a];
+    }
+    if (config == 3 || config == 4)
     {
         ws.nmodes = 16; ws.amp = 0.05/std::sqrt(16.0);
         for (int m = 0; m < 16; ++m)
@@ -977,6 +1023,66 @@ int hc_level_use_graph(hc_level* level, int enable)
     return 0;
 }
 
+int hc_level_set_tile(hc_level* level, int tile)
+{
+    if (!level) return fail("hc_level_set_tile: null level");
+    if (!level->l4) return fail("hc_level_set_tile: only a fourth-order level has a box table");
+    if (tile < 8) return fail("hc_level_set_tile: tile must be >= 8 cells");
+    try
+    {
+        if (hc::level4Tile(*level->l4) == tile) return 0;
+        level->dropGraph();
+        HC_CHECK(hipDeviceSynchronize());
+        hc::level4Retile(*level->l4, tile);
+        return 0;
+    }
+    catch (const std::exception& e) { return fail(std::string("hc_level_set_tile: ") + e.what()); }
+}
+
+int hc_level_tile(const hc_level* level) { return level && level->l4 ? hc::level4Tile(*level->l4) : 0; }
+
+int hc_level_autotune(hc_level* level, const double* U, const double* inner, int* tile, double* ms_per_stage,
+                      hc_stream stream)
+{
+    if (!level || !U || !inner || !tile) return fail("hc_level_autotune: null argument");
+    if (!level->l4) return fail("hc_level_autotune: only a fourth-order level has a box table");
+    try
+    {
+        hc_level& L = *level;
+        L.ensureWorkspace();
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        const int nmax = std::max(L.dims.nr, L.dims.na);
+        std::vector<int> cand;
+        for (int t : {32, 64, nmax}) if (t <= nmax && std::find(cand.begin(), cand.end(), t) == cand.end()) cand.push_back(t);
+        hipEvent_t e0, e1;
+        HC_CHECK(hipEventCreate(&e0));
+        HC_CHECK(hipEventCreate(&e1));
+        int best = cand.front();
+        double bestMs = 1e300;
+        for (int t : cand)
+        {
+            if (hc::level4Tile(*L.l4) != t) { L.dropGraph(); HC_CHECK(hipDeviceSynchronize()); hc::level4Retile(*L.l4, t); }
+            // one stage (preparation + RHS into the level's K; U is not modified), warm, then timed twice
+            prepareStage(L, U, nullptr, 0.0, inner, L.W, s);
+            stageRhs(L, L.W, L.K, nullptr, 0.0, nullptr, nullptr, nullptr, s);
+            HC_CHECK(hipEventRecord(e0, s));
+            for (int r = 0; r < 2; ++r) stageRhs(L, L.W, L.K, nullptr, 0.0, nullptr, nullptr, nullptr, s);
+            HC_CHECK(hipEventRecord(e1, s));
+            HC_CHECK(hipEventSynchronize(e1));
+            float ms = 0.f;
+            HC_CHECK(hipEventElapsedTime(&ms, e0, e1));
+            if (ms/2 < bestMs) { bestMs = ms/2; best = t; }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (hc::level4Tile(*L.l4) != best) { L.dropGraph(); HC_CHECK(hipDeviceSynchronize()); hc::level4Retile(*L.l4, best); }
+        *tile = best;
+        if (ms_per_stage) *ms_per_stage = bestMs;
+        return 0;
+    }
+    catch (const std::exception& e) { return fail(std::string("hc_level_autotune: ") + e.what()); }
+}
+
 int hc_level_dt_cfl(hc_level* level, const double* U, double cone_boundary_density, double* dt, hc_stream stream)
 {
     if (!level || !U || !dt) return fail("hc_level_dt_cfl: null argument");
@@ -1103,7 +1209,7 @@ int hc_init_native_boxes(const hc_shell* shell, const hc_options* options, int c
                          const hc_box* boxes, int nbox, int quad, double* USph)
 {
     if (!shell || !options || !boxes || !USph || nbox <= 0) return fail("hc_init_native_boxes: bad argument");
-    if (config < 0 || config > 4) return fail("hc_init_native_boxes: config must be 0..4");
+    if (config < 0 || config > 5) return fail("hc_init_native_boxes: config must be 0..5");
     if (quad < 1 || quad > 8) return fail("hc_init_native_boxes: quad must be 1..8");
     const int nr = shell->n_rad, na = shell->n_ang, g = 5;
     const hc::ShellDims dims{nr, na, g, shell->inner_radius, shell->outer_radius, shell->radial_stretch};
@@ -1123,7 +1229,7 @@ int hc_init_native_boxes(const hc_shell* shell, const hc_options* options, int c
             const int i = lo[0]+ii, j = lo[1]+jj, k = lo[2]+kk;
             double avg[8] = {0, 0, 0, 0, 0, 0, 0, 0};
             double noise = 1.0;
-            if (config >= 3)
+            if (config == 3 || config == 4)
             {
                 // lognormal density factor per cell, sigma 0.05, mean-preserving
                 SplitMix cr{seed ^ (0x9E3779B97F4A7C15ull*((((unsigned long long)b.block*4096+(k+8))*4096+(j+8))*65536+(i+8)))};
@@ -1173,7 +1279,7 @@ int hc_init_level4(const hc_shell* shell, const hc_options* options, int config,
                    int quad, double* JU, double* inner)
 {
     if (!shell || !options || !JU || !inner) return fail("hc_init_level4: null argument");
-    if (config < 0 || config > 4) return fail("hc_init_level4: config must be 0..4");
+    if (config < 0 || config > 5) return fail("hc_init_level4: config must be 0..5");
     if (quad < 1 || quad > 8) return fail("hc_init_level4: quad must be 1..8");
     const int nr = shell->n_rad, na = shell->n_ang, g = 5;
     // inner radial ghosts: native averages of the analytic state
@@ -1215,7 +1321,7 @@ int hc_init_level4(const hc_shell* shell, const hc_options* options, int config,
         for (int i = 0; i < nr; ++i)
         {
             double noise = 1.0;
-            if (config >= 3)
+            if (config == 3 || config == 4)
             {
                 SplitMix cr{seed ^ (0x9E3779B97F4A7C15ull*((((unsigned long long)p*4096+(k+8))*4096+(j+8))*65536+(i+8)))};
                 const double u1 = std::max(cr.uniform(), 1e-300), u2 = cr.uniform();

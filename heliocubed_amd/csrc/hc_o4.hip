@@ -1417,6 +1417,7 @@ struct Level4
     hc_options opt{};
     Prep4View pv{};
     hc_boxop* box = nullptr;
+    int tile = 32;                 // edge of the box-table tiles over the level
     int ncol = 0;
     long long* colDst = nullptr;
     int* colStart = nullptr;
@@ -1484,63 +1485,16 @@ static std::vector<AvgDonor> averageDonors(int na, int p, int j, int k)
     return out;
 }
 
-Level4* level4Create(const hc_shell& shell, const hc_options& opt, int boxSize)
+// Box table over a fourth-order level: tiles of boxSize^3 cells (clipped to
+// the panel) viewing the level arrays, with the seam pairs of their face
+// fluxes. Every tile evaluates the same per-cell arithmetic, so the level
+// result does not depend on boxSize; only speed and scratch size do.
+static hc_boxop* levelBoxTable(const Level4& Lv, int boxSize)
 {
-    std::unique_ptr<Level4> L(new Level4());
-    L->opt = opt;
-    L->dims = {shell.n_rad, shell.n_ang, kG, shell.inner_radius, shell.outer_radius, shell.radial_stretch};
-    const int nr = shell.n_rad, na = shell.n_ang;
-    Prep4View& v = L->pv;
-    v.nr = nr; v.na = na; v.NE = na+12;
-    v.vcs = (long long)nr*na*na; v.vps = 8*v.vcs;
-    v.cJ = na+2*kGC; v.ccs = v.cJ*v.cJ*nr; v.cps = 8*v.ccs;
-    v.sI = nr+10; v.sJ = na+10; v.scs = v.sI*v.sJ*(na+10); v.sps = 8*v.scs;
-    v.gamma = opt.gamma;
-    std::vector<double> det((size_t)v.NE*v.NE), Aavg(9ull*6*v.NE*v.NE), Ainv(9ull*6*v.NE*v.NE);
-    std::vector<double> gx, gw;
-    gaussLegendre01(10, gx, gw);
-    const double h = 1.0/na;
-    for (int k = -6; k < na+6; ++k) for (int j = -6; j < na+6; ++j) det[(size_t)(k+6)*v.NE+(j+6)] = cellDetA(na, j, k);
-    for (int p = 0; p < 6; ++p) for (int k = -6; k < na+6; ++k) for (int j = -6; j < na+6; ++j)
-    {
-        double M[3][3] = {}, Mi[3][3];
-        for (int a = 0; a < (int)gx.size(); ++a) for (int b = 0; b < (int)gx.size(); ++b)
-        {
-            double A[3][3];
-            pointBasis(p, xiToAngle((j+gx[a])*h), xiToAngle((k+gx[b])*h), A);
-            for (int r = 0; r < 3; ++r) for (int c = 0; c < 3; ++c) M[r][c] += gw[a]*gw[b]*A[r][c];
-        }
-        invert3(M, Mi);
-        const size_t x = 9*(((size_t)p*v.NE+(k+6))*v.NE+(j+6));
-        for (int r = 0; r < 3; ++r) for (int c = 0; c < 3; ++c) { Aavg[x+3*r+c] = M[r][c]; Ainv[x+3*r+c] = Mi[r][c]; }
-    }
-    std::vector<double> dvol(nr+13), rmid(nr+13);
-    for (int i = -6; i <= nr+6; ++i) { const RadialCell c = radialCell(L->dims, i); dvol[i+6] = c.dvol; rmid[i+6] = c.rmid; }
-    v.det = upload(det, L->owned); v.Aavg = upload(Aavg, L->owned); v.Ainv = upload(Ainv, L->owned);
-    v.dvol = upload(dvol, L->owned); v.rmid = upload(rmid, L->owned);
-    std::vector<long long> colDst, donorSrc;
-    std::vector<int> colStart{0};
-    std::vector<double> donorW;
-    for (int p = 0; p < 6; ++p) for (int k = -kGC; k < na+kGC; ++k) for (int j = -kGC; j < na+kGC; ++j)
-    {
-        if (j >= 0 && j < na && k >= 0 && k < na) continue;
-        colDst.push_back(p*v.cps + ((long long)(k+kGC)*v.cJ + (j+kGC))*nr);
-        for (const auto& d : averageDonors(na, p, j, k))
-        {
-            donorSrc.push_back(d.q*v.cps + ((long long)(d.k+kGC)*v.cJ + (d.j+kGC))*nr);
-            donorW.push_back(d.w);
-        }
-        colStart.push_back((int)donorW.size());
-    }
-    L->ncol = (int)colDst.size();
-    L->colDst = upload(colDst, L->owned);
-    L->colStart = upload(colStart, L->owned);
-    L->donorSrc = upload(donorSrc, L->owned);
-    L->donorW = upload(donorW, L->owned);
-    HC_THROW(hipMalloc(&L->ang, 6ull*8*na*na*nr*sizeof(double))); L->owned.push_back(L->ang);
-    HC_THROW(hipMalloc(&L->cart, 6ull*v.cps*sizeof(double))); L->owned.push_back(L->cart);
-    HC_THROW(hipMemset(L->cart, 0, 6ull*v.cps*sizeof(double)));
-    // box table over the level: tiles of boxSize^3 viewing the level arrays
+    const Level4* L = &Lv;
+    const Prep4View& v = L->pv;
+    const int nr = v.nr, na = v.na;
+    const hc_options& opt = L->opt;
     std::vector<hc_box> boxes;
     const int bs[3] = {std::min(boxSize, nr), std::min(boxSize, na), std::min(boxSize, na)};
     for (int p = 0; p < 6; ++p)
@@ -1605,11 +1559,80 @@ Level4* level4Create(const hc_shell& shell, const hc_options& opt, int boxSize)
     }
     B->view.seams = upload(pairs, B->owned);
     B->view.nseams = (int)pairs.size();
-    L->box = B.release();
+    return B.release();
+}
+
+Level4* level4Create(const hc_shell& shell, const hc_options& opt, int boxSize)
+{
+    std::unique_ptr<Level4> L(new Level4());
+    L->opt = opt;
+    L->dims = {shell.n_rad, shell.n_ang, kG, shell.inner_radius, shell.outer_radius, shell.radial_stretch};
+    const int nr = shell.n_rad, na = shell.n_ang;
+    Prep4View& v = L->pv;
+    v.nr = nr; v.na = na; v.NE = na+12;
+    v.vcs = (long long)nr*na*na; v.vps = 8*v.vcs;
+    v.cJ = na+2*kGC; v.ccs = v.cJ*v.cJ*nr; v.cps = 8*v.ccs;
+    v.sI = nr+10; v.sJ = na+10; v.scs = v.sI*v.sJ*(na+10); v.sps = 8*v.scs;
+    v.gamma = opt.gamma;
+    std::vector<double> det((size_t)v.NE*v.NE), Aavg(9ull*6*v.NE*v.NE), Ainv(9ull*6*v.NE*v.NE);
+    std::vector<double> gx, gw;
+    gaussLegendre01(10, gx, gw);
+    const double h = 1.0/na;
+    for (int k = -6; k < na+6; ++k) for (int j = -6; j < na+6; ++j) det[(size_t)(k+6)*v.NE+(j+6)] = cellDetA(na, j, k);
+    for (int p = 0; p < 6; ++p) for (int k = -6; k < na+6; ++k) for (int j = -6; j < na+6; ++j)
+    {
+        double M[3][3] = {}, Mi[3][3];
+        for (int a = 0; a < (int)gx.size(); ++a) for (int b = 0; b < (int)gx.size(); ++b)
+        {
+            double A[3][3];
+            pointBasis(p, xiToAngle((j+gx[a])*h), xiToAngle((k+gx[b])*h), A);
+            for (int r = 0; r < 3; ++r) for (int c = 0; c < 3; ++c) M[r][c] += gw[a]*gw[b]*A[r][c];
+        }
+        invert3(M, Mi);
+        const size_t x = 9*(((size_t)p*v.NE+(k+6))*v.NE+(j+6));
+        for (int r = 0; r < 3; ++r) for (int c = 0; c < 3; ++c) { Aavg[x+3*r+c] = M[r][c]; Ainv[x+3*r+c] = Mi[r][c]; }
+    }
+    std::vector<double> dvol(nr+13), rmid(nr+13);
+    for (int i = -6; i <= nr+6; ++i) { const RadialCell c = radialCell(L->dims, i); dvol[i+6] = c.dvol; rmid[i+6] = c.rmid; }
+    v.det = upload(det, L->owned); v.Aavg = upload(Aavg, L->owned); v.Ainv = upload(Ainv, L->owned);
+    v.dvol = upload(dvol, L->owned); v.rmid = upload(rmid, L->owned);
+    std::vector<long long> colDst, donorSrc;
+    std::vector<int> colStart{0};
+    std::vector<double> donorW;
+    for (int p = 0; p < 6; ++p) for (int k = -kGC; k < na+kGC; ++k) for (int j = -kGC; j < na+kGC; ++j)
+    {
+        if (j >= 0 && j < na && k >= 0 && k < na) continue;
+        colDst.push_back(p*v.cps + ((long long)(k+kGC)*v.cJ + (j+kGC))*nr);
+        for (const auto& d : averageDonors(na, p, j, k))
+        {
+            donorSrc.push_back(d.q*v.cps + ((long long)(d.k+kGC)*v.cJ + (d.j+kGC))*nr);
+            donorW.push_back(d.w);
+        }
+        colStart.push_back((int)donorW.size());
+    }
+    L->ncol = (int)colDst.size();
+    L->colDst = upload(colDst, L->owned);
+    L->colStart = upload(colStart, L->owned);
+    L->donorSrc = upload(donorSrc, L->owned);
+    L->donorW = upload(donorW, L->owned);
+    HC_THROW(hipMalloc(&L->ang, 6ull*8*na*na*nr*sizeof(double))); L->owned.push_back(L->ang);
+    HC_THROW(hipMalloc(&L->cart, 6ull*v.cps*sizeof(double))); L->owned.push_back(L->cart);
+    HC_THROW(hipMemset(L->cart, 0, 6ull*v.cps*sizeof(double)));
+    L->box = levelBoxTable(*L, boxSize);
+    L->tile = boxSize;
     return L.release();
 }
 
+void level4Retile(Level4& L, int boxSize)
+{
+    hc_boxop* B = levelBoxTable(L, boxSize);
+    delete L.box;
+    L.box = B;
+    L.tile = boxSize;
+}
+
 void level4Destroy(Level4* L) { delete L; }
+int level4Tile(const Level4& L) { return L.tile; }
 
 void level4Prepare(Level4& L, const double* U0, const double* K, double a_dt, const double* inner, double* US,
                    hipStream_t s)

@@ -138,6 +138,22 @@ int hc_rk4_advance(hc_level* level, double* U, const double* inner_ghosts, doubl
  * individually. No reference counterpart (Proto launches per box). */
 int hc_level_use_graph(hc_level* level, int enable);
 
+/* Tiles of the fourth-order level's box table: the level evaluates its RHS as
+ * a table of tile^3 boxes (clipped to the panel; 32 by default, the box size
+ * of the reference's production layouts). Every tile evaluates the same
+ * per-cell arithmetic on its own ghosted copy, so results are identical for
+ * every tile size; larger tiles recompute fewer ghost cells and take more
+ * scratch. hc_level_set_tile fails on a second-order level (no box table).
+ * hc_level_autotune times one stage RHS of the state U (inner ghosts as for
+ * hc_rk4_advance; U is not modified) for tiles 32, 64 and the whole panel,
+ * keeps the fastest and returns it (and its time per stage, may be NULL).
+ * Box layout is chosen by the user in the reference (MBDisjointBoxLayout);
+ * these have no reference counterpart. */
+int hc_level_set_tile(hc_level* level, int tile);
+int hc_level_tile(const hc_level* level);
+int hc_level_autotune(hc_level* level, const double* U, const double* inner_ghosts, int* tile,
+                      double* ms_per_stage, hc_stream stream);
+
 /* CFL time step of the level state U (valid layout): 1 / max over the valid
  * cells of sum_d (sqrt(gamma p/rho + |B_d|^2/(4 pi rho)) + |v_d|) / width_d,
  * each cell converted to native primitives at second order. The caller
@@ -211,6 +227,9 @@ int hc_init_state(const hc_shell* shell, const hc_options* options, int config,
  * 0.05, angular wavenumbers <= 8, dB = -dv sqrt(4 pi rho)) and a lognormal
  * density factor exp(0.05 g - 0.00125) per cell, and config 4 = config 3 plus
  * a low-beta cone (p = 1e-4 |B|^2/2 within 15 degrees of a seeded axis),
+ * config 5 = the radial wind without pulse with B_r signed by four azimuthal
+ * sectors and a magnetized blob (radius 0.05 (r1-r0) at a seeded point of the
+ * middle radius; density x4, v_r = 2, toroidal B of amplitude 3 |B_r|),
  * averaged over each ghosted
  * cell by a quad^3 Gauss-Legendre rule in xi. Writes the hc_boxop USph layout
  * into a HOST array. */

@@ -78,3 +78,11 @@ def test_multilevel_box_table_validation_without_device():
     lev = (ctypes.c_int * 1)(2)          # no level 2
     assert hc.lib().hc_boxop_create_levels(shells, 2, ctypes.byref(op), tab, lev, 1, ctypes.byref(h)) != 0
     assert b"level" in hc.lib().hc_last_error()
+
+
+def test_tile_calls_reject_null_level_without_device():
+    assert hc.lib().hc_level_set_tile(None, 32) != 0
+    assert b"null" in hc.lib().hc_last_error()
+    assert hc.lib().hc_level_tile(None) == 0
+    t, ms = ctypes.c_int(), ctypes.c_double()
+    assert hc.lib().hc_level_autotune(None, None, None, ctypes.byref(t), ctypes.byref(ms), None) != 0

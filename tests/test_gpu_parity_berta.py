@@ -65,7 +65,8 @@ def _dev(a):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("nr,na,config,eta_d", [(16, 16, 3, 0.02), (12, 20, 1, 0.02), (16, 16, 0, 10.0)])
+@pytest.mark.parametrize("nr,na,config,eta_d", [(16, 16, 3, 0.02), (12, 20, 1, 0.02), (16, 16, 0, 10.0),
+                                               (24, 16, 5, 0.02)])
 def test_berta_level_rhs_matches_oracle(device, nr, na, config, eta_d):
     shell, opts, ju, inner, O = _setup(nr, na, config, eta_d=eta_d)
     W = O.prepare(ju, inner)
