@@ -212,7 +212,7 @@ __global__ void __launch_bounds__(kB) k_b_wp(O4View v, const double* __restrict_
 // floors, the Roe repair, then the point fluxes and total pressures of both
 // sides summed. Reconstruction arithmetic in S (the normalization and the
 // rescaling stay FP64).
-template <int D, typename S_>
+template <int D, typename S_, bool ROE>
 __global__ void __launch_bounds__(kB) k_b_faces(O4View v, double* __restrict__ S)
 {
     const BoxDesc& b = blockBoxDesc(v);
@@ -275,7 +275,7 @@ __global__ void __launch_bounds__(kB) k_b_faces(O4View v, double* __restrict__ S
     }
     l[0] = refMax(l[0], hc::kRhoFloor); r[0] = refMax(r[0], hc::kRhoFloor);
     l[4] = refMax(l[4], 1.0e-12); r[4] = refMax(r[4], 1.0e-12);
-    if (v.roe && !(usablePrimitive(l) && usablePrimitive(r)))
+    if (ROE && !(usablePrimitive(l) && usablePrimitive(r)))
     {
         double cl[8], cr[8];
 #pragma unroll
@@ -302,7 +302,7 @@ __global__ void __launch_bounds__(kB) k_b_faces(O4View v, double* __restrict__ S
     }
     P[I(F_THF, x[0], x[1], x[2])] = thF;
     P[I(F_PTOT, x[0], x[1], x[2])] = (l[4] + meL) + (r[4] + meR);
-    if (!v.roe)
+    if constexpr (!ROE)
     {
         // Rusanov: pointwise conserved jump R - L times the radial point area
         // and the speed of the point states (:1589-1613, :1653-1725)
@@ -406,7 +406,7 @@ __global__ void __launch_bounds__(kB) k_b_map(O4View v, double* __restrict__ S)
 // mapped products, pressure force ntrow <(p_tot averaged) adjDr>, the
 // theta-aware averaged Roe dissipation with its metric products, and
 // F = (Sum - D)/2.
-template <int D>
+template <int D, bool ROE>
 __global__ void __launch_bounds__(kB) k_b_flux(O4View v, double* __restrict__ S)
 {
     const BoxDesc& b = blockBoxDesc(v);
@@ -483,7 +483,7 @@ __global__ void __launch_bounds__(kB) k_b_flux(O4View v, double* __restrict__ S)
         Davg[c] = smooth ? Fv(F_DPT+c, x) + lap/24.0 : Fv(F_DPT+c, x);
     }
     double Fd[8];
-    if (!v.roe)
+    if constexpr (!ROE)
     {
         // Rusanov: the averaged jump mapped by the face metric averages, times the speed
         double dAA[9];
@@ -528,11 +528,13 @@ __global__ void __launch_bounds__(kB) k_b_flux(O4View v, double* __restrict__ S)
 template <int D, typename S_>
 void bertaDirectionPass(const O4View& v, dim3 grid, hipStream_t s, double* scratch)
 {
-    hipLaunchKernelGGL((k_b_faces<D, S_>), grid, dim3(kB), 0, s, v, scratch);
+    if (v.roe) hipLaunchKernelGGL((k_b_faces<D, S_, true>), grid, dim3(kB), 0, s, v, scratch);
+    else hipLaunchKernelGGL((k_b_faces<D, S_, false>), grid, dim3(kB), 0, s, v, scratch);
     hipLaunchKernelGGL(k_b_avg<D>, grid, dim3(kB), 0, s, v, scratch);
     if (v.roe) hipLaunchKernelGGL((k_o4_roe<D, S_>), grid, dim3(kB), 0, s, v, scratch);
     hipLaunchKernelGGL(k_b_map<D>, grid, dim3(kB), 0, s, v, scratch);
-    hipLaunchKernelGGL(k_b_flux<D>, grid, dim3(kB), 0, s, v, scratch);
+    if (v.roe) hipLaunchKernelGGL((k_b_flux<D, true>), grid, dim3(kB), 0, s, v, scratch);
+    else hipLaunchKernelGGL((k_b_flux<D, false>), grid, dim3(kB), 0, s, v, scratch);
 }
 
 // Cell stage and the three direction passes of the Berta operator.

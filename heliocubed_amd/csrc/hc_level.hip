@@ -214,7 +214,7 @@ k_panel_states(LevelView v, const double* __restrict__ W, double* __restrict__ P
 }
 
 // 5. Face fluxes of direction D for every face of the level.
-template <int D>
+template <int D, bool ROE>
 __global__ void __launch_bounds__(kBlock)
 k_face_flux(LevelView v, const double* __restrict__ W, const double* __restrict__ PS,
             double* __restrict__ F)
@@ -253,7 +253,7 @@ k_face_flux(LevelView v, const double* __restrict__ W, const double* __restrict_
     const double* fr = v.frame[D] + 10LL*(((long long)p*fk + k)*fj + j);
     const double area = (D == 0 ? rad(v,i)[2] : rad(v,i)[3])*fr[9];
     double out[8];
-    hc::o2FaceFluxMixed<double>(out, l, rr, fr, area, v.gamma, v.entropy, v.roe != 0);
+    hc::o2FaceFluxMixed<double, ROE ? 2 : 1>(out, l, rr, fr, area, v.gamma, v.entropy, ROE);
     const long long cstride = (long long)fi*fj*fk;
     const long long base = (long long)p*8*cstride + ((long long)k*fj + j)*fi + i;
 #pragma unroll
@@ -333,7 +333,7 @@ k_divergence(LevelView v, const double* __restrict__ F0, const double* __restric
 // 8. Seam faces: both panels' fluxes of every physical seam face from the
 // two interior states, reconciled 1/2 (F_local + sign F_adjacent), stored for
 // both panels in SF[p][d-1][s][c][t][i]. Each physical face is handled once.
-template <typename S>
+template <typename S, bool ROE>
 __global__ void __launch_bounds__(kBlock)
 k_seam_flux(LevelView v, const double* __restrict__ PS, double* __restrict__ SF)
 {
@@ -363,8 +363,8 @@ k_seam_flux(LevelView v, const double* __restrict__ PS, double* __restrict__ SF)
         const int jj = dd == 1 ? f : tg, kk = dd == 1 ? tg : f;
         const double* fr = v.frame[dd] + 10LL*(((long long)pp*fk + kk)*fj + jj);
         const double area = rad(v,i)[3]*fr[9];
-        if (ss == 0) hc::o2FaceFluxMixed<S>(out, adj, own, fr, area, v.gamma, v.entropy, v.roe != 0);
-        else hc::o2FaceFluxMixed<S>(out, own, adj, fr, area, v.gamma, v.entropy, v.roe != 0);
+        if (ss == 0) hc::o2FaceFluxMixed<S, ROE ? 2 : 1>(out, adj, own, fr, area, v.gamma, v.entropy, ROE);
+        else hc::o2FaceFluxMixed<S, ROE ? 2 : 1>(out, own, adj, fr, area, v.gamma, v.entropy, ROE);
     };
     double fp[8], fq[8];
     flux(p, d, s, tt, sp, sqs, fp);
@@ -387,7 +387,7 @@ constexpr int kTileCells = kTI*kTJ*kTK;
 constexpr int kMax2(int a, int b) { return a > b ? a : b; }
 constexpr int kMaxFaces = kMax2(kMax2((kTI+1)*kTJ*kTK, kTI*(kTJ+1)*kTK), kTI*kTJ*(kTK+1));
 
-template <typename S>
+template <typename S, bool ROE>
 __global__ void __launch_bounds__(kTileCells)
 k_rhs_fused(LevelView v, const double* __restrict__ W, const double* __restrict__ SF,
             double* __restrict__ K, double* __restrict__ U, double b_dt, int tilesI, int tilesJ, int tilesK)
@@ -438,7 +438,7 @@ k_rhs_fused(LevelView v, const double* __restrict__ W, const double* __restrict_
                     const int fjd = v.na + (d == 1), fkd = v.na + (d == 2);
                     const double* fr = v.frame[d] + 10LL*(((long long)p*fkd + k)*fjd + j);
                     const double area = (d == 0 ? rad(v,i)[2] : rad(v,i)[3])*fr[9];
-                    hc::o2FaceFluxMixed<S>(out, l, rr, fr, area, v.gamma, v.entropy, v.roe != 0);
+                    hc::o2FaceFluxMixed<S, ROE ? 2 : 1>(out, l, rr, fr, area, v.gamma, v.entropy, ROE);
                 }
             }
             else
@@ -704,10 +704,13 @@ void stageRhsFused(hc_level& L, const double* W, double* K, double* U, double b_
     P.run("k_panel_states", s, [&] {
         hipLaunchKernelGGL(k_panel_states, dim3(blocksFor(24LL*v.na*v.nr)), dim3(kBlock), 0, s, v, W, L.PS); });
     P.run("k_seam_flux", s, [&] {
-        hipLaunchKernelGGL(k_seam_flux<S>, dim3(blocksFor(24LL*v.na*v.nr)), dim3(kBlock), 0, s, v, L.PS, L.SF); });
+        if (v.roe) hipLaunchKernelGGL((k_seam_flux<S, true>), dim3(blocksFor(24LL*v.na*v.nr)), dim3(kBlock), 0, s, v, L.PS, L.SF);
+        else hipLaunchKernelGGL((k_seam_flux<S, false>), dim3(blocksFor(24LL*v.na*v.nr)), dim3(kBlock), 0, s, v, L.PS, L.SF); });
     const int tI = (v.nr + kTI - 1)/kTI, tJ = (v.na + kTJ - 1)/kTJ, tK = (v.na + kTK - 1)/kTK;
     P.run("k_rhs_fused", s, [&] {
-        hipLaunchKernelGGL(k_rhs_fused<S>, dim3(6u*tI*tJ*tK), dim3(kTileCells), 0, s,
+        if (v.roe) hipLaunchKernelGGL((k_rhs_fused<S, true>), dim3(6u*tI*tJ*tK), dim3(kTileCells), 0, s,
+                           v, W, L.SF, K, U, b_dt, tI, tJ, tK);
+        else hipLaunchKernelGGL((k_rhs_fused<S, false>), dim3(6u*tI*tJ*tK), dim3(kTileCells), 0, s,
                            v, W, L.SF, K, U, b_dt, tI, tJ, tK); });
     HC_CHECK(hipGetLastError());
 }
@@ -734,11 +737,14 @@ void stageRhs(hc_level& L, const double* W, double* K, double* U, double b_dt,
     P.run("k_panel_states", s, [&] {
         hipLaunchKernelGGL(k_panel_states, dim3(blocksFor(24LL*v.na*v.nr)), dim3(kBlock), 0, s, v, W, L.PS); });
     P.run("k_face_flux_0", s, [&] {
-        hipLaunchKernelGGL(k_face_flux<0>, dim3(blocksFor(6LL*(v.nr+1)*v.na*v.na)), dim3(kBlock), 0, s, v, W, L.PS, F[0]); });
+        if (v.roe) hipLaunchKernelGGL((k_face_flux<0, true>), dim3(blocksFor(6LL*(v.nr+1)*v.na*v.na)), dim3(kBlock), 0, s, v, W, L.PS, F[0]);
+        else hipLaunchKernelGGL((k_face_flux<0, false>), dim3(blocksFor(6LL*(v.nr+1)*v.na*v.na)), dim3(kBlock), 0, s, v, W, L.PS, F[0]); });
     P.run("k_face_flux_1", s, [&] {
-        hipLaunchKernelGGL(k_face_flux<1>, dim3(blocksFor(6LL*v.nr*(v.na+1)*v.na)), dim3(kBlock), 0, s, v, W, L.PS, F[1]); });
+        if (v.roe) hipLaunchKernelGGL((k_face_flux<1, true>), dim3(blocksFor(6LL*v.nr*(v.na+1)*v.na)), dim3(kBlock), 0, s, v, W, L.PS, F[1]);
+        else hipLaunchKernelGGL((k_face_flux<1, false>), dim3(blocksFor(6LL*v.nr*(v.na+1)*v.na)), dim3(kBlock), 0, s, v, W, L.PS, F[1]); });
     P.run("k_face_flux_2", s, [&] {
-        hipLaunchKernelGGL(k_face_flux<2>, dim3(blocksFor(6LL*v.nr*v.na*(v.na+1))), dim3(kBlock), 0, s, v, W, L.PS, F[2]); });
+        if (v.roe) hipLaunchKernelGGL((k_face_flux<2, true>), dim3(blocksFor(6LL*v.nr*v.na*(v.na+1))), dim3(kBlock), 0, s, v, W, L.PS, F[2]);
+        else hipLaunchKernelGGL((k_face_flux<2, false>), dim3(blocksFor(6LL*v.nr*v.na*(v.na+1))), dim3(kBlock), 0, s, v, W, L.PS, F[2]); });
     P.run("k_seam_reconcile", s, [&] {
         hipLaunchKernelGGL(k_seam_reconcile, dim3(blocksFor(24LL*v.na*v.nr)), dim3(kBlock), 0, s, v, F[1], F[2]); });
     P.run("k_divergence", s, [&] {

@@ -527,7 +527,7 @@ __device__ __forceinline__ void faceBar(const BoxIdx& I, const double* P, int d,
 
 // 4. Limited, flattened and repaired face states; point states by transverse
 // deconvolution. Faces [lo_d, hi_d+1] x transverse [lo-2, hi+2].
-template <int D>
+template <int D, bool ROE>
 __global__ void __launch_bounds__(kB) k_o4_faces(O4View v, double* __restrict__ S)
 {
     const BoxDesc& b = blockBoxDesc(v);
@@ -593,7 +593,7 @@ __global__ void __launch_bounds__(kB) k_o4_faces(O4View v, double* __restrict__ 
 #pragma unroll
         for (int q = 0; q < 8; ++q) corr[q] = lapb[q]*(1.0/24.0);
     }
-    if (v.roe)
+    if constexpr (ROE)
     {   // the Roe face-state repair (riemann_solver 2 only)
         double pl[8], pr[8], cl[8], cr[8];
         int m1[3] = {x[0], x[1], x[2]}; m1[D] -= 1;
@@ -617,7 +617,7 @@ __global__ void __launch_bounds__(kB) k_o4_faces(O4View v, double* __restrict__ 
     }
     P[I(F_P0L, x[0], x[1], x[2])] = lf[4];
     P[I(F_P0R, x[0], x[1], x[2])] = rf[4];
-    if (!v.roe)
+    if constexpr (!ROE)
         P[I(F_SPD, x[0], x[1], x[2])] = rusanovSpeedO4(lf, rf, v.faceA[D] + 9*angIdx(b, x[1], x[2]),
                                                        v.gamma, D, false);
 }
@@ -853,7 +853,7 @@ __global__ void __launch_bounds__(kB) k_o4_map(O4View v, double* __restrict__ S)
 // radial-area tensor product of the mapped products, the pressure force
 // through the cofactor row, the averaged Roe dissipation, and
 // F = (F_L + F_R - D)/2.
-template <int D>
+template <int D, bool ROE>
 __global__ void __launch_bounds__(kB) k_o4_flux(O4View v, double* __restrict__ S)
 {
     const BoxDesc& b = blockBoxDesc(v);
@@ -918,7 +918,7 @@ __global__ void __launch_bounds__(kB) k_o4_flux(O4View v, double* __restrict__ S
         for (int c = 0; c < 8; ++c) Fsum[c] = side == 0 ? Fs[c] : Fsum[c] + Fs[c];
     }
     double Fd[8];
-    if (!v.roe)
+    if constexpr (!ROE)
     {
         // Rusanov: face-averaged conserved values of both sides, their
         // difference mapped by the face metric averages, times the speed
@@ -1194,13 +1194,15 @@ void buildBoxop(hc_boxop& B, const hc_box* boxes, const int* boxLevel, int nbox)
 template <int D, typename S_>
 void directionPass(const O4View& v, dim3 grid, hipStream_t s, double* scratch)
 {
-    hipLaunchKernelGGL(k_o4_faces<D>, grid, dim3(kB), 0, s, v, scratch);
+    if (v.roe) hipLaunchKernelGGL((k_o4_faces<D, true>), grid, dim3(kB), 0, s, v, scratch);
+    else hipLaunchKernelGGL((k_o4_faces<D, false>), grid, dim3(kB), 0, s, v, scratch);
     hipLaunchKernelGGL(k_o4_bar<D>, grid, dim3(kB), 0, s, v, scratch);
     hipLaunchKernelGGL(k_o4_phys<D>, grid, dim3(kB), 0, s, v, scratch);
     if (v.roe) hipLaunchKernelGGL((k_o4_roe<D, S_>), grid, dim3(kB), 0, s, v, scratch);
     else hipLaunchKernelGGL(k_o4_rusq<D>, grid, dim3(kB), 0, s, v, scratch);
     hipLaunchKernelGGL(k_o4_map<D>, grid, dim3(kB), 0, s, v, scratch);
-    hipLaunchKernelGGL(k_o4_flux<D>, grid, dim3(kB), 0, s, v, scratch);
+    if (v.roe) hipLaunchKernelGGL((k_o4_flux<D, true>), grid, dim3(kB), 0, s, v, scratch);
+    else hipLaunchKernelGGL((k_o4_flux<D, false>), grid, dim3(kB), 0, s, v, scratch);
 }
 
 template <typename S_>

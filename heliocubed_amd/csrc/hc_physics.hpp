@@ -395,7 +395,9 @@ HC_HD FaceFrame makeFaceFrame(const double n[3])
 // with the Riemann dissipation evaluated in precision S: frame
 // rotation, central flux, area and the back-rotation stay FP64. S = double
 // reproduces o2FaceFlux operation for operation.
-template <typename S>
+// RS selects the dissipation at compile time: 1 Rusanov, 2 Roe (Rusanov
+// where the Roe fan is inadmissible), 0 by the run-time flag roe.
+template <typename S, int RS = 0>
 HC_HD void o2FaceFluxMixed(double out[8], const double l[8], const double r[8], const double frame[9],
                            double area, double gamma, double entropy, bool roe)
 {
@@ -417,7 +419,12 @@ HC_HD void o2FaceFluxMixed(double out[8], const double l[8], const double r[8], 
         }
     }
     S ds[8];
-    if (!roe || !roe8WaveDissipation<S>(ds, loc[0], loc[1], gamma, entropy))
+    if constexpr (RS == 1) rusanovDissipation<S>(ds, loc[0], loc[1], gamma);
+    else if constexpr (RS == 2)
+    {
+        if (!roe8WaveDissipation<S>(ds, loc[0], loc[1], gamma, entropy)) rusanovDissipation<S>(ds, loc[0], loc[1], gamma);
+    }
+    else if (!roe || !roe8WaveDissipation<S>(ds, loc[0], loc[1], gamma, entropy))
         rusanovDissipation<S>(ds, loc[0], loc[1], gamma);
     double num[8];
 #pragma unroll
