@@ -191,6 +191,9 @@ def main():
     ap.add_argument("--workload", default="shell-6x128x64x64-o4-roe", choices=sorted(WORKLOADS))
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--only-timed", action="store_true",
+                    help="run the warmup and timed steps only (no graph replay, autotuner, CFL or mixed-error "
+                         "runs and no CPU baseline): the command profiled under rocprofv3")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -233,16 +236,17 @@ def main():
     L.profile(False)
     if not torch.isfinite(U).all():
         raise RuntimeError("non-finite state after the timed steps")
-    # the same steps replayed from the captured hipGraph of one RK4 step
-    # (per-kernel events are off in this mode), outside the timed region above
-    L.rk4_advance(U, G, dt, 1)
-    torch.cuda.synchronize()
-    tg = time.perf_counter()
-    L.rk4_advance(U, G, dt, args.steps)
-    torch.cuda.synchronize()
-    graph_ms = (time.perf_counter() - tg) / args.steps * 1e3
-    tuned = None
-    if wl["order"] == 4:
+    graph_ms = tuned = dt_cfl = cfl_ms = None
+    if not args.only_timed:
+        # the same steps replayed from the captured hipGraph of one RK4 step
+        # (per-kernel events are off in this mode), outside the timed region above
+        L.rk4_advance(U, G, dt, 1)
+        torch.cuda.synchronize()
+        tg = time.perf_counter()
+        L.rk4_advance(U, G, dt, args.steps)
+        torch.cuda.synchronize()
+        graph_ms = (time.perf_counter() - tg) / args.steps * 1e3
+    if wl["order"] == 4 and not args.only_timed:
         # box-table tile picked by the autotuner (results are identical for
         # every tile); timed like the steps above, outside the timed region
         tile, tune_ms = L.autotune(U, G)
@@ -254,16 +258,17 @@ def main():
         ta = time.perf_counter() - ta
         tuned = {"tile": tile, "rhs_ms_per_stage": tune_ms, "ms_per_step": ta / args.steps * 1e3,
                  "value": 6 * wl["n_rad"] * wl["n_ang"] ** 2 * 4 * args.steps / ta}
-    # level CFL time step (the reference's dtCFL, taken once per step by the
-    # driver), timed on its own with the host read-back of dt
-    L.dt_cfl(U)
-    tc = time.perf_counter()
-    for _ in range(10):
-        dt_cfl = L.dt_cfl(U)
-    cfl_ms = (time.perf_counter() - tc) / 10 * 1e3
+    if not args.only_timed:
+        # level CFL time step (the reference's dtCFL, taken once per step by
+        # the driver), timed on its own with the host read-back of dt
+        L.dt_cfl(U)
+        tc = time.perf_counter()
+        for _ in range(10):
+            dt_cfl = L.dt_cfl(U)
+        cfl_ms = (time.perf_counter() - tc) / 10 * 1e3
 
     mixed_err = None
-    if U_init is not None:
+    if U_init is not None and not args.only_timed:
         # mixed precision against FP64 from the same initial state, after 1
         # and after 100 RK4 steps: per-variable relative L-infinity and L2
         # error of the state (outside the timed regions)
@@ -345,7 +350,7 @@ def main():
                              "frac": flop / avg_s / 1e12 / 78.6, "flop_per_launch": flop, "source": pmc_path}
     if mixed_err is not None:
         line["mixed_vs_fp64"] = mixed_err
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and not args.only_timed:
         line["cpu_baseline"] = cpu_baseline(wl, ju, inner, dt)
     print(json.dumps(line), flush=True)
 
