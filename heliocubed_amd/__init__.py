@@ -74,6 +74,7 @@ _SIGNATURES = [
     ("hc_level_tile", _I, [_P]),
     ("hc_level_autotune", _I, [_P, _P, _P, _P, _P, _P]),
     ("hc_level_valid_state", _I, [_P, _P, _P, _P, _P]),
+    ("hc_level_inner_ghosts_from_bc", _I, [_P, _P, _P, _P]),
     ("hc_init_state", _I, [ctypes.POINTER(HcShell), ctypes.POINTER(HcOptions), _I, ctypes.c_ulonglong, _P, _P]),
     ("hc_geom_cell_detA", _D, [_I, _I, _I]),
     ("hc_geom_face_normal", None, [_I, _I, _I, _I, _I, _P]),
@@ -310,6 +311,15 @@ class Level:
         _check(lib().hc_level_dt_cfl(self._handle, _ptr(U), float(cone_boundary_density), ctypes.byref(out),
                                      self._stream()))
         return out.value
+
+    def inner_ghosts_from_bc(self, bc, inner):
+        """Inner radial ghosts (the rk4_advance / prepare_stage `inner` array)
+        from one radial strip of boundary-file primitives bc[6][8][na+2g][na+2g]."""
+        g = self.ghost()
+        n = self.shell.n_ang + 2 * g
+        self._require(bc, (6, 8, n, n), "bc")
+        self._require(inner, (6, 8, n, n, g), "inner")
+        _check(lib().hc_level_inner_ghosts_from_bc(self._handle, _ptr(bc), _ptr(inner), self._stream()))
 
     def valid_state(self, U):
         """(valid, first_invalid): every value finite and every density > 0

@@ -53,6 +53,12 @@ CflTable* cflCreate(const ShellDims& dims, double gamma);
 void cflDestroy(CflTable* T);
 const double* cflLaunch(CflTable& T, const double* JU, double coneDensity, hipStream_t s);
 const long long* stateCheckLaunch(CflTable& T, const double* U, hipStream_t s);
+
+// Inner radial ghosts from boundary-file data (defined with its kernel).
+struct BcTable;
+BcTable* bcCreate(const ShellDims& dims, int order, double gamma);
+void bcDestroy(BcTable* T);
+void bcLaunch(BcTable& T, const double* bc, double* inner, hipStream_t s);
 } // namespace hc
 
 #define HC_THROW(call)                                                              \

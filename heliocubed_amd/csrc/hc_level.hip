@@ -517,6 +517,7 @@ struct hc_level
     KernelProfile profile;
     hc::Level4* l4 = nullptr;     // fourth-order level (spatial_order 4)
     hc::CflTable* cfl = nullptr;  // CFL geometry, built on the first time-step query
+    hc::BcTable* bcTable = nullptr; // boundary-file ghost conversion, built on first use
     hc::ShellDims dims{};
     hc_options opt{};
     LevelView view{};
@@ -557,6 +558,7 @@ struct hc_level
         if (capStream) (void)hipStreamDestroy(capStream);
         hc::level4Destroy(l4);
         hc::cflDestroy(cfl);
+        hc::bcDestroy(bcTable);
         for (void* p : owned) (void)hipFree(p);
     }
     template <typename T> T* own(T* p) { owned.push_back(p); return p; }
@@ -1106,6 +1108,19 @@ int hc_level_dt_cfl(hc_level* level, const double* U, double cone_boundary_densi
         return 0;
     }
     catch (const std::exception& e) { return fail(std::string("hc_level_dt_cfl: ") + e.what()); }
+}
+
+int hc_level_inner_ghosts_from_bc(hc_level* level, const double* bc, double* inner, hc_stream stream)
+{
+    if (!level || !bc || !inner) return fail("hc_level_inner_ghosts_from_bc: null argument");
+    try
+    {
+        hc_level& L = *level;
+        if (!L.bcTable) L.bcTable = hc::bcCreate(L.dims, L.opt.spatial_order, L.opt.gamma);
+        hc::bcLaunch(*L.bcTable, bc, inner, static_cast<hipStream_t>(stream));
+        return 0;
+    }
+    catch (const std::exception& e) { return fail(std::string("hc_level_inner_ghosts_from_bc: ") + e.what()); }
 }
 
 int hc_level_valid_state(hc_level* level, const double* U, int* valid, long long* first_invalid, hc_stream stream)

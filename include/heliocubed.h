@@ -169,6 +169,18 @@ int hc_level_autotune(hc_level* level, const double* U, const double* inner_ghos
 int hc_level_dt_cfl(hc_level* level, const double* U, double cone_boundary_density, double* dt,
                     hc_stream stream);
 
+/* Inner radial ghosts from boundary-file data: bc[6][8][na+2g][na+2g] is one
+ * radial strip of primitive values per ghosted angular cell, in the file's
+ * units and order (rho [amu/cm^3], v_r, v_phi, v_theta [cm/s], p [pdyne/cm^2],
+ * B_r, B_phi, B_theta [uG]), as MHDReader::BoxData_to_BC supplies it for
+ * BC_conserved_averages 0. Writes the level's inner ghosts
+ * inner[6][8][na+2g][na+2g][g] (native conserved averages at order 4,
+ * Cartesian primitives at order 2), radially scaled from r_BC = inner_radius
+ * to each ghost layer. Replaces the file branch of PreStagePatch
+ * (src/BoxOp_PreStage.H:161-317) and, at order 2, the ghost conversion of
+ * src/BoxOp_SecondOrderPreStage.H:194-214. Stream-ordered. */
+int hc_level_inner_ghosts_from_bc(hc_level* level, const double* bc, double* inner_ghosts, hc_stream stream);
+
 /* Conserved-state check of the level state U (valid layout): *valid = 1 when
  * every value is finite and every density is > 0, else 0, with
  * *first_invalid (may be NULL) the smallest flat index of an offending value
@@ -194,7 +206,9 @@ int hc_level_valid_state(hc_level* level, const double* U, int* valid, long long
  *         components, 8*n0*n1*n2 per box; ACCUMULATED into (zero it first);
  *   F_d   face fluxes of the valid box extruded in d (optional, NULL skips).
  * hc_boxop_count returns the element count of USph (what 0), rhs (1) and
- * F_0..F_2 (2..4). */
+ * F_0..F_2 (2..4). Every box must lie inside its panel and span at least 4
+ * cells in each direction (the width of the limiter's one-sided stencils);
+ * an empty table is rejected. */
 int hc_boxop_create(const hc_shell* shell, const hc_options* options, const hc_box* boxes, int nbox,
                     hc_boxop** out);
 /* Box table over several refinement levels of an AMR hierarchy (the batched

@@ -79,6 +79,7 @@ def lib() -> ctypes.CDLL:
             "orc_plm_berta": (None, [P, P]),
             "orc_dt_cfl": (D, [cfgp, I, P, D]),
             "orc_first_invalid": (ctypes.c_longlong, [P, I, I]),
+            "orc_inner_ghosts_from_bc": (None, [cfgp, I, P, P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -233,6 +234,19 @@ def first_invalid(JU):
     ju = np.ascontiguousarray(JU, dtype=np.float64)
     na, nr = ju.shape[2], ju.shape[4]
     return int(lib().orc_first_invalid(_p(ju), nr, na))
+
+
+def inner_ghosts_from_bc(n_rad, n_ang, bc, order=4, gamma=5.0 / 3.0, r0=21.5, r1=215.0):
+    """Inner radial ghosts [6][8][na+2g][na+2g][g] of a level (g = 2 at order 2,
+    5 at order 4) from one strip of boundary-file primitives bc[6][8][na+2g][na+2g]."""
+    g = 2 if order == 2 else 5
+    cfg = OrcCfg(n_rad, n_ang, g, r0, r1, 0.0, gamma, 1, 2, 0.3, 1, 0.02, 0)
+    n = n_ang + 2 * g
+    b = np.ascontiguousarray(bc, dtype=np.float64)
+    assert b.shape == (6, 8, n, n), b.shape
+    out = np.zeros((6, 8, n, n, g))
+    lib().orc_inner_ghosts_from_bc(ctypes.byref(cfg), order, _p(b), _p(out))
+    return out
 
 
 def basis(p, alpha, beta):

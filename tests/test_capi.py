@@ -86,3 +86,12 @@ def test_tile_calls_reject_null_level_without_device():
     assert hc.lib().hc_level_tile(None) == 0
     t, ms = ctypes.c_int(), ctypes.c_double()
     assert hc.lib().hc_level_autotune(None, None, None, ctypes.byref(t), ctypes.byref(ms), None) != 0
+
+
+def test_empty_box_table_is_rejected_without_device():
+    h = ctypes.c_void_p()
+    sh = hc.Shell(32, 16).c()
+    op = hc.Options(spatial_order=4, limiter=1, riemann_solver=2).c()
+    tab = hc.box_table([(0, (0, 0, 0), (7, 7, 7))])
+    assert hc.lib().hc_boxop_create(ctypes.byref(sh), ctypes.byref(op), tab, 0, ctypes.byref(h)) != 0
+    assert b"bad argument" in hc.lib().hc_last_error()
