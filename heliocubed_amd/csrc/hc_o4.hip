@@ -525,15 +525,27 @@ __device__ __forceinline__ void faceBar(const BoxIdx& I, const double* P, int d,
     for (int q = 0; q < 8; ++q) wf[q] = 0.5*P[I(F_WB+q, a, b, c)] + 0.5*P[I(F_WB+q, i, j, k)];
 }
 
+// Transverse extent of the face states and point fluxes of direction D: the
+// mapped products (k_o4_map) take one more transverse neighbour on each side of
+// faces that reach one radial layer beyond the box when D is angular, so the
+// point fluxes are needed on [lo-1, hi+1] transversally and [lo-2, hi+2] in the
+// radial index when it is transverse (D = 1, 2).
+template <int D>
+__device__ __forceinline__ void pointFaceRegion(const BoxDesc& b, int grow, int L[3], int H[3])
+{
+    for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]-1-grow; H[d] = b.hi[d]+1+grow; }
+    if (D != 0) { L[0] -= 1; H[0] += 1; }
+    L[D] = b.lo[D]; H[D] = b.hi[D]+1;
+}
+
 // 4. Limited, flattened and repaired face states; point states by transverse
-// deconvolution. Faces [lo_d, hi_d+1] x transverse [lo-2, hi+2].
+// deconvolution, on the point-flux faces (pointFaceRegion).
 template <int D, bool ROE>
 __global__ void __launch_bounds__(kB) k_o4_faces(O4View v, double* __restrict__ S)
 {
     const BoxDesc& b = blockBoxDesc(v);
     int L[3], H[3], x[3];
-    for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]-2; H[d] = b.hi[d]+2; }
-    L[D] = b.lo[D]; H[D] = b.hi[D]+1;
+    pointFaceRegion<D>(b, 0, L, H);
     if (!regionPoint(b, L, H, boxThread(v, b), x)) return;
     const BoxIdx I(b);
     double* const P = S + b.sOff;
@@ -638,15 +650,14 @@ __device__ void barFlux(const O4View& v, const BoxDesc& b, const BoxIdx& I, cons
 }
 
 // 5a. Physical flux of the face-average state WFaceBar (F_FB), its magnetic
-// energy (F_MEF) and total pressure pBar + meBar (F_P0B) on the faces
-// [lo_d, hi_d+1] x transverse [lo-3, hi+3].
+// energy (F_MEF) and total pressure pBar + meBar (F_P0B) on the point-flux
+// faces grown by one transversally (their Laplacian in k_o4_phys).
 template <int D>
 __global__ void __launch_bounds__(kB) k_o4_bar(O4View v, double* __restrict__ S)
 {
     const BoxDesc& b = blockBoxDesc(v);
     int L[3], H[3], x[3];
-    for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]-3; H[d] = b.hi[d]+3; }
-    L[D] = b.lo[D]; H[D] = b.hi[D]+1;
+    pointFaceRegion<D>(b, 1, L, H);
     if (!regionPoint(b, L, H, boxThread(v, b), x)) return;
     const BoxIdx I(b);
     double* const P = S + b.sOff;
@@ -659,15 +670,14 @@ __global__ void __launch_bounds__(kB) k_o4_bar(O4View v, double* __restrict__ S)
 }
 
 // 5b. Physical fluxes of both face states (point flux + transverse Laplacian
-// of the face-average flux / 24) and the face-average total pressure p + me.
-// Faces [lo_d, hi_d+1] x transverse [lo-2, hi+2].
+// of the face-average flux / 24) and the face-average total pressure p + me,
+// on the point-flux faces (pointFaceRegion).
 template <int D>
 __global__ void __launch_bounds__(kB) k_o4_phys(O4View v, double* __restrict__ S)
 {
     const BoxDesc& b = blockBoxDesc(v);
     int L[3], H[3], x[3];
-    for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]-2; H[d] = b.hi[d]+2; }
-    L[D] = b.lo[D]; H[D] = b.hi[D]+1;
+    pointFaceRegion<D>(b, 0, L, H);
     if (!regionPoint(b, L, H, boxThread(v, b), x)) return;
     const BoxIdx I(b);
     double* const P = S + b.sOff;
