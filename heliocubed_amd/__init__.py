@@ -75,6 +75,7 @@ _SIGNATURES = [
     ("hc_level_autotune", _I, [_P, _P, _P, _P, _P, _P]),
     ("hc_level_valid_state", _I, [_P, _P, _P, _P, _P]),
     ("hc_level_inner_ghosts_from_bc", _I, [_P, _P, _P, _P]),
+    ("hc_level_inner_ghosts_outflow", _I, [_P, _D, _D, _P, _P]),
     ("hc_init_state", _I, [ctypes.POINTER(HcShell), ctypes.POINTER(HcOptions), _I, ctypes.c_ulonglong, _P, _P]),
     ("hc_geom_cell_detA", _D, [_I, _I, _I]),
     ("hc_geom_face_normal", None, [_I, _I, _I, _I, _I, _P]),
@@ -320,6 +321,14 @@ class Level:
         self._require(bc, (6, 8, n, n), "bc")
         self._require(inner, (6, 8, n, n, g), "inner")
         _check(lib().hc_level_inner_ghosts_from_bc(self._handle, _ptr(bc), _ptr(inner), self._stream()))
+
+    def inner_ghosts_outflow(self, inner, strength: float, time: float):
+        """Inner radial ghosts of the radial-outflow test boundary at `time` (s)."""
+        g = self.ghost()
+        n = self.shell.n_ang + 2 * g
+        self._require(inner, (6, 8, n, n, g), "inner")
+        _check(lib().hc_level_inner_ghosts_outflow(self._handle, float(strength), float(time), _ptr(inner),
+                                                   self._stream()))
 
     def valid_state(self, U):
         """(valid, first_invalid): every value finite and every density > 0

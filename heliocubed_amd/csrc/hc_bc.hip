@@ -17,7 +17,9 @@
 //     energy from the Cartesian speed and the scaled field magnitude.
 // The fourth-order level takes these native conserved values as its inner
 // ghosts; the second-order level converts them back with the point basis to
-// Cartesian primitives (density floor, pressure clipped at 0).
+// Cartesian primitives (density floor, pressure clipped at 0). The
+// radial-outflow test boundary (init_condition_type 1) uses the same ghost
+// geometry and conversion.
 #include "hc_common.hpp"
 #include "hc_geometry.hpp"
 #include "hc_physics.hpp"
@@ -40,6 +42,36 @@ struct BcView
     const double* col;       // [6][NJ][NJ][kBcCol]
     const double* rad;       // [g] radius of ghost layer i = -g + ig
 };
+
+// Writes the native conserved ghost u (fourth order), or its Cartesian
+// primitives through the point basis (second order).
+__device__ inline void storeGhost(const BcView& v, const double* c, double u[8], double* __restrict__ out,
+                                  long long p, long long col, int ig)
+{
+    if (v.order == 2)
+    {
+        double cart[8];
+        cart[0] = u[0]; cart[4] = u[4];
+#pragma unroll
+        for (int grp = 0; grp < 2; ++grp)
+        {
+            const int st0 = grp == 0 ? 1 : 5;
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+            {
+                double a = 0.0;
+#pragma unroll
+                for (int j = 0; j < 3; ++j) a += c[12+3*d+j]*u[st0+j];
+                cart[st0+d] = a;
+            }
+        }
+        conservedToPrimitive<double>(u, cart, 1.0, v.gamma);
+    }
+    const long long nn = (long long)v.NJ*v.NJ;
+    const long long o = ((p*8)*nn + col)*v.g + ig;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) out[o + (long long)q*nn*v.g] = u[q];
+}
 
 __global__ void __launch_bounds__(kBcBlock) k_inner_bc(BcView v, const double* __restrict__ bc,
                                                        double* __restrict__ out)
@@ -94,29 +126,45 @@ __global__ void __launch_bounds__(kBcBlock) k_inner_bc(BcView v, const double* _
     const double Bsc2 = 1.0e-12*Bnon2*pow(s, 2.0) + 1.0e-12*Brad2*pow(s, 4.0);
     const double e = pres/(v.gamma - 1.0) + rho*V2/2.0 + Bsc2/8.0/kPi;
     double u[8] = {rho, rho*Vs[0], rho*Vs[1], rho*Vs[2], e, Bx, By, Bz};
-    if (v.order == 2)
-    {
-        // native conserved -> Cartesian conserved with the point basis -> primitives
-        double cart[8];
-        cart[0] = u[0]; cart[4] = u[4];
-#pragma unroll
-        for (int grp = 0; grp < 2; ++grp)
-        {
-            const int st0 = grp == 0 ? 1 : 5;
-#pragma unroll
-            for (int d = 0; d < 3; ++d)
-            {
-                double a = 0.0;
-#pragma unroll
-                for (int j = 0; j < 3; ++j) a += c[12+3*d+j]*u[st0+j];
-                cart[st0+d] = a;
-            }
-        }
-        conservedToPrimitive<double>(u, cart, 1.0, v.gamma);
-    }
-    const long long o = ((p*8)*nn + col)*v.g + ig;
-#pragma unroll
-    for (int q = 0; q < 8; ++q) out[o + (long long)q*nn*v.g] = u[q];
+    storeGhost(v, c, u, out, p, col, ig);
+}
+
+// Radial-outflow inner boundary (init_condition_type 1, f_radialoutflowIBCs,
+// src/BoxOp_PreStage.H:5-48): rho = 700 m_p s^2, p = 1e-7 s^(2 gamma) with
+// s = r_BC / r, a radial speed 500 km/s less strength/100/3 of it times the
+// co-rotating pattern sin^5(theta) sin(5 phi_t) + sin^10(theta) sin(10 phi_t),
+// phi_t = phi - Omega t (Carrington period 25.38 days), and no field.
+__global__ void __launch_bounds__(kBcBlock) k_inner_outflow(BcView v, double strength, double time,
+                                                            double* __restrict__ out)
+{
+    const long long nn = (long long)v.NJ*v.NJ;
+    const long long n = 6LL*nn*v.g;
+    const long long t = (long long)blockIdx.x*blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int ig = (int)(t % v.g);
+    const long long colAll = t / v.g;
+    const long long p = colAll / nn, col = colAll - p*nn;
+    const double* c = v.col + colAll*kBcCol;
+    const double rad = v.rad[ig];
+    const double X[3] = {rad*c[0], rad*c[1], rad*c[2]};
+    const double r = sqrt(X[0]*X[0] + X[1]*X[1] + X[2]*X[2]);
+    const double theta = atan2(sqrt(X[0]*X[0] + X[1]*X[1]), X[2]);
+    const double phi = atan2(X[1], X[0]);
+    const double p0 = 1.0e-7, rho0 = 700*1.67262192e-24, u0 = 500.0*1e5;
+    const double rho = rho0*pow(v.rBC/r, 2.0);
+    const double pres = p0*pow(v.rBC/r, 2.0*v.gamma);
+    const double carr = 25.38*24*60*60;
+    const double omega = 2.0*kPi/carr;
+    const double phit = phi - omega*time;
+    const double scale = strength/100.0;
+    const double st = sin(theta);
+    const double pattern = pow(st, 5)*sin(5*phit) + pow(st, 10)*sin(10*phit);
+    const double ur = u0 - scale*u0/3*pattern;
+    double u[8] = {rho, ur*rho, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    const double umag = sqrt(u[1]*u[1] + u[2]*u[2] + u[3]*u[3])/u[0];
+    const double bmag = sqrt(u[5]*u[5] + u[6]*u[6] + u[7]*u[7]);
+    u[4] = pres/(v.gamma - 1.0) + 0.5*u[0]*umag*umag + bmag*bmag/8.0/kPi;
+    storeGhost(v, c, u, out, p, col, ig);
 }
 } // namespace
 
@@ -185,7 +233,7 @@ BcTable* bcCreate(const ShellDims& dims, int order, double gamma)
 
 void bcDestroy(BcTable* T) { delete T; }
 
-void bcLaunch(BcTable& T, const double* bc, double* inner, hipStream_t s)
+static BcView bcView(const BcTable& T)
 {
     BcView v;
     v.na = T.dims.na;
@@ -196,6 +244,21 @@ void bcLaunch(BcTable& T, const double* bc, double* inner, hipStream_t s)
     v.rBC = T.dims.r0;
     v.col = T.col;
     v.rad = T.rad;
+    return v;
+}
+
+void outflowLaunch(BcTable& T, double strength, double time, double* inner, hipStream_t s)
+{
+    const BcView v = bcView(T);
+    const long long n = 6LL*v.NJ*v.NJ*v.g;
+    hipLaunchKernelGGL(k_inner_outflow, dim3((unsigned)((n + kBcBlock - 1)/kBcBlock)), dim3(kBcBlock), 0, s,
+                       v, strength, time, inner);
+    HC_THROW(hipGetLastError());
+}
+
+void bcLaunch(BcTable& T, const double* bc, double* inner, hipStream_t s)
+{
+    const BcView v = bcView(T);
     const long long n = 6LL*v.NJ*v.NJ*v.g;
     hipLaunchKernelGGL(k_inner_bc, dim3((unsigned)((n + kBcBlock - 1)/kBcBlock)), dim3(kBcBlock), 0, s, v, bc, inner);
     HC_THROW(hipGetLastError());

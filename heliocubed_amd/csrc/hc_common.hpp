@@ -59,6 +59,7 @@ struct BcTable;
 BcTable* bcCreate(const ShellDims& dims, int order, double gamma);
 void bcDestroy(BcTable* T);
 void bcLaunch(BcTable& T, const double* bc, double* inner, hipStream_t s);
+void outflowLaunch(BcTable& T, double strength, double time, double* inner, hipStream_t s);
 } // namespace hc
 
 #define HC_THROW(call)                                                              \

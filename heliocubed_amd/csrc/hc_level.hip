@@ -1123,6 +1123,19 @@ int hc_level_inner_ghosts_from_bc(hc_level* level, const double* bc, double* inn
     catch (const std::exception& e) { return fail(std::string("hc_level_inner_ghosts_from_bc: ") + e.what()); }
 }
 
+int hc_level_inner_ghosts_outflow(hc_level* level, double strength, double time, double* inner, hc_stream stream)
+{
+    if (!level || !inner) return fail("hc_level_inner_ghosts_outflow: null argument");
+    try
+    {
+        hc_level& L = *level;
+        if (!L.bcTable) L.bcTable = hc::bcCreate(L.dims, L.opt.spatial_order, L.opt.gamma);
+        hc::outflowLaunch(*L.bcTable, strength, time, inner, static_cast<hipStream_t>(stream));
+        return 0;
+    }
+    catch (const std::exception& e) { return fail(std::string("hc_level_inner_ghosts_outflow: ") + e.what()); }
+}
+
 int hc_level_valid_state(hc_level* level, const double* U, int* valid, long long* first_invalid, hc_stream stream)
 {
     if (!level || !U || !valid) return fail("hc_level_valid_state: null argument");

@@ -181,6 +181,15 @@ int hc_level_dt_cfl(hc_level* level, const double* U, double cone_boundary_densi
  * src/BoxOp_SecondOrderPreStage.H:194-214. Stream-ordered. */
 int hc_level_inner_ghosts_from_bc(hc_level* level, const double* bc, double* inner_ghosts, hc_stream stream);
 
+/* Inner radial ghosts of the radial-outflow test boundary (init_condition_type
+ * 1, f_radialoutflowIBCs, src/BoxOp_PreStage.H:5-48 as applied at :216-227):
+ * rho = 700 m_p (r_BC/r)^2, p = 1e-7 (r_BC/r)^(2 gamma), radial speed
+ * 5e7 cm/s less strength/300 of it times the pattern co-rotating with the
+ * Carrington period, no field; time in seconds. Same output layout as
+ * hc_level_inner_ghosts_from_bc. Stream-ordered. */
+int hc_level_inner_ghosts_outflow(hc_level* level, double strength, double time, double* inner_ghosts,
+                                  hc_stream stream);
+
 /* Conserved-state check of the level state U (valid layout): *valid = 1 when
  * every value is finite and every density is > 0, else 0, with
  * *first_invalid (may be NULL) the smallest flat index of an offending value

@@ -80,6 +80,7 @@ def lib() -> ctypes.CDLL:
             "orc_dt_cfl": (D, [cfgp, I, P, D]),
             "orc_first_invalid": (ctypes.c_longlong, [P, I, I]),
             "orc_inner_ghosts_from_bc": (None, [cfgp, I, P, P]),
+            "orc_inner_ghosts_outflow": (None, [cfgp, I, D, D, P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -246,6 +247,16 @@ def inner_ghosts_from_bc(n_rad, n_ang, bc, order=4, gamma=5.0 / 3.0, r0=21.5, r1
     assert b.shape == (6, 8, n, n), b.shape
     out = np.zeros((6, 8, n, n, g))
     lib().orc_inner_ghosts_from_bc(ctypes.byref(cfg), order, _p(b), _p(out))
+    return out
+
+
+def inner_ghosts_outflow(n_rad, n_ang, strength, time, order=4, gamma=5.0 / 3.0, r0=21.5, r1=215.0):
+    """Inner radial ghosts of the radial-outflow test boundary at `time` (s)."""
+    g = 2 if order == 2 else 5
+    cfg = OrcCfg(n_rad, n_ang, g, r0, r1, 0.0, gamma, 1, 2, 0.3, 1, 0.02, 0)
+    n = n_ang + 2 * g
+    out = np.zeros((6, 8, n, n, g))
+    lib().orc_inner_ghosts_outflow(ctypes.byref(cfg), order, strength, time, _p(out))
     return out
 
 

@@ -122,3 +122,33 @@ def test_density_only_strip_gives_scaled_proton_density():
     for ig in range(g):
         assert np.allclose(got[:, 0, :, :, ig], MP * (R0 / rad[ig]) ** 2, rtol=1e-14)
     assert np.abs(got[:, 1:]).max() == 0.0
+
+
+def test_outflow_boundary_closed_form_and_rotation():
+    """Radial-outflow test boundary (f_radialoutflowIBCs, src/BoxOp_PreStage.H:5-48):
+    at strength 0 the ghosts are 700 m_p (r_BC/r)^2, 5e7 cm/s radially and
+    1e-7 (r_BC/r)^(2 gamma); the perturbation co-rotates, so advancing time by
+    a full Carrington period reproduces the state and a seventh of it does not."""
+    nr, na, g = 12, 8, 2
+    rad = _radii(nr, g)
+    base = orc.inner_ghosts_outflow(nr, na, 0.0, 0.0, order=2, r1=R1)
+    for p in range(6):
+        for k in range(-g, na + g):
+            for j in range(-g, na + g):
+                A, rhat, that, phat = _frame(p, j, k, na)
+                for ig in range(g):
+                    s = R0 / rad[ig]
+                    o = base[p, :, k + g, j + g, ig]
+                    assert abs(o[0] - 700 * MP * s ** 2) <= 1e-13 * o[0]
+                    assert np.abs(o[1:4] - 5e7 * rhat).max() <= 1e-12 * 5e7
+                    assert abs(o[4] - 1e-7 * s ** (2 * GAMMA)) <= 1e-11 * o[4]
+                    assert np.abs(o[5:8]).max() == 0.0
+    period = 25.38 * 24 * 60 * 60
+    a = orc.inner_ghosts_outflow(nr, na, 20.0, 1000.0, order=4, r1=R1)
+    b = orc.inner_ghosts_outflow(nr, na, 20.0, 1000.0 + period, order=4, r1=R1)
+    c = orc.inner_ghosts_outflow(nr, na, 20.0, 1000.0 + period / 7, order=4, r1=R1)
+    assert np.abs(a - b).max() <= 1e-9 * np.abs(a).max()
+    assert np.abs(a[:, 1] - c[:, 1]).max() > 1e-3 * np.abs(a[:, 1]).max()
+    # the speed perturbation is bounded by strength/300 * 2 of u0
+    speed = a[:, 1] / a[:, 0]
+    assert np.abs(speed / 5e7 - 1).max() <= 20.0 / 300 * 2
