@@ -76,6 +76,8 @@ _SIGNATURES = [
     ("hc_level_valid_state", _I, [_P, _P, _P, _P, _P]),
     ("hc_level_inner_ghosts_from_bc", _I, [_P, _P, _P, _P]),
     ("hc_level_inner_ghosts_outflow", _I, [_P, _D, _D, _P, _P]),
+    ("hc_amr_interp4", _I, [_P, _P, _I, _P, _P, _P, _P]),
+    ("hc_amr_ghosts2", _I, [_P, _P, _P, _P, _P, _D, _P, _P]),
     ("hc_init_state", _I, [ctypes.POINTER(HcShell), ctypes.POINTER(HcOptions), _I, ctypes.c_ulonglong, _P, _P]),
     ("hc_geom_cell_detA", _D, [_I, _I, _I]),
     ("hc_geom_face_normal", None, [_I, _I, _I, _I, _I, _P]),
@@ -339,6 +341,55 @@ class Level:
         _check(lib().hc_level_valid_state(self._handle, _ptr(U), ctypes.byref(ok), ctypes.byref(first),
                                           self._stream()))
         return bool(ok.value), first.value
+
+
+def _patch(lo, hi):
+    b = HcBox()
+    b.block = 0
+    for d in range(3):
+        b.lo[d] = int(lo[d])
+        b.hi[d] = int(hi[d])
+    return b
+
+
+def _patch_shape(lo, hi, nc):
+    return (nc, hi[2] - lo[2] + 1, hi[1] - lo[1] + 1, hi[0] - lo[0] + 1)
+
+
+def _check_patch(t, shape, name):
+    import torch
+
+    if t.dtype != torch.float64 or not t.is_cuda or not t.is_contiguous() or tuple(t.shape) != tuple(shape):
+        raise ValueError(f"{name}: need a contiguous float64 device tensor of shape {tuple(shape)}")
+
+
+def amr_interp4(coarse, coarse_lo, coarse_hi, ratio, fine, fine_lo, fine_hi):
+    """Fourth-order fine averages fine[nc][k][j][i] (fine_lo..fine_hi) from the
+    coarse averages coarse[nc][k][j][i] (coarse_lo..coarse_hi), refinement
+    ratio per direction; cells whose stencil leaves the coarse patch are NaN."""
+    import torch
+
+    nc = coarse.shape[0]
+    _check_patch(coarse, _patch_shape(coarse_lo, coarse_hi, nc), "coarse")
+    _check_patch(fine, _patch_shape(fine_lo, fine_hi, nc), "fine")
+    cb, fb = _patch(coarse_lo, coarse_hi), _patch(fine_lo, fine_hi)
+    r = (ctypes.c_int * 3)(*[int(x) for x in ratio])
+    _check(lib().hc_amr_interp4(_ptr(coarse), ctypes.byref(cb), nc, r, ctypes.byref(fb), _ptr(fine),
+                                ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
+
+
+def amr_ghosts2(coarse_ju_j, coarse_lo, coarse_hi, ratio, fine_w, fine_lo, fine_hi, owned_lo, owned_hi,
+                gamma=5.0 / 3.0):
+    """Second-order fine ghost cells (Cartesian primitives) of fine_w[8][..]
+    outside owned_lo..owned_hi from coarse J*U and J (coarse_ju_j[9][..])."""
+    import torch
+
+    _check_patch(coarse_ju_j, _patch_shape(coarse_lo, coarse_hi, 9), "coarse_ju_j")
+    _check_patch(fine_w, _patch_shape(fine_lo, fine_hi, 8), "fine_w")
+    cb, fb, ob = _patch(coarse_lo, coarse_hi), _patch(fine_lo, fine_hi), _patch(owned_lo, owned_hi)
+    r = (ctypes.c_int * 3)(*[int(x) for x in ratio])
+    _check(lib().hc_amr_ghosts2(_ptr(coarse_ju_j), ctypes.byref(cb), r, ctypes.byref(fb), ctypes.byref(ob),
+                                float(gamma), _ptr(fine_w), ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
 
 
 def box_table(boxes):

@@ -1,0 +1,269 @@
+// AMR coarse-to-fine transfers of one patch (the data the AMR driver moves
+// between levels before each stage):
+//
+//  * hc_amr_interp4: fourth-order averages of the fine cells from the coarse
+//    cell averages, amrInterpolateFourthOrder (src/MHD_AMRInterpolation.H:39-161).
+//    Along each refined direction the five coarse averages around the parent
+//    define the quartic with those averages; its average over the child cell
+//    is a fixed five-weight combination (the weights depend on the ratio and
+//    the child index only). The tensor product is applied as three
+//    one-dimensional passes, radial first, as the reference does.
+//  * hc_amr_ghosts2: second-order fine ghost cells, src/MHD_AMR.H:1282-1393:
+//    the tensor quadratic average (q = x^2 + (1/r^2 - 1)/12, weights
+//    ((q - x)/2, 1 - q, (q + x)/2)) of the coarse J*U and of J, their quotient,
+//    a positivity check that falls back to the parent cell, then Cartesian
+//    primitives.
+//
+// Patches use the Proto BoxData layout: component-major, index 0 fastest,
+// inclusive lo..hi in the level's (possibly negative ghost) index space.
+#include "hc_common.hpp"
+#include "hc_geometry.hpp"
+#include "hc_physics.hpp"
+
+#include <cmath>
+#include <utility>
+
+namespace hc
+{
+namespace
+{
+constexpr int kAmrBlock = 256;
+constexpr int kMaxRatio = 8;
+
+struct Patch
+{
+    int lo[3], hi[3];
+    __host__ __device__ long long n(int d) const { return hi[d] - lo[d] + 1; }
+    __host__ __device__ long long count() const { return n(0)*n(1)*n(2); }
+    __host__ __device__ bool contains(int a, int d) const { return a >= lo[d] && a <= hi[d]; }
+    __host__ __device__ long long at(int c, int i, int j, int k) const
+    { return ((c*n(2) + (k - lo[2]))*n(1) + (j - lo[1]))*n(0) + (i - lo[0]); }
+};
+
+__host__ __device__ inline int floorDiv(int p, int r) { return (int)floor(double(p)/r); }
+
+struct Interp4View
+{
+    Patch src, dst;
+    int ncomp;
+    int ratio[3];
+    double w[3][kMaxRatio][5];   // child weights per direction
+};
+
+__global__ void __launch_bounds__(kAmrBlock) k_amr_interp4(Interp4View v, const double* __restrict__ src,
+                                                           double* __restrict__ dst)
+{
+    const long long t = (long long)blockIdx.x*blockDim.x + threadIdx.x;
+    if (t >= v.dst.count()) return;
+    int p[3];
+    p[0] = v.dst.lo[0] + (int)(t % v.dst.n(0));
+    p[1] = v.dst.lo[1] + (int)((t / v.dst.n(0)) % v.dst.n(1));
+    p[2] = v.dst.lo[2] + (int)(t / (v.dst.n(0)*v.dst.n(1)));
+    int q[3], rad[3];
+    const double* w[3];
+    bool inside = true;
+    for (int d = 0; d < 3; ++d)
+    {
+        q[d] = floorDiv(p[d], v.ratio[d]);
+        w[d] = v.w[d][p[d] - q[d]*v.ratio[d]];
+        rad[d] = v.ratio[d] > 1 ? 2 : 0;
+        for (int k = -rad[d]; k <= rad[d]; ++k)
+            if (w[d][k+2] != 0.0 && !v.src.contains(q[d]+k, d)) inside = false;
+    }
+    for (int c = 0; c < v.ncomp; ++c)
+    {
+        double value = NAN;
+        if (inside)
+        {
+            double s2 = 0.0;
+            for (int k2 = -rad[2]; k2 <= rad[2]; ++k2)
+            {
+                if (w[2][k2+2] == 0.0) continue;
+                double s1 = 0.0;
+                for (int k1 = -rad[1]; k1 <= rad[1]; ++k1)
+                {
+                    if (w[1][k1+2] == 0.0) continue;
+                    double s0 = 0.0;
+                    for (int k0 = -rad[0]; k0 <= rad[0]; ++k0)
+                    {
+                        if (w[0][k0+2] == 0.0) continue;
+                        s0 += w[0][k0+2]*src[v.src.at(c, q[0]+k0, q[1]+k1, q[2]+k2)];
+                    }
+                    s1 += w[1][k1+2]*s0;
+                }
+                s2 += w[2][k2+2]*s1;
+            }
+            value = s2;
+        }
+        dst[v.dst.at(c, p[0], p[1], p[2])] = value;
+    }
+}
+
+struct Ghost2View
+{
+    Patch src, dst, owned;
+    int ratio[3];
+    double gamma;
+};
+
+__global__ void __launch_bounds__(kAmrBlock) k_amr_ghosts2(Ghost2View v, const double* __restrict__ src,
+                                                           double* __restrict__ dst)
+{
+    const long long t = (long long)blockIdx.x*blockDim.x + threadIdx.x;
+    if (t >= v.dst.count()) return;
+    int p[3];
+    p[0] = v.dst.lo[0] + (int)(t % v.dst.n(0));
+    p[1] = v.dst.lo[1] + (int)((t / v.dst.n(0)) % v.dst.n(1));
+    p[2] = v.dst.lo[2] + (int)(t / (v.dst.n(0)*v.dst.n(1)));
+    if (v.owned.contains(p[0], 0) && v.owned.contains(p[1], 1) && v.owned.contains(p[2], 2)) return;
+    int q[3];
+    double w[3][3];
+    for (int d = 0; d < 3; ++d)
+    {
+        const int r = v.ratio[d];
+        q[d] = floorDiv(p[d], r);
+        const double x = (p[d] + 0.5)/r - (q[d] + 0.5);
+        const double quadratic = x*x + (1.0/(double(r)*r) - 1)/12;
+        w[d][0] = 0.5*(quadratic - x);
+        w[d][1] = 1 - quadratic;
+        w[d][2] = 0.5*(quadratic + x);
+        // every donor of the stencil must lie in the coarse patch
+        const int reach = r == 1 ? 0 : 1;
+        if (!v.src.contains(q[d]-reach, d) || !v.src.contains(q[d]+reach, d)) return;
+    }
+    if (!isfinite(src[v.src.at(8, q[0], q[1], q[2])])) return;
+    double u[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double j = 0.0;
+    for (int o2 = -1; o2 <= 1; ++o2)
+        for (int o1 = -1; o1 <= 1; ++o1)
+            for (int o0 = -1; o0 <= 1; ++o0)
+            {
+                const int off[3] = {o0, o1, o2};
+                bool used = true;
+                for (int d = 0; d < 3; ++d) if (v.ratio[d] == 1 && off[d]) used = false;
+                if (!used) continue;
+                double weight = 1;
+                for (int d = 0; d < 3; ++d) weight *= w[d][off[d]+1];
+                if (weight == 0) continue;
+                const int a = q[0]+o0, b = q[1]+o1, c3 = q[2]+o2;
+                j += weight*src[v.src.at(8, a, b, c3)];
+                for (int c = 0; c < 8; ++c) u[c] += weight*src[v.src.at(c, a, b, c3)];
+            }
+    bool valid = j > 0 && isfinite(j);
+    for (int c = 0; c < 8; ++c) { u[c] /= j; valid &= (bool)isfinite(u[c]); }
+    double thermal = u[4];
+    if (u[0] > 0)
+        for (int d = 0; d < 3; ++d) thermal -= 0.5*u[1+d]*u[1+d]/u[0] + u[5+d]*u[5+d]/(8*kPi);
+    valid &= u[0] >= kRhoFloor && thermal >= 0 && isfinite(thermal);
+    if (!valid)
+    {
+        const double jq = src[v.src.at(8, q[0], q[1], q[2])];
+        for (int c = 0; c < 8; ++c) u[c] = src[v.src.at(c, q[0], q[1], q[2])]/jq;
+    }
+    double out[8];
+    conservedToPrimitive<double>(out, u, 1.0, v.gamma);
+    for (int c = 0; c < 8; ++c) dst[v.dst.at(c, p[0], p[1], p[2])] = out[c];
+}
+
+// Average over the child interval [x - h/2, x + h/2] (coarse cell units,
+// parent centred at 0) of the quartic whose averages over the coarse cells
+// [k - 1/2, k + 1/2], k = -2..2, are the data: solve sum_k w_k <xi^n>_k =
+// <xi^n>_child for n = 0..4 (Gaussian elimination with partial pivoting).
+void childWeights(int ratio, int child, double out[5])
+{
+    if (ratio == 1) { for (int k = 0; k < 5; ++k) out[k] = k == 2 ? 1.0 : 0.0; return; }
+    const double h = 1.0/ratio, x = (child + 0.5)/ratio - 0.5;
+    auto avgPow = [](double a, double b, int n) {
+        return (std::pow(b, n+1) - std::pow(a, n+1))/((n+1)*(b - a)); };
+    double M[5][6];
+    for (int n = 0; n < 5; ++n)
+    {
+        for (int k = 0; k < 5; ++k) M[n][k] = avgPow(k - 2 - 0.5, k - 2 + 0.5, n);
+        M[n][5] = avgPow(x - h/2, x + h/2, n);
+    }
+    for (int col = 0; col < 5; ++col)
+    {
+        int piv = col;
+        for (int r = col+1; r < 5; ++r) if (std::fabs(M[r][col]) > std::fabs(M[piv][col])) piv = r;
+        for (int c = 0; c < 6; ++c) std::swap(M[col][c], M[piv][c]);
+        for (int r = 0; r < 5; ++r)
+        {
+            if (r == col) continue;
+            const double f = M[r][col]/M[col][col];
+            for (int c = col; c < 6; ++c) M[r][c] -= f*M[col][c];
+        }
+    }
+    for (int k = 0; k < 5; ++k) out[k] = M[k][5]/M[k][k];
+}
+
+Patch toPatch(const hc_box& b)
+{
+    Patch p;
+    for (int d = 0; d < 3; ++d) { p.lo[d] = b.lo[d]; p.hi[d] = b.hi[d]; }
+    return p;
+}
+
+bool validBox(const hc_box* b)
+{
+    return b && b->hi[0] >= b->lo[0] && b->hi[1] >= b->lo[1] && b->hi[2] >= b->lo[2];
+}
+} // namespace
+} // namespace hc
+
+extern "C" {
+
+int hc_amr_interp4(const double* coarse, const hc_box* coarse_box, int ncomp, const int* ratio,
+                   const hc_box* fine_box, double* fine, hc_stream stream)
+{
+    using namespace hc;
+    if (!coarse || !fine || !ratio || !validBox(coarse_box) || !validBox(fine_box))
+        return fail("hc_amr_interp4: null argument or empty box");
+    if (ncomp < 1) return fail("hc_amr_interp4: ncomp must be >= 1");
+    for (int d = 0; d < 3; ++d)
+        if (ratio[d] < 1 || ratio[d] > kMaxRatio) return fail("hc_amr_interp4: ratio must be 1..8 per direction");
+    try
+    {
+        Interp4View v{};
+        v.src = toPatch(*coarse_box);
+        v.dst = toPatch(*fine_box);
+        v.ncomp = ncomp;
+        for (int d = 0; d < 3; ++d)
+        {
+            v.ratio[d] = ratio[d];
+            for (int c = 0; c < ratio[d]; ++c) childWeights(ratio[d], c, v.w[d][c]);
+        }
+        const long long n = v.dst.count();
+        hipLaunchKernelGGL(k_amr_interp4, dim3((unsigned)((n + kAmrBlock - 1)/kAmrBlock)), dim3(kAmrBlock), 0,
+                           static_cast<hipStream_t>(stream), v, coarse, fine);
+        HC_THROW(hipGetLastError());
+        return 0;
+    }
+    catch (const std::exception& e) { return fail(std::string("hc_amr_interp4: ") + e.what()); }
+}
+
+int hc_amr_ghosts2(const double* coarse_ju_j, const hc_box* coarse_box, const int* ratio, const hc_box* fine_box,
+                   const hc_box* owned_box, double gamma, double* fine_w, hc_stream stream)
+{
+    using namespace hc;
+    if (!coarse_ju_j || !fine_w || !ratio || !validBox(coarse_box) || !validBox(fine_box) || !validBox(owned_box))
+        return fail("hc_amr_ghosts2: null argument or empty box");
+    for (int d = 0; d < 3; ++d)
+        if (ratio[d] < 1) return fail("hc_amr_ghosts2: ratio must be >= 1 per direction");
+    try
+    {
+        Ghost2View v{};
+        v.src = toPatch(*coarse_box);
+        v.dst = toPatch(*fine_box);
+        v.owned = toPatch(*owned_box);
+        for (int d = 0; d < 3; ++d) v.ratio[d] = ratio[d];
+        v.gamma = gamma;
+        const long long n = v.dst.count();
+        hipLaunchKernelGGL(k_amr_ghosts2, dim3((unsigned)((n + kAmrBlock - 1)/kAmrBlock)), dim3(kAmrBlock), 0,
+                           static_cast<hipStream_t>(stream), v, coarse_ju_j, fine_w);
+        HC_THROW(hipGetLastError());
+        return 0;
+    }
+    catch (const std::exception& e) { return fail(std::string("hc_amr_ghosts2: ") + e.what()); }
+}
+
+} // extern "C"

@@ -288,6 +288,32 @@ int hc_level_profile_read(hc_level* level, int slot, char* name, int name_len,
 
 const char* hc_last_error(void);
 
+/* AMR coarse-to-fine transfers of one patch. Patches use the Proto BoxData
+ * layout: component-major, index 0 fastest, over the inclusive lo..hi of an
+ * hc_box (block ignored) in the level's index space; ghost indices may be
+ * negative. ratio[d] is the refinement ratio per direction (1 = unrefined).
+ *
+ * hc_amr_interp4: fine cell averages fine[ncomp][fine_box] from coarse
+ * averages coarse[ncomp][coarse_box] by the separable fourth-order rule of
+ * amrInterpolateFourthOrder (src/MHD_AMRInterpolation.H:39-161): per refined
+ * direction the average over the child of the quartic matching the five
+ * coarse averages around the parent (ratio <= 8). A fine cell whose donors
+ * leave coarse_box is NaN, as in the reference's pass form. */
+int hc_amr_interp4(const double* coarse, const hc_box* coarse_box, int ncomp, const int* ratio,
+                   const hc_box* fine_box, double* fine, hc_stream stream);
+
+/* hc_amr_ghosts2: second-order fine ghost cells of a patch
+ * (src/MHD_AMR.H:1282-1393): coarse_ju_j[9][coarse_box] holds J*U (8 Cartesian
+ * conserved components) and J of the coarse cells; every cell of fine_box
+ * outside owned_box gets the Cartesian primitives of the tensor-quadratic
+ * average of J*U divided by that of J, or of the parent cell's J*U/J when the
+ * average is not a positive state (density < RHO00 or negative thermal
+ * energy). Cells whose parent J is not finite, or whose stencil leaves
+ * coarse_box, are left unchanged. The constant-field test background is not
+ * part of this path. */
+int hc_amr_ghosts2(const double* coarse_ju_j, const hc_box* coarse_box, const int* ratio, const hc_box* fine_box,
+                   const hc_box* owned_box, double gamma, double* fine_w, hc_stream stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -81,6 +81,9 @@ def lib() -> ctypes.CDLL:
             "orc_first_invalid": (ctypes.c_longlong, [P, I, I]),
             "orc_inner_ghosts_from_bc": (None, [cfgp, I, P, P]),
             "orc_inner_ghosts_outflow": (None, [cfgp, I, D, D, P]),
+            "orc_amr4_weights": (None, [I, I, P]),
+            "orc_amr_interp4": (None, [P, P, P, I, P, P, P, P]),
+            "orc_amr_ghosts2": (None, [P, P, P, P, P, P, P, P, D, P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -257,6 +260,46 @@ def inner_ghosts_outflow(n_rad, n_ang, strength, time, order=4, gamma=5.0 / 3.0,
     n = n_ang + 2 * g
     out = np.zeros((6, 8, n, n, g))
     lib().orc_inner_ghosts_outflow(ctypes.byref(cfg), order, strength, time, _p(out))
+    return out
+
+
+def _ivec(v):
+    return np.ascontiguousarray(v, dtype=np.int32)
+
+
+def _box_shape(lo, hi, nc):
+    return (nc, hi[2] - lo[2] + 1, hi[1] - lo[1] + 1, hi[0] - lo[0] + 1)
+
+
+def amr4_weights(ratio, child):
+    w = np.zeros(5)
+    lib().orc_amr4_weights(ratio, child, _p(w))
+    return w
+
+
+def amr_interp4(coarse, coarse_lo, coarse_hi, ratio, fine_lo, fine_hi):
+    """Fine averages [nc][k][j][i] on fine_lo..fine_hi from coarse averages
+    [nc][k][j][i] on coarse_lo..coarse_hi (amrInterpolateFourthOrder)."""
+    src = np.ascontiguousarray(coarse, dtype=np.float64)
+    nc = src.shape[0]
+    assert src.shape == _box_shape(coarse_lo, coarse_hi, nc)
+    out = np.zeros(_box_shape(fine_lo, fine_hi, nc))
+    args = [_ivec(x) for x in (coarse_lo, coarse_hi, ratio, fine_lo, fine_hi)]
+    lib().orc_amr_interp4(_p(src), args[0].ctypes.data, args[1].ctypes.data, nc, args[2].ctypes.data,
+                          args[3].ctypes.data, args[4].ctypes.data, _p(out))
+    return out
+
+
+def amr_ghosts2(coarse_ju_j, coarse_lo, coarse_hi, ratio, fine_lo, fine_hi, owned_lo, owned_hi, fine_init,
+                gamma=5.0 / 3.0):
+    """Second-order fine ghost cells (Cartesian primitives) written into a copy
+    of fine_init [8][k][j][i] on fine_lo..fine_hi."""
+    src = np.ascontiguousarray(coarse_ju_j, dtype=np.float64)
+    assert src.shape == _box_shape(coarse_lo, coarse_hi, 9)
+    out = np.ascontiguousarray(fine_init, dtype=np.float64).copy()
+    assert out.shape == _box_shape(fine_lo, fine_hi, 8)
+    args = [_ivec(x) for x in (coarse_lo, coarse_hi, ratio, fine_lo, fine_hi, owned_lo, owned_hi)]
+    lib().orc_amr_ghosts2(_p(src), *[a.ctypes.data for a in args], gamma, _p(out))
     return out
 
 
