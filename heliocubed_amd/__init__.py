@@ -78,6 +78,7 @@ _SIGNATURES = [
     ("hc_level_inner_ghosts_outflow", _I, [_P, _D, _D, _P, _P]),
     ("hc_amr_interp4", _I, [_P, _P, _I, _P, _P, _P, _P]),
     ("hc_amr_ghosts2", _I, [_P, _P, _P, _P, _P, _D, _P, _P]),
+    ("hc_amr_average_down", _I, [_P, _P, _I, _P, _P, _P, _P]),
     ("hc_init_state", _I, [ctypes.POINTER(HcShell), ctypes.POINTER(HcOptions), _I, ctypes.c_ulonglong, _P, _P]),
     ("hc_geom_cell_detA", _D, [_I, _I, _I]),
     ("hc_geom_face_normal", None, [_I, _I, _I, _I, _I, _P]),
@@ -390,6 +391,19 @@ def amr_ghosts2(coarse_ju_j, coarse_lo, coarse_hi, ratio, fine_w, fine_lo, fine_
     r = (ctypes.c_int * 3)(*[int(x) for x in ratio])
     _check(lib().hc_amr_ghosts2(_ptr(coarse_ju_j), ctypes.byref(cb), r, ctypes.byref(fb), ctypes.byref(ob),
                                 float(gamma), _ptr(fine_w), ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
+
+
+def amr_average_down(fine, fine_lo, fine_hi, ratio, coarse, coarse_lo, coarse_hi):
+    """coarse[nc][..] = mean of each coarse cell's children in fine[nc][..]."""
+    import torch
+
+    nc = fine.shape[0]
+    _check_patch(fine, _patch_shape(fine_lo, fine_hi, nc), "fine")
+    _check_patch(coarse, _patch_shape(coarse_lo, coarse_hi, nc), "coarse")
+    fb, cb = _patch(fine_lo, fine_hi), _patch(coarse_lo, coarse_hi)
+    r = (ctypes.c_int * 3)(*[int(x) for x in ratio])
+    _check(lib().hc_amr_average_down(_ptr(fine), ctypes.byref(fb), nc, r, ctypes.byref(cb), _ptr(coarse),
+                                     ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
 
 
 def box_table(boxes):

@@ -13,6 +13,8 @@
 //    ((q - x)/2, 1 - q, (q + x)/2)) of the coarse J*U and of J, their quotient,
 //    a positivity check that falls back to the parent cell, then Cartesian
 //    primitives.
+//  * hc_amr_average_down: restriction of fine averages to their parents
+//    (Stencil::AvgDown in restrictTo, src/MHD_AMR.H:596-606).
 //
 // Patches use the Proto BoxData layout: component-major, index 0 fastest,
 // inclusive lo..hi in the level's (possibly negative ghost) index space.
@@ -165,6 +167,36 @@ __global__ void __launch_bounds__(kAmrBlock) k_amr_ghosts2(Ghost2View v, const d
     for (int c = 0; c < 8; ++c) dst[v.dst.at(c, p[0], p[1], p[2])] = out[c];
 }
 
+struct AvgDownView
+{
+    Patch src, dst;
+    int ncomp;
+    int ratio[3];
+};
+
+// Restriction of fine averages to their parents (Stencil::AvgDown, applied by
+// restrictTo, src/MHD_AMR.H:596-606): the mean of the r0 r1 r2 children.
+__global__ void __launch_bounds__(kAmrBlock) k_amr_avgdown(AvgDownView v, const double* __restrict__ src,
+                                                           double* __restrict__ dst)
+{
+    const long long t = (long long)blockIdx.x*blockDim.x + threadIdx.x;
+    if (t >= v.dst.count()) return;
+    int q[3];
+    q[0] = v.dst.lo[0] + (int)(t % v.dst.n(0));
+    q[1] = v.dst.lo[1] + (int)((t / v.dst.n(0)) % v.dst.n(1));
+    q[2] = v.dst.lo[2] + (int)(t / (v.dst.n(0)*v.dst.n(1)));
+    const double inv = 1.0/(double(v.ratio[0])*v.ratio[1]*v.ratio[2]);
+    for (int c = 0; c < v.ncomp; ++c)
+    {
+        double sum = 0.0;
+        for (int c2 = 0; c2 < v.ratio[2]; ++c2)
+            for (int c1 = 0; c1 < v.ratio[1]; ++c1)
+                for (int c0 = 0; c0 < v.ratio[0]; ++c0)
+                    sum += inv*src[v.src.at(c, q[0]*v.ratio[0]+c0, q[1]*v.ratio[1]+c1, q[2]*v.ratio[2]+c2)];
+        dst[v.dst.at(c, q[0], q[1], q[2])] = sum;
+    }
+}
+
 // Average over the child interval [x - h/2, x + h/2] (coarse cell units,
 // parent centred at 0) of the quartic whose averages over the coarse cells
 // [k - 1/2, k + 1/2], k = -2..2, are the data: solve sum_k w_k <xi^n>_k =
@@ -264,6 +296,37 @@ int hc_amr_ghosts2(const double* coarse_ju_j, const hc_box* coarse_box, const in
         return 0;
     }
     catch (const std::exception& e) { return fail(std::string("hc_amr_ghosts2: ") + e.what()); }
+}
+
+int hc_amr_average_down(const double* fine, const hc_box* fine_box, int ncomp, const int* ratio,
+                        const hc_box* coarse_box, double* coarse, hc_stream stream)
+{
+    using namespace hc;
+    if (!fine || !coarse || !ratio || !validBox(fine_box) || !validBox(coarse_box))
+        return fail("hc_amr_average_down: null argument or empty box");
+    if (ncomp < 1) return fail("hc_amr_average_down: ncomp must be >= 1");
+    for (int d = 0; d < 3; ++d)
+    {
+        if (ratio[d] < 1) return fail("hc_amr_average_down: ratio must be >= 1 per direction");
+        // every child of every coarse cell must lie in the fine patch
+        if ((long long)coarse_box->lo[d]*ratio[d] < fine_box->lo[d] ||
+            (long long)coarse_box->hi[d]*ratio[d] + ratio[d] - 1 > fine_box->hi[d])
+            return fail("hc_amr_average_down: coarse box not covered by the fine patch");
+    }
+    try
+    {
+        AvgDownView v{};
+        v.src = toPatch(*fine_box);
+        v.dst = toPatch(*coarse_box);
+        v.ncomp = ncomp;
+        for (int d = 0; d < 3; ++d) v.ratio[d] = ratio[d];
+        const long long n = v.dst.count();
+        hipLaunchKernelGGL(k_amr_avgdown, dim3((unsigned)((n + kAmrBlock - 1)/kAmrBlock)), dim3(kAmrBlock), 0,
+                           static_cast<hipStream_t>(stream), v, fine, coarse);
+        HC_THROW(hipGetLastError());
+        return 0;
+    }
+    catch (const std::exception& e) { return fail(std::string("hc_amr_average_down: ") + e.what()); }
 }
 
 } // extern "C"

@@ -314,6 +314,13 @@ int hc_amr_interp4(const double* coarse, const hc_box* coarse_box, int ncomp, co
 int hc_amr_ghosts2(const double* coarse_ju_j, const hc_box* coarse_box, const int* ratio, const hc_box* fine_box,
                    const hc_box* owned_box, double gamma, double* fine_w, hc_stream stream);
 
+/* hc_amr_average_down: coarse[ncomp][coarse_box] = mean of the ratio[0] x
+ * ratio[1] x ratio[2] children in fine[ncomp][fine_box] (the restriction
+ * Stencil::AvgDown of restrictTo, src/MHD_AMR.H:596-606); every child must lie
+ * in fine_box. */
+int hc_amr_average_down(const double* fine, const hc_box* fine_box, int ncomp, const int* ratio,
+                        const hc_box* coarse_box, double* coarse, hc_stream stream);
+
 #ifdef __cplusplus
 }
 #endif

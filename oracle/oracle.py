@@ -84,6 +84,7 @@ def lib() -> ctypes.CDLL:
             "orc_amr4_weights": (None, [I, I, P]),
             "orc_amr_interp4": (None, [P, P, P, I, P, P, P, P]),
             "orc_amr_ghosts2": (None, [P, P, P, P, P, P, P, P, D, P]),
+            "orc_amr_average_down": (None, [P, P, P, I, P, P, P, P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -300,6 +301,16 @@ def amr_ghosts2(coarse_ju_j, coarse_lo, coarse_hi, ratio, fine_lo, fine_hi, owne
     assert out.shape == _box_shape(fine_lo, fine_hi, 8)
     args = [_ivec(x) for x in (coarse_lo, coarse_hi, ratio, fine_lo, fine_hi, owned_lo, owned_hi)]
     lib().orc_amr_ghosts2(_p(src), *[a.ctypes.data for a in args], gamma, _p(out))
+    return out
+
+
+def amr_average_down(fine, fine_lo, fine_hi, ratio, coarse_lo, coarse_hi):
+    src = np.ascontiguousarray(fine, dtype=np.float64)
+    nc = src.shape[0]
+    out = np.zeros(_box_shape(coarse_lo, coarse_hi, nc))
+    args = [_ivec(x) for x in (fine_lo, fine_hi, ratio, coarse_lo, coarse_hi)]
+    lib().orc_amr_average_down(_p(src), args[0].ctypes.data, args[1].ctypes.data, nc, args[2].ctypes.data,
+                               args[3].ctypes.data, args[4].ctypes.data, _p(out))
     return out
 
 

@@ -58,3 +58,20 @@ def test_ghosts2_match_oracle(device, case):
     err = np.abs(got - ref).max() / np.abs(ref).max()
     print("ghosts2", case, "err", err)
     assert np.isfinite(got).all() and err <= 1e-12
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("ratio", [(2, 2, 2), (2, 3, 1), (4, 2, 2)])
+def test_average_down_matches_oracle(device, ratio):
+    rng = np.random.default_rng(sum(ratio))
+    clo, chi = (-2, 1, 0), (3, 4, 5)
+    flo = tuple(ratio[d] * clo[d] - 1 for d in range(3))
+    fhi = tuple(ratio[d] * chi[d] + ratio[d] for d in range(3))
+    fine = rng.random((8,) + tuple(fhi[d] - flo[d] + 1 for d in (2, 1, 0)))
+    ref = orc.amr_average_down(fine, flo, fhi, ratio, clo, chi)
+    out = torch.zeros(ref.shape, dtype=torch.float64, device=device)
+    hc.amr_average_down(_dev(fine), flo, fhi, ratio, out, clo, chi)
+    torch.cuda.synchronize()
+    err = np.abs(out.cpu().numpy() - ref).max() / np.abs(ref).max()
+    print("average_down", ratio, err)
+    assert err <= 1e-13

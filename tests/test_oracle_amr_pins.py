@@ -140,3 +140,21 @@ def test_ghosts2_falls_back_to_the_parent_and_skips_missing_parents():
     got2 = orc.amr_ghosts2(coarse2, clo, chi, R2, FLO, FHI, OLO, OHI, init)
     assert (got2[(slice(None),) + fine((8, 4, 4))] == -7.0).all()
     assert (got2[(slice(None),) + fine((9, 5, 5))] == -7.0).all()
+
+
+def test_average_down_inverts_interp_and_conserves():
+    """Restriction (Stencil::AvgDown): the mean of the children. The fourth-order
+    interpolation is conservative, so restricting its output returns the coarse
+    data; the fine total equals the coarse total times the ratio product."""
+    rng = np.random.default_rng(5)
+    clo, chi, r = (0, 0, 0), (7, 7, 7), (2, 3, 2)
+    coarse = rng.random((2, 8, 8, 8)) + 1
+    flo = tuple(r[d] * 2 for d in range(3))
+    fhi = tuple(r[d] * 6 - 1 for d in range(3))
+    fine = orc.amr_interp4(coarse, clo, chi, r, flo, fhi)
+    back = orc.amr_average_down(fine, flo, fhi, r, (2, 2, 2), (5, 5, 5))
+    assert np.allclose(back, coarse[:, 2:6, 2:6, 2:6], rtol=1e-14)
+    f2 = rng.random((3, 6, 9, 4))
+    c2 = orc.amr_average_down(f2, (0, 0, 0), (3, 8, 5), (2, 3, 3), (0, 0, 0), (1, 2, 1))
+    assert np.allclose(c2.sum(axis=(1, 2, 3)) * 18, f2.sum(axis=(1, 2, 3)), rtol=1e-13)
+    assert np.allclose(c2[:, 1, 2, 0], f2[:, 3:6, 6:9, 0:2].mean(axis=(1, 2, 3)), rtol=1e-14)
