@@ -114,14 +114,19 @@ k_seam_gather(LevelView v, int ncol, const long long* __restrict__ colDst,
     const int col = t / v.nr;
     const int d0 = colStart[col], d1 = colStart[col+1];
     const long long dst = colDst[col] + i;
-    for (int c = 0; c < 8; ++c)
+    // donors outer, components inner (each component sums in table order)
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int d = d0; d < d1; ++d)
     {
-        double s = 0;
-        for (int d = d0; d < d1; ++d) s += donorW[d]*W[donorSrc[d] + c*v.cs + i];
-        if (c == 0) s = fmax(s, hc::kRhoFloor);
-        if (c == 4) s = fmax(s, 0.0);
-        W[dst + c*v.cs] = s;
+        const double wd = donorW[d];
+        const double* sp = W + donorSrc[d] + i;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) acc[c] += wd*sp[c*v.cs];
     }
+    acc[0] = fmax(acc[0], hc::kRhoFloor);
+    acc[4] = fmax(acc[4], 0.0);
+#pragma unroll
+    for (int c = 0; c < 8; ++c) W[dst + c*v.cs] = acc[c];
 }
 
 // 3. Radial boundary ghosts: inner fixed primitives, outer copy of the last

@@ -1337,12 +1337,18 @@ __global__ void __launch_bounds__(kB) k4_seam(Prep4View v, int ncol, const long 
     const int i = t % v.nr;
     const int col = t / v.nr;
     const int d0 = start[col], d1 = start[col+1];
-    for (int c = 0; c < 8; ++c)
+    // donors outer, components inner: each donor's index and weight are read
+    // once; every component still sums its donors in table order
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int d = d0; d < d1; ++d)
     {
-        double s = 0;
-        for (int d = d0; d < d1; ++d) s += w[d]*cart[src[d] + c*v.ccs + i];
-        cart[dst[col] + c*v.ccs + i] = s;
+        const double wd = w[d];
+        const double* sp = cart + src[d] + i;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) acc[c] += wd*sp[c*v.ccs];
     }
+#pragma unroll
+    for (int c = 0; c < 8; ++c) cart[dst[col] + c*v.ccs + i] = acc[c];
 }
 
 // Native components by the fourth-order matrix quotient on the USph cells
