@@ -79,6 +79,7 @@ _SIGNATURES = [
     ("hc_amr_interp4", _I, [_P, _P, _I, _P, _P, _P, _P]),
     ("hc_amr_ghosts2", _I, [_P, _P, _P, _P, _P, _D, _P, _P]),
     ("hc_amr_average_down", _I, [_P, _P, _I, _P, _P, _P, _P]),
+    ("hc_amr_prolong4", _I, [_P, _P, _P, _P, _P, _P, _D, _P, _P, _P]),
     ("hc_init_state", _I, [ctypes.POINTER(HcShell), ctypes.POINTER(HcOptions), _I, ctypes.c_ulonglong, _P, _P]),
     ("hc_geom_cell_detA", _D, [_I, _I, _I]),
     ("hc_geom_face_normal", None, [_I, _I, _I, _I, _I, _P]),
@@ -404,6 +405,27 @@ def amr_average_down(fine, fine_lo, fine_hi, ratio, coarse, coarse_lo, coarse_hi
     r = (ctypes.c_int * 3)(*[int(x) for x in ratio])
     _check(lib().hc_amr_average_down(_ptr(fine), ctypes.byref(fb), nc, r, ctypes.byref(cb), _ptr(coarse),
                                      ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
+
+
+def amr_prolong4(interp, fine_j, fine_lo, fine_hi, coarse, coarse_lo, coarse_hi, ratio, fine_ju,
+                 gamma=5.0 / 3.0):
+    """Conservative fourth-order refinement (fine J*U in fine_ju[8][..]) from
+    interp = amr_interp4 of the coarse J*U and J (9 components), the fine
+    Jacobian fine_j[1][..] and the coarse J*U and J coarse[9][..]. Returns the
+    number of invalid parents (0 on success; HcError is raised otherwise)."""
+    import torch
+
+    _check_patch(interp, _patch_shape(fine_lo, fine_hi, 9), "interp")
+    _check_patch(fine_j, _patch_shape(fine_lo, fine_hi, 1), "fine_j")
+    _check_patch(coarse, _patch_shape(coarse_lo, coarse_hi, 9), "coarse")
+    _check_patch(fine_ju, _patch_shape(fine_lo, fine_hi, 8), "fine_ju")
+    fb, cb = _patch(fine_lo, fine_hi), _patch(coarse_lo, coarse_hi)
+    r = (ctypes.c_int * 3)(*[int(x) for x in ratio])
+    bad = ctypes.c_int()
+    _check(lib().hc_amr_prolong4(_ptr(interp), _ptr(fine_j), ctypes.byref(fb), _ptr(coarse), ctypes.byref(cb), r,
+                                 float(gamma), _ptr(fine_ju), ctypes.byref(bad),
+                                 ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
+    return bad.value
 
 
 def box_table(boxes):

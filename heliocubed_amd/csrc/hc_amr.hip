@@ -15,6 +15,9 @@
 //    primitives.
 //  * hc_amr_average_down: restriction of fine averages to their parents
 //    (Stencil::AvgDown in restrictTo, src/MHD_AMR.H:596-606).
+//  * hc_amr_prolong4: conservative fourth-order refinement of new fine cells
+//    with the positivity-preserving bisection (prolongFourthOrder,
+//    src/MHD_AMRFourthOrder.H:226-314).
 //
 // Patches use the Proto BoxData layout: component-major, index 0 fastest,
 // inclusive lo..hi in the level's (possibly negative ghost) index space.
@@ -24,6 +27,7 @@
 
 #include <cmath>
 #include <utility>
+#include <vector>
 
 namespace hc
 {
@@ -197,6 +201,122 @@ __global__ void __launch_bounds__(kAmrBlock) k_amr_avgdown(AvgDownView v, const 
     }
 }
 
+struct Prolong4View
+{
+    Patch fine, coarse, parents;   // parents: the coarse cells whose children fill `fine`
+    int ratio[3];
+    double gamma;
+};
+
+// Admissible parent state: density >= RHO00 (64 ulp slack) and non-negative
+// thermal energy within 64 ulp of the energy scale, amrAdmissibleParent
+// (src/MHD_AMR.H:58-73).
+__device__ inline bool admissibleParent(const double u[8])
+{
+    const double eps = 2.220446049250313e-16;
+    if (!(u[0] >= kRhoFloor*(1 - 64*eps)) || !isfinite(u[0])) return false;
+    double thermal = u[4], scale = fabs(u[4]);
+    for (int d = 0; d < 3; ++d)
+    {
+        const double energy = 0.5*u[1+d]*u[1+d]/u[0] + u[5+d]*u[5+d]/(8*kPi);
+        thermal -= energy; scale += energy;
+    }
+    return isfinite(thermal) && thermal >= -64*eps*scale;
+}
+
+// Conservative fourth-order refinement of one parent (prolongFourthOrder,
+// src/MHD_AMRFourthOrder.H:253-307): the children's physical values are the
+// interpolated J*U over the interpolated J; their J-weighted mean is replaced
+// by the parent's J*U (times the child count over the children's J sum), and
+// the deviations are scaled by the largest theta in [0, 1] (48 bisection
+// steps) that keeps every child's density >= RHO00 and thermal energy >= 0.
+// status[parent] = 0 when done, 1 when a child's interpolated values are not
+// usable or the parent itself is not admissible.
+__global__ void __launch_bounds__(kAmrBlock) k_amr_prolong4(Prolong4View v, const double* __restrict__ interp,
+                                                            const double* __restrict__ fineJ,
+                                                            const double* __restrict__ coarse,
+                                                            double* __restrict__ out, int* __restrict__ status)
+{
+    const long long t = (long long)blockIdx.x*blockDim.x + threadIdx.x;
+    if (t >= v.parents.count()) return;
+    int q[3];
+    q[0] = v.parents.lo[0] + (int)(t % v.parents.n(0));
+    q[1] = v.parents.lo[1] + (int)((t / v.parents.n(0)) % v.parents.n(1));
+    q[2] = v.parents.lo[2] + (int)(t / (v.parents.n(0)*v.parents.n(1)));
+    const int r0 = v.ratio[0], r1 = v.ratio[1], r2 = v.ratio[2];
+    auto child = [&](int n, int p[3]) {
+        p[0] = q[0]*r0 + n % r0; p[1] = q[1]*r1 + (n / r0) % r1; p[2] = q[2]*r2 + n / (r0*r1); };
+    const int nch = r0*r1*r2;
+    double sumJ = 0.0, mean[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    bool available = true;
+    for (int n = 0; n < nch && available; ++n)
+    {
+        int p[3];
+        child(n, p);
+        const double iJ = interp[v.fine.at(8, p[0], p[1], p[2])];
+        if (!isfinite(iJ) || !(iJ > 0)) { available = false; break; }
+        const double j = fineJ[v.fine.at(0, p[0], p[1], p[2])];
+        sumJ += j;
+        for (int c = 0; c < 8; ++c)
+        {
+            const double physical = interp[v.fine.at(c, p[0], p[1], p[2])]/iJ;
+            if (!isfinite(physical)) available = false;
+            out[v.fine.at(c, p[0], p[1], p[2])] = physical;
+            mean[c] += j*physical;
+        }
+    }
+    if (!available) { status[t] = 1; return; }
+    const double cells = double(r0)*r1*r2;
+    double center[8];
+    for (int c = 0; c < 8; ++c)
+    {
+        mean[c] /= sumJ;
+        center[c] = cells*coarse[v.coarse.at(c, q[0], q[1], q[2])]/sumJ;
+    }
+    auto admissible = [&](double theta) {
+        for (int n = 0; n < nch; ++n)
+        {
+            int p[3];
+            child(n, p);
+            double u[8];
+            for (int c = 0; c < 8; ++c) u[c] = center[c] + theta*(out[v.fine.at(c, p[0], p[1], p[2])] - mean[c]);
+            if (theta == 0)
+            {
+                if (!admissibleParent(u)) return false;
+                continue;
+            }
+            if (!(u[0] >= kRhoFloor)) return false;
+            double thermal = u[4];
+            for (int d = 0; d < 3; ++d) thermal -= 0.5*u[1+d]*u[1+d]/u[0] + u[5+d]*u[5+d]/(8*kPi);
+            if (!isfinite(thermal) || thermal < 0) return false;
+        }
+        return true; };
+    double theta = 1;
+    if (!admissible(theta))
+    {
+        if (!admissible(0)) { status[t] = 1; return; }
+        double lo = 0, hi = 1;
+        for (int n = 0; n < 48; ++n)
+        {
+            const double trial = 0.5*(lo + hi);
+            if (admissible(trial)) lo = trial; else hi = trial;
+        }
+        theta = lo;
+    }
+    for (int n = 0; n < nch; ++n)
+    {
+        int p[3];
+        child(n, p);
+        const double j = fineJ[v.fine.at(0, p[0], p[1], p[2])];
+        for (int c = 0; c < 8; ++c)
+        {
+            const long long o = v.fine.at(c, p[0], p[1], p[2]);
+            out[o] = j*(center[c] + theta*(out[o] - mean[c]));
+        }
+    }
+    status[t] = 0;
+}
+
 // Average over the child interval [x - h/2, x + h/2] (coarse cell units,
 // parent centred at 0) of the quartic whose averages over the coarse cells
 // [k - 1/2, k + 1/2], k = -2..2, are the data: solve sum_k w_k <xi^n>_k =
@@ -327,6 +447,53 @@ int hc_amr_average_down(const double* fine, const hc_box* fine_box, int ncomp, c
         return 0;
     }
     catch (const std::exception& e) { return fail(std::string("hc_amr_average_down: ") + e.what()); }
+}
+
+int hc_amr_prolong4(const double* interp, const double* fine_j, const hc_box* fine_box, const double* coarse,
+                    const hc_box* coarse_box, const int* ratio, double gamma, double* fine_ju, int* invalid,
+                    hc_stream stream)
+{
+    using namespace hc;
+    if (!interp || !fine_j || !coarse || !fine_ju || !ratio || !validBox(fine_box) || !validBox(coarse_box))
+        return fail("hc_amr_prolong4: null argument or empty box");
+    try
+    {
+        Prolong4View v{};
+        v.fine = toPatch(*fine_box);
+        v.coarse = toPatch(*coarse_box);
+        v.gamma = gamma;
+        for (int d = 0; d < 3; ++d)
+        {
+            if (ratio[d] < 1) return fail("hc_amr_prolong4: ratio must be >= 1 per direction");
+            v.ratio[d] = ratio[d];
+            // the fine box must be whole parents, all inside the coarse box
+            if (floorDiv(fine_box->lo[d], ratio[d])*ratio[d] != fine_box->lo[d] ||
+                floorDiv(fine_box->hi[d] + 1, ratio[d])*ratio[d] != fine_box->hi[d] + 1)
+                return fail("hc_amr_prolong4: the fine box must cover whole coarse cells");
+            v.parents.lo[d] = floorDiv(fine_box->lo[d], ratio[d]);
+            v.parents.hi[d] = floorDiv(fine_box->hi[d] + 1, ratio[d]) - 1;
+            if (v.parents.lo[d] < coarse_box->lo[d] || v.parents.hi[d] > coarse_box->hi[d])
+                return fail("hc_amr_prolong4: parents outside the coarse box");
+        }
+        const long long n = v.parents.count();
+        int* status = nullptr;
+        HC_THROW(hipMalloc(&status, n*sizeof(int)));
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        hipLaunchKernelGGL(k_amr_prolong4, dim3((unsigned)((n + kAmrBlock - 1)/kAmrBlock)), dim3(kAmrBlock), 0, s,
+                           v, interp, fine_j, coarse, fine_ju, status);
+        std::vector<int> host(n);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipMemcpyAsync(host.data(), status, n*sizeof(int), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        (void)hipFree(status);
+        HC_THROW(e);
+        int bad = 0;
+        for (long long x = 0; x < n; ++x) bad += host[x];
+        if (invalid) *invalid = bad;
+        if (bad) return fail("hc_amr_prolong4: invalid or unavailable parent data for conservative refinement");
+        return 0;
+    }
+    catch (const std::exception& e) { return fail(std::string("hc_amr_prolong4: ") + e.what()); }
 }
 
 } // extern "C"

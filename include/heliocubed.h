@@ -321,6 +321,23 @@ int hc_amr_ghosts2(const double* coarse_ju_j, const hc_box* coarse_box, const in
 int hc_amr_average_down(const double* fine, const hc_box* fine_box, int ncomp, const int* ratio,
                         const hc_box* coarse_box, double* coarse, hc_stream stream);
 
+/* hc_amr_prolong4: conservative fourth-order refinement of new fine cells
+ * (prolongFourthOrder, src/MHD_AMRFourthOrder.H:226-314). interp[9][fine_box]
+ * is hc_amr_interp4 of the coarse J*U and J (9 components) onto fine_box,
+ * fine_j[fine_box] the fine Jacobian, coarse[9][coarse_box] the coarse J*U
+ * and J. fine_box must consist of whole coarse cells inside coarse_box. Each
+ * parent's children get J_fine (center + theta (u - mean)): u the
+ * interpolated J*U / J, mean their J-weighted mean, center the parent's J*U
+ * times the child count over the children's J sum (so the children's J*U sum
+ * to the count times the parent's), theta the largest value in [0, 1] (48
+ * bisection steps) keeping every child's density >= RHO00 and thermal
+ * energy >= 0. Synchronizes the stream; *invalid (may be NULL) counts parents
+ * whose children are unusable or which are not admissible themselves, and the
+ * call then fails, as the reference throws. */
+int hc_amr_prolong4(const double* interp, const double* fine_j, const hc_box* fine_box, const double* coarse,
+                    const hc_box* coarse_box, const int* ratio, double gamma, double* fine_ju, int* invalid,
+                    hc_stream stream);
+
 #ifdef __cplusplus
 }
 #endif

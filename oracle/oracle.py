@@ -85,6 +85,7 @@ def lib() -> ctypes.CDLL:
             "orc_amr_interp4": (None, [P, P, P, I, P, P, P, P]),
             "orc_amr_ghosts2": (None, [P, P, P, P, P, P, P, P, D, P]),
             "orc_amr_average_down": (None, [P, P, P, I, P, P, P, P]),
+            "orc_amr_prolong4": (I, [P, P, P, P, P, P, P, P, P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -312,6 +313,18 @@ def amr_average_down(fine, fine_lo, fine_hi, ratio, coarse_lo, coarse_hi):
     lib().orc_amr_average_down(_p(src), args[0].ctypes.data, args[1].ctypes.data, nc, args[2].ctypes.data,
                                args[3].ctypes.data, args[4].ctypes.data, _p(out))
     return out
+
+
+def amr_prolong4(interp, fine_j, fine_lo, fine_hi, coarse, coarse_lo, coarse_hi, ratio):
+    """(fine J*U [8][..], invalid parent count) of the conservative fourth-order refinement."""
+    it = np.ascontiguousarray(interp, dtype=np.float64)
+    fj = np.ascontiguousarray(fine_j, dtype=np.float64)
+    co = np.ascontiguousarray(coarse, dtype=np.float64)
+    out = np.zeros(_box_shape(fine_lo, fine_hi, 8))
+    args = [_ivec(x) for x in (fine_lo, fine_hi, coarse_lo, coarse_hi, ratio)]
+    bad = lib().orc_amr_prolong4(_p(it), _p(fj), args[0].ctypes.data, args[1].ctypes.data, _p(co),
+                                 args[2].ctypes.data, args[3].ctypes.data, args[4].ctypes.data, _p(out))
+    return out, bad
 
 
 def basis(p, alpha, beta):

@@ -75,3 +75,34 @@ def test_average_down_matches_oracle(device, ratio):
     err = np.abs(out.cpu().numpy() - ref).max() / np.abs(ref).max()
     print("average_down", ratio, err)
     assert err <= 1e-13
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("jump", [False, True])
+def test_prolong4_matches_oracle(device, jump):
+    from test_oracle_amr_pins import _prolong_case
+    coarse, interp, fine_j, _, clo, chi, flo, fhi, r, J = _prolong_case(23, jump=jump)
+    ref, bad = orc.amr_prolong4(interp, fine_j, flo, fhi, coarse, clo, chi, r)
+    assert bad == 0
+    # the device chain: interpolation then conservative refinement
+    it = torch.zeros((9, 8, 8, 8), dtype=torch.float64, device=device)
+    hc.amr_interp4(_dev(coarse), clo, chi, r, it, flo, fhi)
+    out = torch.zeros((8, 8, 8, 8), dtype=torch.float64, device=device)
+    nbad = hc.amr_prolong4(it, _dev(fine_j), flo, fhi, _dev(coarse), clo, chi, r, out)
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    print("prolong4", jump, nbad, err)
+    assert nbad == 0 and err <= 1e-12
+
+
+@pytest.mark.gpu
+def test_prolong4_rejects_inadmissible_parent(device):
+    from test_oracle_amr_pins import _prolong_case
+    coarse, interp, fine_j, _, clo, chi, flo, fhi, r, J = _prolong_case(29)
+    coarse[0, 3, 3, 3] = -1.0
+    it = torch.zeros((9, 8, 8, 8), dtype=torch.float64, device=device)
+    hc.amr_interp4(_dev(coarse), clo, chi, r, it, flo, fhi)
+    out = torch.zeros((8, 8, 8, 8), dtype=torch.float64, device=device)
+    with pytest.raises(hc.HcError, match="invalid"):
+        hc.amr_prolong4(it, _dev(fine_j), flo, fhi, _dev(coarse), clo, chi, r, out)

@@ -158,3 +158,57 @@ def test_average_down_inverts_interp_and_conserves():
     c2 = orc.amr_average_down(f2, (0, 0, 0), (3, 8, 5), (2, 3, 3), (0, 0, 0), (1, 2, 1))
     assert np.allclose(c2.sum(axis=(1, 2, 3)) * 18, f2.sum(axis=(1, 2, 3)), rtol=1e-13)
     assert np.allclose(c2[:, 1, 2, 0], f2[:, 3:6, 6:9, 0:2].mean(axis=(1, 2, 3)), rtol=1e-14)
+
+
+def _prolong_case(seed, jump=False):
+    rng = np.random.default_rng(seed)
+    clo, chi, r = (-2, -2, -2), (5, 5, 5), (2, 2, 2)
+    J = 1.5
+    terms = _state_terms(rng)
+    coarse = np.zeros((9, 8, 8, 8))
+    for c in range(8):
+        coarse[c] = J * SCALARS[c] * _box_avgs(terms[c], clo, chi, (1, 1, 1))
+    coarse[8] = J
+    if jump:
+        # a density and energy cliff: the quartic undershoots next to it
+        coarse[0:5, :, :, 4:] *= 1e-3
+        coarse[5:8, :, :, 4:] *= np.sqrt(1e-3)
+    flo, fhi = (0, 0, 0), (7, 7, 7)                     # parents 0..3, donors -2..5
+    interp = orc.amr_interp4(coarse, clo, chi, r, flo, fhi)
+    fine_j = np.full((1, 8, 8, 8), J)
+    return coarse, interp, fine_j, terms, clo, chi, flo, fhi, r, J
+
+
+def _child_sums(fine, r=2):
+    return fine.reshape(fine.shape[0], 4, r, 4, r, 4, r).sum(axis=(2, 4, 6))
+
+
+def test_prolong4_is_conservative_and_exact_for_smooth_data():
+    coarse, interp, fine_j, terms, clo, chi, flo, fhi, r, J = _prolong_case(13)
+    fine, bad = orc.amr_prolong4(interp, fine_j, flo, fhi, coarse, clo, chi, r)
+    assert bad == 0
+    parents = coarse[:8, 2:6, 2:6, 2:6]
+    assert np.allclose(_child_sums(fine), 8 * parents, rtol=1e-13)
+    exact = np.stack([J * SCALARS[c] * _box_avgs(terms[c], flo, fhi, r) for c in range(8)])
+    assert np.abs(fine - exact).max() <= 1e-12 * np.abs(exact).max()
+
+
+def test_prolong4_limits_toward_the_parent_to_stay_positive():
+    coarse, interp, fine_j, _, clo, chi, flo, fhi, r, J = _prolong_case(17, jump=True)
+    phys = interp[:8] / interp[8]
+    assert (phys[0] < 0).any() or ((phys[4] - 0.5 * (phys[1:4] ** 2).sum(0) / phys[0]
+                                    - (phys[5:8] ** 2).sum(0) / (8 * np.pi)) < 0).any()
+    fine, bad = orc.amr_prolong4(interp, fine_j, flo, fhi, coarse, clo, chi, r)
+    assert bad == 0
+    u = fine / J
+    thermal = u[4] - 0.5 * (u[1:4] ** 2).sum(0) / u[0] - (u[5:8] ** 2).sum(0) / (8 * np.pi)
+    assert (u[0] >= 1.67262192e-26).all() and (thermal >= 0).all()
+    assert np.allclose(_child_sums(fine), 8 * coarse[:8, 2:6, 2:6, 2:6], rtol=1e-12)
+
+
+def test_prolong4_reports_inadmissible_parents():
+    coarse, interp, fine_j, _, clo, chi, flo, fhi, r, J = _prolong_case(19)
+    coarse[0, 3, 3, 3] = -1.0          # parent (1, 1, 1): negative density
+    interp = orc.amr_interp4(coarse, clo, chi, r, flo, fhi)
+    _, bad = orc.amr_prolong4(interp, fine_j, flo, fhi, coarse, clo, chi, r)
+    assert bad >= 1
