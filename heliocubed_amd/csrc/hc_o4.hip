@@ -538,8 +538,9 @@ __device__ __forceinline__ void pointFaceRegion(const BoxDesc& b, int grow, int 
     L[D] = b.lo[D]; H[D] = b.hi[D]+1;
 }
 
-// 4. Limited, flattened and repaired face states; point states by transverse
-// deconvolution, on the point-flux faces (pointFaceRegion).
+// 5. Limited, flattened and repaired face states; point states by transverse
+// deconvolution; their physical fluxes and face-average total pressures, on
+// the point-flux faces (pointFaceRegion). Runs after k_o4_bar.
 template <int D, bool ROE>
 __global__ void __launch_bounds__(kB) k_o4_faces(O4View v, double* __restrict__ S)
 {
@@ -627,11 +628,44 @@ __global__ void __launch_bounds__(kB) k_o4_faces(O4View v, double* __restrict__ 
         P[I(F_PTL+q, x[0], x[1], x[2])] = lf[q]-lapb[q]/24.0;
         P[I(F_PTR+q, x[0], x[1], x[2])] = rf[q]-lapb[q]/24.0;
     }
-    P[I(F_P0L, x[0], x[1], x[2])] = lf[4];
-    P[I(F_P0R, x[0], x[1], x[2])] = rf[4];
     if constexpr (!ROE)
         P[I(F_SPD, x[0], x[1], x[2])] = rusanovSpeedO4(lf, rf, v.faceA[D] + 9*angIdx(b, x[1], x[2]),
                                                        v.gamma, D, false);
+    // Physical fluxes of both point states (point flux + transverse Laplacian
+    // of the face-average flux / 24, k_o4_bar having run first) and the
+    // face-average total pressure p + me + Lap(meBar)/24.
+    double lapF[8], lapMe = 0;
+    {
+        const double mex = P[I(F_MEF, x[0], x[1], x[2])];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) lapF[q] = 0;
+#pragma unroll
+        for (int t = 0; t < 2; ++t)
+        {
+            const int tt = t == 0 ? t1 : t2;
+            int y[3] = {x[0], x[1], x[2]}, z[3] = {x[0], x[1], x[2]};
+            y[tt] += 1; z[tt] -= 1;
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+                lapF[q] += P[I(F_FB+q, y[0], y[1], y[2])] - 2.0*P[I(F_FB+q, x[0], x[1], x[2])]
+                         + P[I(F_FB+q, z[0], z[1], z[2])];
+            lapMe += P[I(F_MEF, y[0], y[1], y[2])] - 2.0*mex + P[I(F_MEF, z[0], z[1], z[2])];
+        }
+    }
+    const double* A = v.faceA[D] + 9*angIdx(b, x[1], x[2]);
+#pragma unroll 1
+    for (int side = 0; side < 2; ++side)
+    {
+        double w[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) w[q] = (side == 0 ? lf[q] : rf[q]) - lapb[q]/24.0;
+        double ke, me, vb, f[8];
+        energiesNative(A, w, ke, me, vb);
+        primFlux(f, w, ke, me, vb, v.gamma, D);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) P[I((side == 0 ? F_PHL : F_PHR)+q, x[0], x[1], x[2])] = f[q]+lapF[q]/24.0;
+        P[I(side == 0 ? F_P0L : F_P0R, x[0], x[1], x[2])] = (side == 0 ? lf[4] : rf[4]) + (me + lapMe/24.0);
+    }
 }
 
 // Energies (ke, me, v.B) and the physical flux of the face-average state at
@@ -649,9 +683,10 @@ __device__ void barFlux(const O4View& v, const BoxDesc& b, const BoxIdx& I, cons
     pBar = wf[4];
 }
 
-// 5a. Physical flux of the face-average state WFaceBar (F_FB), its magnetic
+// 4. Physical flux of the face-average state WFaceBar (F_FB), its magnetic
 // energy (F_MEF) and total pressure pBar + meBar (F_P0B) on the point-flux
-// faces grown by one transversally (their Laplacian in k_o4_phys).
+// faces grown by one transversally (their Laplacian in k_o4_faces). Needs only
+// the cell averages, so it runs first in each direction pass.
 template <int D>
 __global__ void __launch_bounds__(kB) k_o4_bar(O4View v, double* __restrict__ S)
 {
@@ -667,52 +702,6 @@ __global__ void __launch_bounds__(kB) k_o4_bar(O4View v, double* __restrict__ S)
     for (int q = 0; q < 8; ++q) P[I(F_FB+q, x[0], x[1], x[2])] = f[q];
     P[I(F_MEF, x[0], x[1], x[2])] = me;
     P[I(F_P0B, x[0], x[1], x[2])] = p + me;
-}
-
-// 5b. Physical fluxes of both face states (point flux + transverse Laplacian
-// of the face-average flux / 24) and the face-average total pressure p + me,
-// on the point-flux faces (pointFaceRegion).
-template <int D>
-__global__ void __launch_bounds__(kB) k_o4_phys(O4View v, double* __restrict__ S)
-{
-    const BoxDesc& b = blockBoxDesc(v);
-    int L[3], H[3], x[3];
-    pointFaceRegion<D>(b, 0, L, H);
-    if (!regionPoint(b, L, H, boxThread(v, b), x)) return;
-    const BoxIdx I(b);
-    double* const P = S + b.sOff;
-    constexpr int t1 = D == 0 ? 1 : 0, t2 = D == 2 ? 1 : 2;
-    // transverse Laplacian of the face-average flux and magnetic energy
-    double lapF[8], lapMe = 0;
-    const double mex = P[I(F_MEF, x[0], x[1], x[2])];
-#pragma unroll
-    for (int q = 0; q < 8; ++q) lapF[q] = 0;
-#pragma unroll
-    for (int t = 0; t < 2; ++t)
-    {
-        const int tt = t == 0 ? t1 : t2;
-        int y[3] = {x[0], x[1], x[2]}, z[3] = {x[0], x[1], x[2]};
-        y[tt] += 1; z[tt] -= 1;
-#pragma unroll
-        for (int q = 0; q < 8; ++q)
-            lapF[q] += P[I(F_FB+q, y[0], y[1], y[2])] - 2.0*P[I(F_FB+q, x[0], x[1], x[2])] + P[I(F_FB+q, z[0], z[1], z[2])];
-        lapMe += P[I(F_MEF, y[0], y[1], y[2])] - 2.0*mex + P[I(F_MEF, z[0], z[1], z[2])];
-    }
-    const double* A = v.faceA[D] + 9*angIdx(b, x[1], x[2]);
-#pragma unroll 1
-    for (int side = 0; side < 2; ++side)
-    {
-        double w[8];
-#pragma unroll
-        for (int q = 0; q < 8; ++q) w[q] = P[I((side == 0 ? F_PTL : F_PTR)+q, x[0], x[1], x[2])];
-        double ke, me, vb, f[8];
-        energiesNative(A, w, ke, me, vb);
-        primFlux(f, w, ke, me, vb, v.gamma, D);
-#pragma unroll
-        for (int q = 0; q < 8; ++q) P[I((side == 0 ? F_PHL : F_PHR)+q, x[0], x[1], x[2])] = f[q]+lapF[q]/24.0;
-        const int fp = side == 0 ? F_P0L : F_P0R;
-        P[I(fp, x[0], x[1], x[2])] = P[I(fp, x[0], x[1], x[2])] + (me + lapMe/24.0);
-    }
 }
 
 // 5c. Point Roe dissipation of the face states on the faces [lo_d, hi_d+1] x
@@ -1204,10 +1193,9 @@ void buildBoxop(hc_boxop& B, const hc_box* boxes, const int* boxLevel, int nbox)
 template <int D, typename S_>
 void directionPass(const O4View& v, dim3 grid, hipStream_t s, double* scratch)
 {
+    hipLaunchKernelGGL(k_o4_bar<D>, grid, dim3(kB), 0, s, v, scratch);
     if (v.roe) hipLaunchKernelGGL((k_o4_faces<D, true>), grid, dim3(kB), 0, s, v, scratch);
     else hipLaunchKernelGGL((k_o4_faces<D, false>), grid, dim3(kB), 0, s, v, scratch);
-    hipLaunchKernelGGL(k_o4_bar<D>, grid, dim3(kB), 0, s, v, scratch);
-    hipLaunchKernelGGL(k_o4_phys<D>, grid, dim3(kB), 0, s, v, scratch);
     if (v.roe) hipLaunchKernelGGL((k_o4_roe<D, S_>), grid, dim3(kB), 0, s, v, scratch);
     else hipLaunchKernelGGL(k_o4_rusq<D>, grid, dim3(kB), 0, s, v, scratch);
     hipLaunchKernelGGL(k_o4_map<D>, grid, dim3(kB), 0, s, v, scratch);
