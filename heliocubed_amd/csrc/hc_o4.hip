@@ -824,9 +824,10 @@ struct FaceOps
 };
 
 // 6a. Mapped products of both sides' physical point fluxes into the face
-// state slots (dead once k_o4_roe has run): faces [lo_d, hi_d+1] x [lo, hi],
-// grown by one in the radial index when it is transverse (d = 1, 2), where
-// the radial-area tensor product needs them.
+// state slots (dead once k_o4_roe has run), for the angular directions
+// (d = 1, 2): faces [lo_d, hi_d+1] x [lo, hi] grown by one in the radial
+// index, where the radial-area tensor product needs them. Radial faces
+// (d = 0) need them only at the face itself; k_o4_flux<0> evaluates them.
 template <int D>
 __global__ void __launch_bounds__(kB) k_o4_map(O4View v, double* __restrict__ S)
 {
@@ -892,13 +893,30 @@ __global__ void __launch_bounds__(kB) k_o4_flux(O4View v, double* __restrict__ S
         const int base = side == 0 ? F_MPL : F_MPR;
         const int y[3] = {x[0]+1, x[1], x[2]}, z[3] = {x[0]-1, x[1], x[2]};
         double Fs[8];
+        if constexpr (D == 0)
+        {
+            // radial faces need the mapped products only at the face itself:
+            // evaluate them here instead of in k_o4_map
+            double mp[8];
+            op.mapped(side == 0 ? F_PHL : F_PHR, x, mp);
+#pragma unroll
+            for (int c = 0; c < 8; ++c)
+            {
+                double val = 0;
+                val += adjX*mp[c];
+                Fs[c] = val;
+            }
+        }
+        else
+        {
 #pragma unroll
         for (int c = 0; c < 8; ++c)
         {
             double val = 0;
-            if (D != 0) val += adjD1*(0.5*Fv(base+c, y) - 0.5*Fv(base+c, z))*(1.0/12.0);
+            val += adjD1*(0.5*Fv(base+c, y) - 0.5*Fv(base+c, z))*(1.0/12.0);
             val += adjX*Fv(base+c, x);
             Fs[c] = val;
+        }
         }
         // pressure force: p0 (face average) with p0Bar = pBar + meBar
         const int fp = side == 0 ? F_P0L : F_P0R;
@@ -1198,7 +1216,7 @@ void directionPass(const O4View& v, dim3 grid, hipStream_t s, double* scratch)
     else hipLaunchKernelGGL((k_o4_faces<D, false>), grid, dim3(kB), 0, s, v, scratch);
     if (v.roe) hipLaunchKernelGGL((k_o4_roe<D, S_>), grid, dim3(kB), 0, s, v, scratch);
     else hipLaunchKernelGGL(k_o4_rusq<D>, grid, dim3(kB), 0, s, v, scratch);
-    hipLaunchKernelGGL(k_o4_map<D>, grid, dim3(kB), 0, s, v, scratch);
+    if (D != 0) hipLaunchKernelGGL(k_o4_map<D>, grid, dim3(kB), 0, s, v, scratch);
     if (v.roe) hipLaunchKernelGGL((k_o4_flux<D, true>), grid, dim3(kB), 0, s, v, scratch);
     else hipLaunchKernelGGL((k_o4_flux<D, false>), grid, dim3(kB), 0, s, v, scratch);
 }
