@@ -541,8 +541,8 @@ __device__ __forceinline__ void pointFaceRegion(const BoxDesc& b, int grow, int 
 // 5. Limited, flattened and repaired face states; point states by transverse
 // deconvolution; their physical fluxes and face-average total pressures, on
 // the point-flux faces (pointFaceRegion). Runs after k_o4_bar.
-template <int D, bool ROE>
-__global__ void __launch_bounds__(kB) k_o4_faces(O4View v, double* __restrict__ S)
+template <int D, bool ROE, typename S_>
+__global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(2))) k_o4_faces(O4View v, double* __restrict__ S)
 {
     const BoxDesc& b = blockBoxDesc(v);
     int L[3], H[3], x[3];
@@ -622,15 +622,17 @@ __global__ void __launch_bounds__(kB) k_o4_faces(O4View v, double* __restrict__ 
 #pragma unroll
             for (int q = 0; q < 8; ++q) { lf[q] = cl[q]+corr[q]; rf[q] = cr[q]+corr[q]; }
     }
-#pragma unroll
-    for (int q = 0; q < 8; ++q)
-    {
-        P[I(F_PTL+q, x[0], x[1], x[2])] = lf[q]-lapb[q]/24.0;
-        P[I(F_PTR+q, x[0], x[1], x[2])] = rf[q]-lapb[q]/24.0;
-    }
     if constexpr (!ROE)
+    {
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+        {
+            P[I(F_PTL+q, x[0], x[1], x[2])] = lf[q]-lapb[q]/24.0;
+            P[I(F_PTR+q, x[0], x[1], x[2])] = rf[q]-lapb[q]/24.0;
+        }
         P[I(F_SPD, x[0], x[1], x[2])] = rusanovSpeedO4(lf, rf, v.faceA[D] + 9*angIdx(b, x[1], x[2]),
                                                        v.gamma, D, false);
+    }
     // Physical fluxes of both point states (point flux + transverse Laplacian
     // of the face-average flux / 24, k_o4_bar having run first) and the
     // face-average total pressure p + me + Lap(meBar)/24.
@@ -665,6 +667,23 @@ __global__ void __launch_bounds__(kB) k_o4_faces(O4View v, double* __restrict__ 
 #pragma unroll
         for (int q = 0; q < 8; ++q) P[I((side == 0 ? F_PHL : F_PHR)+q, x[0], x[1], x[2])] = f[q]+lapF[q]/24.0;
         P[I(side == 0 ? F_P0L : F_P0R, x[0], x[1], x[2])] = (side == 0 ? lf[4] : rf[4]) + (me + lapMe/24.0);
+    }
+    if constexpr (ROE)
+    {
+        // point Roe dissipation of the two point states on the faces the flux
+        // kernel averages ([lo-1, hi+1] transversally)
+        bool roeFace = true;
+        for (int d = 0; d < 3; ++d)
+            if (d != D && (x[d] < b.lo[d]-1 || x[d] > b.hi[d]+1)) roeFace = false;
+        if (roeFace)
+        {
+            double wl[8], wr[8], out[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) { wl[q] = lf[q]-lapb[q]/24.0; wr[q] = rf[q]-lapb[q]/24.0; }
+            factoredRoe<S_>(out, wl, wr, A, radialPoint(v, b, D, x[0]), D, v.gamma, v.entropy);
+#pragma unroll
+            for (int q = 0; q < 8; ++q) P[I(F_DPT+q, x[0], x[1], x[2])] = out[q];
+        }
     }
 }
 
@@ -1212,10 +1231,9 @@ template <int D, typename S_>
 void directionPass(const O4View& v, dim3 grid, hipStream_t s, double* scratch)
 {
     hipLaunchKernelGGL(k_o4_bar<D>, grid, dim3(kB), 0, s, v, scratch);
-    if (v.roe) hipLaunchKernelGGL((k_o4_faces<D, true>), grid, dim3(kB), 0, s, v, scratch);
-    else hipLaunchKernelGGL((k_o4_faces<D, false>), grid, dim3(kB), 0, s, v, scratch);
-    if (v.roe) hipLaunchKernelGGL((k_o4_roe<D, S_>), grid, dim3(kB), 0, s, v, scratch);
-    else hipLaunchKernelGGL(k_o4_rusq<D>, grid, dim3(kB), 0, s, v, scratch);
+    if (v.roe) hipLaunchKernelGGL((k_o4_faces<D, true, S_>), grid, dim3(kB), 0, s, v, scratch);
+    else hipLaunchKernelGGL((k_o4_faces<D, false, double>), grid, dim3(kB), 0, s, v, scratch);
+    if (!v.roe) hipLaunchKernelGGL(k_o4_rusq<D>, grid, dim3(kB), 0, s, v, scratch);
     if (D != 0) hipLaunchKernelGGL(k_o4_map<D>, grid, dim3(kB), 0, s, v, scratch);
     if (v.roe) hipLaunchKernelGGL((k_o4_flux<D, true>), grid, dim3(kB), 0, s, v, scratch);
     else hipLaunchKernelGGL((k_o4_flux<D, false>), grid, dim3(kB), 0, s, v, scratch);
