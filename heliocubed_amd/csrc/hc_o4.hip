@@ -38,7 +38,8 @@ enum Field : int
     F_MEL = 36, F_MER = 37, F_PHL = 38, F_PHR = 46, F_DPT = 54, F_FLUX = 62,
     F_FB = 86, F_MEF = 94, F_COUNT = 95,
     // mapped products of the two sides' point fluxes; reuse the face state
-    // slots, which are dead once k_o4_roe has run
+    // slots (the standard operator keeps its face states in registers; the
+    // Berta operator's are dead once k_o4_roe has run)
     F_MPL = F_PTL, F_MPR = F_PTR,
     // Rusanov (riemann_solver 1): the speed of a face, and the conserved point
     // values of both sides (the dissipation and face-average flux slots)
@@ -723,8 +724,9 @@ __global__ void __launch_bounds__(kB) k_o4_bar(O4View v, double* __restrict__ S)
     P[I(F_P0B, x[0], x[1], x[2])] = p + me;
 }
 
-// 5c. Point Roe dissipation of the face states on the faces [lo_d, hi_d+1] x
-// transverse [lo-1, hi+1].
+// 5c. Point Roe dissipation of the stored face states on the faces
+// [lo_d, hi_d+1] x transverse [lo-1, hi+1] (the Berta operator's pass; the
+// standard operator evaluates it inside k_o4_faces).
 template <int D, typename S_>
 __global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(2))) k_o4_roe(O4View v, double* __restrict__ S)
 {
@@ -843,7 +845,7 @@ struct FaceOps
 };
 
 // 6a. Mapped products of both sides' physical point fluxes into the face
-// state slots (dead once k_o4_roe has run), for the angular directions
+// state slots, for the angular directions
 // (d = 1, 2): faces [lo_d, hi_d+1] x [lo, hi] grown by one in the radial
 // index, where the radial-area tensor product needs them. Radial faces
 // (d = 0) need them only at the face itself; k_o4_flux<0> evaluates them.
