@@ -37,10 +37,6 @@ enum Field : int
     F_WB = 0, F_MEB = 8, F_W = 9, F_ETA = 17, F_PTL = 18, F_PTR = 26, F_P0L = 34, F_P0R = 35,
     F_MEL = 36, F_MER = 37, F_PHL = 38, F_PHR = 46, F_DPT = 54, F_FLUX = 62,
     F_FB = 86, F_MEF = 94, F_COUNT = 95,
-    // mapped products of the two sides' point fluxes; reuse the face state
-    // slots (the standard operator keeps its face states in registers; the
-    // Berta operator's are dead once k_o4_roe has run)
-    F_MPL = F_PTL, F_MPR = F_PTR,
     // Rusanov (riemann_solver 1): the speed of a face, and the conserved point
     // values of both sides (the dissipation and face-average flux slots)
     F_SPD = F_MER, F_QL = F_DPT, F_QR = F_FB,
@@ -527,15 +523,13 @@ __device__ __forceinline__ void faceBar(const BoxIdx& I, const double* P, int d,
 }
 
 // Transverse extent of the face states and point fluxes of direction D: the
-// mapped products (k_o4_map) take one more transverse neighbour on each side of
-// faces that reach one radial layer beyond the box when D is angular, so the
-// point fluxes are needed on [lo-1, hi+1] transversally and [lo-2, hi+2] in the
-// radial index when it is transverse (D = 1, 2).
+// flux kernel's mapped products and their radial differences take one
+// transverse neighbour on each side of the box's faces, so the point fluxes are
+// needed on [lo-1, hi+1] transversally.
 template <int D>
 __device__ __forceinline__ void pointFaceRegion(const BoxDesc& b, int grow, int L[3], int H[3])
 {
     for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]-1-grow; H[d] = b.hi[d]+1+grow; }
-    if (D != 0) { L[0] -= 1; H[0] += 1; }
     L[D] = b.lo[D]; H[D] = b.hi[D]+1;
 }
 
@@ -671,20 +665,14 @@ __global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(2))) k_
     }
     if constexpr (ROE)
     {
-        // point Roe dissipation of the two point states on the faces the flux
-        // kernel averages ([lo-1, hi+1] transversally)
-        bool roeFace = true;
-        for (int d = 0; d < 3; ++d)
-            if (d != D && (x[d] < b.lo[d]-1 || x[d] > b.hi[d]+1)) roeFace = false;
-        if (roeFace)
-        {
-            double wl[8], wr[8], out[8];
+        // point Roe dissipation of the two point states (the flux kernel
+        // averages it over the transverse neighbours)
+        double wl[8], wr[8], out[8];
 #pragma unroll
-            for (int q = 0; q < 8; ++q) { wl[q] = lf[q]-lapb[q]/24.0; wr[q] = rf[q]-lapb[q]/24.0; }
-            factoredRoe<S_>(out, wl, wr, A, radialPoint(v, b, D, x[0]), D, v.gamma, v.entropy);
+        for (int q = 0; q < 8; ++q) { wl[q] = lf[q]-lapb[q]/24.0; wr[q] = rf[q]-lapb[q]/24.0; }
+        factoredRoe<S_>(out, wl, wr, A, radialPoint(v, b, D, x[0]), D, v.gamma, v.entropy);
 #pragma unroll
-            for (int q = 0; q < 8; ++q) P[I(F_DPT+q, x[0], x[1], x[2])] = out[q];
-        }
+        for (int q = 0; q < 8; ++q) P[I(F_DPT+q, x[0], x[1], x[2])] = out[q];
     }
 }
 
@@ -810,7 +798,10 @@ struct FaceOps
         a[t] += 1; z[t] -= 1;
         return 0.5*Fv(f, a) - 0.5*Fv(f, z);
     }
-    // <det A F> and <det(A) A F> of the point flux in fields base.. at y
+    // <det A F> and <det(A) A F> of the point flux in fields base.. at y.
+    // For an angular direction t1 is radial, along which the angular metric
+    // is constant: that term of the product rule is exactly +-0 and is left
+    // out.
     __device__ void mapped(int base, const int y[3], double out[8]) const
     {
 #pragma unroll
@@ -824,7 +815,7 @@ struct FaceOps
 #pragma unroll
                 for (int c = 0; c < 3; ++c)
                 {
-                    val += Dm(1+3*r+c, t1, y)*Df(base+c0+c, t1, y)*(1.0/12.0);
+                    if (D == 0) val += Dm(1+3*r+c, t1, y)*Df(base+c0+c, t1, y)*(1.0/12.0);
                     val += Dm(1+3*r+c, t2, y)*Df(base+c0+c, t2, y)*(1.0/12.0);
                     val += M(1+3*r+c, y)*Fv(base+c0+c, y);
                 }
@@ -836,41 +827,60 @@ struct FaceOps
         {
             const int c = s == 0 ? 0 : 4;
             double val = 0;
-            val += Dm(0, t1, y)*Df(base+c, t1, y)*(1.0/12.0);
+            if (D == 0) val += Dm(0, t1, y)*Df(base+c, t1, y)*(1.0/12.0);
             val += Dm(0, t2, y)*Df(base+c, t2, y)*(1.0/12.0);
             val += M(0, y)*Fv(base+c, y);
             out[c] = val;
         }
     }
+    // For an angular direction: half the difference of the mapped products at
+    // the radial neighbours y + e_r and y - e_r. The angular metric does not
+    // vary with the radius (its radial differences Dm(., t1) are exactly
+    // zero), so this is the mapped product of the fluxes' radial difference.
+    __device__ double Gv(int f, const int y[3]) const
+    {
+        const int a[3] = {y[0]+1, y[1], y[2]}, z[3] = {y[0]-1, y[1], y[2]};
+        return 0.5*Fv(f, a) - 0.5*Fv(f, z);
+    }
+    __device__ double DGv(int f, const int y[3]) const
+    {
+        int a[3] = {y[0], y[1], y[2]}, z[3] = {y[0], y[1], y[2]};
+        a[t2] += 1; z[t2] -= 1;
+        return 0.5*Gv(f, a) - 0.5*Gv(f, z);
+    }
+    __device__ void mappedRadialDiff(int base, const int y[3], double out[8]) const
+    {
+        static_assert(D != 0, "radial direction has a radius-dependent metric");
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+        {
+#pragma unroll
+            for (int grp = 0; grp < 2; ++grp)
+            {
+                const int c0 = grp == 0 ? 1 : 5;
+                double val = 0;
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+                {
+                    val += Dm(1+3*r+c, t2, y)*DGv(base+c0+c, y)*(1.0/12.0);
+                    val += M(1+3*r+c, y)*Gv(base+c0+c, y);
+                }
+                out[c0+r] = val;
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+        {
+            const int c = s == 0 ? 0 : 4;
+            double val = 0;
+            val += Dm(0, t2, y)*DGv(base+c, y)*(1.0/12.0);
+            val += M(0, y)*Gv(base+c, y);
+            out[c] = val;
+        }
+    }
 };
 
-// 6a. Mapped products of both sides' physical point fluxes into the face
-// state slots, for the angular directions
-// (d = 1, 2): faces [lo_d, hi_d+1] x [lo, hi] grown by one in the radial
-// index, where the radial-area tensor product needs them. Radial faces
-// (d = 0) need them only at the face itself; k_o4_flux<0> evaluates them.
-template <int D>
-__global__ void __launch_bounds__(kB) k_o4_map(O4View v, double* __restrict__ S)
-{
-    const BoxDesc& b = blockBoxDesc(v);
-    int L[3], H[3], x[3];
-    for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]; H[d] = b.hi[d]; }
-    H[D] = b.hi[D]+1;
-    if (D != 0) { L[0] -= 1; H[0] += 1; }
-    if (!regionPoint(b, L, H, boxThread(v, b), x)) return;
-    const BoxIdx I(b);
-    double* const P = S + b.sOff;
-    const FaceOps<D> op(v, b, I, P);
-    double out[8];
-    op.mapped(F_PHL, x, out);
-#pragma unroll
-    for (int c = 0; c < 8; ++c) P[I(F_MPL+c, x[0], x[1], x[2])] = out[c];
-    op.mapped(F_PHR, x, out);
-#pragma unroll
-    for (int c = 0; c < 8; ++c) P[I(F_MPR+c, x[0], x[1], x[2])] = out[c];
-}
-
-// 6b. Mapped face flux on the flux faces [lo_d, hi_d+1] x [lo, hi]: the
+// 6. Mapped face flux on the flux faces [lo_d, hi_d+1] x [lo, hi]: the
 // radial-area tensor product of the mapped products, the pressure force
 // through the cofactor row, the averaged Roe dissipation, and
 // F = (F_L + F_R - D)/2.
@@ -911,13 +921,10 @@ __global__ void __launch_bounds__(kB) k_o4_flux(O4View v, double* __restrict__ S
     double Fsum[8];
     for (int side = 0; side < 2; ++side)
     {
-        const int base = side == 0 ? F_MPL : F_MPR;
-        const int y[3] = {x[0]+1, x[1], x[2]}, z[3] = {x[0]-1, x[1], x[2]};
         double Fs[8];
         if constexpr (D == 0)
         {
-            // radial faces need the mapped products only at the face itself:
-            // evaluate them here instead of in k_o4_map
+            // radial faces: the mapped products at the face itself
             double mp[8];
             op.mapped(side == 0 ? F_PHL : F_PHR, x, mp);
 #pragma unroll
@@ -930,14 +937,19 @@ __global__ void __launch_bounds__(kB) k_o4_flux(O4View v, double* __restrict__ S
         }
         else
         {
+            // angular faces: the mapped products at the face and the half
+            // radial difference of those at its radial neighbours
+            double mp[8], dmp[8];
+            op.mapped(side == 0 ? F_PHL : F_PHR, x, mp);
+            op.mappedRadialDiff(side == 0 ? F_PHL : F_PHR, x, dmp);
 #pragma unroll
-        for (int c = 0; c < 8; ++c)
-        {
-            double val = 0;
-            val += adjD1*(0.5*Fv(base+c, y) - 0.5*Fv(base+c, z))*(1.0/12.0);
-            val += adjX*Fv(base+c, x);
-            Fs[c] = val;
-        }
+            for (int c = 0; c < 8; ++c)
+            {
+                double val = 0;
+                val += adjD1*dmp[c]*(1.0/12.0);
+                val += adjX*mp[c];
+                Fs[c] = val;
+            }
         }
         // pressure force: p0 (face average) with p0Bar = pBar + meBar
         const int fp = side == 0 ? F_P0L : F_P0R;
@@ -1032,7 +1044,7 @@ __global__ void __launch_bounds__(kB) k_o4_flux(O4View v, double* __restrict__ S
         P[I(F_FLUX+8*D+c, x[0], x[1], x[2])] = 0.5*(Fsum[c]-Fd[c]);
 }
 
-// 6b. Seam flux reconciliation 1/2 (F_a + sign F_b) on both boxes' copies
+// 6a. Seam flux reconciliation 1/2 (F_a + sign F_b) on both boxes' copies
 // of each physical seam face, src/MHD_AMR.H:1484-1496.
 __global__ void __launch_bounds__(kB) k_o4_seams(O4View v, double* __restrict__ S)
 {
@@ -1236,7 +1248,6 @@ void directionPass(const O4View& v, dim3 grid, hipStream_t s, double* scratch)
     if (v.roe) hipLaunchKernelGGL((k_o4_faces<D, true, S_>), grid, dim3(kB), 0, s, v, scratch);
     else hipLaunchKernelGGL((k_o4_faces<D, false, double>), grid, dim3(kB), 0, s, v, scratch);
     if (!v.roe) hipLaunchKernelGGL(k_o4_rusq<D>, grid, dim3(kB), 0, s, v, scratch);
-    if (D != 0) hipLaunchKernelGGL(k_o4_map<D>, grid, dim3(kB), 0, s, v, scratch);
     if (v.roe) hipLaunchKernelGGL((k_o4_flux<D, true>), grid, dim3(kB), 0, s, v, scratch);
     else hipLaunchKernelGGL((k_o4_flux<D, false>), grid, dim3(kB), 0, s, v, scratch);
 }
