@@ -350,19 +350,19 @@ __global__ void __launch_bounds__(kB) k_b_avg(O4View v, double* __restrict__ S)
     P[I(F_PAVG, x[0], x[1], x[2])] = avg(F_PTOT);
 }
 
-// 6. Mapped products of the averaged flux (bertaMapHostBox, :35-98): per
-// matrix column one term (transverse corrections, then the product), the terms
-// summed over columns. The radial correction is dropped for d = 1, 2, where the
-// angular metric is constant along the radius. Faces [lo_d, hi_d+1] x [lo, hi],
-// grown by one in the radial index when it is transverse.
+// 6. Mapped products of the averaged flux on the radial faces
+// (bertaMapHostBox, :35-98): per matrix column one term (transverse
+// corrections, then the product), the terms summed over columns. Faces
+// [lo_0, hi_0+1] x [lo, hi]. The angular directions evaluate theirs in
+// k_b_flux.
 template <int D>
 __global__ void __launch_bounds__(kB) k_b_map(O4View v, double* __restrict__ S)
 {
+    static_assert(D == 0, "angular mapped products are evaluated in k_b_flux");
     const BoxDesc& b = blockBoxDesc(v);
     int L[3], H[3], x[3];
     for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]; H[d] = b.hi[d]; }
     H[D] = b.hi[D]+1;
-    if (D != 0) { L[0] -= 1; H[0] += 1; }
     if (!regionPoint(b, L, H, boxThread(v, b), x)) return;
     const BoxIdx I(b);
     double* const P = S + b.sOff;
@@ -421,19 +421,35 @@ __global__ void __launch_bounds__(kB) k_b_flux(O4View v, double* __restrict__ S)
     auto Fv = [&](int f, const int y[3]) { return op.Fv(f, y); };
     const double adjX = areaOf(v, b, D, x[0]);
     auto dAdj = [&](int i) { return 0.5*areaOf(v, b, D, i+1) - 0.5*areaOf(v, b, D, i-1); };
-    const int rp[3] = {x[0]+1, x[1], x[2]}, rm[3] = {x[0]-1, x[1], x[2]};
     double Fsum[8];
-#pragma unroll
-    for (int c = 0; c < 8; ++c)
+    if constexpr (D == 0)
     {
-        double val = 0;
-        if (D != 0)
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
         {
-            const double dv = 0.5*Fv(F_BMP+c, rp) - 0.5*Fv(F_BMP+c, rm);
-            val += dv*dAdj(x[0])*(1.0/12.0);
+            double val = 0;
+            val += Fv(F_BMP+c, x)*adjX*1.0;
+            Fsum[c] = val;
         }
-        val += Fv(F_BMP+c, x)*adjX*1.0;
-        Fsum[c] = val;
+    }
+    else
+    {
+        // angular faces: the mapped products of the averaged flux at the face
+        // and of its half radial difference (the angular metric is constant
+        // along the radius, so the latter is the radial difference of the
+        // mapped products)
+        double mp[8], dmp[8];
+        op.mapped(F_PFA, x, mp);
+        op.mappedRadialDiff(F_PFA, x, dmp);
+        const double dA = dAdj(x[0]);
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+        {
+            double val = 0;
+            val += dmp[c]*dA*(1.0/12.0);
+            val += mp[c]*adjX*1.0;
+            Fsum[c] = val;
+        }
     }
     // <p_tot adjDr> at y: radial tensor product of the averaged total pressure
     auto tmp3 = [&](const int y[3]) {
@@ -532,7 +548,7 @@ void bertaDirectionPass(const O4View& v, dim3 grid, hipStream_t s, double* scrat
     else hipLaunchKernelGGL((k_b_faces<D, S_, false>), grid, dim3(kB), 0, s, v, scratch);
     hipLaunchKernelGGL(k_b_avg<D>, grid, dim3(kB), 0, s, v, scratch);
     if (v.roe) hipLaunchKernelGGL((k_o4_roe<D, S_>), grid, dim3(kB), 0, s, v, scratch);
-    hipLaunchKernelGGL(k_b_map<D>, grid, dim3(kB), 0, s, v, scratch);
+    if constexpr (D == 0) hipLaunchKernelGGL(k_b_map<D>, grid, dim3(kB), 0, s, v, scratch);
     if (v.roe) hipLaunchKernelGGL((k_b_flux<D, true>), grid, dim3(kB), 0, s, v, scratch);
     else hipLaunchKernelGGL((k_b_flux<D, false>), grid, dim3(kB), 0, s, v, scratch);
 }
