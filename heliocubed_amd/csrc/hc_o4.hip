@@ -37,6 +37,9 @@ enum Field : int
     F_WB = 0, F_MEB = 8, F_W = 9, F_ETA = 17, F_PTL = 18, F_PTR = 26, F_P0L = 34, F_P0R = 35,
     F_MEL = 36, F_MER = 37, F_PHL = 38, F_PHR = 46, F_DPT = 54, F_FLUX = 62,
     F_FB = 86, F_MEF = 94, F_COUNT = 95,
+    // the standard operator stores the sums of both sides' point fluxes
+    // (F_PHL + F_PHR) and face-average total pressures (F_P0L + F_P0R)
+    F_PHS = F_PHL, F_P0S = F_P0L,
     // Rusanov (riemann_solver 1): the speed of a face, and the conserved point
     // values of both sides (the dissipation and face-average flux slots)
     F_SPD = F_MER, F_QL = F_DPT, F_QR = F_FB,
@@ -534,8 +537,9 @@ __device__ __forceinline__ void pointFaceRegion(const BoxDesc& b, int grow, int 
 }
 
 // 5. Limited, flattened and repaired face states; point states by transverse
-// deconvolution; their physical fluxes and face-average total pressures, on
-// the point-flux faces (pointFaceRegion). Runs after k_o4_bar.
+// deconvolution; the sums of both sides' physical fluxes and face-average
+// total pressures, on the point-flux faces (pointFaceRegion). Runs after
+// k_o4_bar.
 template <int D, bool ROE, typename S_>
 __global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(2))) k_o4_faces(O4View v, double* __restrict__ S)
 {
@@ -650,6 +654,7 @@ __global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(2))) k_
         }
     }
     const double* A = v.faceA[D] + 9*angIdx(b, x[1], x[2]);
+    double phs[8], p0s = 0;
 #pragma unroll 1
     for (int side = 0; side < 2; ++side)
     {
@@ -659,10 +664,14 @@ __global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(2))) k_
         double ke, me, vb, f[8];
         energiesNative(A, w, ke, me, vb);
         primFlux(f, w, ke, me, vb, v.gamma, D);
+        const double p0 = (side == 0 ? lf[4] : rf[4]) + (me + lapMe/24.0);
 #pragma unroll
-        for (int q = 0; q < 8; ++q) P[I((side == 0 ? F_PHL : F_PHR)+q, x[0], x[1], x[2])] = f[q]+lapF[q]/24.0;
-        P[I(side == 0 ? F_P0L : F_P0R, x[0], x[1], x[2])] = (side == 0 ? lf[4] : rf[4]) + (me + lapMe/24.0);
+        for (int q = 0; q < 8; ++q) phs[q] = side == 0 ? f[q]+lapF[q]/24.0 : phs[q] + (f[q]+lapF[q]/24.0);
+        p0s = side == 0 ? p0 : p0s + p0;
     }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) P[I(F_PHS+q, x[0], x[1], x[2])] = phs[q];
+    P[I(F_P0S, x[0], x[1], x[2])] = p0s;
     if constexpr (ROE)
     {
         // point Roe dissipation of the two point states (the flux kernel
@@ -918,54 +927,53 @@ __global__ void __launch_bounds__(kB) k_o4_flux(O4View v, double* __restrict__ S
         p0bD[t] = 0.5*pa - 0.5*pz;
         radPD[t] = 0.5*(pa*areaOf(v, b, D, y[0])) - 0.5*(pz*areaOf(v, b, D, z[0]));
     }
+    // Both sides at once: the mapped products and the pressure force are linear
+    // in the point flux and the face-average total pressure, so they act on the
+    // sums PHL + PHR and P0L + P0R that k_o4_faces stores; the terms of the
+    // face-average total pressure p0Bar, common to both sides, count twice.
     double Fsum[8];
-    for (int side = 0; side < 2; ++side)
+    if constexpr (D == 0)
     {
-        double Fs[8];
-        if constexpr (D == 0)
-        {
-            // radial faces: the mapped products at the face itself
-            double mp[8];
-            op.mapped(side == 0 ? F_PHL : F_PHR, x, mp);
+        // radial faces: the mapped products at the face itself
+        double mp[8];
+        op.mapped(F_PHS, x, mp);
 #pragma unroll
-            for (int c = 0; c < 8; ++c)
-            {
-                double val = 0;
-                val += adjX*mp[c];
-                Fs[c] = val;
-            }
-        }
-        else
+        for (int c = 0; c < 8; ++c)
         {
-            // angular faces: the mapped products at the face and the half
-            // radial difference of those at its radial neighbours
-            double mp[8], dmp[8];
-            op.mapped(side == 0 ? F_PHL : F_PHR, x, mp);
-            op.mappedRadialDiff(side == 0 ? F_PHL : F_PHR, x, dmp);
-#pragma unroll
-            for (int c = 0; c < 8; ++c)
-            {
-                double val = 0;
-                val += adjD1*dmp[c]*(1.0/12.0);
-                val += adjX*mp[c];
-                Fs[c] = val;
-            }
+            double val = 0;
+            val += adjX*mp[c];
+            Fsum[c] = val;
         }
+    }
+    else
+    {
+        // angular faces: the mapped products at the face and the half radial
+        // difference of those at its radial neighbours
+        double mp[8], dmp[8];
+        op.mapped(F_PHS, x, mp);
+        op.mappedRadialDiff(F_PHS, x, dmp);
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+        {
+            double val = 0;
+            val += adjD1*dmp[c]*(1.0/12.0);
+            val += adjX*mp[c];
+            Fsum[c] = val;
+        }
+    }
+    {
         // pressure force: p0 (face average) with p0Bar = pBar + meBar
-        const int fp = side == 0 ? F_P0L : F_P0R;
         double tmp3 = 0;
-        if (D != 0) tmp3 += adjD1*p0bD[0]*(1.0/12.0);
-        tmp3 += adjX*Fv(fp, x);
+        if (D != 0) tmp3 += 2.0*(adjD1*p0bD[0]*(1.0/12.0));
+        tmp3 += adjX*Fv(F_P0S, x);
         for (int r = 0; r < 3; ++r)
         {
             double pf = 0;
-            pf += radPD[0]*Dm(10+r, t1, x)*(1.0/12.0);
-            pf += radPD[1]*Dm(10+r, t2, x)*(1.0/12.0);
+            pf += 2.0*(radPD[0]*Dm(10+r, t1, x)*(1.0/12.0));
+            pf += 2.0*(radPD[1]*Dm(10+r, t2, x)*(1.0/12.0));
             pf += tmp3*M(10+r, x);
-            Fs[1+r] += pf;
+            Fsum[1+r] += pf;
         }
-#pragma unroll
-        for (int c = 0; c < 8; ++c) Fsum[c] = side == 0 ? Fs[c] : Fsum[c] + Fs[c];
     }
     double Fd[8];
     if constexpr (!ROE)
