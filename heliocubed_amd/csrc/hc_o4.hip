@@ -250,15 +250,19 @@ __device__ double etaTilde(const double* pb2, const double* pb, const double* pa
                            double vb1, double va1)
 {
     const double z1 = 0.85, z0 = 0.75, delta = 0.33;
-    double si = 0;
+    // (lhs1 > 0 && lhs2 > delta) ? si : 1, with the divisions of lhs2 and si
+    // evaluated only where they decide the result
+    const double lhs1 = vb1-va1;
+    if (!(lhs1 > 0.0)) return 1.0;
     const double p_a = pa[0], p_b = pb[0], p_a2 = pa2[0], p_b2 = pb2[0];
+    const double lhs2 = fabs(p_a-p_b)/fmin(p_a, p_b);
+    if (!(lhs2 > delta)) return 1.0;
+    double si = 0;
     const double arg = fabs(p_a-p_b)/fabs(p_a2-p_b2);
     if (arg > z1) si = 0.0;
     if (arg < z0) si = 1.0;
     if ((arg <= z1) && (arg >= z0)) si = 1.0-(arg-z0)/(z1-z0);
-    const double lhs1 = vb1-va1;
-    const double lhs2 = fabs(p_a-p_b)/fmin(p_a, p_b);
-    return (lhs1 > 0.0 && lhs2 > delta) ? si : 1.0;
+    return si;
 }
 
 // Mapped and factored Roe dissipation at a face point, src/MHD_Roe8Wave.H:634-740.
@@ -493,23 +497,26 @@ __global__ void __launch_bounds__(kB) k_o4_eta(O4View v, double* __restrict__ S)
     const BoxIdx I(b);
     double* const P = S + b.sOff;
     double e = 0;
+#pragma unroll
     for (int dd = 0; dd < 3; ++dd)
     {
+        // total pressure and velocity on the line x + q e_dd, q = -3..3; the
+        // five-point window of the cell x + oo e_dd starts at index oo+1
+        double pr[7], vel[7];
+#pragma unroll
+        for (int q = -3; q <= 3; ++q)
+        {
+            int z[3] = {x[0], x[1], x[2]};
+            z[dd] += q;
+            pr[q+3] = P[I(F_WB+4, z[0], z[1], z[2])] + P[I(F_MEB, z[0], z[1], z[2])];
+            vel[q+3] = P[I(F_WB+1+dd, z[0], z[1], z[2])];
+        }
         double ed = 0;
+#pragma unroll
         for (int o = 0; o < 3; ++o)
         {
-            const int oo = o == 0 ? 0 : (o == 1 ? 1 : -1);
-            int y[3] = {x[0], x[1], x[2]};
-            y[dd] += oo;
-            double pr[5], vel[5];
-            for (int q = -2; q <= 2; ++q)
-            {
-                int z[3] = {y[0], y[1], y[2]};
-                z[dd] += q;
-                pr[q+2] = P[I(F_WB+4, z[0], z[1], z[2])] + P[I(F_MEB, z[0], z[1], z[2])];
-                vel[q+2] = P[I(F_WB+1+dd, z[0], z[1], z[2])];
-            }
-            const double et = etaTilde(&pr[0], &pr[1], &pr[3], &pr[4], vel[1], vel[3]);
+            const int s0 = (o == 0 ? 0 : (o == 1 ? 1 : -1)) + 1;
+            const double et = etaTilde(&pr[s0], &pr[s0+1], &pr[s0+3], &pr[s0+4], vel[s0+1], vel[s0+3]);
             ed = o == 0 ? et : fmin(ed, et);
         }
         e = dd == 0 ? ed : fmin(ed, e);
