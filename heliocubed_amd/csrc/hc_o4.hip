@@ -1333,49 +1333,55 @@ __device__ __forceinline__ double deriv3(F q, int x, int lo, int hi)
     return 0.5*(q(x+1)-q(x-1));
 }
 
-// Angular quotient with the fourth-order correction, valid cells.
+// Angular quotient with the fourth-order correction, valid cells. One thread
+// per cell: the metric differences and 1/det are shared by its 8 components.
 __global__ void __launch_bounds__(kB) k4_angular(Prep4View v, const double* __restrict__ U0, const double* __restrict__ K,
                                                  double a_dt, double* __restrict__ ang)
 {
-    const long long n = 6LL*8*v.na*v.na*v.nr;
-    const long long t = (long long)blockIdx.x*blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    const int i = t % v.nr;
-    long long r = t / v.nr;
-    const int j = r % v.na; r /= v.na;
-    const int k = r % v.na; r /= v.na;
-    const int c = r % 8;
-    const int p = r / 8;
-    const long long base = p*v.vps + c*v.vcs;
-    auto ju = [&](int jj, int kk) {
-        const long long x = base + ((long long)kk*v.na + jj)*v.nr + i;
-        return K ? U0[x] + a_dt*K[x] : U0[x]; };
-    auto q1 = [&](int jj) { return ju(jj, k)/det4(v, jj, k); };
-    auto q2 = [&](int kk) { return ju(j, kk)/det4(v, j, kk); };
-    double corr = 0;
-    corr += deriv3(q1, j, 0, v.na-1)*(0.5*(det4(v, j+1, k)-det4(v, j-1, k)));
-    corr += deriv3(q2, k, 0, v.na-1)*(0.5*(det4(v, j, k+1)-det4(v, j, k-1)));
-    ang[base + ((long long)k*v.na + j)*v.nr + i] = (ju(j, k)-corr/12.0)*(1.0/det4(v, j, k));
+    const int nr = v.nr, na = v.na;
+    const int t = blockIdx.x*blockDim.x + threadIdx.x;
+    if (t >= nr*na) return;
+    const int i = t % nr, j = t / nr;
+    const int k = blockIdx.y % na, p = blockIdx.y / na;
+    const double djd = 0.5*(det4(v, j+1, k)-det4(v, j-1, k));
+    const double dkd = 0.5*(det4(v, j, k+1)-det4(v, j, k-1));
+    const double idet = 1.0/det4(v, j, k);
+#pragma unroll 2
+    for (int c = 0; c < 8; ++c)
+    {
+        const long long base = p*v.vps + c*v.vcs;
+        auto ju = [&](int jj, int kk) {
+            const long long x = base + ((long long)kk*na + jj)*nr + i;
+            return K ? U0[x] + a_dt*K[x] : U0[x]; };
+        auto q1 = [&](int jj) { return ju(jj, k)/det4(v, jj, k); };
+        auto q2 = [&](int kk) { return ju(j, kk)/det4(v, j, kk); };
+        double corr = 0;
+        corr += deriv3(q1, j, 0, na-1)*djd;
+        corr += deriv3(q2, k, 0, na-1)*dkd;
+        ang[base + ((long long)k*na + j)*nr + i] = (ju(j, k)-corr/12.0)*idet;
+    }
 }
 
-// Radial quotient with its correction, into the Cartesian buffer.
+// Radial quotient with its correction, into the Cartesian buffer. One thread
+// per cell: the volume difference and 1/dv are shared by its 8 components.
 __global__ void __launch_bounds__(kB) k4_radial(Prep4View v, const double* __restrict__ ang, double* __restrict__ cart)
 {
-    const long long n = 6LL*8*v.na*v.na*v.nr;
-    const long long t = (long long)blockIdx.x*blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    const int i = t % v.nr;
-    long long r = t / v.nr;
-    const int j = r % v.na; r /= v.na;
-    const int k = r % v.na; r /= v.na;
-    const int c = r % 8;
-    const int p = r / 8;
-    const double* a = ang + p*v.vps + c*v.vcs + ((long long)k*v.na + j)*v.nr;
+    const int nr = v.nr, na = v.na;
+    const int t = blockIdx.x*blockDim.x + threadIdx.x;
+    if (t >= nr*na) return;
+    const int i = t % nr, j = t / nr;
+    const int k = blockIdx.y % na, p = blockIdx.y / na;
     auto dv = [&](int ii) { return v.dvol[ii+6]; };
-    auto rq = [&](int ii) { return a[ii]/dv(ii); };
-    const double dq = deriv3(rq, i, 0, v.nr-1);
     const double dd = 0.5*(dv(i+1)-dv(i-1));
-    cart[p*v.cps + c*v.ccs + ((long long)(k+kGC)*v.cJ + (j+kGC))*v.nr + i] = (a[i]-dq*dd/12.0)*(1.0/dv(i));
+    const double idv = 1.0/dv(i);
+#pragma unroll 2
+    for (int c = 0; c < 8; ++c)
+    {
+        const double* a = ang + p*v.vps + c*v.vcs + ((long long)k*na + j)*nr;
+        auto rq = [&](int ii) { return a[ii]/dv(ii); };
+        const double dq = deriv3(rq, i, 0, nr-1);
+        cart[p*v.cps + c*v.ccs + ((long long)(k+kGC)*v.cJ + (j+kGC))*nr + i] = (a[i]-dq*dd/12.0)*idv;
+    }
 }
 
 // Seam ghost columns of the Cartesian averages: weighted sums of donor
@@ -1458,24 +1464,27 @@ __global__ void __launch_bounds__(kB) k4_native(Prep4View v, const double* __res
 
 // Radial boundary ghosts of USph: inner fixed native state, outer copy of the
 // last layer scaled by (r_last/r)^2 (r_last/r for the angular B components).
+// One thread per ghost cell (5 inner and 5 outer layers of every column).
 __global__ void __launch_bounds__(kB) k4_radial_bc(Prep4View v, const double* __restrict__ inner, double* __restrict__ US)
 {
     const int NA = v.na+10;
     const int t = blockIdx.x*blockDim.x + threadIdx.x;
-    if (t >= 6*NA*NA) return;
-    const int jj = t % NA, kk = (t / NA) % NA, p = t / (NA*NA);
+    if (t >= 10*6*NA*NA) return;
+    const int layer = t % 10, col = t / 10;
+    const int jj = col % NA, kk = (col / NA) % NA, p = col / (NA*NA);
     auto us = [&](int c, int i) -> double& {
         return US[p*v.sps + c*v.scs + ((long long)kk*v.sJ + jj)*v.sI + (i+5)]; };
-    for (int ig = 0; ig < 5; ++ig)
-        for (int c = 0; c < 8; ++c)
-            us(c, -5+ig) = inner[((((long long)p*8+c)*NA+kk)*NA+jj)*5+ig];
-    const double rlast = v.rmid[v.nr-1+6];
-    for (int i = v.nr; i < v.nr+5; ++i)
+    if (layer < 5)
     {
-        const double ratio = rlast/v.rmid[i+6];
-        for (int c = 0; c < 6; ++c) us(c, i) = us(c, v.nr-1)*(ratio*ratio);
-        for (int c = 6; c < 8; ++c) us(c, i) = us(c, v.nr-1)*ratio;
+        for (int c = 0; c < 8; ++c)
+            us(c, -5+layer) = inner[((((long long)p*8+c)*NA+kk)*NA+jj)*5+layer];
+        return;
     }
+    const int i = v.nr + (layer-5);
+    const double rlast = v.rmid[v.nr-1+6];
+    const double ratio = rlast/v.rmid[i+6];
+    for (int c = 0; c < 6; ++c) us(c, i) = us(c, v.nr-1)*(ratio*ratio);
+    for (int c = 6; c < 8; ++c) us(c, i) = us(c, v.nr-1)*ratio;
 }
 } // namespace
 
@@ -1708,14 +1717,15 @@ void level4Prepare(Level4& L, const double* U0, const double* K, double a_dt, co
                    hipStream_t s)
 {
     const Prep4View& v = L.pv;
-    const long long nv = 6LL*8*v.na*v.na*v.nr;
-    hipLaunchKernelGGL(k4_angular, dim3((nv+kB-1)/kB), dim3(kB), 0, s, v, U0, K, a_dt, L.ang);
-    hipLaunchKernelGGL(k4_radial, dim3((nv+kB-1)/kB), dim3(kB), 0, s, v, L.ang, L.cart);
+    // (i, j) of a cell in x, (k, panel) in y
+    const dim3 cellGrid((v.nr*v.na+kB-1)/kB, 6*v.na);
+    hipLaunchKernelGGL(k4_angular, cellGrid, dim3(kB), 0, s, v, U0, K, a_dt, L.ang);
+    hipLaunchKernelGGL(k4_radial, cellGrid, dim3(kB), 0, s, v, L.ang, L.cart);
     hipLaunchKernelGGL(k4_seam, dim3(((long long)L.ncol*v.nr+kB-1)/kB), dim3(kB), 0, s,
                        v, L.ncol, L.colDst, L.colStart, L.donorSrc, L.donorW, L.cart);
     const long long nn = 6LL*(v.na+10)*(v.na+10)*v.nr;
     hipLaunchKernelGGL(k4_native, dim3((nn+kB-1)/kB), dim3(kB), 0, s, v, L.cart, US);
-    hipLaunchKernelGGL(k4_radial_bc, dim3((6*(v.na+10)*(v.na+10)+kB-1)/kB), dim3(kB), 0, s, v, inner, US);
+    hipLaunchKernelGGL(k4_radial_bc, dim3((10*6*(v.na+10)*(v.na+10)+kB-1)/kB), dim3(kB), 0, s, v, inner, US);
     HC_THROW(hipGetLastError());
 }
 
