@@ -1333,54 +1333,57 @@ __device__ __forceinline__ double deriv3(F q, int x, int lo, int hi)
     return 0.5*(q(x+1)-q(x-1));
 }
 
-// Angular quotient with the fourth-order correction, valid cells. One thread
-// per cell: the metric differences and 1/det are shared by its 8 components.
+// Angular quotient with the fourth-order correction, valid cells. One block
+// per angular column (j = blockIdx.x, k and the panel in blockIdx.y), threads
+// along the radius: the metric terms are uniform over the block and shared by
+// the 8 components.
 __global__ void __launch_bounds__(kB) k4_angular(Prep4View v, const double* __restrict__ U0, const double* __restrict__ K,
                                                  double a_dt, double* __restrict__ ang)
 {
     const int nr = v.nr, na = v.na;
-    const int t = blockIdx.x*blockDim.x + threadIdx.x;
-    if (t >= nr*na) return;
-    const int i = t % nr, j = t / nr;
-    const int k = blockIdx.y % na, p = blockIdx.y / na;
+    const int j = blockIdx.x, k = blockIdx.y % na, p = blockIdx.y / na;
     const double djd = 0.5*(det4(v, j+1, k)-det4(v, j-1, k));
     const double dkd = 0.5*(det4(v, j, k+1)-det4(v, j, k-1));
     const double idet = 1.0/det4(v, j, k);
-#pragma unroll 2
-    for (int c = 0; c < 8; ++c)
+    for (int i = threadIdx.x; i < nr; i += blockDim.x)
     {
-        const long long base = p*v.vps + c*v.vcs;
-        auto ju = [&](int jj, int kk) {
-            const long long x = base + ((long long)kk*na + jj)*nr + i;
-            return K ? U0[x] + a_dt*K[x] : U0[x]; };
-        auto q1 = [&](int jj) { return ju(jj, k)/det4(v, jj, k); };
-        auto q2 = [&](int kk) { return ju(j, kk)/det4(v, j, kk); };
-        double corr = 0;
-        corr += deriv3(q1, j, 0, na-1)*djd;
-        corr += deriv3(q2, k, 0, na-1)*dkd;
-        ang[base + ((long long)k*na + j)*nr + i] = (ju(j, k)-corr/12.0)*idet;
+#pragma unroll 2
+        for (int c = 0; c < 8; ++c)
+        {
+            const long long base = p*v.vps + c*v.vcs;
+            auto ju = [&](int jj, int kk) {
+                const long long x = base + ((long long)kk*na + jj)*nr + i;
+                return K ? U0[x] + a_dt*K[x] : U0[x]; };
+            auto q1 = [&](int jj) { return ju(jj, k)/det4(v, jj, k); };
+            auto q2 = [&](int kk) { return ju(j, kk)/det4(v, j, kk); };
+            double corr = 0;
+            corr += deriv3(q1, j, 0, na-1)*djd;
+            corr += deriv3(q2, k, 0, na-1)*dkd;
+            ang[base + ((long long)k*na + j)*nr + i] = (ju(j, k)-corr/12.0)*idet;
+        }
     }
 }
 
-// Radial quotient with its correction, into the Cartesian buffer. One thread
-// per cell: the volume difference and 1/dv are shared by its 8 components.
+// Radial quotient with its correction, into the Cartesian buffer. One block
+// per angular column, threads along the radius; the volume difference and 1/dv
+// are shared by the 8 components.
 __global__ void __launch_bounds__(kB) k4_radial(Prep4View v, const double* __restrict__ ang, double* __restrict__ cart)
 {
     const int nr = v.nr, na = v.na;
-    const int t = blockIdx.x*blockDim.x + threadIdx.x;
-    if (t >= nr*na) return;
-    const int i = t % nr, j = t / nr;
-    const int k = blockIdx.y % na, p = blockIdx.y / na;
+    const int j = blockIdx.x, k = blockIdx.y % na, p = blockIdx.y / na;
     auto dv = [&](int ii) { return v.dvol[ii+6]; };
-    const double dd = 0.5*(dv(i+1)-dv(i-1));
-    const double idv = 1.0/dv(i);
-#pragma unroll 2
-    for (int c = 0; c < 8; ++c)
+    for (int i = threadIdx.x; i < nr; i += blockDim.x)
     {
-        const double* a = ang + p*v.vps + c*v.vcs + ((long long)k*na + j)*nr;
-        auto rq = [&](int ii) { return a[ii]/dv(ii); };
-        const double dq = deriv3(rq, i, 0, nr-1);
-        cart[p*v.cps + c*v.ccs + ((long long)(k+kGC)*v.cJ + (j+kGC))*nr + i] = (a[i]-dq*dd/12.0)*idv;
+        const double dd = 0.5*(dv(i+1)-dv(i-1));
+        const double idv = 1.0/dv(i);
+#pragma unroll 2
+        for (int c = 0; c < 8; ++c)
+        {
+            const double* a = ang + p*v.vps + c*v.vcs + ((long long)k*na + j)*nr;
+            auto rq = [&](int ii) { return a[ii]/dv(ii); };
+            const double dq = deriv3(rq, i, 0, nr-1);
+            cart[p*v.cps + c*v.ccs + ((long long)(k+kGC)*v.cJ + (j+kGC))*nr + i] = (a[i]-dq*dd/12.0)*idv;
+        }
     }
 }
 
@@ -1410,54 +1413,52 @@ __global__ void __launch_bounds__(kB) k4_seam(Prep4View v, int ncol, const long 
 }
 
 // Native components by the fourth-order matrix quotient on the USph cells
-// i in [0, nr), j,k in [-5, na+5).
+// i in [0, nr), j,k in [-5, na+5). One block per angular column (j+5 =
+// blockIdx.x, k and the panel in blockIdx.y), threads along the radius: the
+// matrix averages and inverses are uniform over the block.
 __global__ void __launch_bounds__(kB) k4_native(Prep4View v, const double* __restrict__ cart, double* __restrict__ US)
 {
     const int NA = v.na+10;
-    const long long n = 6LL*NA*NA*v.nr;
-    const long long t = (long long)blockIdx.x*blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    const int i = t % v.nr;
-    long long r = t / v.nr;
-    const int j = (int)(r % NA) - 5; r /= NA;
-    const int k = (int)(r % NA) - 5;
-    const int p = r / NA;
-    auto ca = [&](int c, int jj, int kk) {
-        return cart[p*v.cps + c*v.ccs + ((long long)(kk+kGC)*v.cJ + (jj+kGC))*v.nr + i]; };
-    auto us = [&](int c) -> double& {
-        return US[p*v.sps + c*v.scs + ((long long)(k+5)*v.sJ + (j+5))*v.sI + (i+5)]; };
-    us(0) = ca(0, j, k);
-    us(4) = ca(4, j, k);
-    for (int grp = 0; grp < 2; ++grp)
+    const int j = (int)blockIdx.x - 5, k = (int)(blockIdx.y % NA) - 5, p = blockIdx.y / NA;
+    for (int i = threadIdx.x; i < v.nr; i += blockDim.x)
     {
-        const int st = grp == 0 ? 1 : 5;
-        auto q0 = [&](int jj, int kk, int row) {
-            const double* Mi = Ainv4(v, p, jj, kk);
-            double val = 0;
-            for (int cc = 0; cc < 3; ++cc) val += Mi[3*row+cc]*ca(st+cc, jj, kk);
-            return val; };
-        double rhs[3];
-        for (int row = 0; row < 3; ++row)
+        auto ca = [&](int c, int jj, int kk) {
+            return cart[p*v.cps + c*v.ccs + ((long long)(kk+kGC)*v.cJ + (jj+kGC))*v.nr + i]; };
+        auto us = [&](int c) -> double& {
+            return US[p*v.sps + c*v.scs + ((long long)(k+5)*v.sJ + (j+5))*v.sI + (i+5)]; };
+        us(0) = ca(0, j, k);
+        us(4) = ca(4, j, k);
+        for (int grp = 0; grp < 2; ++grp)
         {
-            double corr = 0;
-            for (int d = 1; d < 3; ++d)
+            const int st = grp == 0 ? 1 : 5;
+            auto q0 = [&](int jj, int kk, int row) {
+                const double* Mi = Ainv4(v, p, jj, kk);
+                double val = 0;
+                for (int cc = 0; cc < 3; ++cc) val += Mi[3*row+cc]*ca(st+cc, jj, kk);
+                return val; };
+            double rhs[3];
+            for (int row = 0; row < 3; ++row)
             {
-                const int dj = d == 1, dk = d == 2;
-                for (int cc = 0; cc < 3; ++cc)
+                double corr = 0;
+                for (int d = 1; d < 3; ++d)
                 {
-                    const double dA = 0.5*(Aavg4(v, p, j+dj, k+dk)[3*row+cc]-Aavg4(v, p, j-dj, k-dk)[3*row+cc]);
-                    const double dq = 0.5*(q0(j+dj, k+dk, cc)-q0(j-dj, k-dk, cc));
-                    corr += dA*dq;
+                    const int dj = d == 1, dk = d == 2;
+                    for (int cc = 0; cc < 3; ++cc)
+                    {
+                        const double dA = 0.5*(Aavg4(v, p, j+dj, k+dk)[3*row+cc]-Aavg4(v, p, j-dj, k-dk)[3*row+cc]);
+                        const double dq = 0.5*(q0(j+dj, k+dk, cc)-q0(j-dj, k-dk, cc));
+                        corr += dA*dq;
+                    }
                 }
+                rhs[row] = ca(st+row, j, k)-corr/12.0;
             }
-            rhs[row] = ca(st+row, j, k)-corr/12.0;
-        }
-        const double* Mi = Ainv4(v, p, j, k);
-        for (int row = 0; row < 3; ++row)
-        {
-            double val = 0;
-            for (int cc = 0; cc < 3; ++cc) val += Mi[3*row+cc]*rhs[cc];
-            us(st+row) = val;
+            const double* Mi = Ainv4(v, p, j, k);
+            for (int row = 0; row < 3; ++row)
+            {
+                double val = 0;
+                for (int cc = 0; cc < 3; ++cc) val += Mi[3*row+cc]*rhs[cc];
+                us(st+row) = val;
+            }
         }
     }
 }
@@ -1717,14 +1718,14 @@ void level4Prepare(Level4& L, const double* U0, const double* K, double a_dt, co
                    hipStream_t s)
 {
     const Prep4View& v = L.pv;
-    // (i, j) of a cell in x, (k, panel) in y
-    const dim3 cellGrid((v.nr*v.na+kB-1)/kB, 6*v.na);
-    hipLaunchKernelGGL(k4_angular, cellGrid, dim3(kB), 0, s, v, U0, K, a_dt, L.ang);
-    hipLaunchKernelGGL(k4_radial, cellGrid, dim3(kB), 0, s, v, L.ang, L.cart);
+    // one block per angular column (j in x; k and the panel in y), threads
+    // along the radius in multiples of the wavefront, at most kB
+    const dim3 colBlock(std::min(kB, (v.nr+63)/64*64));
+    hipLaunchKernelGGL(k4_angular, dim3(v.na, 6*v.na), colBlock, 0, s, v, U0, K, a_dt, L.ang);
+    hipLaunchKernelGGL(k4_radial, dim3(v.na, 6*v.na), colBlock, 0, s, v, L.ang, L.cart);
     hipLaunchKernelGGL(k4_seam, dim3(((long long)L.ncol*v.nr+kB-1)/kB), dim3(kB), 0, s,
                        v, L.ncol, L.colDst, L.colStart, L.donorSrc, L.donorW, L.cart);
-    const long long nn = 6LL*(v.na+10)*(v.na+10)*v.nr;
-    hipLaunchKernelGGL(k4_native, dim3((nn+kB-1)/kB), dim3(kB), 0, s, v, L.cart, US);
+    hipLaunchKernelGGL(k4_native, dim3(v.na+10, 6*(v.na+10)), colBlock, 0, s, v, L.cart, US);
     hipLaunchKernelGGL(k4_radial_bc, dim3((10*6*(v.na+10)*(v.na+10)+kB-1)/kB), dim3(kB), 0, s, v, inner, US);
     HC_THROW(hipGetLastError());
 }
