@@ -1388,28 +1388,30 @@ __global__ void __launch_bounds__(kB) k4_radial(Prep4View v, const double* __res
 }
 
 // Seam ghost columns of the Cartesian averages: weighted sums of donor
-// columns (valid cells of the panels the ghost cell overlaps).
-__global__ void __launch_bounds__(kB) k4_seam(Prep4View v, int ncol, const long long* __restrict__ dst,
+// columns (valid cells of the panels the ghost cell overlaps). One block per
+// ghost column (blockIdx.x), threads along the radius: the donor table entries
+// are uniform over the block.
+__global__ void __launch_bounds__(kB) k4_seam(Prep4View v, const long long* __restrict__ dst,
                                               const int* __restrict__ start, const long long* __restrict__ src,
                                               const double* __restrict__ w, double* __restrict__ cart)
 {
-    const long long t = (long long)blockIdx.x*blockDim.x + threadIdx.x;
-    if (t >= (long long)ncol*v.nr) return;
-    const int i = t % v.nr;
-    const int col = t / v.nr;
+    const int col = blockIdx.x;
     const int d0 = start[col], d1 = start[col+1];
-    // donors outer, components inner: each donor's index and weight are read
-    // once; every component still sums its donors in table order
-    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int d = d0; d < d1; ++d)
+    for (int i = threadIdx.x; i < v.nr; i += blockDim.x)
     {
-        const double wd = w[d];
-        const double* sp = cart + src[d] + i;
+        // donors outer, components inner: each donor's index and weight are
+        // read once; every component still sums its donors in table order
+        double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (int d = d0; d < d1; ++d)
+        {
+            const double wd = w[d];
+            const double* sp = cart + src[d] + i;
 #pragma unroll
-        for (int c = 0; c < 8; ++c) acc[c] += wd*sp[c*v.ccs];
+            for (int c = 0; c < 8; ++c) acc[c] += wd*sp[c*v.ccs];
+        }
+#pragma unroll
+        for (int c = 0; c < 8; ++c) cart[dst[col] + c*v.ccs + i] = acc[c];
     }
-#pragma unroll
-    for (int c = 0; c < 8; ++c) cart[dst[col] + c*v.ccs + i] = acc[c];
 }
 
 // Native components by the fourth-order matrix quotient on the USph cells
@@ -1723,8 +1725,9 @@ void level4Prepare(Level4& L, const double* U0, const double* K, double a_dt, co
     const dim3 colBlock(std::min(kB, (v.nr+63)/64*64));
     hipLaunchKernelGGL(k4_angular, dim3(v.na, 6*v.na), colBlock, 0, s, v, U0, K, a_dt, L.ang);
     hipLaunchKernelGGL(k4_radial, dim3(v.na, 6*v.na), colBlock, 0, s, v, L.ang, L.cart);
-    hipLaunchKernelGGL(k4_seam, dim3(((long long)L.ncol*v.nr+kB-1)/kB), dim3(kB), 0, s,
-                       v, L.ncol, L.colDst, L.colStart, L.donorSrc, L.donorW, L.cart);
+    if (L.ncol > 0)
+        hipLaunchKernelGGL(k4_seam, dim3(L.ncol), colBlock, 0, s, v, L.colDst, L.colStart, L.donorSrc, L.donorW,
+                           L.cart);
     hipLaunchKernelGGL(k4_native, dim3(v.na+10, 6*(v.na+10)), colBlock, 0, s, v, L.cart, US);
     hipLaunchKernelGGL(k4_radial_bc, dim3((10*6*(v.na+10)*(v.na+10)+kB-1)/kB), dim3(kB), 0, s, v, inner, US);
     HC_THROW(hipGetLastError());
