@@ -125,9 +125,11 @@ __device__ __forceinline__ int blockBoxIndex(const O4View& v)
     return v.blocksPerBox > 0 ? (int)(L / (unsigned)v.blocksPerBox) : v.blockBox[L];
 }
 __device__ __forceinline__ const BoxDesc& blockBoxDesc(const O4View& v) { return v.boxes[blockBoxIndex(v)]; }
-__device__ __forceinline__ long long boxThread(const O4View& v, const BoxDesc& b)
+// (32-bit: a box's launch region, even a whole-panel tile, has far fewer than
+// 2^32 points)
+__device__ __forceinline__ unsigned boxThread(const O4View& v, const BoxDesc& b)
 {
-    return (long long)(logicalBlock(v) - b.block0)*blockDim.x + threadIdx.x;
+    return (logicalBlock(v) - (unsigned)b.block0)*blockDim.x + threadIdx.x;
 }
 
 // Offset (in doubles, 32-bit) of field f at cell (i,j,k) inside a box's
@@ -146,13 +148,14 @@ struct BoxIdx
 };
 
 // Region iteration: thread t of the box's region [lo-gl, hi+gh] (+ext along d).
-__device__ __forceinline__ bool regionPoint(const BoxDesc& b, const int L[3], const int H[3], long long t, int x[3])
+__device__ __forceinline__ bool regionPoint(const BoxDesc& b, const int L[3], const int H[3], unsigned t, int x[3])
 {
-    const int n0 = H[0]-L[0]+1, n1 = H[1]-L[1]+1, n2 = H[2]-L[2]+1;
-    if (t >= (long long)n0*n1*n2) return false;
-    x[0] = L[0] + (int)(t % n0);
-    x[1] = L[1] + (int)((t / n0) % n1);
-    x[2] = L[2] + (int)(t / ((long long)n0*n1));
+    const unsigned n0 = H[0]-L[0]+1, n1 = H[1]-L[1]+1, n2 = H[2]-L[2]+1;
+    if (t >= n0*n1*n2) return false;
+    const unsigned q = t / n0;
+    x[0] = L[0] + (int)(t - q*n0);
+    x[1] = L[1] + (int)(q % n1);
+    x[2] = L[2] + (int)(q / n1);
     return true;
 }
 
