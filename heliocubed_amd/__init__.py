@@ -46,6 +46,7 @@ class HcOptions(ctypes.Structure):
         ("spatial_operator", ctypes.c_int),
         ("berta_eta_d", ctypes.c_double),
         ("berta_fallback", ctypes.c_int),
+        ("divergence_cleaning", ctypes.c_int),
     ]
 
 
@@ -156,10 +157,12 @@ class Options:
     spatial_operator: int = 1  # fourth order: 1 standard, 2 Berta WENOZ
     berta_eta_d: float = 0.02  # Berta Jameson detector threshold
     berta_fallback: int = 0  # Berta fallback: 0 WENO3, 1 PLM
+    divergence_cleaning: int = 0  # 1: Powell source terms
 
     def c(self) -> HcOptions:
         return HcOptions(self.spatial_order, self.limiter, self.riemann_solver, self.entropy_fix, self.gamma,
-                         self.precision, self.spatial_operator, self.berta_eta_d, self.berta_fallback)
+                         self.precision, self.spatial_operator, self.berta_eta_d, self.berta_fallback,
+                         self.divergence_cleaning)
 
 
 def init_state(shell: Shell, options: Options, config: int = 0, seed: int = 20261203):

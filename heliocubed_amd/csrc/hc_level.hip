@@ -40,10 +40,12 @@ struct LevelView
     long long cs, ps;      // stage component / panel strides
     long long vcs, vps;    // valid component / panel strides
     int limiter, roe;
+    int powell;            // divergence_cleaning 1: Powell source terms
     double gamma, entropy;
     const double* radial;  // [nr+2g+1][4]: rmid, dvol, area0, areaT for i = -g ..
     const double* detA;    // [na][na]
     const double* frame[3];// [6][fk][fj][10]: 9 frame entries + |normal|
+    const double* normal[3];// [6][fk][fj][3]: integrated face normal (Powell source only)
     const double* cellA;   // [6][NK][NJ][9]
     const double* cellAi;  // [6][NK][NJ][9]
     const int* edge;       // [6][2][2][4]: q, dq, sq, rev
@@ -471,6 +473,99 @@ k_rhs_fused(LevelView v, const double* __restrict__ W, const double* __restrict_
     }
 }
 
+// 10. Powell source (divergence_cleaning 1), src/BoxOp_SecondOrder.H:317-369.
+// (a) Magnetic flux 1/2 (B_l + B_r).n area of every face of direction D, n the
+// integrated face normal, from the face states of the flux kernels: PLM in
+// precision S, and at a seam face the adjacent panel's state from PS on that
+// side. Each panel keeps its own value at a seam face (not reconciled).
+template <int D, typename S>
+__global__ void __launch_bounds__(kBlock)
+k_powell_bflux(LevelView v, const double* __restrict__ W, const double* __restrict__ PS, double* __restrict__ BF)
+{
+    const int fi = v.nr + (D == 0), fj = v.na + (D == 1), fk = v.na + (D == 2);
+    const long long n = 6LL*fi*fj*fk;
+    const long long t = (long long)blockIdx.x*blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int i = t % fi;
+    long long r = t / fi;
+    const int j = r % fj; r /= fj;
+    const int k = r % fk;
+    const int p = r / fk;
+    constexpr int e0 = D == 0, e1 = D == 1, e2 = D == 2;
+    double mm[8], m[8], c[8], pp[8], l[8], rr[8];
+    loadStage(v, W, p, i-2*e0, j-2*e1, k-2*e2, mm);
+    loadStage(v, W, p, i-e0, j-e1, k-e2, m);
+    loadStage(v, W, p, i, j, k, c);
+    loadStage(v, W, p, i+e0, j+e1, k+e2, pp);
+    hc::plmFaceStatesMixed<S>(l, rr, mm, m, c, pp, v.limiter);
+    if (D > 0)
+    {
+        const int f = D == 1 ? j : k, tt = D == 1 ? k : j;
+        if (f == 0 || f == v.na)
+        {
+            const int s = f == 0 ? 0 : 1;
+            const int* e = v.edge + 4*((p*2 + (D-1))*2 + s);
+            const int tq = e[3] ? v.na-1-tt : tt;
+            const long long base = ((((long long)(e[0]*2+(e[1]-1))*2+e[2])*8)*v.na + tq)*v.nr + i;
+            const long long cstride = (long long)v.na*v.nr;
+            double* dst = s == 0 ? l : rr;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) dst[q] = PS[base + q*cstride];
+        }
+    }
+    const double* nr = v.normal[D] + 3LL*(((long long)p*fk + k)*fj + j);
+    const double area = D == 0 ? rad(v,i)[2] : rad(v,i)[3];
+    double f = 0;
+#pragma unroll
+    for (int x = 0; x < 3; ++x) f += 0.5*(l[5+x]+rr[5+x])*nr[x]*area;
+    BF[((long long)p*fk + k)*fj*fi + (long long)j*fi + i] = f;
+}
+
+// (b) -J div B of the magnetic face fluxes times (B/4pi, v, v.B/4pi) of the
+// cell's Cartesian primitives, added to K; then the RK stage accumulate
+// U += b_dt*K when U is given (the divergence kernel leaves U alone when the
+// Powell source is on, so U takes the complete K in one rounding).
+__global__ void __launch_bounds__(kBlock)
+k_powell_source(LevelView v, const double* __restrict__ W, const double* __restrict__ B0,
+                const double* __restrict__ B1, const double* __restrict__ B2, double* __restrict__ K,
+                double* __restrict__ U, double b_dt)
+{
+    const long long n = 6LL*v.na*v.na*v.nr;
+    const long long t = (long long)blockIdx.x*blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int i = t % v.nr;
+    long long r = t / v.nr;
+    const int j = r % v.na; r /= v.na;
+    const int k = r % v.na;
+    const int p = r / v.na;
+    const double inv[3] = {-1.0/(1.0/v.nr), -1.0/(1.0/v.na), -1.0/(1.0/v.na)};
+    const long long b0 = (((long long)p*v.na + k)*v.na + j)*(v.nr+1) + i;
+    const long long b1 = (((long long)p*v.na + k)*(v.na+1) + j)*v.nr + i;
+    const long long b2 = (((long long)p*(v.na+1) + k)*v.na + j)*v.nr + i;
+    double md = 0;
+    md += inv[0]*(B0[b0 + 1] - B0[b0]);
+    md += inv[1]*(B1[b1 + v.nr] - B1[b1]);
+    md += inv[2]*(B2[b2 + (long long)v.na*v.nr] - B2[b2]);
+    double w[8];
+    loadStage(v, W, p, i, j, k, w);
+    double src[8], vb = 0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+    {
+        const double b = w[5+d];
+        src[1+d] = md*b/(4.0*hc::kPi);
+        src[5+d] = md*w[1+d];
+        vb += w[1+d]*b;
+    }
+    src[4] = md*vb/(4.0*hc::kPi);
+    const long long vbase = validIndex(v, p, 0, i, j, k);
+#pragma unroll
+    for (int c = 1; c < 8; ++c) K[vbase + c*v.vcs] += src[c];
+    if (U)
+#pragma unroll
+        for (int c = 0; c < 8; ++c) U[vbase + c*v.vcs] += b_dt*K[vbase + c*v.vcs];
+}
+
 inline unsigned blocksFor(long long n) { return (unsigned)((n + kBlock - 1)/kBlock); }
 
 template <typename T>
@@ -537,6 +632,7 @@ struct hc_level
     double* F[3] = {nullptr, nullptr, nullptr};
     double* PS = nullptr;
     double* SF = nullptr;
+    double* BF[3] = {nullptr, nullptr, nullptr};   // Powell magnetic face fluxes
     double* U0 = nullptr;
     double* K = nullptr;
     // hipGraph of one RK4 step (state copy + 4 x (prepare, RHS)), captured on a
@@ -590,6 +686,8 @@ struct hc_level
         HC_CHECK(hipMalloc(&SF, 24ull*8*dims.na*dims.nr*sizeof(double))); owned.push_back(SF);
         HC_CHECK(hipMalloc(&U0, validCount()*sizeof(double))); owned.push_back(U0);
         HC_CHECK(hipMalloc(&K, validCount()*sizeof(double))); owned.push_back(K);
+        if (view.powell)
+            for (int d = 0; d < 3; ++d) { HC_CHECK(hipMalloc(&BF[d], faceCount(d)/8*sizeof(double))); owned.push_back(BF[d]); }
     }
 };
 
@@ -605,6 +703,7 @@ void buildLevel(hc_level& L)
     v.vcs = (long long)nr*na*na; v.vps = 8*v.vcs;
     v.limiter = L.opt.limiter;
     v.roe = L.opt.riemann_solver == 2;
+    v.powell = L.opt.divergence_cleaning == 1;
     v.gamma = L.opt.gamma;
     v.entropy = L.opt.entropy_fix;
 
@@ -624,7 +723,7 @@ void buildLevel(hc_level& L)
     for (int d = 0; d < 3; ++d)
     {
         const int fj = na + (d == 1), fk = na + (d == 2);
-        std::vector<double> fr(10ull*6*fj*fk);
+        std::vector<double> fr(10ull*6*fj*fk), nrm(3ull*6*fj*fk);
         for (int p = 0; p < 6; ++p) for (int k = 0; k < fk; ++k) for (int j = 0; j < fj; ++j)
         {
             const hc::Vec3 n = hc::faceNormal(na, p, d, j, k);
@@ -632,8 +731,10 @@ void buildLevel(hc_level& L)
             double* o = &fr[10*(((size_t)p*fk+k)*fj+j)];
             for (int a = 0; a < 3; ++a) for (int b = 0; b < 3; ++b) o[3*a+b] = F.f[a][b];
             o[9] = F.norm;
+            for (int a = 0; a < 3; ++a) nrm[3*(((size_t)p*fk+k)*fj+j)+a] = n.x[a];
         }
         v.frame[d] = L.own(deviceCopy(fr));
+        v.normal[d] = v.powell ? L.own(deviceCopy(nrm)) : nullptr;
     }
 
     const int NJ = v.NJ, NK = v.NK;
@@ -703,6 +804,22 @@ void prepareStage(hc_level& L, const double* U0, const double* K, double a_dt,
     HC_CHECK(hipGetLastError());
 }
 
+// Powell source after the flux divergence (needs the panel states in L.PS).
+template <typename S>
+void powellSource(hc_level& L, const double* W, double* K, double* U, double b_dt, hipStream_t s)
+{
+    const LevelView& v = L.view;
+    if (!v.powell) return;
+    KernelProfile& P = L.profile;
+    P.run("k_powell_bflux", s, [&] {
+        hipLaunchKernelGGL((k_powell_bflux<0, S>), dim3(blocksFor(6LL*(v.nr+1)*v.na*v.na)), dim3(kBlock), 0, s, v, W, L.PS, L.BF[0]);
+        hipLaunchKernelGGL((k_powell_bflux<1, S>), dim3(blocksFor(6LL*v.nr*(v.na+1)*v.na)), dim3(kBlock), 0, s, v, W, L.PS, L.BF[1]);
+        hipLaunchKernelGGL((k_powell_bflux<2, S>), dim3(blocksFor(6LL*v.nr*v.na*(v.na+1))), dim3(kBlock), 0, s, v, W, L.PS, L.BF[2]); });
+    P.run("k_powell_source", s, [&] {
+        hipLaunchKernelGGL(k_powell_source, dim3(blocksFor(6LL*v.na*v.na*v.nr)), dim3(kBlock), 0, s,
+                           v, W, L.BF[0], L.BF[1], L.BF[2], K, U, b_dt); });
+}
+
 template <typename S>
 void stageRhsFused(hc_level& L, const double* W, double* K, double* U, double b_dt, hipStream_t s)
 {
@@ -714,11 +831,13 @@ void stageRhsFused(hc_level& L, const double* W, double* K, double* U, double b_
         if (v.roe) hipLaunchKernelGGL((k_seam_flux<S, true>), dim3(blocksFor(24LL*v.na*v.nr)), dim3(kBlock), 0, s, v, L.PS, L.SF);
         else hipLaunchKernelGGL((k_seam_flux<S, false>), dim3(blocksFor(24LL*v.na*v.nr)), dim3(kBlock), 0, s, v, L.PS, L.SF); });
     const int tI = (v.nr + kTI - 1)/kTI, tJ = (v.na + kTJ - 1)/kTJ, tK = (v.na + kTK - 1)/kTK;
+    double* const Ud = v.powell ? nullptr : U;   // with Powell, U is accumulated after the source
     P.run("k_rhs_fused", s, [&] {
         if (v.roe) hipLaunchKernelGGL((k_rhs_fused<S, true>), dim3(6u*tI*tJ*tK), dim3(kTileCells), 0, s,
-                           v, W, L.SF, K, U, b_dt, tI, tJ, tK);
+                           v, W, L.SF, K, Ud, b_dt, tI, tJ, tK);
         else hipLaunchKernelGGL((k_rhs_fused<S, false>), dim3(6u*tI*tJ*tK), dim3(kTileCells), 0, s,
-                           v, W, L.SF, K, U, b_dt, tI, tJ, tK); });
+                           v, W, L.SF, K, Ud, b_dt, tI, tJ, tK); });
+    powellSource<S>(L, W, K, U, b_dt, s);
     HC_CHECK(hipGetLastError());
 }
 
@@ -756,7 +875,8 @@ void stageRhs(hc_level& L, const double* W, double* K, double* U, double b_dt,
         hipLaunchKernelGGL(k_seam_reconcile, dim3(blocksFor(24LL*v.na*v.nr)), dim3(kBlock), 0, s, v, F[1], F[2]); });
     P.run("k_divergence", s, [&] {
         hipLaunchKernelGGL(k_divergence, dim3(blocksFor(6LL*v.na*v.na*v.nr)), dim3(kBlock), 0, s,
-                           v, F[0], F[1], F[2], K, U, b_dt); });
+                           v, F[0], F[1], F[2], K, v.powell ? nullptr : U, b_dt); });
+    powellSource<double>(L, W, K, U, b_dt, s);
     HC_CHECK(hipGetLastError());
 }
 

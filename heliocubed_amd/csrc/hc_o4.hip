@@ -45,8 +45,14 @@ enum Field : int
     F_SPD = F_MER, F_QL = F_DPT, F_QR = F_FB,
     // face-average total pressure pBar + meBar of the faces of the direction in
     // progress; reuses the cell meBar slot, which is dead once k_o4_eta has run
-    F_P0B = F_MEB
+    F_P0B = F_MEB,
+    // Powell source (standard operator, after the direction passes): the radial
+    // area products of <det A B_n> of the three directions, -J div B, and the
+    // point values and cell averages of the coefficient components 1..7 (slot
+    // + component); reuses the per-direction face slots 18..37
+    F_GB = 18, F_MJD = 21, F_CP = 22, F_CA = 30
 };
+static_assert(F_GB > F_W + 7 && F_CA + 7 < F_FLUX, "Powell slots overlap the primitives or the face fluxes");
 
 // One box of the table. U (ghosted native conserved state) and R (RHS) are
 // addressed through strides so a box can view either its own dense BoxData
@@ -77,6 +83,7 @@ struct O4View
     double gamma, entropy;
     int op, fallback;        // 1 standard operator, 2 Berta WENOZ; Berta fallback 0 WENO3, 1 PLM
     int roe;                 // 1 Roe 8-wave dissipation, 0 Rusanov
+    int powell;              // divergence_cleaning 1: Powell source (standard operator)
     double etaD;             // Berta Jameson detector threshold
     const BoxDesc* boxes;
     const double* cellA;     // [6][naE][naE][9]
@@ -1078,9 +1085,146 @@ __global__ void __launch_bounds__(kB) k_o4_seams(O4View v, double* __restrict__ 
     }
 }
 
+// 6b. Powell source, standard operator (src/BoxOp_DivBcleaning.H:31-168,
+// src/BoxOp_EulerCubedSphereImplem.H:152-155,203-211,385-390). On the cells
+// [lo-2, hi+2]: the point values of the coefficient (B/4pi, v, v.B/4pi) in
+// Cartesian components from the deconvolved primitives W - Lap(W)/24 and the
+// cell-centre basis (F_CP + 1..7); on the faces [lo_d-1, hi_d+2] x [lo-1, hi+1]
+// of each direction d: G_d = <adjDr_d <det A B_d>>, with B_d the unlimited
+// fourth-order face value of the native field component d and <ab> the face
+// product rule. The metric does not vary with the radius and the radial area
+// not across a panel: those differences are exactly zero and their product
+// terms (+-0) are left out.
+__device__ __forceinline__ double faceValueW(const BoxIdx& I, const double* P, int c, int d, const int y[3])
+{
+    auto W = [&](int o) { int z[3] = {y[0], y[1], y[2]}; z[d] += o; return P[I(F_W+c, z[0], z[1], z[2])]; };
+    return (7.0/12.0)*(W(-1)+W(0)) - (1.0/12.0)*(W(-2)+W(1));
+}
+
+template <int D>
+__device__ double powellFaceProduct(const O4View& v, const BoxDesc& b, const BoxIdx& I, const double* P, const int x[3])
+{
+    constexpr int t1 = D == 0 ? 1 : 0, t2 = D == 2 ? 1 : 2;
+    const double* met = v.fmet[D];
+    // <det A B_D> at face y: derivative terms along the angular transverse
+    // directions, then det A B_D
+    auto detAB = [&](const int y[3]) {
+        int a[3] = {y[0], y[1], y[2]}, z[3] = {y[0], y[1], y[2]};
+        double val = 0;
+        if (D == 0)
+        {
+            a[t1] += 1; z[t1] -= 1;
+            const double dm = 0.5*met[13*angIdx(b, a[1], a[2])] - 0.5*met[13*angIdx(b, z[1], z[2])];
+            const double db = 0.5*faceValueW(I, P, 5+D, D, a) - 0.5*faceValueW(I, P, 5+D, D, z);
+            val += dm*db*(1.0/12.0);
+            a[t1] -= 1; z[t1] += 1;
+        }
+        a[t2] += 1; z[t2] -= 1;
+        const double dm = 0.5*met[13*angIdx(b, a[1], a[2])] - 0.5*met[13*angIdx(b, z[1], z[2])];
+        const double db = 0.5*faceValueW(I, P, 5+D, D, a) - 0.5*faceValueW(I, P, 5+D, D, z);
+        val += dm*db*(1.0/12.0);
+        val += met[13*angIdx(b, y[1], y[2])]*faceValueW(I, P, 5+D, D, y);
+        return val;
+    };
+    const double adjX = areaOf(v, b, D, x[0]);
+    double val = 0;
+    if (D != 0)
+    {
+        // radial transverse direction (t1): the radial area varies with it
+        const int a[3] = {x[0]+1, x[1], x[2]}, z[3] = {x[0]-1, x[1], x[2]};
+        const double dadj = 0.5*areaOf(v, b, D, x[0]+1) - 0.5*areaOf(v, b, D, x[0]-1);
+        val += dadj*(0.5*detAB(a) - 0.5*detAB(z))*(1.0/12.0);
+    }
+    val += adjX*detAB(x);
+    return val;
+}
+
+__global__ void __launch_bounds__(kB) k_o4_pw_face(O4View v, double* __restrict__ S)
+{
+    const BoxDesc& b = blockBoxDesc(v);
+    int L[3], H[3], x[3];
+    for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]-2; H[d] = b.hi[d]+2; }
+    if (!regionPoint(b, L, H, boxThread(v, b), x)) return;
+    const BoxIdx I(b);
+    double* const P = S + b.sOff;
+    {
+        double w[8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+        {
+            const int f = F_W+c;
+            const double lap = P[I(f,x[0]+1,x[1],x[2])]+P[I(f,x[0]-1,x[1],x[2])]+P[I(f,x[0],x[1]+1,x[2])]
+                              +P[I(f,x[0],x[1]-1,x[2])]+P[I(f,x[0],x[1],x[2]+1)]+P[I(f,x[0],x[1],x[2]-1)]
+                              -6.0*P[I(f,x[0],x[1],x[2])];
+            w[c] = P[I(f,x[0],x[1],x[2])]-lap/24.0;
+        }
+        const double* A = v.cellA + 9*angIdx(b, x[1], x[2]);
+        double vel[3] = {0, 0, 0}, mag[3] = {0, 0, 0};
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) { vel[i] += A[3*i+j]*w[1+j]; mag[i] += A[3*i+j]*w[5+j]; }
+        double ce = 0;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+        {
+            P[I(F_CP+1+i, x[0], x[1], x[2])] = mag[i]/(4.0*hc::kPi);
+            P[I(F_CP+5+i, x[0], x[1], x[2])] = vel[i];
+            ce += vel[i]*mag[i]/(4.0*hc::kPi);
+        }
+        P[I(F_CP+4, x[0], x[1], x[2])] = ce;
+    }
+    auto inFaces = [&](int d) {
+        for (int a = 0; a < 3; ++a)
+        {
+            const int lo = b.lo[a]-1, hi = a == d ? b.hi[a]+2 : b.hi[a]+1;
+            if (x[a] < lo || x[a] > hi) return false;
+        }
+        return true;
+    };
+    if (inFaces(0)) P[I(F_GB+0, x[0], x[1], x[2])] = powellFaceProduct<0>(v, b, I, P, x);
+    if (inFaces(1)) P[I(F_GB+1, x[0], x[1], x[2])] = powellFaceProduct<1>(v, b, I, P, x);
+    if (inFaces(2)) P[I(F_GB+2, x[0], x[1], x[2])] = powellFaceProduct<2>(v, b, I, P, x);
+}
+
+// 6c. -J div B = sum_d -(1/dxi_d) (G_d(x+e_d) - G_d(x)) and the cell
+// averages C + Lap(C)/24 of the coefficient, on the cells [lo-1, hi+1].
+__global__ void __launch_bounds__(kB) k_o4_pw_cell(O4View v, double* __restrict__ S)
+{
+    const BoxDesc& b = blockBoxDesc(v);
+    int L[3], H[3], x[3];
+    for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]-1; H[d] = b.hi[d]+1; }
+    if (!regionPoint(b, L, H, boxThread(v, b), x)) return;
+    const BoxIdx I(b);
+    double* const P = S + b.sOff;
+    double md = 0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+    {
+        const double dxInv = -1.0/(d == 0 ? 1.0/b.nr : 1.0/b.na);
+        int a[3] = {x[0], x[1], x[2]}; a[d] += 1;
+        double inc = 0;
+        inc += (-1.0*dxInv)*P[I(F_GB+d, x[0], x[1], x[2])];
+        inc += (1.0*dxInv)*P[I(F_GB+d, a[0], a[1], a[2])];
+        md += inc;
+    }
+    P[I(F_MJD, x[0], x[1], x[2])] = md;
+#pragma unroll
+    for (int c = 1; c < 8; ++c)
+    {
+        const int f = F_CP+c;
+        const double lap = P[I(f,x[0]+1,x[1],x[2])]+P[I(f,x[0]-1,x[1],x[2])]+P[I(f,x[0],x[1]+1,x[2])]
+                          +P[I(f,x[0],x[1]-1,x[2])]+P[I(f,x[0],x[1],x[2]+1)]+P[I(f,x[0],x[1],x[2]-1)]
+                          -6.0*P[I(f,x[0],x[1],x[2])];
+        P[I(F_CA+c, x[0], x[1], x[2])] = P[I(f,x[0],x[1],x[2])]+lap/24.0;
+    }
+}
+
 // 7. Flux divergence into the box RHS: accumulated (overwrite = 0, the
 // BoxOp convention; the caller zeroes it) or overwritten; optionally the RK
-// stage accumulate U += b_dt * RHS with U in the RHS layout.
+// stage accumulate U += b_dt * RHS with U in the RHS layout. With the Powell
+// source on, the cell product <(-J div B) C_c> = sum_d D_d(-J div B) D_d C_c / 12
+// + (-J div B) C_c is added to components 1..7 after the divergence.
 __global__ void __launch_bounds__(kB) k_o4_div(O4View v, const double* __restrict__ S, double* __restrict__ R,
                                                int overwrite, double* __restrict__ Uacc, double b_dt)
 {
@@ -1102,6 +1246,20 @@ __global__ void __launch_bounds__(kB) k_o4_div(O4View v, const double* __restric
             inc += (-1.0*dxInv)*P[I(F_FLUX+8*d+c, x[0], x[1], x[2])];
             inc += (1.0*dxInv)*P[I(F_FLUX+8*d+c, a[0], a[1], a[2])];
             acc += inc;
+        }
+        if (v.powell && c > 0)
+        {
+            double val = 0;
+            for (int d = 0; d < 3; ++d)
+            {
+                int a[3] = {x[0], x[1], x[2]}, z[3] = {x[0], x[1], x[2]};
+                a[d] += 1; z[d] -= 1;
+                const double ds = 0.5*P[I(F_MJD, a[0], a[1], a[2])] - 0.5*P[I(F_MJD, z[0], z[1], z[2])];
+                const double dc = 0.5*P[I(F_CA+c, a[0], a[1], a[2])] - 0.5*P[I(F_CA+c, z[0], z[1], z[2])];
+                val += ds*dc*(1.0/12.0);
+            }
+            val += P[I(F_MJD, x[0], x[1], x[2])]*P[I(F_CA+c, x[0], x[1], x[2])];
+            acc += val;
         }
         R[r + n] = acc;
         if (Uacc) Uacc[r + n] += b_dt*acc;
@@ -1167,6 +1325,7 @@ void buildBoxop(hc_boxop& B, const hc_box* boxes, const int* boxLevel, int nbox)
     v.gamma = B.opt.gamma; v.entropy = B.opt.entropy_fix;
     v.op = B.opt.spatial_operator == 2 ? 2 : 1;
     v.roe = B.opt.riemann_solver == 2;
+    v.powell = B.opt.divergence_cleaning == 1;
     v.fallback = B.opt.berta_fallback;
     v.etaD = B.opt.berta_eta_d;
     std::vector<double> cellA, faceA[3], fmet[3], radial;
@@ -1288,6 +1447,11 @@ void applyBoxop(hc_boxop& B, const double* U, double* R, double* const F[3], hip
     }
     if (v.nseams > 0)
         hipLaunchKernelGGL(k_o4_seams, dim3((v.nseams + kB - 1)/kB), dim3(kB), 0, s, v, B.scratch);
+    if (v.powell)
+    {
+        hipLaunchKernelGGL(k_o4_pw_face, grid, dim3(kB), 0, s, v, B.scratch);
+        hipLaunchKernelGGL(k_o4_pw_cell, grid, dim3(kB), 0, s, v, B.scratch);
+    }
     hipLaunchKernelGGL(k_o4_div, grid, dim3(kB), 0, s, v, B.scratch, R, overwrite, Uacc, b_dt);
     for (int d = 0; d < 3; ++d)
         if (F[d]) hipLaunchKernelGGL(k_o4_fluxout, grid, dim3(kB), 0, s, v, d, B.scratch, F[d], B.fOff[d]);

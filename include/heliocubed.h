@@ -73,6 +73,9 @@ typedef struct hc_options
     int spatial_operator;   /* 0 or 1: standard operator, 2: Berta WENOZ (fourth order) */
     double berta_eta_d;     /* Berta Jameson detector threshold (reference default 0.02) */
     int berta_fallback;     /* Berta fallback reconstruction: 0 WENO3, 1 PLM */
+    int divergence_cleaning;/* 1: Powell source -J div(B) (B/4pi, v, v.B/4pi) added to the RHS
+                               (src/BoxOp_SecondOrder.H:317-369 at order 2,
+                               src/BoxOp_DivBcleaning.H:31-168 at order 4); 0: off */
 } hc_options;
 
 /* A box of the shell: panel (block) 0..5 and inclusive cell range lo..hi in

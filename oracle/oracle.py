@@ -31,6 +31,7 @@ class OrcCfg(ctypes.Structure):
         ("spatial_operator", ctypes.c_int),
         ("berta_eta_d", ctypes.c_double),
         ("berta_fallback", ctypes.c_int),
+        ("divergence_cleaning", ctypes.c_int),
     ]
 
 
@@ -104,8 +105,9 @@ class Oracle:
     """O2 operator of one shell configuration on the CPU (FP64)."""
 
     def __init__(self, n_rad, n_ang, r0=21.5, r1=215.0, radial_stretch=0.0, gamma=5.0 / 3.0,
-                 limiter=1, riemann=1, entropy_fix=0.3, ghost=2):
+                 limiter=1, riemann=1, entropy_fix=0.3, ghost=2, divergence_cleaning=0):
         self.cfg = OrcCfg(n_rad, n_ang, ghost, r0, r1, radial_stretch, gamma, limiter, riemann, entropy_fix)
+        self.cfg.divergence_cleaning = divergence_cleaning
         self.nr, self.na, self.g = n_rad, n_ang, ghost
         self._h = ctypes.c_void_p(lib().orc_create(ctypes.byref(self.cfg)))
 
@@ -334,14 +336,14 @@ def basis(p, alpha, beta):
 
 
 def o4_rhs_boxes(n_rad, n_ang, boxes, USph, limiter=1, entropy_fix=0.3, gamma=5.0 / 3.0, r0=21.5, r1=215.0,
-                 with_fluxes=False, spatial_operator=1, eta_d=0.02, fallback=0, riemann=2):
+                 with_fluxes=False, spatial_operator=1, eta_d=0.02, fallback=0, riemann=2, divergence_cleaning=0):
     """Fourth-order RHS (and optionally fluxes) of every box, concatenated like hc_boxop.
 
     spatial_operator 1 is the standard operator, 2 the Berta WENOZ operator
     (Jameson detector threshold eta_d, fallback 0 WENO3 / 1 PLM); riemann 2
     Roe 8-wave, 1 Rusanov."""
     cfg = OrcCfg(n_rad, n_ang, 5, r0, r1, 0.0, gamma, limiter, riemann, entropy_fix, spatial_operator, eta_d,
-                 fallback)
+                 fallback, divergence_cleaning)
     g = 5
     rhs_parts, flux_parts, off = [], [[], [], []], 0
     for blk, lo, hi in boxes:
@@ -368,9 +370,9 @@ class Oracle4:
     """Fourth-order level (standard or Berta operator + Roe, state_conversion 1) on the CPU."""
 
     def __init__(self, n_rad, n_ang, limiter=1, entropy_fix=0.3, gamma=5.0 / 3.0, r0=21.5, r1=215.0,
-                 spatial_operator=1, eta_d=0.02, fallback=0, riemann=2):
+                 spatial_operator=1, eta_d=0.02, fallback=0, riemann=2, divergence_cleaning=0):
         self.cfg = OrcCfg(n_rad, n_ang, 5, r0, r1, 0.0, gamma, limiter, riemann, entropy_fix, spatial_operator,
-                          eta_d, fallback)
+                          eta_d, fallback, divergence_cleaning)
         self.nr, self.na = n_rad, n_ang
         self._h = ctypes.c_void_p(lib().orc4_create(ctypes.byref(self.cfg)))
 
