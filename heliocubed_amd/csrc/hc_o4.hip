@@ -166,6 +166,10 @@ __device__ __forceinline__ bool regionPoint(const BoxDesc& b, const int L[3], co
     return true;
 }
 
+// FAST (mixed precision only): constant divisions as products with the
+// reciprocal (within an ulp; the FP64 form keeps the divisions of the CPU
+// restatement).
+template <bool FAST = false>
 __device__ void energiesNative(const double* A, const double w[8], double& ke, double& me, double& vb)
 {
     double vc[3] = {0, 0, 0}, bc[3] = {0, 0, 0};
@@ -176,7 +180,7 @@ __device__ void energiesNative(const double* A, const double w[8], double& ke, d
 #pragma unroll
         for (int b = 0; b < 3; ++b) { vc[a] += A[3*a+b]*w[1+b]; bc[a] += A[3*a+b]*w[5+b]; }
         ke += vc[a]*vc[a];
-        me += bc[a]*bc[a]/8.0/hc::kPi;
+        me += FAST ? bc[a]*bc[a]*hc::kInv8Pi : bc[a]*bc[a]/8.0/hc::kPi;
         vb += vc[a]*bc[a];
     }
     ke *= w[0]*.5;
@@ -197,16 +201,18 @@ __device__ __forceinline__ bool usablePrimitive(const double w[8])
     return f && w[0] >= hc::kRhoFloor && w[4] >= 0.0;
 }
 
+template <bool FAST = false>
 __device__ void primFlux(double f[8], const double w[8], double ke, double me, double vb, double gamma, int d)
 {
     const int t1 = (d+1)%3, t2 = (d+2)%3;
     const double vn = w[d+1], vt1 = w[t1+1], vt2 = w[t2+1];
     const double bn = w[d+5], bt1 = w[t1+5], bt2 = w[t2+5];
+    auto q4pi = [](double x) { return FAST ? x*hc::kInv4Pi : x/4.0/hc::kPi; };
     f[0] = w[0]*vn;
-    f[d+1] = f[0]*vn - bn*bn/4.0/hc::kPi;
-    f[t1+1] = f[0]*vt1 - bn*bt1/4.0/hc::kPi;
-    f[t2+1] = f[0]*vt2 - bn*bt2/4.0/hc::kPi;
-    f[4] = (2.0*me + ke + w[4]*gamma/(gamma-1.0))*vn - bn*vb/4.0/hc::kPi;
+    f[d+1] = f[0]*vn - q4pi(bn*bn);
+    f[t1+1] = f[0]*vt1 - q4pi(bn*bt1);
+    f[t2+1] = f[0]*vt2 - q4pi(bn*bt2);
+    f[4] = (2.0*me + ke + w[4]*gamma/(gamma-1.0))*vn - q4pi(bn*vb);
     f[5] = vn*w[5] - w[1]*bn;
     f[6] = vn*w[6] - w[2]*bn;
     f[7] = vn*w[7] - w[3]*bn;
@@ -297,9 +303,17 @@ __device__ void factoredRoe(double out[8], const double wl[8], const double wr[8
         normal2 += frame[0][i]*frame[0][i];
         tangent2 += frame[1][i]*frame[1][i];
     }
+    constexpr bool FAST = !std::is_same<S, double>::value;
     const double nm = sqrt(normal2), tm = sqrt(tangent2);
+    if constexpr (FAST)
+    {
+        const double inm = 1.0/nm, itm = 1.0/tm;
 #pragma unroll
-    for (int i = 0; i < 3; ++i) { frame[0][i] /= nm; frame[1][i] /= tm; }
+        for (int i = 0; i < 3; ++i) { frame[0][i] *= inm; frame[1][i] *= itm; }
+    }
+    else
+#pragma unroll
+        for (int i = 0; i < 3; ++i) { frame[0][i] /= nm; frame[1][i] /= tm; }
 #pragma unroll
     for (int i = 0; i < 3; ++i) { const int j = (i+1)%3, k = (i+2)%3; frame[2][i] = frame[0][j]*frame[1][k]-frame[0][k]*frame[1][j]; }
     double loc[2][8];
@@ -346,6 +360,19 @@ __device__ void factoredRoe(double out[8], const double wl[8], const double wr[8
         for (int i = 0; i < 3; ++i) { const int j = (i+1)%3, k = (i+2)%3; adj[d][i] = A[j][u1]*A[k][u2]-A[k][u1]*A[j][u2]; }
     }
     const double det = A[0][0]*adj[0][0]+A[1][0]*adj[0][1]+A[2][0]*adj[0][2];
+    if constexpr (FAST)
+    {
+        const double idet = 1.0/det, idet2 = 1.0/(det*det);
+        out[0] = mapped[0]*idet; out[4] = mapped[4]*idet;
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+        {
+            out[1+d] = 0; out[5+d] = 0;
+#pragma unroll
+            for (int i = 0; i < 3; ++i) { out[1+d] += adj[d][i]*mapped[1+i]*idet2; out[5+d] += adj[d][i]*mapped[5+i]*idet2; }
+        }
+        return;
+    }
     out[0] = mapped[0]/det; out[4] = mapped[4]/det;
 #pragma unroll
     for (int d = 0; d < 3; ++d)
@@ -567,6 +594,8 @@ __global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(2))) k_
     const BoxIdx I(b);
     double* const P = S + b.sOff;
     constexpr int t1 = D == 0 ? 1 : 0, t2 = D == 2 ? 1 : 2;
+    constexpr bool FAST = !std::is_same<S_, double>::value;   // mixed precision
+    auto by24 = [](double x) { return FAST ? x*(1.0/24.0) : x/24.0; };
     auto Wat = [&](int c, int o) {
         int y[3] = {x[0], x[1], x[2]}; y[D] += o;
         return P[I(F_W+c, y[0], y[1], y[2])]; };
@@ -629,8 +658,8 @@ __global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(2))) k_
 #pragma unroll
         for (int q = 0; q < 8; ++q)
         {
-            pl[q] = lf[q]-lapb[q]/24.0;
-            pr[q] = rf[q]-lapb[q]/24.0;
+            pl[q] = lf[q]-by24(lapb[q]);
+            pr[q] = rf[q]-by24(lapb[q]);
             cl[q] = P[I(F_WB+q, m1[0], m1[1], m1[2])];
             cr[q] = P[I(F_WB+q, x[0], x[1], x[2])];
         }
@@ -643,8 +672,8 @@ __global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(2))) k_
 #pragma unroll
         for (int q = 0; q < 8; ++q)
         {
-            P[I(F_PTL+q, x[0], x[1], x[2])] = lf[q]-lapb[q]/24.0;
-            P[I(F_PTR+q, x[0], x[1], x[2])] = rf[q]-lapb[q]/24.0;
+            P[I(F_PTL+q, x[0], x[1], x[2])] = lf[q]-by24(lapb[q]);
+            P[I(F_PTR+q, x[0], x[1], x[2])] = rf[q]-by24(lapb[q]);
         }
         P[I(F_SPD, x[0], x[1], x[2])] = rusanovSpeedO4(lf, rf, v.faceA[D] + 9*angIdx(b, x[1], x[2]),
                                                        v.gamma, D, false);
@@ -677,13 +706,13 @@ __global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(2))) k_
     {
         double w[8];
 #pragma unroll
-        for (int q = 0; q < 8; ++q) w[q] = (side == 0 ? lf[q] : rf[q]) - lapb[q]/24.0;
+        for (int q = 0; q < 8; ++q) w[q] = (side == 0 ? lf[q] : rf[q]) - by24(lapb[q]);
         double ke, me, vb, f[8];
-        energiesNative(A, w, ke, me, vb);
-        primFlux(f, w, ke, me, vb, v.gamma, D);
-        const double p0 = (side == 0 ? lf[4] : rf[4]) + (me + lapMe/24.0);
+        energiesNative<FAST>(A, w, ke, me, vb);
+        primFlux<FAST>(f, w, ke, me, vb, v.gamma, D);
+        const double p0 = (side == 0 ? lf[4] : rf[4]) + (me + by24(lapMe));
 #pragma unroll
-        for (int q = 0; q < 8; ++q) phs[q] = side == 0 ? f[q]+lapF[q]/24.0 : phs[q] + (f[q]+lapF[q]/24.0);
+        for (int q = 0; q < 8; ++q) phs[q] = side == 0 ? f[q]+by24(lapF[q]) : phs[q] + (f[q]+by24(lapF[q]));
         p0s = side == 0 ? p0 : p0s + p0;
     }
 #pragma unroll
@@ -695,7 +724,7 @@ __global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(2))) k_
         // averages it over the transverse neighbours)
         double wl[8], wr[8], out[8];
 #pragma unroll
-        for (int q = 0; q < 8; ++q) { wl[q] = lf[q]-lapb[q]/24.0; wr[q] = rf[q]-lapb[q]/24.0; }
+        for (int q = 0; q < 8; ++q) { wl[q] = lf[q]-by24(lapb[q]); wr[q] = rf[q]-by24(lapb[q]); }
         factoredRoe<S_>(out, wl, wr, A, radialPoint(v, b, D, x[0]), D, v.gamma, v.entropy);
 #pragma unroll
         for (int q = 0; q < 8; ++q) P[I(F_DPT+q, x[0], x[1], x[2])] = out[q];

@@ -179,23 +179,30 @@ HC_HD bool roe8WaveDissipation(S out[8], const double WL[8], const double WR[8],
     const S rs2 = S(0.707106781186547524400844362104849039);
     if (!(WL[0] > 0.0 && WR[0] > 0.0 && WL[4] > 0.0 && WR[4] > 0.0)) return false;
     const double hgd = gamma_d-1.0;
+    // mixed precision (S = float): the FP64 divisions by gamma-1 and by the
+    // average density become products with their reciprocals
+    constexpr bool FAST = !std::is_same<S, double>::value;
+    const double ihgd = FAST ? 1.0/hgd : 0.0;
+    auto byH = [&](double x) { return FAST ? x*ihgd : x/hgd; };
     const double rL = WL[0], rR = WR[0];
     const double mL0 = rL*WL[1], mL1 = rL*WL[2], mL2 = rL*WL[3];
     const double mR0 = rR*WR[1], mR1 = rR*WR[2], mR2 = rR*WR[3];
     const double bbL = kInv8Pi*(WL[5]*WL[5]+WL[6]*WL[6]+WL[7]*WL[7]);
     const double keL = 0.5*(mL0*WL[1]+mL1*WL[2]+mL2*WL[3]);
-    const double EL = WL[4]/hgd+keL+bbL, a2L = gamma_d*WL[4]/rL;
+    const double EL = byH(WL[4])+keL+bbL, a2L = gamma_d*WL[4]/rL;
     const double bbR = kInv8Pi*(WR[5]*WR[5]+WR[6]*WR[6]+WR[7]*WR[7]);
     const double keR = 0.5*(mR0*WR[1]+mR1*WR[2]+mR2*WR[3]);
-    const double ER = WR[4]/hgd+keR+bbR, a2R = gamma_d*WR[4]/rR;
+    const double ER = byH(WR[4])+keR+bbR, a2R = gamma_d*WR[4]/rR;
     const double Rhod = 0.5*(rL+rR);
+    const double iRhod = FAST ? 1.0/Rhod : 0.0;
+    auto byR = [&](double x) { return FAST ? x*iRhod : x/Rhod; };
     const double Ud = 0.5*(WL[1]+WR[1]), Vd = 0.5*(WL[2]+WR[2]), Wd = 0.5*(WL[3]+WR[3]);
     const double BXd = 0.5*(WL[5]+WR[5]), BYd = 0.5*(WL[6]+WR[6]), BZd = 0.5*(WL[7]+WR[7]);
     const double Pd = 0.5*((WL[4]+bbL)+(WR[4]+bbR));
     const double BX2d = BXd*BXd, BYZ2d = BYd*BYd+BZd*BZd;
     const double BxBHd = kInv8Pi*(BX2d+BYZ2d);
     const double kkd = 0.5*(Ud*Ud+Vd*Vd+Wd*Wd);
-    double ccd = gamma_d*(Pd-BxBHd)/Rhod;
+    double ccd = byR(gamma_d*(Pd-BxBHd));
     const double ccmi = a2L > a2R ? a2R : a2L, ccma = a2L > a2R ? a2L : a2R;
     if (ccd < ccmi) ccd = ccmi;
     if (ccd > ccma) ccd = ccma;
@@ -218,14 +225,14 @@ HC_HD bool roe8WaveDissipation(S out[8], const double WL[8], const double WR[8],
     // Jump in primitive-like variables (left minus right), FP64.
     double dud[8] = {rL-rR, mL0-mR0, mL1-mR1, mL2-mR2, EL-ER, WL[5]-WR[5], WL[6]-WR[6], WL[7]-WR[7]};
     dud[4] = -hgd*(-kkd*dud[0]+Ud*dud[1]+Vd*dud[2]+Wd*dud[3]-dud[4]+(BXd*dud[5]+BYd*dud[6]+BZd*dud[7])*kInv4Pi);
-    dud[1] = (-Ud*dud[0]+dud[1])/Rhod;
-    dud[2] = (-Vd*dud[0]+dud[2])/Rhod;
-    dud[3] = (-Wd*dud[0]+dud[3])/Rhod;
+    dud[1] = byR(-Ud*dud[0]+dud[1]);
+    dud[2] = byR(-Vd*dud[0]+dud[2]);
+    dud[3] = byR(-Wd*dud[0]+dud[3]);
     // Wave speeds and eigenvector coefficients: FP64 (the slow speed and the
     // fast/slow mixing factors cancel catastrophically at low beta); the
     // eigenvector entries, projections and products below are in S.
     const double rs2d = 0.707106781186547524400844362104849039, smallBd = 1.0e-9;
-    const double bbd = kInv4Pi*BX2d/Rhod, bd = sqrt(bbd);
+    const double bbd = byR(kInv4Pi*BX2d), bd = sqrt(bbd);
     const double cd = sqrt(ccd);
     double hyyd, hzzd, aafd, aasd, afd, asd;
     if (BYZ2d < smallBd*smallBd)
@@ -238,7 +245,7 @@ HC_HD bool roe8WaveDissipation(S out[8], const double WL[8], const double WR[8],
     {
         const double hyz = sqrt(BYZ2d);
         hyyd = BYd/hyz; hzzd = BZd/hyz;
-        const double szb = ccd+(BxBHd+BxBHd)/Rhod;
+        const double szb = ccd+byR(BxBHd+BxBHd);
         const double hh = sqrt(fmax(0.0, szb*szb-4.0*ccd*bbd));
         aasd = 0.5*(szb-hh);
         if (aasd <= 0.0) { aasd = 0; asd = 0; }
@@ -321,7 +328,7 @@ HC_HD bool roe8WaveDissipation(S out[8], const double WL[8], const double WR[8],
             double dl[8];
 #pragma unroll
             for (int q = 0; q < 8; ++q) dl[q] = double(amp*col[q]);
-            dl[4] = kkd*dl[0]+Rhod*(Ud*dl[1]+Vd*dl[2]+Wd*dl[3])+dl[4]/hgd+kInv4Pi*(BXd*dl[5]+BYd*dl[6]+BZd*dl[7]);
+            dl[4] = kkd*dl[0]+Rhod*(Ud*dl[1]+Vd*dl[2]+Wd*dl[3])+byH(dl[4])+kInv4Pi*(BXd*dl[5]+BYd*dl[6]+BZd*dl[7]);
             dl[1] = Ud*dl[0]+Rhod*dl[1];
             dl[2] = Vd*dl[0]+Rhod*dl[2];
             dl[3] = Wd*dl[0]+Rhod*dl[3];
@@ -353,7 +360,7 @@ HC_HD bool roe8WaveDissipation(S out[8], const double WL[8], const double WR[8],
     double rd[8];
 #pragma unroll
     for (int q = 0; q < 8; ++q) rd[q] = double(ra[q]);
-    rd[4] = rd[0]*kkd+Rhod*(Ud*rd[1]+Vd*rd[2]+Wd*rd[3])+rd[4]/hgd+kInv4Pi*(BXd*rd[5]+BYd*rd[6]+BZd*rd[7]);
+    rd[4] = rd[0]*kkd+Rhod*(Ud*rd[1]+Vd*rd[2]+Wd*rd[3])+byH(rd[4])+kInv4Pi*(BXd*rd[5]+BYd*rd[6]+BZd*rd[7]);
     rd[1] = Ud*rd[0]+Rhod*rd[1];
     rd[2] = Vd*rd[0]+Rhod*rd[2];
     rd[3] = Wd*rd[0]+Rhod*rd[3];
