@@ -218,7 +218,8 @@ __global__ void __launch_bounds__(kB) k_b_faces(O4View v, double* __restrict__ S
     const BoxDesc& b = blockBoxDesc(v);
     int L[3], H[3], x[3];
     for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]-2; H[d] = b.hi[d]+2; }
-    L[D] = b.lo[D]; H[D] = b.hi[D]+1;
+    // the Powell face products also take the faces lo_d-1 and hi_d+2
+    L[D] = b.lo[D] - (v.powell ? 1 : 0); H[D] = b.hi[D] + (v.powell ? 2 : 1);
     if (!regionPoint(b, L, H, boxThread(v, b), x)) return;
     const BoxIdx I(b);
     double* const P = S + b.sOff;
@@ -541,11 +542,82 @@ __global__ void __launch_bounds__(kB) k_b_flux(O4View v, double* __restrict__ S)
         P[I(F_FLUX+8*D+c, x[0], x[1], x[2])] = 0.5*(Fsum[c]-Fd[c]);
 }
 
+// 4b. Powell face products of direction D (divergence_cleaning 1,
+// src/BoxOp_BertaCubedSphereImplem.H:1911-1929 with src/BoxOp_DivBcleaning.H:31-71)
+// on the faces [lo_d-1, hi_d+2] x [lo-1, hi+1]: G_D = <adjDr_D <det A B>>, with
+// <det A B> the face product of det A and the fourth-order face value of the
+// native field average in <U>, or det A times the centred point value
+// (B_L + B_R)/2 of the face states where the face's flux flag is <= 1/2. The
+// products keep G_D in slots that outlive the direction passes.
+template <int D>
+__global__ void __launch_bounds__(kB) k_b_pw_face(O4View v, const double* __restrict__ U, double* __restrict__ S)
+{
+    const BoxDesc& b = blockBoxDesc(v);
+    int L[3], H[3], x[3];
+    for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]-1; H[d] = b.hi[d]+1; }
+    H[D] = b.hi[D]+2;
+    if (!regionPoint(b, L, H, boxThread(v, b), x)) return;
+    const BoxIdx I(b);
+    double* const P = S + b.sOff;
+    constexpr int t1 = D == 0 ? 1 : 0, t2 = D == 2 ? 1 : 2;
+    const double* met = v.fmet[D];
+    auto bFace = [&](const int y[3]) {
+        auto u = [&](int o) {
+            int z[3] = {y[0], y[1], y[2]}; z[D] += o;
+            return U[b.uOff + (z[2]-b.lo[2]+kG)*b.us[1] + (z[1]-b.lo[1]+kG)*b.us[0] + (z[0]-b.lo[0]+kG)
+                     + (5+D)*b.us[2]]; };
+        return (7.0/12.0)*(u(-1)+u(0)) - (1.0/12.0)*(u(-2)+u(1));
+    };
+    auto detAB = [&](const int y[3]) {
+        const double dA = met[13*angIdx(b, y[1], y[2])];
+        if (P[I(F_THF, y[0], y[1], y[2])] <= 0.5)
+            return dA*((P[I(F_BWL+5+D, y[0], y[1], y[2])] + P[I(F_BWR+5+D, y[0], y[1], y[2])])*0.5);
+        int a[3] = {y[0], y[1], y[2]}, z[3] = {y[0], y[1], y[2]};
+        double val = 0;
+        if (D == 0)
+        {
+            a[t1] += 1; z[t1] -= 1;
+            const double dm = 0.5*met[13*angIdx(b, a[1], a[2])] - 0.5*met[13*angIdx(b, z[1], z[2])];
+            const double db = 0.5*bFace(a) - 0.5*bFace(z);
+            val += dm*db*(1.0/12.0);
+            a[t1] -= 1; z[t1] += 1;
+        }
+        a[t2] += 1; z[t2] -= 1;
+        const double dm = 0.5*met[13*angIdx(b, a[1], a[2])] - 0.5*met[13*angIdx(b, z[1], z[2])];
+        const double db = 0.5*bFace(a) - 0.5*bFace(z);
+        val += dm*db*(1.0/12.0);
+        val += dA*bFace(y);
+        return val;
+    };
+    double val = 0;
+    if (D != 0)
+    {
+        const int a[3] = {x[0]+1, x[1], x[2]}, z[3] = {x[0]-1, x[1], x[2]};
+        const double dadj = 0.5*areaOf(v, b, D, x[0]+1) - 0.5*areaOf(v, b, D, x[0]-1);
+        val += dadj*(0.5*detAB(a) - 0.5*detAB(z))*(1.0/12.0);
+    }
+    val += areaOf(v, b, D, x[0])*detAB(x);
+    P[I(F_GBB+D, x[0], x[1], x[2])] = val;
+}
+
+// 4c. Point values of the Powell coefficient from the point primitives on the
+// cells [lo-2, hi+2] (DivBPowell2 with a_cellAverages false).
+__global__ void __launch_bounds__(kB) k_b_pw_coef(O4View v, double* __restrict__ S)
+{
+    const BoxDesc& b = blockBoxDesc(v);
+    int L[3], H[3], x[3];
+    for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]-2; H[d] = b.hi[d]+2; }
+    if (!regionPoint(b, L, H, boxThread(v, b), x)) return;
+    const BoxIdx I(b);
+    powellCoefficient<false>(v, b, I, S + b.sOff, x);
+}
+
 template <int D, typename S_>
-void bertaDirectionPass(const O4View& v, dim3 grid, hipStream_t s, double* scratch)
+void bertaDirectionPass(const O4View& v, dim3 grid, hipStream_t s, const double* U, double* scratch)
 {
     if (v.roe) hipLaunchKernelGGL((k_b_faces<D, S_, true>), grid, dim3(kB), 0, s, v, scratch);
     else hipLaunchKernelGGL((k_b_faces<D, S_, false>), grid, dim3(kB), 0, s, v, scratch);
+    if (v.powell) hipLaunchKernelGGL(k_b_pw_face<D>, grid, dim3(kB), 0, s, v, U, scratch);
     hipLaunchKernelGGL(k_b_avg<D>, grid, dim3(kB), 0, s, v, scratch);
     if (v.roe) hipLaunchKernelGGL((k_o4_roe<D, S_>), grid, dim3(kB), 0, s, v, scratch);
     if constexpr (D == 0) hipLaunchKernelGGL(k_b_map<D>, grid, dim3(kB), 0, s, v, scratch);
@@ -561,7 +633,7 @@ void bertaFluxes(const O4View& v, dim3 grid, hipStream_t s, const double* U, dou
     hipLaunchKernelGGL(k_b_thraw, grid, dim3(kB), 0, s, v, scratch);
     hipLaunchKernelGGL(k_b_theta, grid, dim3(kB), 0, s, v, scratch);
     hipLaunchKernelGGL(k_b_wp, grid, dim3(kB), 0, s, v, U, scratch);
-    bertaDirectionPass<0, S_>(v, grid, s, scratch);
-    bertaDirectionPass<1, S_>(v, grid, s, scratch);
-    bertaDirectionPass<2, S_>(v, grid, s, scratch);
+    bertaDirectionPass<0, S_>(v, grid, s, U, scratch);
+    bertaDirectionPass<1, S_>(v, grid, s, U, scratch);
+    bertaDirectionPass<2, S_>(v, grid, s, U, scratch);
 }

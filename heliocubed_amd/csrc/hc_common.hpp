@@ -31,8 +31,6 @@ inline const char* checkOperatorOptions(const hc_options& o)
     }
     if (o.divergence_cleaning != 0 && o.divergence_cleaning != 1)
         return "divergence_cleaning must be 0 (off) or 1 (Powell source)";
-    if (o.divergence_cleaning == 1 && o.spatial_order == 4 && o.spatial_operator == 2)
-        return "the Powell source is built for the second-order and the standard fourth-order operators";
     return nullptr;
 }
 

@@ -50,7 +50,10 @@ enum Field : int
     // area products of <det A B_n> of the three directions, -J div B, and the
     // point values and cell averages of the coefficient components 1..7 (slot
     // + component); reuses the per-direction face slots 18..37
-    F_GB = 18, F_MJD = 21, F_CP = 22, F_CA = 30
+    F_GB = 18, F_MJD = 21, F_CP = 22, F_CA = 30,
+    // the Berta operator forms its Powell face products inside each direction
+    // pass, so they need slots that outlive the passes: three extra fields
+    F_GBB = F_COUNT, F_COUNT_BERTA_POWELL = F_COUNT + 3
 };
 static_assert(F_GB > F_W + 7 && F_CA + 7 < F_FLUX, "Powell slots overlap the primitives or the face fluxes");
 
@@ -1168,6 +1171,43 @@ __device__ double powellFaceProduct(const O4View& v, const BoxDesc& b, const Box
     return val;
 }
 
+// Point value of the coefficient (B/4pi, v, v.B/4pi) at cell x from the
+// primitives in slots F_W..: deconvolved W - Lap(W)/24 (standard operator,
+// cell averages) or taken as they are (Berta, point values).
+template <bool DECONV>
+__device__ void powellCoefficient(const O4View& v, const BoxDesc& b, const BoxIdx& I, double* P, const int x[3])
+{
+    double w[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+    {
+        const int f = F_W+c;
+        if (DECONV)
+        {
+            const double lap = P[I(f,x[0]+1,x[1],x[2])]+P[I(f,x[0]-1,x[1],x[2])]+P[I(f,x[0],x[1]+1,x[2])]
+                              +P[I(f,x[0],x[1]-1,x[2])]+P[I(f,x[0],x[1],x[2]+1)]+P[I(f,x[0],x[1],x[2]-1)]
+                              -6.0*P[I(f,x[0],x[1],x[2])];
+            w[c] = P[I(f,x[0],x[1],x[2])]-lap/24.0;
+        }
+        else w[c] = P[I(f,x[0],x[1],x[2])];
+    }
+    const double* A = v.cellA + 9*angIdx(b, x[1], x[2]);
+    double vel[3] = {0, 0, 0}, mag[3] = {0, 0, 0};
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) { vel[i] += A[3*i+j]*w[1+j]; mag[i] += A[3*i+j]*w[5+j]; }
+    double ce = 0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+    {
+        P[I(F_CP+1+i, x[0], x[1], x[2])] = mag[i]/(4.0*hc::kPi);
+        P[I(F_CP+5+i, x[0], x[1], x[2])] = vel[i];
+        ce += vel[i]*mag[i]/(4.0*hc::kPi);
+    }
+    P[I(F_CP+4, x[0], x[1], x[2])] = ce;
+}
+
 __global__ void __launch_bounds__(kB) k_o4_pw_face(O4View v, double* __restrict__ S)
 {
     const BoxDesc& b = blockBoxDesc(v);
@@ -1176,33 +1216,7 @@ __global__ void __launch_bounds__(kB) k_o4_pw_face(O4View v, double* __restrict_
     if (!regionPoint(b, L, H, boxThread(v, b), x)) return;
     const BoxIdx I(b);
     double* const P = S + b.sOff;
-    {
-        double w[8];
-#pragma unroll
-        for (int c = 0; c < 8; ++c)
-        {
-            const int f = F_W+c;
-            const double lap = P[I(f,x[0]+1,x[1],x[2])]+P[I(f,x[0]-1,x[1],x[2])]+P[I(f,x[0],x[1]+1,x[2])]
-                              +P[I(f,x[0],x[1]-1,x[2])]+P[I(f,x[0],x[1],x[2]+1)]+P[I(f,x[0],x[1],x[2]-1)]
-                              -6.0*P[I(f,x[0],x[1],x[2])];
-            w[c] = P[I(f,x[0],x[1],x[2])]-lap/24.0;
-        }
-        const double* A = v.cellA + 9*angIdx(b, x[1], x[2]);
-        double vel[3] = {0, 0, 0}, mag[3] = {0, 0, 0};
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-#pragma unroll
-            for (int j = 0; j < 3; ++j) { vel[i] += A[3*i+j]*w[1+j]; mag[i] += A[3*i+j]*w[5+j]; }
-        double ce = 0;
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-        {
-            P[I(F_CP+1+i, x[0], x[1], x[2])] = mag[i]/(4.0*hc::kPi);
-            P[I(F_CP+5+i, x[0], x[1], x[2])] = vel[i];
-            ce += vel[i]*mag[i]/(4.0*hc::kPi);
-        }
-        P[I(F_CP+4, x[0], x[1], x[2])] = ce;
-    }
+    powellCoefficient<true>(v, b, I, P, x);
     auto inFaces = [&](int d) {
         for (int a = 0; a < 3; ++a)
         {
@@ -1217,8 +1231,9 @@ __global__ void __launch_bounds__(kB) k_o4_pw_face(O4View v, double* __restrict_
 }
 
 // 6c. -J div B = sum_d -(1/dxi_d) (G_d(x+e_d) - G_d(x)) and the cell
-// averages C + Lap(C)/24 of the coefficient, on the cells [lo-1, hi+1].
-__global__ void __launch_bounds__(kB) k_o4_pw_cell(O4View v, double* __restrict__ S)
+// averages C + Lap(C)/24 of the coefficient, on the cells [lo-1, hi+1]; G_d
+// in slot gBase + d.
+__global__ void __launch_bounds__(kB) k_o4_pw_cell(O4View v, double* __restrict__ S, int gBase)
 {
     const BoxDesc& b = blockBoxDesc(v);
     int L[3], H[3], x[3];
@@ -1233,8 +1248,8 @@ __global__ void __launch_bounds__(kB) k_o4_pw_cell(O4View v, double* __restrict_
         const double dxInv = -1.0/(d == 0 ? 1.0/b.nr : 1.0/b.na);
         int a[3] = {x[0], x[1], x[2]}; a[d] += 1;
         double inc = 0;
-        inc += (-1.0*dxInv)*P[I(F_GB+d, x[0], x[1], x[2])];
-        inc += (1.0*dxInv)*P[I(F_GB+d, a[0], a[1], a[2])];
+        inc += (-1.0*dxInv)*P[I(gBase+d, x[0], x[1], x[2])];
+        inc += (1.0*dxInv)*P[I(gBase+d, a[0], a[1], a[2])];
         md += inc;
     }
     P[I(F_MJD, x[0], x[1], x[2])] = md;
@@ -1398,6 +1413,7 @@ void buildBoxop(hc_boxop& B, const hc_box* boxes, const int* boxLevel, int nbox)
     long long u = 0, r = 0, sc = 0, blk = 0;
     for (int d = 0; d < 3; ++d) B.fOffHost[d].clear();
     long long f[3] = {0, 0, 0};
+    const long long nfields = v.powell && v.op == 2 ? F_COUNT_BERTA_POWELL : F_COUNT;
     std::vector<int> blockBox;
     for (int n = 0; n < nbox; ++n)
     {
@@ -1410,11 +1426,11 @@ void buildBoxop(hc_boxop& B, const hc_box* boxes, const int* boxLevel, int nbox)
         bd.radOff = levRad[lev];
         bd.angOff = levAng[lev] + (long long)bd.p*bd.naE*bd.naE;
         const long long e = (long long)(bd.n[0]+2*kG)*(bd.n[1]+2*kG)*(bd.n[2]+2*kG);
-        if (F_COUNT*e > 0x7fffffffLL) throw std::runtime_error("box too large: scratch offsets exceed 32 bits");
+        if (nfields*e > 0x7fffffffLL) throw std::runtime_error("box too large: scratch offsets exceed 32 bits");
         bd.uOff = u; bd.rOff = r; bd.sOff = sc;
         bd.us[0] = bd.n[0]+2*kG; bd.us[1] = (long long)(bd.n[0]+2*kG)*(bd.n[1]+2*kG); bd.us[2] = e;
         bd.rs[0] = bd.n[0]; bd.rs[1] = (long long)bd.n[0]*bd.n[1]; bd.rs[2] = (long long)bd.n[0]*bd.n[1]*bd.n[2];
-        u += 8*e; r += 8LL*bd.n[0]*bd.n[1]*bd.n[2]; sc += F_COUNT*e;
+        u += 8*e; r += 8LL*bd.n[0]*bd.n[1]*bd.n[2]; sc += nfields*e;
         for (int d = 0; d < 3; ++d)
         {
             B.fOffHost[d].push_back(f[d]);
@@ -1478,8 +1494,9 @@ void applyBoxop(hc_boxop& B, const double* U, double* R, double* const F[3], hip
         hipLaunchKernelGGL(k_o4_seams, dim3((v.nseams + kB - 1)/kB), dim3(kB), 0, s, v, B.scratch);
     if (v.powell)
     {
-        hipLaunchKernelGGL(k_o4_pw_face, grid, dim3(kB), 0, s, v, B.scratch);
-        hipLaunchKernelGGL(k_o4_pw_cell, grid, dim3(kB), 0, s, v, B.scratch);
+        if (v.op == 2) hipLaunchKernelGGL(k_b_pw_coef, grid, dim3(kB), 0, s, v, B.scratch);
+        else hipLaunchKernelGGL(k_o4_pw_face, grid, dim3(kB), 0, s, v, B.scratch);
+        hipLaunchKernelGGL(k_o4_pw_cell, grid, dim3(kB), 0, s, v, B.scratch, v.op == 2 ? (int)F_GBB : (int)F_GB);
     }
     hipLaunchKernelGGL(k_o4_div, grid, dim3(kB), 0, s, v, B.scratch, R, overwrite, Uacc, b_dt);
     for (int d = 0; d < 3; ++d)

@@ -75,7 +75,9 @@ typedef struct hc_options
     int berta_fallback;     /* Berta fallback reconstruction: 0 WENO3, 1 PLM */
     int divergence_cleaning;/* 1: Powell source -J div(B) (B/4pi, v, v.B/4pi) added to the RHS
                                (src/BoxOp_SecondOrder.H:317-369 at order 2,
-                               src/BoxOp_DivBcleaning.H:31-168 at order 4); 0: off */
+                               src/BoxOp_DivBcleaning.H:31-168 at order 4, with
+                               src/BoxOp_BertaCubedSphereImplem.H:1911-1929,2071-2074
+                               for Berta); 0: off */
 } hc_options;
 
 /* A box of the shell: panel (block) 0..5 and inclusive cell range lo..hi in

@@ -101,15 +101,18 @@ R1_THIN = 60.0
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("riemann,config", [(2, 3), (1, 0)])
-def test_o4_powell_boxop_matches_oracle(device, riemann, config):
-    print(f"o4 powell boxop riemann={riemann} config={config}", flush=True)
+@pytest.mark.parametrize("op,riemann,config", [(1, 2, 3), (1, 1, 0), (2, 2, 4), (2, 1, 3), (2, 2, 0)])
+def test_o4_powell_boxop_matches_oracle(device, op, riemann, config):
+    # op 2 (Berta) with config 4: the low-beta cone flags troubled cells, so
+    # both branches of the flux-flag switch and the Roe repair are exercised
+    print(f"o4 powell boxop op={op} riemann={riemann} config={config}", flush=True)
     nr, na = 32, 16
     shell = hc.Shell(nr, na)
-    opts = hc.Options(spatial_order=4, limiter=1, riemann_solver=riemann, divergence_cleaning=1)
+    opts = hc.Options(spatial_order=4, limiter=1, riemann_solver=riemann, divergence_cleaning=1,
+                      spatial_operator=op)
     U = hc.init_native_boxes(shell, opts, O4_BOXES, config=config, seed=5, quad=3)
-    ref = orc.o4_rhs_boxes(nr, na, O4_BOXES, U, riemann=riemann, divergence_cleaning=1)
-    ref0 = orc.o4_rhs_boxes(nr, na, O4_BOXES, U, riemann=riemann, divergence_cleaning=0)
+    ref = orc.o4_rhs_boxes(nr, na, O4_BOXES, U, riemann=riemann, divergence_cleaning=1, spatial_operator=op)
+    ref0 = orc.o4_rhs_boxes(nr, na, O4_BOXES, U, riemann=riemann, divergence_cleaning=0, spatial_operator=op)
     assert np.abs(ref - ref0).max() > 0
     op = hc.BoxOp(shell, opts, O4_BOXES)
     R = torch.zeros(op.count(1), dtype=torch.float64, device=device)
@@ -122,14 +125,14 @@ def test_o4_powell_boxop_matches_oracle(device, riemann, config):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("config", [0, 1])
-def test_o4_powell_level_rhs_matches_oracle(device, config):
+@pytest.mark.parametrize("op,config", [(1, 0), (1, 1), (2, 4)])
+def test_o4_powell_level_rhs_matches_oracle(device, op, config):
     nr, na = 16, 16
-    print(f"o4 powell level rhs nr={nr} na={na} config={config}", flush=True)
+    print(f"o4 powell level rhs nr={nr} na={na} op={op} config={config}", flush=True)
     shell = hc.Shell(nr, na, 21.5, R1_THIN)
-    opts = hc.Options(spatial_order=4, limiter=1, riemann_solver=2, divergence_cleaning=1)
+    opts = hc.Options(spatial_order=4, limiter=1, riemann_solver=2, divergence_cleaning=1, spatial_operator=op)
     ju, inner = hc.init_level4(shell, opts, config, 3, 3)
-    O = orc.Oracle4(nr, na, r1=R1_THIN, divergence_cleaning=1)
+    O = orc.Oracle4(nr, na, r1=R1_THIN, divergence_cleaning=1, spatial_operator=op)
     W = O.prepare(ju, inner)
     ref = O.rhs(W)
     L = hc.Level(shell, opts)
