@@ -167,3 +167,26 @@ def test_o4_powell_level_rk4_matches_oracle(device):
     # increments of density, momentum and energy; the monopole field barely
     # changes, so its increment is rounding-sized and checked through the state
     assert rel_linf_per_var(got - ju, uref - ju)[:5].max() <= 1e-10
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("op,config", [(1, 1), (2, 4)])
+def test_o4_powell_mixed_precision_vs_fp64_oracle(device, op, config):
+    """Mixed precision with the Powell source (the source itself is FP64)
+    stays within 1e-6 of the FP64 oracle after one RK4 step."""
+    nr, na = 16, 16
+    print(f"o4 powell mixed op={op} config={config}", flush=True)
+    shell = hc.Shell(nr, na, 21.5, R1_THIN)
+    opts = hc.Options(spatial_order=4, limiter=1, riemann_solver=2, divergence_cleaning=1, spatial_operator=op,
+                      precision=1)
+    ju, inner = hc.init_level4(shell, opts, config, 3, 3)
+    O = orc.Oracle4(nr, na, r1=R1_THIN, divergence_cleaning=1, spatial_operator=op)
+    dt = 0.2 * (21.5 * np.pi / 2 / na) / 1.6
+    ref = O.rk4(ju, inner, dt, 1)
+    L = hc.Level(shell, opts)
+    U = _dev(ju)
+    L.rk4_advance(U, _dev(inner), dt, 1)
+    torch.cuda.synchronize()
+    err = rel_linf_per_var(U.cpu().numpy(), ref)
+    print("o4 powell mixed state err", err)
+    assert err.max() <= 1e-6, err
