@@ -252,8 +252,9 @@ class Level:
         self._require(U0, self.valid_shape, "U0")
         self._require(K, self.valid_shape, "K")
         self._require(W, self.stage_shape, "W")
-        g = self.ghost()
-        self._require(inner, (6, 8, self.shell.n_ang + 2 * g, self.shell.n_ang + 2 * g, g), "inner")
+        self._require(inner, self.inner_shape, "inner")
+        if inner is None:
+            raise ValueError("inner: the inner radial ghosts are required")
         _check(lib().hc_prepare_stage(self._handle, _ptr(U0), _ptr(K), a_dt, _ptr(inner), _ptr(W), self._stream()))
 
     def stage_rhs(self, W, K, U=None, b_dt: float = 0.0, fluxes=None):
@@ -285,10 +286,18 @@ class Level:
             out[name.value.decode()] = (ms.value, n.value)
             slot += 1
 
+    @property
+    def inner_shape(self):
+        s, g = self.shell, self.ghost()
+        return (6, 8, s.n_ang + 2 * g, s.n_ang + 2 * g, g)
+
     def rk4_advance(self, U, inner, dt: float, nsteps: int = 1):
         """MBLevelRK4::advance: nsteps classical RK4 steps in place on U
         (replayed from a captured hipGraph of one step unless use_graph(False))."""
         self._require(U, self.valid_shape, "U")
+        self._require(inner, self.inner_shape, "inner")
+        if inner is None:
+            raise ValueError("inner: the inner radial ghosts are required")
         _check(lib().hc_rk4_advance(self._handle, _ptr(U), _ptr(inner), dt, nsteps, self._stream()))
 
     def use_graph(self, enable: bool = True):
@@ -306,6 +315,9 @@ class Level:
         """Times one stage RHS of U for tiles 32, 64 and the whole panel and
         keeps the fastest; returns (tile, ms per stage). U is not modified."""
         self._require(U, self.valid_shape, "U")
+        self._require(inner, self.inner_shape, "inner")
+        if inner is None:
+            raise ValueError("inner: the inner radial ghosts are required")
         t, ms = ctypes.c_int(), ctypes.c_double()
         _check(lib().hc_level_autotune(self._handle, _ptr(U), _ptr(inner), ctypes.byref(t), ctypes.byref(ms),
                                        self._stream()))
@@ -503,12 +515,13 @@ class BoxOp:
     def apply(self, USph, rhs, fluxes=None):
         import torch
 
-        for t, what, name in ((USph, 0, "USph"), (rhs, 1, "rhs")):
-            if t.dtype != torch.float64 or not t.is_cuda or not t.is_contiguous() or t.numel() != self.count(what):
-                raise ValueError(f"{name}: need a contiguous float64 device tensor of {self.count(what)} elements")
         f = list(fluxes) if fluxes is not None else [None, None, None]
-        for d in range(3):
-            if f[d] is not None and (f[d].numel() != self.count(2 + d) or f[d].dtype != torch.float64):
-                raise ValueError(f"F{d}: need {self.count(2 + d)} float64 elements")
+        if len(f) != 3:
+            raise ValueError("fluxes: need three face-flux tensors (or None entries)")
+        checks = [(USph, 0, "USph"), (rhs, 1, "rhs")] + [(f[d], 2 + d, f"F{d}") for d in range(3) if f[d] is not None]
+        for t, what, name in checks:
+            if t is None or t.dtype != torch.float64 or not t.is_cuda or not t.is_contiguous() \
+                    or t.numel() != self.count(what):
+                raise ValueError(f"{name}: need a contiguous float64 device tensor of {self.count(what)} elements")
         stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
         _check(lib().hc_boxop_apply(self._handle, _ptr(USph), _ptr(rhs), _ptr(f[0]), _ptr(f[1]), _ptr(f[2]), stream))

@@ -1137,7 +1137,17 @@ int hc_rk4_advance(hc_level* level, double* U, const double* inner, double dt, i
             HC_CHECK(hipStreamSynchronize(L.capStream));
             (void)hipEventDestroy(ready);
             HC_CHECK(hipStreamBeginCapture(L.capStream, hipStreamCaptureModeThreadLocal));
-            step(L.capStream);
+            try { step(L.capStream); }
+            catch (...)
+            {
+                // end the capture so the stream is usable again, drop the
+                // partial graph, then report the error
+                hipGraph_t partial = nullptr;
+                (void)hipStreamEndCapture(L.capStream, &partial);
+                if (partial) (void)hipGraphDestroy(partial);
+                (void)hipGetLastError();
+                throw;
+            }
             HC_CHECK(hipStreamEndCapture(L.capStream, &L.graph));
             HC_CHECK(hipGraphInstantiate(&L.graphExec, L.graph, nullptr, nullptr, 0));
             L.gU = U; L.gInner = inner; L.gDt = dt;

@@ -1976,8 +1976,10 @@ int hc_boxop_create_levels(const hc_shell* shells, int nlevels, const hc_options
     for (int l = 0; l < nlevels; ++l)
     {
         const hc_shell& sh = shells[l];
-        if (sh.n_rad < 4 || sh.n_ang < 4 || !(sh.outer_radius > sh.inner_radius && sh.inner_radius > 0))
-            return hc::fail("hc_boxop_create: bad level shell (need n_rad, n_ang >= 4 and 0 < r0 < r1)");
+        // n_ang >= 16: the metric tables span the angular cells -5 .. n_ang+5,
+        // which must stay inside the panel's horizon (|xi - 1/2| < 1)
+        if (sh.n_rad < 4 || sh.n_ang < 16 || !(sh.outer_radius > sh.inner_radius && sh.inner_radius > 0))
+            return hc::fail("hc_boxop_create: bad level shell (need n_rad >= 4, n_ang >= 16 and 0 < r0 < r1)");
     }
     for (int n = 0; n < nbox; ++n)
     {

@@ -206,9 +206,10 @@ int hc_level_inner_ghosts_outflow(hc_level* level, double strength, double time,
 int hc_level_valid_state(hc_level* level, const double* U, int* valid, long long* first_invalid,
                          hc_stream stream);
 
-/* Fourth-order box operator (spatial_order 4, standard operator, limiter 0
- * or 1, riemann_solver 2) over a table of boxes, evaluated for every box in
- * the same launches. Replaces BoxOp_EulerCubedSphere::operator()
+/* Fourth-order box operator (spatial_order 4; the standard operator with
+ * limiter 0 or 1, or the Berta WENOZ operator, spatial_operator 2; Roe 8-wave,
+ * riemann_solver 2, or Rusanov, 1; optional Powell source) over a table of
+ * boxes, evaluated for every box in the same launches. Replaces BoxOp_EulerCubedSphere::operator()
  * (src/BoxOp_EulerCubedSphere.H:412-421, src/BoxOp_EulerCubedSphereImplem.H:4-402)
  * as called per box by StageLevel (:955-1012).
  * Arrays concatenate the boxes in table order, each box component-major
@@ -222,7 +223,9 @@ int hc_level_valid_state(hc_level* level, const double* U, int* valid, long long
  * hc_boxop_count returns the element count of USph (what 0), rhs (1) and
  * F_0..F_2 (2..4). Every box must lie inside its panel and span at least 4
  * cells in each direction (the width of the limiter's one-sided stencils);
- * an empty table is rejected. */
+ * every level needs n_ang >= 16 (the five angular ghost layers of a box at a
+ * panel edge must stay inside the panel's horizon, where the metric tables
+ * are defined); an empty table is rejected. */
 int hc_boxop_create(const hc_shell* shell, const hc_options* options, const hc_box* boxes, int nbox,
                     hc_boxop** out);
 /* Box table over several refinement levels of an AMR hierarchy (the batched

@@ -95,3 +95,15 @@ def test_empty_box_table_is_rejected_without_device():
     tab = hc.box_table([(0, (0, 0, 0), (7, 7, 7))])
     assert hc.lib().hc_boxop_create(ctypes.byref(sh), ctypes.byref(op), tab, 0, ctypes.byref(h)) != 0
     assert b"bad argument" in hc.lib().hc_last_error()
+
+
+def test_box_table_rejects_small_angular_levels_without_device():
+    """The metric tables of a box table span the angular cells -5 .. n_ang+5,
+    which leave the panel's horizon below n_ang = 16: such levels are rejected."""
+    h = ctypes.c_void_p()
+    op = hc.Options(spatial_order=4, limiter=1, riemann_solver=2).c()
+    tab = hc.box_table([(0, (0, 0, 0), (7, 7, 7))])
+    for na in (8, 12, 15):
+        sh = hc.Shell(32, na).c()
+        assert hc.lib().hc_boxop_create(ctypes.byref(sh), ctypes.byref(op), tab, 1, ctypes.byref(h)) != 0
+        assert b"n_ang >= 16" in hc.lib().hc_last_error()

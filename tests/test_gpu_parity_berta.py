@@ -113,6 +113,7 @@ def test_berta_mixed_precision_error(device):
     err = rel_linf_per_var(got, ref)
     errd = rel_linf_per_var(got - ju, ref - ju)
     print("berta mixed state err", err, "increment err", errd)
+    assert errd.max() <= 1e-6, errd
     assert err.max() <= 1e-6, err
 
 
@@ -132,6 +133,9 @@ def test_mixed_precision_low_beta_cone(device, operator):
     U = _dev(ju)
     L.rk4_advance(U, _dev(inner), dt, 1)
     torch.cuda.synchronize()
-    err = rel_linf_per_var(U.cpu().numpy(), ref)
-    print("mixed low-beta state err", operator, err)
+    got = U.cpu().numpy()
+    err = rel_linf_per_var(got, ref)
+    errd = rel_linf_per_var(got - ju, ref - ju)
+    print("mixed low-beta state err", operator, err, "increment err", errd)
+    assert errd.max() <= 1e-6, errd
     assert err.max() <= 1e-6, err

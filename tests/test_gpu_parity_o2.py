@@ -145,4 +145,5 @@ def test_mixed_precision_error_vs_fp64_oracle(device, riemann):
     err = rel_linf_per_var(got, ref)
     errd = rel_linf_per_var(got - ju, ref - ju)
     print("mixed riemann", riemann, "state err", err, "increment err", errd)
+    assert errd.max() <= 1e-6, errd
     assert err.max() <= 1e-6, err

@@ -89,6 +89,7 @@ def test_o4_mixed_precision_error(device):
     err = rel_linf_per_var(got, ref)
     errd = rel_linf_per_var(got - ju, ref - ju)
     print("o4 mixed state err", err, "increment err", errd)
+    assert errd.max() <= 1e-6, errd
     assert err.max() <= 1e-6, err
 
 

@@ -187,6 +187,9 @@ def test_o4_powell_mixed_precision_vs_fp64_oracle(device, op, config):
     U = _dev(ju)
     L.rk4_advance(U, _dev(inner), dt, 1)
     torch.cuda.synchronize()
-    err = rel_linf_per_var(U.cpu().numpy(), ref)
-    print("o4 powell mixed state err", err)
+    got = U.cpu().numpy()
+    err = rel_linf_per_var(got, ref)
+    errd = rel_linf_per_var(got - ju, ref - ju)
+    print("o4 powell mixed state err", err, "increment err", errd)
+    assert errd.max() <= 1e-6, errd
     assert err.max() <= 1e-6, err
