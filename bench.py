@@ -3,8 +3,21 @@
 
 One step = one classical RK4 step (4 stage preparations + 4 RHS evaluations
 + the stage accumulate) of the whole six-panel shell, inputs resident in HBM.
-Prints one JSON line. The CPU baseline is the FP64 oracle (oracle/) timed on
-this host's cores on one RK4 step of the same grid.
+The default workload is the 25M-cell shell (6 x 256 x 128 x 128, Berta WENOZ +
+Roe 8-wave): the batched mixed-precision level is `value`, the plain FP64 level
+is reported beside it, both timed from the captured hipGraph of one step (no
+per-kernel events inside the timed region), with
+
+* the per-variable relative L-infinity / L2 error of one mixed-precision stage
+  RHS against the FP64 one over the whole shell, and against the FP64 CPU
+  oracle on a sample of interior boxes;
+* the roofline of the stage RHS (a separate pass with per-kernel HIP events);
+* the reference's calling pattern (one BoxOp call per box, FP64) on a sample
+  of boxes, scaled to the whole table;
+* the CPU baseline: the FP64 oracle's box operator on a sample of the same
+  boxes, one box per OpenMP thread at a time.
+
+Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -18,20 +31,21 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFS = 78.6
+METRIC = "cell-updates/s per RK stage, mixed vs FP64 at stated L-inf error; % HBM peak"
 
-# Workloads: grid, operator options, and the algorithmic bytes per unit of
-# the kernels that can dominate (unit = face for the face-flux kernels, cell
-# for the others).
+# Workloads: grid, operator options and the seeded state (config numbers of
+# hc_init_level4 / hc_init_state).
 WORKLOADS = {
-    # 6 panels x 128 radial x 64x64 angular, fourth-order standard operator +
-    # Roe 8-wave, radial wind with density noise and Alfvenic perturbations
+    # 6 x 256 radial x 128x128 angular (25.2M cells, 768 boxes of 32^3), Berta
+    # WENOZ + Roe; four-sector B_r with a magnetized blob moving at v_r = 2
+    "shell-6x256x128x128-berta-roe": dict(n_rad=256, n_ang=128, limiter=1, riemann=2, order=4, config=5, op=2),
+    # 6 x 128 radial x 64x64 angular, fourth-order standard operator + Roe,
+    # radial wind with density noise and Alfvenic perturbations
     "shell-6x128x64x64-o4-roe": dict(n_rad=128, n_ang=64, limiter=1, riemann=2, order=4, config=3),
     # same grid, Berta WENOZ (Jameson detector, WENO3 fallback) + Roe, with a
     # seeded low-beta cone (p = 1e-4 |B|^2/2 within 15 degrees of an axis)
     "shell-6x128x64x64-berta-roe": dict(n_rad=128, n_ang=64, limiter=1, riemann=2, order=4, config=4, op=2),
-    # 6 panels x 256 radial x 128x128 angular (25.2M cells, 768 boxes of 32^3),
-    # Berta WENOZ + Roe; four-sector B_r with a magnetized blob moving at v_r = 2
-    "shell-6x256x128x128-berta-roe": dict(n_rad=256, n_ang=128, limiter=1, riemann=2, order=4, config=5, op=2),
     "shell-6x128x64x64-o2-minmod-roe": dict(n_rad=128, n_ang=64, limiter=1, riemann=2, order=2, config=0),
     "shell-6x32x32x32-o2-minmod-rusanov": dict(n_rad=32, n_ang=32, limiter=1, riemann=1, order=2, config=0),
     # three-level AMR box table in one launch per kernel: base 6 x 64 x 48x48 in
@@ -39,6 +53,13 @@ WORKLOADS = {
     # 8^3 boxes at a shock front inside the wedge; 652 boxes, RHS of every box
     "amr-3level-652boxes-o4-roe": dict(n_rad=64, n_ang=48, limiter=1, riemann=2, order=4, config=3, amr=True),
 }
+DEFAULT_WORKLOAD = "shell-6x256x128x128-berta-roe"
+
+# Algorithmic HBM bytes per cell-update of one RK stage (8 FP64 components =
+# 64 B): read U_stage + read U_n + read-modify-write the RK accumulator +
+# write the next stage = 320 B; the RHS alone reads the state and writes K.
+STAGE_BYTES = 320
+RHS_BYTES = 128
 
 
 def amr_table():
@@ -62,26 +83,11 @@ def amr_table():
             for i in range(96, 160, 8):
                 table.append((2, (1, (i, j, k), (i + 7, j + 7, k + 7))))
     return levels, table
-# Algorithmic HBM bytes per unit for each kernel (8 FP64 components = 64 B):
-#   face flux: read the 8-component cell state once + write the 8-component face flux;
-#   divergence: read 3 face fluxes + write K + read-modify-write U;
-#   prepare: read U0 and K, write W.
-ALGO_BYTES = {
-    # fourth-order stage: read U_stage (as JU and U0+K), read-modify-write U, write the next stage
-    "k4_rhs": 320, "k4_prepare": 320,
-    # fused RHS: read the 8-component stage state, write K, read-modify-write U
-    "k_rhs_fused": 64 + 64 + 128,
-    "k_face_flux_0": 128, "k_face_flux_1": 128, "k_face_flux_2": 128,
-    "k_divergence": 64 * 3 + 64 + 128,
-    "k_prepare_valid": 64 * 3,
-}
 
 
 def pmc_summary(kernel, workload):
     """HBM traffic (bytes) and FP64 work (flop) per launch of `kernel` on
-    `workload` from the committed rocprofv3 PMC summary (FETCH_SIZE and
-    WRITE_SIZE scaled by the copy-kernel calibration, FP64 VALU instruction
-    counts), or (None, None) when no summary exists for the pair."""
+    `workload` from the committed rocprofv3 PMC summary, or Nones."""
     import re
     path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_pmc.txt")
     if not os.path.exists(path):
@@ -93,38 +99,304 @@ def pmc_summary(kernel, workload):
         os.path.relpath(path, ROOT)
 
 
-def units_per_launch(name, nr, na):
-    if name == "k_face_flux_0":
-        return 6 * (nr + 1) * na * na
-    if name in ("k_face_flux_1", "k_face_flux_2"):
-        return 6 * nr * (na + 1) * na
-    return 6 * nr * na * na
+def algorithmic_flops(workload):
+    """Algorithmic FP64 flops per cell-update of one stage RHS from the
+    committed op count of the oracle (profiles/opcount_<workload>.txt), or None."""
+    import re
+    path = os.path.join(ROOT, "profiles", f"opcount_{workload}.txt")
+    if not os.path.exists(path):
+        return None, None
+    m = re.search(r"flop per cell-stage: ([0-9.]+)", open(path).read())
+    return (float(m.group(1)) if m else None), os.path.relpath(path, ROOT)
 
 
-def cpu_baseline(wl, ju, inner, dt):
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
-    threads = min(threads, 16)
+def host_threads():
+    """Threads for the CPU leg: OMP_NUM_THREADS (the box's CPU share for one
+    GPU) if set, else the cores this process may run on."""
+    env = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return (env or avail), avail
+
+
+def rel_err(test, ref):
+    """Per-variable relative L-infinity and L2 error of two [6][8][..] level
+    arrays (torch tensors on the device)."""
+    d = (test - ref).abs()
+    dims = tuple(x for x in range(ref.dim()) if x != 1)
+    linf = (d.amax(dim=dims) / ref.abs().amax(dim=dims).clamp_min(1e-300)).cpu().tolist()
+    l2 = (d.square().sum(dim=dims).sqrt() / ref.square().sum(dim=dims).sqrt().clamp_min(1e-300)).cpu().tolist()
+    return linf, l2
+
+
+def timed_steps(L, U, G, dt, steps, warmup):
+    """Warmup steps, then `steps` RK4 steps replayed from the level's captured
+    hipGraph, bracketed by device synchronisation; returns seconds."""
+    import torch
+    for _ in range(warmup):
+        L.rk4_advance(U, G, dt, 1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    L.rk4_advance(U, G, dt, steps)
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+
+def profiled(L, U, G, dt, steps):
+    """Per-kernel-group milliseconds per step from HIP events on the launch
+    stream (a separate pass: the events bypass the graph)."""
+    import torch
+    L.profile(True)
+    L.rk4_advance(U, G, dt, steps)
+    torch.cuda.synchronize()
+    prof = L.profile_read()
+    L.profile(False)
+    return prof
+
+
+def interior_boxes(wl, count):
+    """`count` 32^3 boxes of the level's box table that touch no panel edge
+    (their RHS needs no seam reconciliation), spread over panels and radii."""
+    nr, na = wl["n_rad"], wl["n_ang"]
+    out = []
+    for p in range(6):
+        for k in range(32, na - 32, 32):
+            for j in range(32, na - 32, 32):
+                for i in range(0, nr, 32):
+                    out.append((p, (i, j, k), (i + 31, j + 31, k + 31)))
+    if not out:
+        return []
+    step = max(1, len(out) // count)
+    return out[::step][:count]
+
+
+def box_windows(W, boxes):
+    """Ghost-5 windows of the stage level W[6][8][na+10][na+10][nr+10] for
+    each box, concatenated in the hc_boxop USph layout (device tensor)."""
+    import torch
+    parts = []
+    for p, lo, hi in boxes:
+        parts.append(W[p, :, lo[2]:hi[2] + 11, lo[1]:hi[1] + 11, lo[0]:hi[0] + 11].reshape(-1))
+    return torch.cat(parts)
+
+
+def box_valid(K, boxes):
+    import torch
+    return torch.cat([K[p, :, lo[2]:hi[2] + 1, lo[1]:hi[1] + 1, lo[0]:hi[0] + 1].reshape(-1)
+                      for p, lo, hi in boxes])
+
+
+def per_var(flat, boxes):
+    import numpy as np
+    out, off = [[] for _ in range(8)], 0
+    for _, lo, hi in boxes:
+        n = int(np.prod([hi[d] - lo[d] + 1 for d in range(3)]))
+        blk = flat[off:off + 8 * n].reshape(8, n)
+        for c in range(8):
+            out[c].append(blk[c])
+        off += 8 * n
+    return np.array([np.concatenate(o) for o in out])
+
+
+def cpu_baseline(wl, boxes, Ub, reps=5, warm=3):
+    """The FP64 oracle's box operator on `boxes` (one box per OpenMP thread
+    at a time): `warm` untimed runs, then the median of `reps`. Returns the
+    baseline record and the oracle RHS of the boxes."""
+    import numpy as np
+    threads, avail = host_threads()
     os.environ["OMP_NUM_THREADS"] = str(threads)
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle as orc  # the oracle is the timed CPU baseline only
+    import oracle as orc  # the oracle is the timed CPU baseline and the checker only
 
-    cells = 6 * wl["n_rad"] * wl["n_ang"] ** 2
-    if wl["order"] == 4:
-        O = orc.Oracle4(wl["n_rad"], wl["n_ang"], limiter=wl["limiter"], spatial_operator=wl.get("op", 1))
+    kw = dict(limiter=wl["limiter"], spatial_operator=wl.get("op", 1), riemann=wl["riemann"])
+    ref = None
+    times = []
+    for r in range(warm + reps):
         t0 = time.perf_counter()
-        O.rhs(O.prepare(ju, inner))
-        t = time.perf_counter() - t0
-        return {"value": cells / t, "unit": "cell-updates/s per RK stage", "cores": threads, "kind": "port",
-                "sample": f"1 RK stage (prepare + RHS) of the same shell on the CPU FP64 oracle, "
-                          f"{threads} OpenMP threads, {t:.2f} s"}
-    O = orc.Oracle(wl["n_rad"], wl["n_ang"], limiter=wl["limiter"], riemann=wl["riemann"])
-    O.rk4(ju, inner, dt, 1)  # first call builds the OpenMP pool and caches
-    t0 = time.perf_counter()
-    O.rk4(ju, inner, dt, 1)
-    t = time.perf_counter() - t0
-    return {"value": cells * 4 / t, "unit": "cell-updates/s per RK stage", "cores": threads,
-            "kind": "port",
-            "sample": f"1 RK4 step of the same shell on the CPU FP64 oracle, {threads} OpenMP threads, {t:.2f} s"}
+        ref = orc.o4_rhs_table(wl["n_rad"], wl["n_ang"], boxes, Ub, **kw)
+        if r >= warm:
+            times.append(time.perf_counter() - t0)
+    t = float(np.median(times))
+    cells = sum((hi[0] - lo[0] + 1) * (hi[1] - lo[1] + 1) * (hi[2] - lo[2] + 1) for _, lo, hi in boxes)
+    rec = {"value": cells / t, "unit": "cell-updates/s per RK stage (stage RHS)", "cores": threads,
+           "kind": "port",
+           "sample": f"stage RHS of {len(boxes)} interior 32^3 boxes of the same shell and state on the CPU FP64 "
+                     f"oracle (box operator, one box per OpenMP thread, {threads} threads of {avail} available, "
+                     f"{os.cpu_count()} online); median of {reps} runs after {warm} warm-ups, {t:.2f} s per run; "
+                     f"the stage preparation is not in the sample"}
+    return rec, ref
+
+
+def plain_per_box(hc, shell, opts, W, boxes, nbox_total):
+    """The reference's calling pattern: one BoxOp evaluation per box
+    (operator() per box in StageLevel), FP64, each box its own table; the
+    per-box time of the sample times the table's box count."""
+    import torch
+    ops = [hc.BoxOp(shell, opts, [b]) for b in boxes]
+    ins = [box_windows(W, [b]).contiguous() for b in boxes]
+    outs = [torch.zeros(op.count(1), dtype=torch.float64, device=W.device) for op in ops]
+    for op, u, r in zip(ops, ins, outs):
+        op.apply(u, r)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for op, u, r in zip(ops, ins, outs):
+        op.apply(u, r)
+    e1.record()
+    torch.cuda.synchronize()
+    per_box_ms = e0.elapsed_time(e1) / len(boxes)
+    del ops, ins, outs
+    return per_box_ms, per_box_ms * nbox_total
+
+
+def bench_level(args, wl):
+    import dataclasses
+    import numpy as np
+    import torch
+    import heliocubed_amd as hc
+
+    shell = hc.Shell(wl["n_rad"], wl["n_ang"])
+    base = hc.Options(spatial_order=wl["order"], limiter=wl["limiter"], riemann_solver=wl["riemann"],
+                      spatial_operator=wl.get("op", 1))
+    o64, omix = dataclasses.replace(base, precision=0), dataclasses.replace(base, precision=1)
+    if wl["order"] == 4:
+        # cell averages by a 2^3 Gauss rule (midpoint values on the 25M-cell
+        # shell, where the host-side averaging would dominate the run)
+        ju, inner = hc.init_level4(shell, base, wl["config"], 20261203, 1 if wl["n_rad"] > 128 else 2)
+    else:
+        ju, inner = hc.init_state(shell, base, wl["config"], 20261203)
+    dt = 0.25 * (shell.inner_radius * np.pi / 2 / shell.n_ang) / 1.6
+    dev = torch.device("cuda", 0)
+    U0 = torch.from_numpy(ju).to(dev)
+    G = torch.from_numpy(inner).to(dev)
+    cells = 6 * wl["n_rad"] * wl["n_ang"] ** 2
+    precisions = ["fp64", "mixed"] if args.precision == "both" else [args.precision]
+
+    res = {}
+    for prec in precisions:
+        L = hc.Level(shell, o64 if prec == "fp64" else omix)
+        if args.tile:
+            L.set_tile(args.tile)
+        U = U0.clone()
+        t = timed_steps(L, U, G, dt, args.steps, args.warmup)
+        if not torch.isfinite(U).all():
+            raise RuntimeError(f"non-finite state after the timed steps ({prec})")
+        res[prec] = {"L": L, "elapsed": t, "value": cells * 4 * args.steps / t,
+                     "ms_per_step": t / args.steps * 1e3}
+        del U
+    head = "mixed" if "mixed" in res else precisions[0]
+    value, elapsed = res[head]["value"], res[head]["elapsed"]
+
+    line = {
+        "metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if head == "fp64" else (
+            "f64 state, f32 reconstruction weights and Riemann eigen-system" if wl.get("op", 1) == 2
+            else "f64 state, f32 slopes / Riemann eigen-system"),
+        "data": ("synthetic seeded radial-wind shell with four-sector B_r and a magnetized blob"
+                 if wl["config"] == 5 else
+                 "synthetic seeded radial-wind shell with a Gaussian pressure pulse"
+                 + (", lognormal density noise and Alfvenic perturbations" if wl["config"] in (3, 4) else "")
+                 + (", low-beta cone" if wl["config"] == 4 else "")),
+        "config": {"workload": args.workload, "cells": cells, "panels": 6, "n_rad": wl["n_rad"],
+                   "n_ang": wl["n_ang"], "spatial_order": wl["order"],
+                   "operator": "berta-wenoz" if wl.get("op", 1) == 2 else ("standard" if wl["order"] == 4 else "plm"),
+                   "limiter": wl["limiter"], "riemann_solver": wl["riemann"], "precision": head,
+                   "boxes": (f"{res[head]['L'].tile}^3 box table" if wl["order"] == 4 else "panel arrays")},
+    }
+    for prec in precisions:
+        line[f"{prec}_batched"] = {"value": res[prec]["value"], "ms_per_step": res[prec]["ms_per_step"]}
+    if len(res) == 2:
+        line["mixed_speedup_vs_fp64_batched"] = res["mixed"]["value"] / res["fp64"]["value"]
+
+    if args.only_timed:
+        print(json.dumps(line), flush=True)
+        return
+
+    # per-kernel timing of the headline precision and of FP64 (separate
+    # passes: events on the launch stream around every launch group)
+    nprof = min(args.steps, 5)
+    kernels = {}
+    for prec in precisions:
+        U = U0.clone()
+        prof = profiled(res[prec]["L"], U, G, dt, nprof)
+        kernels[prec] = {k: v[0] / nprof for k, v in prof.items()}
+        res[prec]["prof"] = prof
+        del U
+    line["kernels_ms_per_step"] = kernels
+
+    dominant = "k4_rhs" if wl["order"] == 4 else "k_rhs_fused"
+    rhs_ms = res[head]["prof"][dominant][0] / res[head]["prof"][dominant][1]
+    algo_bytes = STAGE_BYTES * cells
+    achieved = algo_bytes / (rhs_ms / 1e3) / 1e9
+    traffic, pmc_flop, pmc_path = pmc_summary(dominant, args.workload)
+    fpc, fpc_path = algorithmic_flops(args.workload)
+    roof = {"bound": "hbm", "kernel": f"{dominant} (every launch of one stage RHS, {head})",
+            "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic, "avg_launch_us": rhs_ms * 1e3,
+            "algorithmic_bytes_per_launch": algo_bytes}
+    if traffic is not None:
+        roof["traffic_source"] = pmc_path
+    if fpc is not None:
+        # the fourth-order operators are FP64-VALU-bound: algorithmic flops of
+        # the oracle's op count over the measured stage RHS time
+        tf = fpc * cells / (rhs_ms / 1e3) / 1e12
+        line["roofline"] = {"bound": "fp64-valu", "kernel": roof["kernel"], "achieved": tf, "peak": FP64_PEAK_TFS,
+                            "unit": "TFLOP/s", "frac": tf / FP64_PEAK_TFS, "traffic": traffic,
+                            "avg_launch_us": rhs_ms * 1e3, "flop_per_cell_stage": fpc, "flop_source": fpc_path,
+                            "hbm": {k: roof[k] for k in ("achieved", "peak", "unit", "frac")}}
+    else:
+        line["roofline"] = roof
+    if pmc_flop is not None:
+        line["valu_fp64_counters"] = {"achieved": pmc_flop / (rhs_ms / 1e3) / 1e12, "peak": FP64_PEAK_TFS,
+                                      "unit": "TFLOP/s", "flop_per_launch": pmc_flop, "source": pmc_path}
+
+    # one stage RHS from the same prepared stage input in both precisions
+    if wl["order"] == 4 and len(res) == 2:
+        L64, Lm = res["fp64"]["L"], res["mixed"]["L"]
+        W = torch.zeros(L64.stage_shape, dtype=torch.float64, device=dev)
+        L64.prepare_stage(U0, G, W)
+        K64 = torch.zeros(L64.valid_shape, dtype=torch.float64, device=dev)
+        Km = torch.zeros_like(K64)
+        L64.stage_rhs(W, K64)
+        Lm.stage_rhs(W, Km)
+        torch.cuda.synchronize()
+        linf, l2 = rel_err(Km, K64)
+        line["mixed_rhs_error_vs_fp64_gpu"] = {"rel_linf": linf, "rel_l2": l2, "max_rel_linf": max(linf),
+                                               "cells": cells}
+        sample = interior_boxes(wl, args.cpu_boxes or 2 * host_threads()[0])
+        if sample and not args.no_cpu_baseline:
+            Ub = box_windows(W, sample).cpu().numpy()
+            rec, kref = cpu_baseline(wl, sample, Ub)
+            line["cpu_baseline"] = rec
+            kref = per_var(kref, sample)
+            from_gpu = {p: per_var(box_valid(K, sample).cpu().numpy(), sample) for p, K in (("fp64", K64),
+                                                                                           ("mixed", Km))}
+            scale = np.abs(kref).max(axis=1)
+            scale[scale == 0] = 1.0
+            line["rhs_error_vs_oracle"] = {
+                "sample": f"{len(sample)} interior 32^3 boxes (the CPU baseline's sample)",
+                **{f"{p}_rel_linf": (np.abs(from_gpu[p] - kref).max(axis=1) / scale).tolist() for p in from_gpu}}
+        if not args.no_plain:
+            nbox = len(hc.shell_boxes(shell))
+            psample = interior_boxes(wl, 8)
+            if psample:
+                per_box, total = plain_per_box(hc, shell, o64, W, psample, nbox)
+                b64 = res["fp64"]["prof"]["k4_rhs"]
+                batched64 = b64[0] / b64[1]
+                bm = res["mixed"]["prof"]["k4_rhs"]
+                batched_m = bm[0] / bm[1]
+                line["plain_per_box_fp64"] = {
+                    "rhs_ms_per_stage": total, "per_box_ms": per_box,
+                    "sample": f"{len(psample)} boxes, one hc_boxop_apply per box (each box its own table), "
+                              f"scaled to {nbox} boxes",
+                    "batched_fp64_rhs_ms_per_stage": batched64, "batched_mixed_rhs_ms_per_stage": batched_m,
+                    "batched_mixed_rhs_speedup_vs_plain_fp64": total / batched_m}
+        del W, K64, Km
+    print(json.dumps(line), flush=True)
 
 
 def bench_amr(args, wl):
@@ -137,8 +409,9 @@ def bench_amr(args, wl):
 
     levels, table = amr_table()
     shells = [hc.Shell(nr, na) for nr, na in levels]
+    prec = "mixed" if args.precision == "mixed" else "fp64"
     opts = hc.Options(spatial_order=4, limiter=wl["limiter"], riemann_solver=wl["riemann"],
-                      precision=1 if args.precision == "mixed" else 0)
+                      precision=1 if prec == "mixed" else 0)
     boxes = [b for _, b in table]
     lev = [l for l, _ in table]
     U = np.concatenate([hc.init_native_boxes(shells[l], opts, [b], config=wl["config"], seed=20261203, quad=1)
@@ -166,8 +439,7 @@ def bench_amr(args, wl):
     if not args.only_timed:
         # the same evaluation captured once in a hipGraph (torch's graph
         # capture of the library's launches on the current stream) and
-        # replayed, outside the timed region above; one replay is checked
-        # against an eager evaluation bit for bit
+        # replayed; one replay is checked against an eager evaluation bit for bit
         Rg = torch.zeros_like(R)
         graph = torch.cuda.CUDAGraph()
         side = torch.cuda.Stream()
@@ -193,18 +465,18 @@ def bench_amr(args, wl):
         del graph, Rg, Re
     cells = sum((hi[0] - lo[0] + 1) * (hi[1] - lo[1] + 1) * (hi[2] - lo[2] + 1) for _, lo, hi in boxes)
     value = cells * args.steps / elapsed
-    algo = 128 * cells   # RHS only: read the 8-component state, write the 8-component RHS
+    algo = RHS_BYTES * cells
     achieved = algo / (ev_ms / 1e3) / 1e9
     line = {
-        "metric": f"cell-updates/s per RK stage ({args.precision})", "value": value, "unit": "cell-updates/s",
-        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3,
+        "metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f64 state, f32 Riemann dissipation",
+        "dtype": "f64" if prec == "fp64" else "f64 state, f32 Riemann eigen-system",
         "data": "synthetic seeded radial wind with density noise and Alfvenic perturbations on every box",
         "config": {"workload": args.workload, "cells": cells, "boxes": len(boxes),
                    "levels": [list(x) for x in levels], "box_sizes": "32x16x16, 16^3, 8^3",
                    "operator": "standard", "limiter": wl["limiter"], "riemann_solver": wl["riemann"],
-                   "precision": args.precision},
+                   "precision": prec},
         "roofline": {"bound": "hbm", "kernel": "box-table RHS (all launches of one evaluation)",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": None, "avg_launch_us": ev_ms * 1e3},
@@ -218,199 +490,25 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="shell-6x128x64x64-o4-roe", choices=sorted(WORKLOADS))
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--workload", default=DEFAULT_WORKLOAD, choices=sorted(WORKLOADS))
+    ap.add_argument("--precision", default="both", choices=["both", "fp64", "mixed"],
+                    help="both: mixed is `value`, FP64 is reported beside it")
+    ap.add_argument("--tile", type=int, default=0, help="box-table tile edge of the level (0: the default 32)")
+    ap.add_argument("--cpu-boxes", type=int, default=0, help="boxes in the CPU sample (0: two per thread)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain", action="store_true", help="skip the one-call-per-box FP64 measurement")
     ap.add_argument("--only-timed", action="store_true",
-                    help="run the warmup and timed steps only (no graph replay, autotuner, CFL or mixed-error "
-                         "runs and no CPU baseline): the command profiled under rocprofv3")
+                    help="run the warmup and timed steps only: the command profiled under rocprofv3")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               "(the operator shards by boxes across GPUs, which this build does not run)", file=sys.stderr)
         sys.exit(2)
-
-    import numpy as np
-    import torch
-    import heliocubed_amd as hc
-
     wl = WORKLOADS[args.workload]
     if wl.get("amr"):
         bench_amr(args, wl)
-        return
-    shell = hc.Shell(wl["n_rad"], wl["n_ang"])
-    opts = hc.Options(spatial_order=wl["order"], limiter=wl["limiter"], riemann_solver=wl["riemann"],
-                      precision=1 if args.precision == "mixed" else 0, spatial_operator=wl.get("op", 1))
-    if wl["order"] == 4:
-        # cell averages by a 2^3 Gauss rule (midpoint values on the 25M-cell shell,
-        # where the host-side averaging would dominate the run)
-        ju, inner = hc.init_level4(shell, opts, wl["config"], 20261203, 1 if wl["n_rad"] > 128 else 2)
     else:
-        ju, inner = hc.init_state(shell, opts, wl["config"], 20261203)
-    dt = 0.25 * (shell.inner_radius * np.pi / 2 / shell.n_ang) / 1.6
-    dev = torch.device("cuda", 0)
-    U = torch.from_numpy(ju).to(dev)
-    G = torch.from_numpy(inner).to(dev)
-    L = hc.Level(shell, opts)
-    U_init = U.clone() if args.precision == "mixed" else None
-
-    for _ in range(args.warmup):
-        L.rk4_advance(U, G, dt, 1)
-    torch.cuda.synchronize()
-    L.profile(True)
-    t0 = time.perf_counter()
-    L.rk4_advance(U, G, dt, args.steps)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    prof = L.profile_read()
-    L.profile(False)
-    if not torch.isfinite(U).all():
-        raise RuntimeError("non-finite state after the timed steps")
-    graph_ms = tuned = dt_cfl = cfl_ms = None
-    if not args.only_timed:
-        # the same steps replayed from the captured hipGraph of one RK4 step
-        # (per-kernel events are off in this mode), outside the timed region above
-        L.rk4_advance(U, G, dt, 1)
-        torch.cuda.synchronize()
-        tg = time.perf_counter()
-        L.rk4_advance(U, G, dt, args.steps)
-        torch.cuda.synchronize()
-        graph_ms = (time.perf_counter() - tg) / args.steps * 1e3
-    if wl["order"] == 4 and not args.only_timed:
-        # box-table tile picked by the autotuner (results are identical for
-        # every tile); timed like the steps above, outside the timed region
-        tile, tune_ms = L.autotune(U, G)
-        L.rk4_advance(U, G, dt, 1)
-        torch.cuda.synchronize()
-        ta = time.perf_counter()
-        L.rk4_advance(U, G, dt, args.steps)
-        torch.cuda.synchronize()
-        ta = time.perf_counter() - ta
-        tuned = {"tile": tile, "rhs_ms_per_stage": tune_ms, "ms_per_step": ta / args.steps * 1e3,
-                 "value": 6 * wl["n_rad"] * wl["n_ang"] ** 2 * 4 * args.steps / ta}
-    if not args.only_timed:
-        # level CFL time step (the reference's dtCFL, taken once per step by
-        # the driver), timed on its own with the host read-back of dt
-        L.dt_cfl(U)
-        tc = time.perf_counter()
-        for _ in range(10):
-            dt_cfl = L.dt_cfl(U)
-        cfl_ms = (time.perf_counter() - tc) / 10 * 1e3
-    aux = None
-    if not args.only_timed:
-        # the level's other per-step kernels, each timed with HIP events on the
-        # launch stream over 10 calls: conserved-state check (with its host
-        # read-back) and inner boundary ghosts from a boundary-file strip
-        g = L.ghost()
-        n = wl["n_ang"] + 2 * g
-        strip = torch.zeros((6, 8, n, n), dtype=torch.float64, device=dev)
-        strip[:, 0] = 500.0
-        strip[:, 1] = 4e7
-        strip[:, 4] = 5e3
-        strip[:, 5] = 1e3
-        ghosts = torch.empty((6, 8, n, n, g), dtype=torch.float64, device=dev)
-        L.inner_ghosts_from_bc(strip, ghosts)
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record()
-        for _ in range(10):
-            L.inner_ghosts_from_bc(strip, ghosts)
-        e1.record()
-        torch.cuda.synchronize()
-        bc_ms = e0.elapsed_time(e1) / 10
-        ts = time.perf_counter()
-        for _ in range(10):
-            ok, _first = L.valid_state(U)
-        vs_ms = (time.perf_counter() - ts) / 10 * 1e3
-        aux = {"valid_state_ms": vs_ms, "inner_ghosts_from_bc_ms": bc_ms, "state_valid": ok}
-        del strip, ghosts
-
-    mixed_err = None
-    if U_init is not None and not args.only_timed:
-        # mixed precision against FP64 from the same initial state, after 1
-        # and after 100 RK4 steps: per-variable relative L-infinity and L2
-        # error of the state (outside the timed regions)
-        import dataclasses
-        L64 = hc.Level(shell, dataclasses.replace(opts, precision=0))
-        Um, U64 = U_init.clone(), U_init.clone()
-        mixed_err = {}
-        done = 0
-        for n in (1, 100):
-            L.rk4_advance(Um, G, dt, n - done)
-            L64.rk4_advance(U64, G, dt, n - done)
-            done = n
-            torch.cuda.synchronize()
-            d = (Um - U64).abs()
-            linf = (d.amax(dim=(0, 2, 3, 4)) / U64.abs().amax(dim=(0, 2, 3, 4))).cpu().tolist()
-            l2 = (d.square().sum(dim=(0, 2, 3, 4)).sqrt() / U64.square().sum(dim=(0, 2, 3, 4)).sqrt()).cpu().tolist()
-            mixed_err[f"after_{n}_steps"] = {"rel_linf": linf, "rel_l2": l2}
-        # sensitivity baseline: FP64 from the same state perturbed by a
-        # relative 1e-9 (the size of the mixed-precision error after one
-        # step), against the unperturbed FP64 state after 100 steps
-        gen = torch.Generator(device=dev)
-        gen.manual_seed(20261203)
-        noise = 2 * torch.rand(U_init.shape, generator=gen, device=dev, dtype=torch.float64) - 1
-        Up = U_init * (1 + 1e-9 * noise)
-        del noise
-        L64.rk4_advance(Up, G, dt, 100)
-        torch.cuda.synchronize()
-        d = (Up - U64).abs()
-        mixed_err["fp64_perturbed_1e-9_after_100_steps"] = {
-            "rel_linf": (d.amax(dim=(0, 2, 3, 4)) / U64.abs().amax(dim=(0, 2, 3, 4))).cpu().tolist(),
-            "rel_l2": (d.square().sum(dim=(0, 2, 3, 4)).sqrt() / U64.square().sum(dim=(0, 2, 3, 4)).sqrt()).cpu().tolist()}
-        del L64, Um, U64, Up, d
-
-    cells = 6 * wl["n_rad"] * wl["n_ang"] ** 2
-    value = cells * 4 * args.steps / elapsed
-    dominant = max((k for k in prof if k in ALGO_BYTES), key=lambda k: prof[k][0])
-    ms, launches = prof[dominant]
-    avg_s = ms / launches / 1e3
-    algo = ALGO_BYTES[dominant] * units_per_launch(dominant, wl["n_rad"], wl["n_ang"])
-    achieved = algo / avg_s / 1e9
-    traffic, flop, pmc_path = pmc_summary(dominant, args.workload) if args.precision == "fp64" else (None, None, None)
-    line = {
-        "metric": f"cell-updates/s per RK stage ({args.precision})",
-        "value": value,
-        "unit": "cell-updates/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": elapsed / args.steps * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else (
-            "f64 state, f32 reconstruction weights and Riemann dissipation" if wl.get("op", 1) == 2
-            else "f64 state, f32 slopes and Riemann dissipation"),
-        "data": ("synthetic seeded radial-wind shell with four-sector B_r and a magnetized blob"
-                 if wl["config"] == 5 else
-                 "synthetic seeded radial-wind shell with a Gaussian pressure pulse"
-                 + (", lognormal density noise and Alfvenic perturbations" if wl["config"] in (3, 4) else "")
-                 + (", low-beta cone" if wl["config"] == 4 else "")),
-        "config": {"workload": args.workload, "cells": cells, "panels": 6, "n_rad": wl["n_rad"],
-                   "n_ang": wl["n_ang"], "spatial_order": wl["order"],
-                   "operator": "berta-wenoz" if wl.get("op", 1) == 2 else "standard",
-                   "limiter": wl["limiter"], "riemann_solver": wl["riemann"], "precision": args.precision,
-                   "boxes": "32^3 box table" if wl["order"] == 4 else "panel arrays"},
-        "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "avg_launch_us": avg_s * 1e6},
-        "kernels_ms_per_step": {k: v[0] / args.steps for k, v in prof.items()},
-        "graph_ms_per_step": graph_ms,
-        "dt_cfl": {"value": dt_cfl, "ms": cfl_ms},
-        "level_aux_kernels": aux,
-        "autotuned_tile": tuned,
-    }
-    if traffic is not None:
-        line["roofline"]["traffic_source"] = pmc_path
-        line["roofline"]["traffic_hbm_gbs"] = traffic / avg_s / 1e9
-    if flop is not None:
-        line["valu_fp64"] = {"achieved": flop / avg_s / 1e12, "peak": 78.6, "unit": "TFLOP/s",
-                             "frac": flop / avg_s / 1e12 / 78.6, "flop_per_launch": flop, "source": pmc_path}
-    if mixed_err is not None:
-        line["mixed_vs_fp64"] = mixed_err
-    if not args.no_cpu_baseline and not args.only_timed:
-        line["cpu_baseline"] = cpu_baseline(wl, ju, inner, dt)
-    print(json.dumps(line), flush=True)
+        bench_level(args, wl)
 
 
 if __name__ == "__main__":

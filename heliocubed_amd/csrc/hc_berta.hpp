@@ -254,23 +254,26 @@ __global__ void __launch_bounds__(kB) k_b_faces(O4View v, double* __restrict__ S
             double s4[4];
 #pragma unroll
             for (int o = -2; o <= 1; ++o) s4[o+2] = at(F_WPT+c, o);
+            // the fallbacks run where the detector flags a non-smooth
+            // stencil: FP64 in both precisions (their weights and slopes act
+            // on the largest jumps)
             if (v.fallback == 0)
             {
                 const double scale = wenoNormalize(s4);
                 if (scale == 0.0) { l[c] = 0.0; r[c] = 0.0; }
                 else
                 {
-                    l[c] = scale*weno3Side<S_>(s4[0], s4[1], s4[2], dx2);
-                    r[c] = scale*weno3Side<S_>(s4[3], s4[2], s4[1], dx2);
+                    l[c] = scale*weno3Side<double>(s4[0], s4[1], s4[2], dx2);
+                    r[c] = scale*weno3Side<double>(s4[3], s4[2], s4[1], dx2);
                 }
             }
             else
             {
-                const S_ th = S_(1.5);
-                const S_ dm = S_(s4[1]-s4[0]), dp = S_(s4[2]-s4[1]), dc = S_(0.5*(s4[2]-s4[0]));
-                l[c] = s4[1] + 0.5*double(minmod3<S_>(th*dm, dc, th*dp));
-                const S_ rdm = S_(s4[2]-s4[1]), rdp = S_(s4[3]-s4[2]), rdc = S_(0.5*(s4[3]-s4[1]));
-                r[c] = s4[2] - 0.5*double(minmod3<S_>(th*rdm, rdc, th*rdp));
+                const double th = 1.5;
+                const double dm = s4[1]-s4[0], dp = s4[2]-s4[1], dc = 0.5*(s4[2]-s4[0]);
+                l[c] = s4[1] + 0.5*minmod3<double>(th*dm, dc, th*dp);
+                const double rdm = s4[2]-s4[1], rdp = s4[3]-s4[2], rdc = 0.5*(s4[3]-s4[1]);
+                r[c] = s4[2] - 0.5*minmod3<double>(th*rdm, rdc, th*rdp);
             }
         }
     }

@@ -306,17 +306,13 @@ __device__ void factoredRoe(double out[8], const double wl[8], const double wr[8
         normal2 += frame[0][i]*frame[0][i];
         tangent2 += frame[1][i]*frame[1][i];
     }
+    // the frame (and with it the face-frame states the Roe solver decides on)
+    // is the same in both precisions; only the back-transform below takes
+    // reciprocal products in mixed precision
     constexpr bool FAST = !std::is_same<S, double>::value;
     const double nm = sqrt(normal2), tm = sqrt(tangent2);
-    if constexpr (FAST)
-    {
-        const double inm = 1.0/nm, itm = 1.0/tm;
 #pragma unroll
-        for (int i = 0; i < 3; ++i) { frame[0][i] *= inm; frame[1][i] *= itm; }
-    }
-    else
-#pragma unroll
-        for (int i = 0; i < 3; ++i) { frame[0][i] /= nm; frame[1][i] /= tm; }
+    for (int i = 0; i < 3; ++i) { frame[0][i] /= nm; frame[1][i] /= tm; }
 #pragma unroll
     for (int i = 0; i < 3; ++i) { const int j = (i+1)%3, k = (i+2)%3; frame[2][i] = frame[0][j]*frame[1][k]-frame[0][k]*frame[1][j]; }
     double loc[2][8];
@@ -338,12 +334,11 @@ __device__ void factoredRoe(double out[8], const double wl[8], const double wr[8
 #pragma unroll
             for (int i = 0; i < 3; ++i) { loc[s][1+d] += frame[d][i]*vc[i]; loc[s][5+d] += frame[d][i]*bc[i]; }
     }
-    S ds[8];
-    if (!hc::roe8WaveDissipation<S>(ds, loc[0], loc[1], gamma, efix))
-        hc::rusanovDissipation<S>(ds, loc[0], loc[1], gamma);
+    // the Rusanov fallback of an inadmissible Roe fan is FP64 in both
+    // precisions: such faces carry the largest jumps
     double diss[8];
-#pragma unroll
-    for (int c = 0; c < 8; ++c) diss[c] = double(ds[c]);
+    if (!hc::roe8WaveDissipation<S>(diss, loc[0], loc[1], gamma, efix))
+        hc::rusanovDissipation<double>(diss, loc[0], loc[1], gamma);
     const double area = radialArea*nm;
     double mapped[8];
     mapped[0] = area*diss[0]; mapped[4] = area*diss[4];
@@ -597,8 +592,13 @@ __global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(2))) k_
     const BoxIdx I(b);
     double* const P = S + b.sOff;
     constexpr int t1 = D == 0 ? 1 : 0, t2 = D == 2 ? 1 : 2;
-    constexpr bool FAST = !std::is_same<S_, double>::value;   // mixed precision
-    auto by24 = [](double x) { return FAST ? x*(1.0/24.0) : x/24.0; };
+    // mixed precision: the point fluxes take reciprocal products (FAST); the
+    // point states stay operation for operation with the FP64 path, so every
+    // decision taken on them (the Roe repair, the Roe fan admissibility and
+    // its FP64 fallback below plasma beta 1/2) is the same in both precisions
+    constexpr bool FAST = !std::is_same<S_, double>::value;
+    auto by24 = [](double x) { return x/24.0; };
+    auto by24f = [](double x) { return FAST ? x*(1.0/24.0) : x/24.0; };
     auto Wat = [&](int c, int o) {
         int y[3] = {x[0], x[1], x[2]}; y[D] += o;
         return P[I(F_W+c, y[0], y[1], y[2])]; };
@@ -713,9 +713,9 @@ __global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(2))) k_
         double ke, me, vb, f[8];
         energiesNative<FAST>(A, w, ke, me, vb);
         primFlux<FAST>(f, w, ke, me, vb, v.gamma, D);
-        const double p0 = (side == 0 ? lf[4] : rf[4]) + (me + by24(lapMe));
+        const double p0 = (side == 0 ? lf[4] : rf[4]) + (me + by24f(lapMe));
 #pragma unroll
-        for (int q = 0; q < 8; ++q) phs[q] = side == 0 ? f[q]+by24(lapF[q]) : phs[q] + (f[q]+by24(lapF[q]));
+        for (int q = 0; q < 8; ++q) phs[q] = side == 0 ? f[q]+by24f(lapF[q]) : phs[q] + (f[q]+by24f(lapF[q]));
         p0s = side == 0 ? p0 : p0s + p0;
     }
 #pragma unroll

@@ -13,6 +13,11 @@
 #ifndef HC_MIXED_FP64_BETA
 #define HC_MIXED_FP64_BETA 0.5
 #endif
+// Relative density or pressure jump across a face above which mixed precision
+// evaluates the Roe eigen-system in FP64 (0 disables the test).
+#ifndef HC_MIXED_FP64_JUMP
+#define HC_MIXED_FP64_JUMP 0.0
+#endif
 
 namespace hc
 {
@@ -135,10 +140,10 @@ HC_HD void plmHighSide(R hi[8], const R m[8], const R c[8], const R p[8], int li
 }
 
 // Local Lax-Friedrichs dissipation speed * (U_R - U_L) in the face frame,
-// src/MHD_Roe8Wave.H:604-629. Conserved states and their difference are
-// formed in FP64; the wave speed and the product are evaluated in S.
+// src/MHD_Roe8Wave.H:604-629. Conserved states, their difference and the
+// product are formed in FP64; the wave speed is evaluated in S.
 template <typename S>
-HC_HD void rusanovDissipation(S out[8], const double l[8], const double r[8], double gamma)
+HC_HD void rusanovDissipation(double out[8], const double l[8], const double r[8], double gamma)
 {
     double cons[2][8];
     S speed = 0;
@@ -158,7 +163,7 @@ HC_HD void rusanovDissipation(S out[8], const double l[8], const double r[8], do
         const S wn = S(w[1]), g = S(gamma), p = S(w[4]), bb = S(b2), rho = S(w[0]);
         speed = fmax(speed, fabs(wn)+sqrt((g*p+S(kInv4Pi)*bb)/rho));
     }
-    for (int c = 0; c < 8; ++c) out[c] = speed*S(cons[1][c]-cons[0][c]);
+    for (int c = 0; c < 8; ++c) out[c] = double(speed)*(cons[1][c]-cons[0][c]);
 }
 
 template <typename S>
@@ -171,10 +176,12 @@ HC_HD S entropyFixedSpeed(S lambda, S del)
 // Roe 8-wave dissipation D = |A| (U_R - U_L), arithmetic averages, eight-wave
 // entropy fix and the wave-fan admissibility walk, src/MHD_Roe8Wave.H:16-600.
 // Returns false when the caller must fall back to Rusanov. The averages, the
-// conserved jump and the walked fan states are FP64; the eigen-system and the
-// projections are evaluated in S (S = double: all FP64).
+// conserved jump, the walked fan states and the result are FP64; the
+// eigen-system and the projections are evaluated in S (S = double: all FP64).
+// The result stays FP64 in both cases, so a face evaluated in FP64 by the
+// mixed path carries no FP32 rounding.
 template <typename S>
-HC_HD bool roe8WaveDissipation(S out[8], const double WL[8], const double WR[8], double gamma_d, double efix_d)
+HC_HD bool roe8WaveDissipation(double out[8], const double WL[8], const double WR[8], double gamma_d, double efix_d)
 {
     const S rs2 = S(0.707106781186547524400844362104849039);
     if (!(WL[0] > 0.0 && WR[0] > 0.0 && WL[4] > 0.0 && WR[4] > 0.0)) return false;
@@ -212,14 +219,12 @@ HC_HD bool roe8WaveDissipation(S out[8], const double WL[8], const double WR[8],
         // Below plasma beta 1/2 (of either side or of the average state) the
         // FP32 eigen-system loses digits to cancellation in the fast/slow
         // recombination: evaluate such faces in FP64.
-        constexpr double betaFp64 = HC_MIXED_FP64_BETA;
-        if (ccd*Rhod < betaFp64*gamma_d*BxBHd || WL[4] < betaFp64*bbL || WR[4] < betaFp64*bbR)
+        constexpr double betaFp64 = HC_MIXED_FP64_BETA, jumpFp64 = HC_MIXED_FP64_JUMP;
+        const bool jump = jumpFp64 > 0.0 && (fabs(rL-rR) > jumpFp64*fmin(rL, rR) ||
+                                             fabs(WL[4]-WR[4]) > jumpFp64*fmin(WL[4], WR[4]));
+        if (jump || ccd*Rhod < betaFp64*gamma_d*BxBHd || WL[4] < betaFp64*bbL || WR[4] < betaFp64*bbR)
         {
-            double o[8];
-            const bool ok = roe8WaveDissipation<double>(o, WL, WR, gamma_d, efix_d);
-#pragma unroll
-            for (int q = 0; q < 8; ++q) out[q] = S(o[q]);
-            return ok;
+            return roe8WaveDissipation<double>(out, WL, WR, gamma_d, efix_d);
         }
     }
     // Jump in primitive-like variables (left minus right), FP64.
@@ -366,7 +371,7 @@ HC_HD bool roe8WaveDissipation(S out[8], const double WL[8], const double WR[8],
     rd[3] = Wd*rd[0]+Rhod*rd[3];
     bool finite = true;
 #pragma unroll
-    for (int q = 0; q < 8; ++q) { out[q] = S(-rd[q]); finite = finite && isfinite(rd[q]); }
+    for (int q = 0; q < 8; ++q) { out[q] = -rd[q]; finite = finite && isfinite(rd[q]); }
     return finite;
 }
 
@@ -425,14 +430,18 @@ HC_HD void o2FaceFluxMixed(double out[8], const double l[8], const double r[8], 
             q[1+i] = v; q[5+i] = bsum;
         }
     }
-    S ds[8];
+    // the Rusanov fallback of an inadmissible Roe fan is FP64 in both
+    // precisions (such faces carry the largest jumps); the Rusanov solver
+    // itself (riemann_solver 1) takes its speed in S
+    double ds[8];
     if constexpr (RS == 1) rusanovDissipation<S>(ds, loc[0], loc[1], gamma);
     else if constexpr (RS == 2)
     {
-        if (!roe8WaveDissipation<S>(ds, loc[0], loc[1], gamma, entropy)) rusanovDissipation<S>(ds, loc[0], loc[1], gamma);
+        if (!roe8WaveDissipation<S>(ds, loc[0], loc[1], gamma, entropy)) rusanovDissipation<double>(ds, loc[0], loc[1], gamma);
     }
-    else if (!roe || !roe8WaveDissipation<S>(ds, loc[0], loc[1], gamma, entropy))
-        rusanovDissipation<S>(ds, loc[0], loc[1], gamma);
+    else if (!roe) rusanovDissipation<S>(ds, loc[0], loc[1], gamma);
+    else if (!roe8WaveDissipation<S>(ds, loc[0], loc[1], gamma, entropy))
+        rusanovDissipation<double>(ds, loc[0], loc[1], gamma);
     double num[8];
 #pragma unroll
     for (int s = 0; s < 2; ++s)
@@ -456,7 +465,7 @@ HC_HD void o2FaceFluxMixed(double out[8], const double l[8], const double r[8], 
         for (int c = 0; c < 8; ++c) num[c] = s == 0 ? f[c] : num[c]+f[c];
     }
 #pragma unroll
-    for (int c = 0; c < 8; ++c) num[c] = 0.5*area*(num[c]-double(ds[c]));
+    for (int c = 0; c < 8; ++c) num[c] = 0.5*area*(num[c]-ds[c]);
     out[0] = num[0]; out[4] = num[4];
 #pragma unroll
     for (int i = 0; i < 3; ++i)

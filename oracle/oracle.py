@@ -67,6 +67,11 @@ def lib() -> ctypes.CDLL:
             "orc_ghost_donors": (I, [cfgp, I, I, I, P, P, P, P]),
             "orc_seam_edge": (None, [cfgp, I, I, I, P]),
             "orc_o4_rhs": (I, [cfgp, I, P, P, P, P, P, P, P]),
+            "orc_o4_rhs_table": (I, [cfgp, I, P, P, P]),
+            "orc_limit_line": (None, [P, P]),
+            "orc_eta_tilde": (D, [P, P]),
+            "orc_prim_flux": (None, [P, P, D, I, P, P]),
+            "orc_factored_roe": (None, [P, P, P, P, D, I, D, D]),
             "orc_face_average": (None, [cfgp, I, I, I, I, I, P]),
             "orc4_create": (P, [cfgp]),
             "orc4_destroy": (None, [P]),
@@ -97,8 +102,15 @@ def lib() -> ctypes.CDLL:
 
 
 def _p(a: np.ndarray):
+    """Data pointer of a C-contiguous float64 array. The caller must keep the
+    array referenced until the library call returns (convert first, then
+    pass the pointer)."""
     assert a.dtype == np.float64 and a.flags.c_contiguous
     return a.ctypes.data
+
+
+def _c(x):
+    return np.ascontiguousarray(x, dtype=np.float64)
 
 
 class Oracle:
@@ -172,15 +184,15 @@ class Oracle:
 
 def roe8(left, right, gamma=5.0 / 3.0, entropy_fix=0.3, check=True):
     out = np.zeros(8)
-    ok = lib().orc_roe8(_p(out), _p(np.ascontiguousarray(left, dtype=np.float64)),
-                        _p(np.ascontiguousarray(right, dtype=np.float64)), gamma, entropy_fix, int(check))
+    l, r = _c(left), _c(right)
+    ok = lib().orc_roe8(_p(out), _p(l), _p(r), gamma, entropy_fix, int(check))
     return bool(ok), out
 
 
 def rusanov(left, right, gamma=5.0 / 3.0):
     out = np.zeros(8)
-    lib().orc_rusanov(_p(out), _p(np.ascontiguousarray(left, dtype=np.float64)),
-                      _p(np.ascontiguousarray(right, dtype=np.float64)), gamma)
+    l, r = _c(left), _c(right)
+    lib().orc_rusanov(_p(out), _p(l), _p(r), gamma)
     return out
 
 
@@ -193,9 +205,8 @@ def plm(mm, m, c, p, limiter=1):
 
 def face_flux(left, right, normal, area, gamma=5.0 / 3.0, entropy_fix=0.3, roe=False):
     out = np.zeros(8)
-    lib().orc_face_flux(_p(out), _p(np.ascontiguousarray(left, dtype=np.float64)),
-                        _p(np.ascontiguousarray(right, dtype=np.float64)),
-                        _p(np.ascontiguousarray(normal, dtype=np.float64)), area, gamma, entropy_fix, int(roe))
+    l, r, n = _c(left), _c(right), _c(normal)
+    lib().orc_face_flux(_p(out), _p(l), _p(r), _p(n), area, gamma, entropy_fix, int(roe))
     return out
 
 
@@ -204,10 +215,11 @@ def berta_theta(n_rad, n_ang, p, lo, hi, U, eta_d=0.02, gamma=5.0 / 3.0, r0=21.5
     cfg = OrcCfg(n_rad, n_ang, 5, r0, r1, 0.0, gamma, 1, 2, 0.3, 2, eta_d, 0)
     n = [hi[d] - lo[d] + 11 for d in range(3)]
     th = np.zeros((n[2], n[1], n[0]))
+    u = _c(U)
     lo_a = np.array(lo, dtype=np.int32)
     hi_a = np.array(hi, dtype=np.int32)
     lib().orc_berta_theta(ctypes.byref(cfg), p, lo_a.ctypes.data, hi_a.ctypes.data,
-                          _p(np.ascontiguousarray(U, dtype=np.float64)), _p(th))
+                          _p(u), _p(th))
     return th
 
 
@@ -366,6 +378,47 @@ def o4_rhs_boxes(n_rad, n_ang, boxes, USph, limiter=1, entropy_fix=0.3, gamma=5.
     return rhs
 
 
+def limit_line(w7):
+    """(left state of the high face, right state of the low face) of the
+    middle cell of a 7-cell line after Limiters_4O's limiter."""
+    out, w = np.zeros(2), np.ascontiguousarray(w7, dtype=np.float64)
+    lib().orc_limit_line(_p(w), _p(out))
+    return out[0], out[1]
+
+
+def eta_tilde(p5, v5):
+    p, v = np.ascontiguousarray(p5, dtype=np.float64), np.ascontiguousarray(v5, dtype=np.float64)
+    return lib().orc_eta_tilde(_p(p), _p(v))
+
+
+def prim_flux(A, w, d, gamma=5.0 / 3.0):
+    """(f[8], (ke, me, v.B)) of the fourth-order point flux of native state w."""
+    f, kmv = np.zeros(8), np.zeros(3)
+    a, s = np.ascontiguousarray(np.ravel(A), dtype=np.float64), np.ascontiguousarray(w, dtype=np.float64)
+    lib().orc_prim_flux(_p(a), _p(s), gamma, d, _p(f), _p(kmv))
+    return f, kmv
+
+
+def factored_roe(wl, wr, A, radial_area, d, gamma=5.0 / 3.0, entropy_fix=0.3):
+    out = np.zeros(8)
+    l, r = np.ascontiguousarray(wl, dtype=np.float64), np.ascontiguousarray(wr, dtype=np.float64)
+    a = np.ascontiguousarray(np.ravel(A), dtype=np.float64)
+    lib().orc_factored_roe(_p(out), _p(l), _p(r), _p(a), radial_area, d, gamma, entropy_fix)
+    return out
+
+
+def o4_rhs_table(n_rad, n_ang, boxes, USph, limiter=1, entropy_fix=0.3, gamma=5.0 / 3.0, r0=21.5, r1=215.0,
+                 spatial_operator=1, eta_d=0.02, fallback=0, riemann=2):
+    """o4_rhs_boxes without fluxes, the boxes spread over the OpenMP threads."""
+    cfg = OrcCfg(n_rad, n_ang, 5, r0, r1, 0.0, gamma, limiter, riemann, entropy_fix, spatial_operator, eta_d,
+                 fallback, 0)
+    tab = np.ascontiguousarray([[blk, *lo, *hi] for blk, lo, hi in boxes], dtype=np.int32)
+    u = np.ascontiguousarray(USph, dtype=np.float64)
+    rhs = np.zeros(sum(8 * np.prod([hi[d] - lo[d] + 1 for d in range(3)]) for _, lo, hi in boxes))
+    lib().orc_o4_rhs_table(ctypes.byref(cfg), len(boxes), tab.ctypes.data, _p(u), _p(rhs))
+    return rhs
+
+
 class Oracle4:
     """Fourth-order level (standard or Berta operator + Roe, state_conversion 1) on the CPU."""
 
@@ -387,15 +440,18 @@ class Oracle4:
 
     def prepare(self, ju, inner):
         W = np.zeros(self.stage_shape)
-        lib().orc4_prepare(self._h, _p(np.ascontiguousarray(ju)), _p(np.ascontiguousarray(inner)), _p(W))
+        a, b = _c(ju), _c(inner)
+        lib().orc4_prepare(self._h, _p(a), _p(b), _p(W))
         return W
 
     def rhs(self, W):
         k = np.zeros((6, 8, self.na, self.na, self.nr))
-        lib().orc4_rhs(self._h, _p(np.ascontiguousarray(W)), _p(k))
+        w = _c(W)
+        lib().orc4_rhs(self._h, _p(w), _p(k))
         return k
 
     def rk4(self, ju, inner, dt, nsteps=1):
         u = np.ascontiguousarray(ju.copy())
-        lib().orc4_rk4(self._h, _p(u), _p(np.ascontiguousarray(inner)), dt, nsteps)
+        g = _c(inner)
+        lib().orc4_rk4(self._h, _p(u), _p(g), dt, nsteps)
         return u

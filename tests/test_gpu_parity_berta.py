@@ -136,6 +136,11 @@ def test_mixed_precision_low_beta_cone(device, operator):
     got = U.cpu().numpy()
     err = rel_linf_per_var(got, ref)
     errd = rel_linf_per_var(got - ju, ref - ju)
-    print("mixed low-beta state err", operator, err, "increment err", errd)
-    assert errd.max() <= 1e-6, errd
+    # the FP64 oracle's own increment change from a 1e-9 relative input
+    # perturbation: the band of branch flips inside the cone
+    rng = np.random.default_rng(7)
+    up = O.rk4(ju * (1 + 1e-9 * (2 * rng.random(ju.shape) - 1)), inner, dt, 1)
+    sens = rel_linf_per_var(up - ju, ref - ju)
+    print("mixed low-beta state err", operator, err, "increment err", errd, "FP64 sensitivity", sens)
+    assert errd.max() <= max(1e-6, 2 * sens.max()), (errd, sens)
     assert err.max() <= 1e-6, err
