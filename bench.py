@@ -283,8 +283,25 @@ def bench_level(args, wl):
         t = timed_steps(L, U, G, dt, args.steps, args.warmup)
         if not torch.isfinite(U).all():
             raise RuntimeError(f"non-finite state after the timed steps ({prec})")
-        res[prec] = {"L": L, "elapsed": t, "value": cells * 4 * args.steps / t,
-                     "ms_per_step": t / args.steps * 1e3}
+        shape = L.tile_shape if wl["order"] == 4 else None
+        r = {"L": L, "elapsed": t, "value": cells * 4 * args.steps / t, "ms_per_step": t / args.steps * 1e3,
+             "tile": shape}
+        if wl["order"] == 4 and not args.tile and not args.no_autotune:
+            # the autotuner's tile shape (identical results for every shape),
+            # timed over the same steps; the faster of the two is kept
+            first = {"tile": list(shape), "ms_per_step": r["ms_per_step"], "value": r["value"]}
+            tuned, tune_ms = L.autotune(U0, G)
+            U = U0.clone()
+            t2 = timed_steps(L, U, G, dt, args.steps, args.warmup)
+            second = {"tile": list(tuned), "ms_per_step": t2 / args.steps * 1e3, "value": cells * 4 * args.steps / t2,
+                      "autotuner_rhs_ms_per_stage": tune_ms}
+            r["tiles_timed"] = [first, second]
+            if t2 < t:
+                r.update({"elapsed": t2, "value": second["value"], "ms_per_step": second["ms_per_step"],
+                          "tile": tuned})
+            else:
+                L.set_tile(tuple(shape))
+        res[prec] = r
         del U
     head = "mixed" if "mixed" in res else precisions[0]
     value, elapsed = res[head]["value"], res[head]["elapsed"]
@@ -305,10 +322,15 @@ def bench_level(args, wl):
                    "n_ang": wl["n_ang"], "spatial_order": wl["order"],
                    "operator": "berta-wenoz" if wl.get("op", 1) == 2 else ("standard" if wl["order"] == 4 else "plm"),
                    "limiter": wl["limiter"], "riemann_solver": wl["riemann"], "precision": head,
-                   "boxes": (f"{res[head]['L'].tile}^3 box table" if wl["order"] == 4 else "panel arrays")},
+                   "boxes": ("x".join(str(x) for x in res[head]["tile"]) + " box table (32^3 configured; "
+                             "the autotuned shape when faster over the timed steps)" if wl["order"] == 4
+                             else "panel arrays")},
     }
     for prec in precisions:
-        line[f"{prec}_batched"] = {"value": res[prec]["value"], "ms_per_step": res[prec]["ms_per_step"]}
+        line[f"{prec}_batched"] = {"value": res[prec]["value"], "ms_per_step": res[prec]["ms_per_step"],
+                                   "tile": list(res[prec]["tile"]) if res[prec]["tile"] else None}
+        if "tiles_timed" in res[prec]:
+            line[f"{prec}_batched"]["tiles_timed"] = res[prec]["tiles_timed"]
     if len(res) == 2:
         line["mixed_speedup_vs_fp64_batched"] = res["mixed"]["value"] / res["fp64"]["value"]
 
@@ -500,7 +522,9 @@ def main():
     ap.add_argument("--workload", default=DEFAULT_WORKLOAD, choices=sorted(WORKLOADS))
     ap.add_argument("--precision", default="both", choices=["both", "fp64", "mixed"],
                     help="both: mixed is `value`, FP64 is reported beside it")
-    ap.add_argument("--tile", type=int, default=0, help="box-table tile edge of the level (0: the default 32)")
+    ap.add_argument("--tile", type=int, default=0, help="box-table tile edge of the level (0: 32, then the "
+                                                          "autotuner's shape if faster)")
+    ap.add_argument("--no-autotune", action="store_true", help="time the configured 32^3 tiles only")
     ap.add_argument("--cpu-boxes", type=int, default=0, help="boxes in the CPU sample (0: two per thread)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the one-call-per-box FP64 measurement")

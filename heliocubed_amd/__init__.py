@@ -72,7 +72,9 @@ _SIGNATURES = [
     ("hc_level_use_graph", _I, [_P, _I]),
     ("hc_level_dt_cfl", _I, [_P, _P, _D, _P, _P]),
     ("hc_level_set_tile", _I, [_P, _I]),
+    ("hc_level_set_tile_shape", _I, [_P, _I, _I, _I]),
     ("hc_level_tile", _I, [_P]),
+    ("hc_level_tile_shape", _I, [_P, _P]),
     ("hc_level_autotune", _I, [_P, _P, _P, _P, _P, _P]),
     ("hc_level_valid_state", _I, [_P, _P, _P, _P, _P]),
     ("hc_level_inner_ghosts_from_bc", _I, [_P, _P, _P, _P]),
@@ -303,25 +305,36 @@ class Level:
     def use_graph(self, enable: bool = True):
         _check(lib().hc_level_use_graph(self._handle, int(bool(enable))))
 
-    def set_tile(self, tile: int):
-        """Edge of the fourth-order level's box-table tiles (results unchanged)."""
-        _check(lib().hc_level_set_tile(self._handle, int(tile)))
+    def set_tile(self, tile):
+        """Fourth-order box-table tiles (results unchanged): an edge for cubes,
+        or a (radial, alpha, beta) shape."""
+        if isinstance(tile, (tuple, list)):
+            _check(lib().hc_level_set_tile_shape(self._handle, *[int(t) for t in tile]))
+        else:
+            _check(lib().hc_level_set_tile(self._handle, int(tile)))
 
     @property
     def tile(self) -> int:
+        """Edge of cubic tiles (-1 for another shape)."""
         return lib().hc_level_tile(self._handle)
 
+    @property
+    def tile_shape(self):
+        t = (ctypes.c_int * 3)()
+        _check(lib().hc_level_tile_shape(self._handle, t))
+        return tuple(t)
+
     def autotune(self, U, inner):
-        """Times one stage RHS of U for tiles 32, 64 and the whole panel and
-        keeps the fastest; returns (tile, ms per stage). U is not modified."""
+        """Times one stage RHS of U for the candidate tile shapes (cubes and
+        radial pencils) and keeps the fastest; returns ((t0, t1, t2), ms per
+        stage). U is not modified."""
         self._require(U, self.valid_shape, "U")
         self._require(inner, self.inner_shape, "inner")
         if inner is None:
             raise ValueError("inner: the inner radial ghosts are required")
-        t, ms = ctypes.c_int(), ctypes.c_double()
-        _check(lib().hc_level_autotune(self._handle, _ptr(U), _ptr(inner), ctypes.byref(t), ctypes.byref(ms),
-                                       self._stream()))
-        return t.value, ms.value
+        t, ms = (ctypes.c_int * 3)(), ctypes.c_double()
+        _check(lib().hc_level_autotune(self._handle, _ptr(U), _ptr(inner), t, ctypes.byref(ms), self._stream()))
+        return tuple(t), ms.value
 
     def dt_cfl(self, U, cone_boundary_density: float = 0.0) -> float:
         """1 / max CFL speed sum of the valid state U (BoxOp dtCFL); the caller

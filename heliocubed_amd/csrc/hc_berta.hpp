@@ -13,7 +13,8 @@
 //     BertaReconstructWithFallback (:747-1239): WENOZ / WENO3 / PLM face points
 //     Roe repair with the cell-average donors (:1817-1844)
 //     point fluxes of both sides summed, theta-aware face averages (:1246-1266, :1904-1946)
-//     BertaPrimPointToMappedFlux (:1334-1534) with the separable-metric shortcut (:1-98)
+//     BertaPrimPointToMappedFlux (:1334-1534) with the separable-metric shortcut (:1-98),
+//       evaluated inside the flux kernel
 //     primToMappedRoeFlux with theta-aware dissipation average (src/BoxOp_RoeCubedSphere.H:29-145),
 //     or BertaPrimPointToMappedRusanovFlux (:1541-1730) for riemann_solver 1
 //     F = (Sum - D)/2 (:2004-2026)
@@ -30,7 +31,6 @@ constexpr int F_PTOT = F_P0R;               // point total pressure, both sides 
 constexpr int F_PAVG = F_MEL;               // its theta-aware face average
 constexpr int F_FPS = F_PHL;                // point physical fluxes, both sides summed
 constexpr int F_PFA = F_PHR;                // their theta-aware face average
-constexpr int F_BMP = F_FB;                 // mapped products of the averaged flux
 
 __device__ __forceinline__ double refMax(double a, double b) { return a < b ? b : a; }
 __device__ __forceinline__ double refMin(double a, double b) { return b < a ? b : a; }
@@ -354,35 +354,24 @@ __global__ void __launch_bounds__(kB) k_b_avg(O4View v, double* __restrict__ S)
     P[I(F_PAVG, x[0], x[1], x[2])] = avg(F_PTOT);
 }
 
-// 6. Mapped products of the averaged flux on the radial faces
-// (bertaMapHostBox, :35-98): per matrix column one term (transverse
-// corrections, then the product), the terms summed over columns. Faces
-// [lo_0, hi_0+1] x [lo, hi]. The angular directions evaluate theirs in
-// k_b_flux.
-template <int D>
-__global__ void __launch_bounds__(kB) k_b_map(O4View v, double* __restrict__ S)
+// 6. Mapped products of the averaged flux at a radial face (bertaMapHostBox,
+// :35-98): per matrix column one term (transverse corrections, then the
+// product), the terms summed over columns. The radial flux kernel evaluates
+// them at its own face; the angular directions use FaceOps::mapped.
+__device__ __forceinline__ void bertaRadialMapped(const FaceOps<0>& op, const int x[3], double bmp[8])
 {
-    static_assert(D == 0, "angular mapped products are evaluated in k_b_flux");
-    const BoxDesc& b = blockBoxDesc(v);
-    int L[3], H[3], x[3];
-    for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]; H[d] = b.hi[d]; }
-    H[D] = b.hi[D]+1;
-    if (!regionPoint(b, L, H, boxThread(v, b), x)) return;
-    const BoxIdx I(b);
-    double* const P = S + b.sOff;
-    constexpr int t1 = D == 0 ? 1 : 0, t2 = D == 2 ? 1 : 2;
-    const FaceOps<D> op(v, b, I, P);
+    constexpr int t1 = 1, t2 = 2;
 #pragma unroll
     for (int s = 0; s < 2; ++s)
     {
         const int c = s == 0 ? 0 : 4;
         double term = 0;
-        if (D == 0) term += op.Dm(0, t1, x)*op.Df(F_PFA+c, t1, x)*(1.0/12.0);
+        term += op.Dm(0, t1, x)*op.Df(F_PFA+c, t1, x)*(1.0/12.0);
         term += op.Dm(0, t2, x)*op.Df(F_PFA+c, t2, x)*(1.0/12.0);
         term += op.M(0, x)*op.Fv(F_PFA+c, x)*1.0;
         double sum = 0;
         sum += term;
-        P[I(F_BMP+c, x[0], x[1], x[2])] = sum;
+        bmp[c] = sum;
     }
 #pragma unroll
     for (int grp = 0; grp < 2; ++grp)
@@ -396,12 +385,12 @@ __global__ void __launch_bounds__(kB) k_b_map(O4View v, double* __restrict__ S)
             for (int col = 0; col < 3; ++col)
             {
                 double term = 0;
-                if (D == 0) term += op.Dm(1+3*r+col, t1, x)*op.Df(F_PFA+c0+col, t1, x)*(1.0/12.0);
+                term += op.Dm(1+3*r+col, t1, x)*op.Df(F_PFA+c0+col, t1, x)*(1.0/12.0);
                 term += op.Dm(1+3*r+col, t2, x)*op.Df(F_PFA+c0+col, t2, x)*(1.0/12.0);
                 term += op.M(1+3*r+col, x)*op.Fv(F_PFA+c0+col, x)*1.0;
                 sum += term;
             }
-            P[I(F_BMP+c0+r, x[0], x[1], x[2])] = sum;
+            bmp[c0+r] = sum;
         }
     }
 }
@@ -428,11 +417,13 @@ __global__ void __launch_bounds__(kB) k_b_flux(O4View v, double* __restrict__ S)
     double Fsum[8];
     if constexpr (D == 0)
     {
+        double bmp[8];
+        bertaRadialMapped(op, x, bmp);
 #pragma unroll
         for (int c = 0; c < 8; ++c)
         {
             double val = 0;
-            val += Fv(F_BMP+c, x)*adjX*1.0;
+            val += bmp[c]*adjX*1.0;
             Fsum[c] = val;
         }
     }
@@ -623,7 +614,6 @@ void bertaDirectionPass(const O4View& v, dim3 grid, hipStream_t s, const double*
     if (v.powell) hipLaunchKernelGGL(k_b_pw_face<D>, grid, dim3(kB), 0, s, v, U, scratch);
     hipLaunchKernelGGL(k_b_avg<D>, grid, dim3(kB), 0, s, v, scratch);
     if (v.roe) hipLaunchKernelGGL((k_o4_roe<D, S_>), grid, dim3(kB), 0, s, v, scratch);
-    if constexpr (D == 0) hipLaunchKernelGGL(k_b_map<D>, grid, dim3(kB), 0, s, v, scratch);
     if (v.roe) hipLaunchKernelGGL((k_b_flux<D, true>), grid, dim3(kB), 0, s, v, scratch);
     else hipLaunchKernelGGL((k_b_flux<D, false>), grid, dim3(kB), 0, s, v, scratch);
 }

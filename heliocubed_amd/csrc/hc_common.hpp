@@ -37,11 +37,12 @@ inline const char* checkOperatorOptions(const hc_options& o)
 // Fourth-order level (defined with the fourth-order kernels): stage
 // preparation JU -> USph (ghost 5) and the box-table RHS over the level.
 struct Level4;
-Level4* level4Create(const hc_shell& shell, const hc_options& opt, int boxSize);
+Level4* level4Create(const hc_shell& shell, const hc_options& opt, const int tile[3]);
 void level4Destroy(Level4* L);
-// Rebuilds the level's box table with tiles of boxSize^3 (results unchanged).
-void level4Retile(Level4& L, int boxSize);
-int level4Tile(const Level4& L);
+// Rebuilds the level's box table with tiles of tile[0] x tile[1] x tile[2]
+// cells (radial, alpha, beta; clipped to the panel; results unchanged).
+void level4Retile(Level4& L, const int tile[3]);
+void level4TileShape(const Level4& L, int tile[3]);
 void level4Prepare(Level4& L, const double* U0, const double* K, double a_dt, const double* inner, double* US,
                    hipStream_t s);
 void level4Rhs(Level4& L, const double* US, double* K, double* U, double b_dt, hipStream_t s);

@@ -13,6 +13,7 @@
 #include <stdexcept>
 #include <string>
 #include <algorithm>
+#include <array>
 #include <vector>
 
 #include "../../include/heliocubed.h"
@@ -1065,7 +1066,8 @@ int hc_level_create(const hc_shell* shell, const hc_options* options, hc_level**
             v.NI = v.nr+2*g; v.NJ = v.na+2*g; v.NK = v.na+2*g;
             v.cs = (long long)v.NI*v.NJ*v.NK; v.ps = 8*v.cs;
             v.vcs = (long long)v.nr*v.na*v.na; v.vps = 8*v.vcs;
-            L->l4 = hc::level4Create(*shell, *options, 32);
+            const int tile[3] = {32, 32, 32};
+            L->l4 = hc::level4Create(*shell, *options, tile);
         }
         else buildLevel(*L);
         *out = L.release();
@@ -1166,46 +1168,87 @@ int hc_level_use_graph(hc_level* level, int enable)
     return 0;
 }
 
+int hc_level_set_tile_shape(hc_level* level, int t0, int t1, int t2)
+{
+    if (!level) return fail("hc_level_set_tile_shape: null level");
+    if (!level->l4) return fail("hc_level_set_tile_shape: only a fourth-order level has a box table");
+    if (t0 < 8 || t1 < 8 || t2 < 8) return fail("hc_level_set_tile_shape: every tile edge must be >= 8 cells");
+    try
+    {
+        int cur[3];
+        hc::level4TileShape(*level->l4, cur);
+        if (cur[0] == t0 && cur[1] == t1 && cur[2] == t2) return 0;
+        level->dropGraph();
+        HC_CHECK(hipDeviceSynchronize());
+        const int tile[3] = {t0, t1, t2};
+        hc::level4Retile(*level->l4, tile);
+        return 0;
+    }
+    catch (const std::exception& e) { return fail(std::string("hc_level_set_tile_shape: ") + e.what()); }
+}
+
 int hc_level_set_tile(hc_level* level, int tile)
 {
     if (!level) return fail("hc_level_set_tile: null level");
     if (!level->l4) return fail("hc_level_set_tile: only a fourth-order level has a box table");
     if (tile < 8) return fail("hc_level_set_tile: tile must be >= 8 cells");
-    try
-    {
-        if (hc::level4Tile(*level->l4) == tile) return 0;
-        level->dropGraph();
-        HC_CHECK(hipDeviceSynchronize());
-        hc::level4Retile(*level->l4, tile);
-        return 0;
-    }
-    catch (const std::exception& e) { return fail(std::string("hc_level_set_tile: ") + e.what()); }
+    return hc_level_set_tile_shape(level, tile, tile, tile);
 }
 
-int hc_level_tile(const hc_level* level) { return level && level->l4 ? hc::level4Tile(*level->l4) : 0; }
+int hc_level_tile(const hc_level* level)
+{
+    if (!level || !level->l4) return 0;
+    int t[3];
+    hc::level4TileShape(*level->l4, t);
+    return t[0] == t[1] && t[1] == t[2] ? t[0] : -1;
+}
 
-int hc_level_autotune(hc_level* level, const double* U, const double* inner, int* tile, double* ms_per_stage,
+int hc_level_tile_shape(const hc_level* level, int* tile3)
+{
+    if (!level || !tile3) return fail("hc_level_tile_shape: null argument");
+    if (!level->l4) return fail("hc_level_tile_shape: only a fourth-order level has a box table");
+    hc::level4TileShape(*level->l4, tile3);
+    return 0;
+}
+
+int hc_level_autotune(hc_level* level, const double* U, const double* inner, int* tile3, double* ms_per_stage,
                       hc_stream stream)
 {
-    if (!level || !U || !inner || !tile) return fail("hc_level_autotune: null argument");
+    if (!level || !U || !inner || !tile3) return fail("hc_level_autotune: null argument");
     if (!level->l4) return fail("hc_level_autotune: only a fourth-order level has a box table");
     try
     {
         hc_level& L = *level;
         L.ensureWorkspace();
         hipStream_t s = static_cast<hipStream_t>(stream);
-        const int nmax = std::max(L.dims.nr, L.dims.na);
-        std::vector<int> cand;
-        for (int t : {32, 64, nmax}) if (t <= nmax && std::find(cand.begin(), cand.end(), t) == cand.end()) cand.push_back(t);
+        const int nr = L.dims.nr, na = L.dims.na;
+        // cubes and radial pencils (long along the unit-stride radius, which
+        // halves the ghost fraction per radial line), clipped to the panel
+        const int shapes[][3] = {{32, 32, 32}, {64, 64, 64}, {128, 128, 128}, {64, 32, 32}, {128, 32, 32},
+                                 {nr, 32, 32}, {128, 64, 64}, {nr, 64, 64}, {nr, na, na}};
+        std::vector<std::array<int, 3>> cand;
+        for (const auto& sh : shapes)
+        {
+            const std::array<int, 3> c = {std::min(sh[0], nr), std::min(sh[1], na), std::min(sh[2], na)};
+            if (std::find(cand.begin(), cand.end(), c) == cand.end()) cand.push_back(c);
+        }
         hipEvent_t e0, e1;
         HC_CHECK(hipEventCreate(&e0));
         HC_CHECK(hipEventCreate(&e1));
-        int best = cand.front();
+        std::array<int, 3> best = cand.front();
         double bestMs = 1e300;
-        for (int t : cand)
+        for (const auto& t : cand)
         {
-            if (hc::level4Tile(*L.l4) != t) { L.dropGraph(); HC_CHECK(hipDeviceSynchronize()); hc::level4Retile(*L.l4, t); }
-            // one stage (preparation + RHS into the level's K; U is not modified), warm, then timed twice
+            int cur[3];
+            hc::level4TileShape(*L.l4, cur);
+            if (cur[0] != t[0] || cur[1] != t[1] || cur[2] != t[2])
+            {
+                L.dropGraph();
+                HC_CHECK(hipDeviceSynchronize());
+                hc::level4Retile(*L.l4, t.data());
+            }
+            // one stage (preparation + RHS into the level's K; U is not
+            // modified), warm, then the RHS timed twice
             prepareStage(L, U, nullptr, 0.0, inner, L.W, s);
             stageRhs(L, L.W, L.K, nullptr, 0.0, nullptr, nullptr, nullptr, s);
             HC_CHECK(hipEventRecord(e0, s));
@@ -1218,8 +1261,15 @@ int hc_level_autotune(hc_level* level, const double* U, const double* inner, int
         }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
-        if (hc::level4Tile(*L.l4) != best) { L.dropGraph(); HC_CHECK(hipDeviceSynchronize()); hc::level4Retile(*L.l4, best); }
-        *tile = best;
+        int cur[3];
+        hc::level4TileShape(*L.l4, cur);
+        if (cur[0] != best[0] || cur[1] != best[1] || cur[2] != best[2])
+        {
+            L.dropGraph();
+            HC_CHECK(hipDeviceSynchronize());
+            hc::level4Retile(*L.l4, best.data());
+        }
+        for (int d = 0; d < 3; ++d) tile3[d] = best[d];
         if (ms_per_stage) *ms_per_stage = bestMs;
         return 0;
     }

@@ -1712,7 +1712,7 @@ struct Level4
     hc_options opt{};
     Prep4View pv{};
     hc_boxop* box = nullptr;
-    int tile = 32;                 // edge of the box-table tiles over the level
+    int tile[3] = {32, 32, 32};    // box-table tile shape over the level (radial, alpha, beta)
     int ncol = 0;
     long long* colDst = nullptr;
     int* colStart = nullptr;
@@ -1780,18 +1780,19 @@ static std::vector<AvgDonor> averageDonors(int na, int p, int j, int k)
     return out;
 }
 
-// Box table over a fourth-order level: tiles of boxSize^3 cells (clipped to
-// the panel) viewing the level arrays, with the seam pairs of their face
-// fluxes. Every tile evaluates the same per-cell arithmetic, so the level
-// result does not depend on boxSize; only speed and scratch size do.
-static hc_boxop* levelBoxTable(const Level4& Lv, int boxSize)
+// Box table over a fourth-order level: tiles of tile[0] x tile[1] x tile[2]
+// cells (clipped to the panel) viewing the level arrays, with the seam pairs
+// of their face fluxes. Every tile evaluates the same per-cell arithmetic, so
+// the level result does not depend on the tile shape; only speed and scratch
+// size do.
+static hc_boxop* levelBoxTable(const Level4& Lv, const int tile[3])
 {
     const Level4* L = &Lv;
     const Prep4View& v = L->pv;
     const int nr = v.nr, na = v.na;
     const hc_options& opt = L->opt;
     std::vector<hc_box> boxes;
-    const int bs[3] = {std::min(boxSize, nr), std::min(boxSize, na), std::min(boxSize, na)};
+    const int bs[3] = {std::min(tile[0], nr), std::min(tile[1], na), std::min(tile[2], na)};
     for (int p = 0; p < 6; ++p)
         for (int k = 0; k < na; k += bs[2]) for (int j = 0; j < na; j += bs[1]) for (int i = 0; i < nr; i += bs[0])
         {
@@ -1857,7 +1858,7 @@ static hc_boxop* levelBoxTable(const Level4& Lv, int boxSize)
     return B.release();
 }
 
-Level4* level4Create(const hc_shell& shell, const hc_options& opt, int boxSize)
+Level4* level4Create(const hc_shell& shell, const hc_options& opt, const int tile[3])
 {
     std::unique_ptr<Level4> L(new Level4());
     L->opt = opt;
@@ -1913,21 +1914,21 @@ Level4* level4Create(const hc_shell& shell, const hc_options& opt, int boxSize)
     HC_THROW(hipMalloc(&L->ang, 6ull*8*na*na*nr*sizeof(double))); L->owned.push_back(L->ang);
     HC_THROW(hipMalloc(&L->cart, 6ull*v.cps*sizeof(double))); L->owned.push_back(L->cart);
     HC_THROW(hipMemset(L->cart, 0, 6ull*v.cps*sizeof(double)));
-    L->box = levelBoxTable(*L, boxSize);
-    L->tile = boxSize;
+    L->box = levelBoxTable(*L, tile);
+    for (int d = 0; d < 3; ++d) L->tile[d] = tile[d];
     return L.release();
 }
 
-void level4Retile(Level4& L, int boxSize)
+void level4Retile(Level4& L, const int tile[3])
 {
-    hc_boxop* B = levelBoxTable(L, boxSize);
+    hc_boxop* B = levelBoxTable(L, tile);
     delete L.box;
     L.box = B;
-    L.tile = boxSize;
+    for (int d = 0; d < 3; ++d) L.tile[d] = tile[d];
 }
 
 void level4Destroy(Level4* L) { delete L; }
-int level4Tile(const Level4& L) { return L.tile; }
+void level4TileShape(const Level4& L, int tile[3]) { for (int d = 0; d < 3; ++d) tile[d] = L.tile[d]; }
 
 void level4Prepare(Level4& L, const double* U0, const double* K, double a_dt, const double* inner, double* US,
                    hipStream_t s)

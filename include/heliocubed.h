@@ -144,19 +144,26 @@ int hc_rk4_advance(hc_level* level, double* U, const double* inner_ghosts, doubl
 int hc_level_use_graph(hc_level* level, int enable);
 
 /* Tiles of the fourth-order level's box table: the level evaluates its RHS as
- * a table of tile^3 boxes (clipped to the panel; 32 by default, the box size
- * of the reference's production layouts). Every tile evaluates the same
- * per-cell arithmetic on its own ghosted copy, so results are identical for
- * every tile size; larger tiles recompute fewer ghost cells and take more
- * scratch. hc_level_set_tile fails on a second-order level (no box table).
- * hc_level_autotune times one stage RHS of the state U (inner ghosts as for
- * hc_rk4_advance; U is not modified) for tiles 32, 64 and the whole panel,
- * keeps the fastest and returns it (and its time per stage, may be NULL).
- * Box layout is chosen by the user in the reference (MBDisjointBoxLayout);
- * these have no reference counterpart. */
+ * a table of t0 x t1 x t2 boxes (radial x alpha x beta cells, clipped to the
+ * panel; 32^3 by default, the box size of the reference's production
+ * layouts). Every tile evaluates the same per-cell arithmetic on its own
+ * ghosted copy, so results are identical for every tile shape; larger tiles
+ * recompute fewer ghost cells and take more scratch. hc_level_set_tile(t) is
+ * hc_level_set_tile_shape(t, t, t); hc_level_tile returns the edge of a cubic
+ * tile (-1 for another shape), hc_level_tile_shape the three edges. These
+ * fail on a second-order level (no box table). hc_level_autotune times one
+ * stage RHS of the state U (inner ghosts as for hc_rk4_advance; U is not
+ * modified) for the cubes 32, 64, 128 and the radial pencils 64x32x32,
+ * 128x32x32, n_rad x 32 x 32, 128x64x64, n_rad x 64 x 64 and whole panels
+ * (clipped to the panel), keeps the fastest and returns its shape in
+ * tile3[3] (and its time per stage, may be NULL). Box layout is chosen by
+ * the user in the reference (MBDisjointBoxLayout); these have no reference
+ * counterpart. */
 int hc_level_set_tile(hc_level* level, int tile);
+int hc_level_set_tile_shape(hc_level* level, int t0, int t1, int t2);
 int hc_level_tile(const hc_level* level);
-int hc_level_autotune(hc_level* level, const double* U, const double* inner_ghosts, int* tile,
+int hc_level_tile_shape(const hc_level* level, int* tile3);
+int hc_level_autotune(hc_level* level, const double* U, const double* inner_ghosts, int* tile3,
                       double* ms_per_stage, hc_stream stream);
 
 /* CFL time step of the level state U (valid layout): 1 / max over the valid

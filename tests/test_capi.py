@@ -84,8 +84,10 @@ def test_tile_calls_reject_null_level_without_device():
     assert hc.lib().hc_level_set_tile(None, 32) != 0
     assert b"null" in hc.lib().hc_last_error()
     assert hc.lib().hc_level_tile(None) == 0
-    t, ms = ctypes.c_int(), ctypes.c_double()
-    assert hc.lib().hc_level_autotune(None, None, None, ctypes.byref(t), ctypes.byref(ms), None) != 0
+    t, ms = (ctypes.c_int * 3)(), ctypes.c_double()
+    assert hc.lib().hc_level_autotune(None, None, None, t, ctypes.byref(ms), None) != 0
+    assert hc.lib().hc_level_set_tile_shape(None, 32, 32, 32) != 0
+    assert hc.lib().hc_level_tile_shape(None, t) != 0
 
 
 def test_empty_box_table_is_rejected_without_device():

@@ -27,9 +27,10 @@ def test_tiles_give_identical_steps(device, operator, precision):
     L = hc.Level(shell, opts)
     assert L.tile == 32
     results = {}
-    for tile in (16, 32, 64, 48):
+    for tile in (16, 32, 64, 48, (48, 16, 16), (16, 32, 8)):
         L.set_tile(tile)
-        assert L.tile == tile
+        assert L.tile == (tile if isinstance(tile, int) else -1)
+        assert L.tile_shape == ((tile,) * 3 if isinstance(tile, int) else tile)
         U = _dev(ju)
         L.rk4_advance(U, G, dt, 2)
         torch.cuda.synchronize()
@@ -50,7 +51,9 @@ def test_autotune_keeps_a_candidate_and_state(device):
     tile, ms = L.autotune(U, G)
     torch.cuda.synchronize()
     print("autotune tile", tile, "ms per stage", ms)
-    assert tile in (32, 64) and L.tile == tile and ms > 0
+    cands = {(32, 32, 32), (64, 32, 32), (64, 64, 64)}   # the candidate shapes clipped to 64 x 32 x 32
+    assert tuple(min(a, b) for a, b in zip(tile, (64, 32, 32))) == tile and L.tile_shape == tile and ms > 0
+    assert tile in {tuple(min(a, b) for a, b in zip(c, (64, 32, 32))) for c in cands}
     assert np.array_equal(U.cpu().numpy(), ju)
     dt = 0.2 * (21.5 * np.pi / 2 / na) / 1.6
     L.rk4_advance(U, G, dt, 1)
