@@ -63,26 +63,9 @@ RHS_BYTES = 128
 
 
 def amr_table():
-    """Levels (n_rad, n_ang) and the (level, (block, lo, hi)) box table of the
-    three-level hierarchy: the wedge is the central 32x32 angular cells of
-    panel 1 at level 1 (30 degrees of the 90-degree panel), the shock front
-    the radial band [96, 160) of level 2 inside it."""
-    levels = [(64, 48), (128, 96), (256, 192)]
-    table = []
-    for p in range(6):
-        for k in range(0, 48, 16):
-            for j in range(0, 48, 16):
-                for i in range(0, 64, 32):
-                    table.append((0, (p, (i, j, k), (i + 31, j + 15, k + 15))))
-    for k in range(32, 64, 16):
-        for j in range(32, 64, 16):
-            for i in range(0, 128, 16):
-                table.append((1, (1, (i, j, k), (i + 15, j + 15, k + 15))))
-    for k in range(64, 128, 8):
-        for j in range(64, 128, 8):
-            for i in range(96, 160, 8):
-                table.append((2, (1, (i, j, k), (i + 7, j + 7, k + 7))))
-    return levels, table
+    """Levels and box table of the three-level AMR workload (652 boxes)."""
+    from heliocubed_amd.workloads import amr_3level_table
+    return amr_3level_table()
 
 
 def pmc_summary(kernel, workload):
@@ -278,7 +261,8 @@ def bench_level(args, wl):
     for prec in precisions:
         L = hc.Level(shell, o64 if prec == "fp64" else omix)
         if args.tile:
-            L.set_tile(args.tile)
+            t = [int(x) for x in args.tile.split(",")]
+            L.set_tile(t[0] if len(t) == 1 else tuple(t))
         U = U0.clone()
         t = timed_steps(L, U, G, dt, args.steps, args.warmup)
         if not torch.isfinite(U).all():
@@ -522,8 +506,8 @@ def main():
     ap.add_argument("--workload", default=DEFAULT_WORKLOAD, choices=sorted(WORKLOADS))
     ap.add_argument("--precision", default="both", choices=["both", "fp64", "mixed"],
                     help="both: mixed is `value`, FP64 is reported beside it")
-    ap.add_argument("--tile", type=int, default=0, help="box-table tile edge of the level (0: 32, then the "
-                                                          "autotuner's shape if faster)")
+    ap.add_argument("--tile", default="", help="box-table tile edge or radial,alpha,beta shape of the level "
+                                                "(default: 32^3, then the autotuner's shape if faster)")
     ap.add_argument("--no-autotune", action="store_true", help="time the configured 32^3 tiles only")
     ap.add_argument("--cpu-boxes", type=int, default=0, help="boxes in the CPU sample (0: two per thread)")
     ap.add_argument("--no-cpu-baseline", action="store_true")

@@ -1546,24 +1546,40 @@ __device__ __forceinline__ double deriv3(F q, int x, int lo, int hi)
     return 0.5*(q(x+1)-q(x-1));
 }
 
-// Angular quotient with the fourth-order correction, valid cells. One block
-// per angular column (j = blockIdx.x, k and the panel in blockIdx.y), threads
-// along the radius: the metric terms are uniform over the block and shared by
-// the 8 components.
-__global__ void __launch_bounds__(kB) k4_angular(Prep4View v, const double* __restrict__ U0, const double* __restrict__ K,
-                                                 double a_dt, double* __restrict__ ang)
+// Logical index of a column block under the XCD-aware order of a 1-D grid:
+// the hardware deals consecutive blocks round-robin to the 8 XCDs, so XCD x
+// runs the contiguous logical range starting at start(x) and neighbouring
+// columns (j +- 1, and k +- 1 one row of na columns away) share its L2.
+__device__ __forceinline__ unsigned xcdColumn()
 {
+    const unsigned b = blockIdx.x, n = gridDim.x, x = b % kXcd, q = n / kXcd, r = n % kXcd;
+    return x*q + (x < r ? x : r) + b / kXcd;
+}
+
+// Angular quotient with the fourth-order correction (the metric terms are
+// uniform over a column), then the radial quotient with its correction, of
+// one angular column of valid cells: the angular quotient of each component
+// along the whole radial line goes to LDS, the radial quotient reads its
+// radial neighbours there and writes the Cartesian buffer. One block per
+// column in the XCD-aware order, threads along the radius; dynamic LDS of nr
+// doubles.
+__global__ void __launch_bounds__(kB) k4_angrad(Prep4View v, const double* __restrict__ U0, const double* __restrict__ K,
+                                                double a_dt, double* __restrict__ cart)
+{
+    extern __shared__ double line[];
     const int nr = v.nr, na = v.na;
-    const int j = blockIdx.x, k = blockIdx.y % na, p = blockIdx.y / na;
+    const unsigned col = xcdColumn();
+    const int j = (int)(col % (unsigned)na), k = (int)((col / (unsigned)na) % (unsigned)na), p = (int)(col / (unsigned)(na*na));
     const double djd = 0.5*(det4(v, j+1, k)-det4(v, j-1, k));
     const double dkd = 0.5*(det4(v, j, k+1)-det4(v, j, k-1));
     const double idet = 1.0/det4(v, j, k);
-    for (int i = threadIdx.x; i < nr; i += blockDim.x)
+    auto dv = [&](int ii) { return v.dvol[ii+6]; };
+#pragma unroll 1
+    for (int c = 0; c < 8; ++c)
     {
-#pragma unroll 2
-        for (int c = 0; c < 8; ++c)
+        const long long base = p*v.vps + c*v.vcs;
+        for (int i = threadIdx.x; i < nr; i += blockDim.x)
         {
-            const long long base = p*v.vps + c*v.vcs;
             auto ju = [&](int jj, int kk) {
                 const long long x = base + ((long long)kk*na + jj)*nr + i;
                 return K ? U0[x] + a_dt*K[x] : U0[x]; };
@@ -1572,31 +1588,18 @@ __global__ void __launch_bounds__(kB) k4_angular(Prep4View v, const double* __re
             double corr = 0;
             corr += deriv3(q1, j, 0, na-1)*djd;
             corr += deriv3(q2, k, 0, na-1)*dkd;
-            ang[base + ((long long)k*na + j)*nr + i] = (ju(j, k)-corr/12.0)*idet;
+            line[i] = (ju(j, k)-corr/12.0)*idet;
         }
-    }
-}
-
-// Radial quotient with its correction, into the Cartesian buffer. One block
-// per angular column, threads along the radius; the volume difference and 1/dv
-// are shared by the 8 components.
-__global__ void __launch_bounds__(kB) k4_radial(Prep4View v, const double* __restrict__ ang, double* __restrict__ cart)
-{
-    const int nr = v.nr, na = v.na;
-    const int j = blockIdx.x, k = blockIdx.y % na, p = blockIdx.y / na;
-    auto dv = [&](int ii) { return v.dvol[ii+6]; };
-    for (int i = threadIdx.x; i < nr; i += blockDim.x)
-    {
-        const double dd = 0.5*(dv(i+1)-dv(i-1));
-        const double idv = 1.0/dv(i);
-#pragma unroll 2
-        for (int c = 0; c < 8; ++c)
+        __syncthreads();
+        for (int i = threadIdx.x; i < nr; i += blockDim.x)
         {
-            const double* a = ang + p*v.vps + c*v.vcs + ((long long)k*na + j)*nr;
-            auto rq = [&](int ii) { return a[ii]/dv(ii); };
+            const double dd = 0.5*(dv(i+1)-dv(i-1));
+            const double idv = 1.0/dv(i);
+            auto rq = [&](int ii) { return line[ii]/dv(ii); };
             const double dq = deriv3(rq, i, 0, nr-1);
-            cart[p*v.cps + c*v.ccs + ((long long)(k+kGC)*v.cJ + (j+kGC))*nr + i] = (a[i]-dq*dd/12.0)*idv;
+            cart[p*v.cps + c*v.ccs + ((long long)(k+kGC)*v.cJ + (j+kGC))*nr + i] = (line[i]-dq*dd/12.0)*idv;
         }
+        __syncthreads();
     }
 }
 
@@ -1634,7 +1637,10 @@ __global__ void __launch_bounds__(kB) k4_seam(Prep4View v, const long long* __re
 __global__ void __launch_bounds__(kB) k4_native(Prep4View v, const double* __restrict__ cart, double* __restrict__ US)
 {
     const int NA = v.na+10;
-    const int j = (int)blockIdx.x - 5, k = (int)(blockIdx.y % NA) - 5, p = blockIdx.y / NA;
+    // 1-D grid over the 6 (na+10)^2 columns in the XCD-aware order
+    const unsigned col = xcdColumn();
+    const int j = (int)(col % (unsigned)NA) - 5, k = (int)((col / (unsigned)NA) % (unsigned)NA) - 5,
+              p = (int)(col / (unsigned)(NA*NA));
     for (int i = threadIdx.x; i < v.nr; i += blockDim.x)
     {
         auto ca = [&](int c, int jj, int kk) {
@@ -1718,7 +1724,6 @@ struct Level4
     int* colStart = nullptr;
     long long* donorSrc = nullptr;
     double* donorW = nullptr;
-    double* ang = nullptr;
     double* cart = nullptr;
     std::vector<void*> owned;
     ~Level4()
@@ -1911,7 +1916,6 @@ Level4* level4Create(const hc_shell& shell, const hc_options& opt, const int til
     L->colStart = upload(colStart, L->owned);
     L->donorSrc = upload(donorSrc, L->owned);
     L->donorW = upload(donorW, L->owned);
-    HC_THROW(hipMalloc(&L->ang, 6ull*8*na*na*nr*sizeof(double))); L->owned.push_back(L->ang);
     HC_THROW(hipMalloc(&L->cart, 6ull*v.cps*sizeof(double))); L->owned.push_back(L->cart);
     HC_THROW(hipMemset(L->cart, 0, 6ull*v.cps*sizeof(double)));
     L->box = levelBoxTable(*L, tile);
@@ -1937,12 +1941,11 @@ void level4Prepare(Level4& L, const double* U0, const double* K, double a_dt, co
     // one block per angular column (j in x; k and the panel in y), threads
     // along the radius in multiples of the wavefront, at most kB
     const dim3 colBlock(std::min(kB, (v.nr+63)/64*64));
-    hipLaunchKernelGGL(k4_angular, dim3(v.na, 6*v.na), colBlock, 0, s, v, U0, K, a_dt, L.ang);
-    hipLaunchKernelGGL(k4_radial, dim3(v.na, 6*v.na), colBlock, 0, s, v, L.ang, L.cart);
+    hipLaunchKernelGGL(k4_angrad, dim3(6u*v.na*v.na), colBlock, v.nr*sizeof(double), s, v, U0, K, a_dt, L.cart);
     if (L.ncol > 0)
         hipLaunchKernelGGL(k4_seam, dim3(L.ncol), colBlock, 0, s, v, L.colDst, L.colStart, L.donorSrc, L.donorW,
                            L.cart);
-    hipLaunchKernelGGL(k4_native, dim3(v.na+10, 6*(v.na+10)), colBlock, 0, s, v, L.cart, US);
+    hipLaunchKernelGGL(k4_native, dim3(6u*(v.na+10)*(v.na+10)), colBlock, 0, s, v, L.cart, US);
     hipLaunchKernelGGL(k4_radial_bc, dim3((10*6*(v.na+10)*(v.na+10)+kB-1)/kB), dim3(kB), 0, s, v, inner, US);
     HC_THROW(hipGetLastError());
 }
