@@ -378,6 +378,33 @@ def o4_rhs_boxes(n_rad, n_ang, boxes, USph, limiter=1, entropy_fix=0.3, gamma=5.
     return rhs
 
 
+_opc = None
+
+
+def opcount_box(n_rad, n_ang, box, USph, spatial_operator=1, limiter=1, riemann=2, eta_d=0.02, gamma=5.0 / 3.0,
+                r0=21.5, r1=215.0):
+    """(rhs, counts) of the box operator on one box from the operation-counting
+    build: counts = {add, mul, div, sqrt, cmp, other} over the box (the
+    geometry helpers excluded)."""
+    global _opc
+    if _opc is None:
+        path = os.path.join(_HERE, "_build", "libhcopcount.so")
+        if not os.path.exists(path):
+            build()
+        _opc = ctypes.CDLL(path)
+        _opc.orc_opcount_box.argtypes = [ctypes.POINTER(OrcCfg), ctypes.c_int] + [ctypes.c_void_p] * 5
+        _opc.orc_opcount_box.restype = None
+    blk, lo, hi = box
+    cfg = OrcCfg(n_rad, n_ang, 5, r0, r1, 0.0, gamma, limiter, riemann, 0.3, spatial_operator, eta_d, 0, 0)
+    lo_a, hi_a = np.array(lo, dtype=np.int32), np.array(hi, dtype=np.int32)
+    u = _c(USph)
+    rhs = np.zeros(8 * np.prod([hi[d] - lo[d] + 1 for d in range(3)]))
+    cnt = np.zeros(6, dtype=np.int64)
+    _opc.orc_opcount_box(ctypes.byref(cfg), blk, lo_a.ctypes.data, hi_a.ctypes.data, _p(u), _p(rhs),
+                         cnt.ctypes.data)
+    return rhs, dict(zip(("add", "mul", "div", "sqrt", "cmp", "other"), cnt.tolist()))
+
+
 def limit_line(w7):
     """(left state of the high face, right state of the low face) of the
     middle cell of a 7-cell line after Limiters_4O's limiter."""

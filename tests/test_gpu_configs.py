@@ -77,17 +77,24 @@ def test_amr_652_box_table_matches_oracle_box_by_box(device, operator):
     ref = _oracle_table(levels, table, U, opts)
     got, gotm = R.cpu().numpy(), Rm.cpu().numpy()
     assert np.isfinite(got).all()
-    # box by box at its own level
-    off, worst = 0, 0.0
+    # box by box at its own level: every box's error, relative to the largest
+    # |RHS| of each variable over the table (the bar's normalization), within
+    # 1e-12; relative to the box's own largest |RHS| (stricter: a near-steady
+    # box amplifies round-off differences of its face fluxes) within 1e-10
+    scale = np.abs(per_var(ref, boxes)).max(axis=1)
+    off, worst_table, worst_box = 0, 0.0, 0.0
     for _, (blk, lo, hi) in table:
         m = 8 * np.prod([hi[d] - lo[d] + 1 for d in range(3)])
-        e = rel_linf_per_var(got[off:off + m].reshape(8, -1), ref[off:off + m].reshape(8, -1), axis_var=0)
-        worst = max(worst, e.max())
+        d = np.abs(got[off:off + m] - ref[off:off + m]).reshape(8, -1).max(axis=1)
+        worst_table = max(worst_table, (d / scale).max())
+        own = np.abs(ref[off:off + m]).reshape(8, -1).max(axis=1)
+        worst_box = max(worst_box, (d / np.where(own == 0, 1.0, own)).max())
         off += m
     errm = rel_linf_per_var(per_var(gotm, boxes), per_var(ref, boxes), axis_var=0)
-    print(f"worst per-box FP64 error {worst:.2e}; mixed over the table {errm.max():.2e} "
-          f"({time.time() - t0:.1f} s)", flush=True)
-    assert worst <= TOL
+    print(f"worst per-box FP64 error {worst_table:.2e} (table scale), {worst_box:.2e} (box scale); "
+          f"mixed over the table {errm.max():.2e} ({time.time() - t0:.1f} s)", flush=True)
+    assert worst_table <= TOL
+    assert worst_box <= 1e-10
     assert errm.max() <= 1e-6, errm
 
 
