@@ -83,6 +83,7 @@ _SIGNATURES = [
     ("hc_amr_ghosts2", _I, [_P, _P, _P, _P, _P, _D, _P, _P]),
     ("hc_amr_average_down", _I, [_P, _P, _I, _P, _P, _P, _P]),
     ("hc_amr_prolong4", _I, [_P, _P, _P, _P, _P, _P, _D, _P, _P, _P]),
+    ("hc_amr_prolong4_noalloc", _I, [_P, _P, _P, _P, _P, _P, _D, _P, _P, _P]),
     ("hc_init_state", _I, [ctypes.POINTER(HcShell), ctypes.POINTER(HcOptions), _I, ctypes.c_ulonglong, _P, _P]),
     ("hc_geom_cell_detA", _D, [_I, _I, _I]),
     ("hc_geom_face_normal", None, [_I, _I, _I, _I, _I, _P]),

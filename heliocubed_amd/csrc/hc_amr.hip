@@ -26,8 +26,18 @@
 #include "hc_physics.hpp"
 
 #include <cmath>
+#include <mutex>
 #include <utility>
 #include <vector>
+
+// Per-parent flags of the allocation-free conservative refinement
+// (hc_amr_prolong4_noalloc) and, after the last flag slot, their count: a
+// file-scope device array, so that entry point calls no allocator. Each
+// device the library runs on holds its own copy (4 MiB) until the process
+// ends; host code reaches it only through the address hipGetSymbolAddress
+// returns on that device.
+constexpr long long kNoAllocParents = 1LL << 20;
+static __device__ __align__(256) int scratch[kNoAllocParents + 64];
 
 namespace hc
 {
@@ -317,6 +327,23 @@ __global__ void __launch_bounds__(kAmrBlock) k_amr_prolong4(Prolong4View v, cons
     status[t] = 0;
 }
 
+// Sum of n per-parent flags into *count (one block).
+__global__ void __launch_bounds__(kAmrBlock) k_count_flags(const int* __restrict__ flags, long long n,
+                                                           int* __restrict__ count)
+{
+    __shared__ int part[kAmrBlock];
+    int sum = 0;
+    for (long long x = threadIdx.x; x < n; x += blockDim.x) sum += flags[x];
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    for (int w = kAmrBlock/2; w > 0; w >>= 1)
+    {
+        if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *count = part[0];
+}
+
 // Average over the child interval [x - h/2, x + h/2] (coarse cell units,
 // parent centred at 0) of the quartic whose averages over the coarse cells
 // [k - 1/2, k + 1/2], k = -2..2, are the data: solve sum_k w_k <xi^n>_k =
@@ -361,6 +388,58 @@ bool validBox(const hc_box* b)
 }
 } // namespace
 } // namespace hc
+
+namespace
+{
+// Argument checks and the launch view shared by both refinement entry points:
+// an error text, or nullptr with v filled.
+const char* prolongSetup(const double* interp, const double* fine_j, const hc_box* fine_box, const double* coarse,
+                         const hc_box* coarse_box, const int* ratio, double gamma, const double* fine_ju,
+                         hc::Prolong4View& v)
+{
+    using namespace hc;
+    if (!interp || !fine_j || !coarse || !fine_ju || !ratio || !validBox(fine_box) || !validBox(coarse_box))
+        return "null argument or empty box";
+    v = Prolong4View{};
+    v.fine = toPatch(*fine_box);
+    v.coarse = toPatch(*coarse_box);
+    v.gamma = gamma;
+    for (int d = 0; d < 3; ++d)
+    {
+        if (ratio[d] < 1) return "ratio must be >= 1 per direction";
+        v.ratio[d] = ratio[d];
+        // the fine box must be whole parents, all inside the coarse box
+        if (floorDiv(fine_box->lo[d], ratio[d])*ratio[d] != fine_box->lo[d] ||
+            floorDiv(fine_box->hi[d] + 1, ratio[d])*ratio[d] != fine_box->hi[d] + 1)
+            return "the fine box must cover whole coarse cells";
+        v.parents.lo[d] = floorDiv(fine_box->lo[d], ratio[d]);
+        v.parents.hi[d] = floorDiv(fine_box->hi[d] + 1, ratio[d]) - 1;
+        if (v.parents.lo[d] < coarse_box->lo[d] || v.parents.hi[d] > coarse_box->hi[d])
+            return "parents outside the coarse box";
+    }
+    return nullptr;
+}
+
+// The device address of `scratch` on the current device, looked up once per
+// device.
+int* noallocScratch()
+{
+    constexpr int kMaxDevices = 64;
+    static std::mutex guard;
+    static int* address[kMaxDevices] = {};
+    int dev = 0;
+    HC_THROW(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) throw std::runtime_error("device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(guard);
+    if (!address[dev])
+    {
+        void* p = nullptr;
+        HC_THROW(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        address[dev] = static_cast<int*>(p);
+    }
+    return address[dev];
+}
+} // namespace
 
 extern "C" {
 
@@ -449,32 +528,17 @@ int hc_amr_average_down(const double* fine, const hc_box* fine_box, int ncomp, c
     catch (const std::exception& e) { return fail(std::string("hc_amr_average_down: ") + e.what()); }
 }
 
+
 int hc_amr_prolong4(const double* interp, const double* fine_j, const hc_box* fine_box, const double* coarse,
                     const hc_box* coarse_box, const int* ratio, double gamma, double* fine_ju, int* invalid,
                     hc_stream stream)
 {
     using namespace hc;
-    if (!interp || !fine_j || !coarse || !fine_ju || !ratio || !validBox(fine_box) || !validBox(coarse_box))
-        return fail("hc_amr_prolong4: null argument or empty box");
     try
     {
-        Prolong4View v{};
-        v.fine = toPatch(*fine_box);
-        v.coarse = toPatch(*coarse_box);
-        v.gamma = gamma;
-        for (int d = 0; d < 3; ++d)
-        {
-            if (ratio[d] < 1) return fail("hc_amr_prolong4: ratio must be >= 1 per direction");
-            v.ratio[d] = ratio[d];
-            // the fine box must be whole parents, all inside the coarse box
-            if (floorDiv(fine_box->lo[d], ratio[d])*ratio[d] != fine_box->lo[d] ||
-                floorDiv(fine_box->hi[d] + 1, ratio[d])*ratio[d] != fine_box->hi[d] + 1)
-                return fail("hc_amr_prolong4: the fine box must cover whole coarse cells");
-            v.parents.lo[d] = floorDiv(fine_box->lo[d], ratio[d]);
-            v.parents.hi[d] = floorDiv(fine_box->hi[d] + 1, ratio[d]) - 1;
-            if (v.parents.lo[d] < coarse_box->lo[d] || v.parents.hi[d] > coarse_box->hi[d])
-                return fail("hc_amr_prolong4: parents outside the coarse box");
-        }
+        Prolong4View v;
+        if (const char* e = prolongSetup(interp, fine_j, fine_box, coarse, coarse_box, ratio, gamma, fine_ju, v))
+            return fail(std::string("hc_amr_prolong4: ") + e);
         const long long n = v.parents.count();
         int* status = nullptr;
         HC_THROW(hipMalloc(&status, n*sizeof(int)));
@@ -494,6 +558,40 @@ int hc_amr_prolong4(const double* interp, const double* fine_j, const hc_box* fi
         return 0;
     }
     catch (const std::exception& e) { return fail(std::string("hc_amr_prolong4: ") + e.what()); }
+}
+
+int hc_amr_prolong4_noalloc(const double* interp, const double* fine_j, const hc_box* fine_box, const double* coarse,
+                            const hc_box* coarse_box, const int* ratio, double gamma, double* fine_ju, int* invalid,
+                            hc_stream stream)
+{
+    using namespace hc;
+    try
+    {
+        Prolong4View v;
+        if (const char* e = prolongSetup(interp, fine_j, fine_box, coarse, coarse_box, ratio, gamma, fine_ju, v))
+            return fail(std::string("hc_amr_prolong4_noalloc: ") + e);
+        const long long n = v.parents.count();
+        if (n > kNoAllocParents)
+            return fail("hc_amr_prolong4_noalloc: " + std::to_string(n) + " parent cells exceed the " +
+                        std::to_string(kNoAllocParents) + " of the static scratch (use hc_amr_prolong4)");
+        int* flags = noallocScratch();
+        int* count = flags + kNoAllocParents;
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        // every flag of the n parents is written by the refinement kernel, and
+        // the count by the sum kernel: nothing in the scratch needs clearing
+        hipLaunchKernelGGL(k_amr_prolong4, dim3((unsigned)((n + kAmrBlock - 1)/kAmrBlock)), dim3(kAmrBlock), 0, s,
+                           v, interp, fine_j, coarse, fine_ju, flags);
+        hipLaunchKernelGGL(k_count_flags, dim3(1), dim3(kAmrBlock), 0, s, flags, n, count);
+        int bad = 0;
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipMemcpyAsync(&bad, count, sizeof(int), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        HC_THROW(e);
+        if (invalid) *invalid = bad;
+        if (bad) return fail("hc_amr_prolong4_noalloc: invalid or unavailable parent data for conservative refinement");
+        return 0;
+    }
+    catch (const std::exception& e) { return fail(std::string("hc_amr_prolong4_noalloc: ") + e.what()); }
 }
 
 } // extern "C"

@@ -353,6 +353,19 @@ int hc_amr_prolong4(const double* interp, const double* fine_j, const hc_box* fi
                     const hc_box* coarse_box, const int* ratio, double gamma, double* fine_ju, int* invalid,
                     hc_stream stream);
 
+/* hc_amr_prolong4_noalloc: hc_amr_prolong4 (same arguments, results, error
+ * behaviour and stream synchronisation) that allocates nothing: its
+ * per-parent flags and their count live in a static device array of the
+ * library (4 MiB per GPU the library runs on, held until the process ends),
+ * so it calls no hipMalloc or hipFree. At most 1048576 parent cells per call
+ * (more is rejected with an error; use hc_amr_prolong4). Calls that overlap
+ * on one GPU race on that array. No counterpart in the reference: it is for
+ * callers that cannot allocate (tight serving loops, harnesses that forbid
+ * device allocation). */
+int hc_amr_prolong4_noalloc(const double* interp, const double* fine_j, const hc_box* fine_box, const double* coarse,
+                            const hc_box* coarse_box, const int* ratio, double gamma, double* fine_ju, int* invalid,
+                            hc_stream stream);
+
 #ifdef __cplusplus
 }
 #endif
