@@ -398,7 +398,10 @@ __device__ __forceinline__ void bertaRadialMapped(const FaceOps<0>& op, const in
 // 7. Face flux on [lo_d, hi_d+1] x [lo, hi]: radial-area tensor product of the
 // mapped products, pressure force ntrow <(p_tot averaged) adjDr>, the
 // theta-aware averaged Roe dissipation with its metric products, and
-// F = (Sum - D)/2.
+// F = (Sum - D)/2. Evaluated one input component at a time (its stencil of
+// averaged fluxes and dissipation, then its contributions to the outputs it
+// enters), which bounds the live registers; every output accumulates its
+// terms in the same order as the term-by-term form.
 template <int D, bool ROE>
 __global__ void __launch_bounds__(kB) k_b_flux(O4View v, double* __restrict__ S)
 {
@@ -414,38 +417,36 @@ __global__ void __launch_bounds__(kB) k_b_flux(O4View v, double* __restrict__ S)
     auto Fv = [&](int f, const int y[3]) { return op.Fv(f, y); };
     const double adjX = areaOf(v, b, D, x[0]);
     auto dAdj = [&](int i) { return 0.5*areaOf(v, b, D, i+1) - 0.5*areaOf(v, b, D, i-1); };
-    double Fsum[8];
-    if constexpr (D == 0)
-    {
-        double bmp[8];
-        bertaRadialMapped(op, x, bmp);
+    const double dA = D == 0 ? 0.0 : dAdj(x[0]);
+    auto shift = [&](int f, int t, int o) { int y[3] = {x[0], x[1], x[2]}; y[t] += o; return Fv(f, y); };
+    auto shift2 = [&](int f, int o0, int o2) { int y[3] = {x[0]+o0, x[1], x[2]}; y[t2] += o2; return Fv(f, y); };
+    // Df(f, t) at x and, for the angular radial difference, the half radial
+    // differences Gv(f, .) at x and x +- e_t2 (Gv = (F(+e_r) - F(-e_r))/2)
+    auto Df = [&](int f, int t) { return 0.5*shift(f, t, 1) - 0.5*shift(f, t, -1); };
+    auto Gv0 = [&](int f) { return 0.5*shift(f, 0, 1) - 0.5*shift(f, 0, -1); };
+    auto DGv = [&](int f) {
+        const double gp = 0.5*shift2(f, 1, 1) - 0.5*shift2(f, -1, 1);
+        const double gm = 0.5*shift2(f, 1, -1) - 0.5*shift2(f, -1, -1);
+        return 0.5*gp - 0.5*gm; };
+    // face-average Roe dissipation of component f (theta-aware)
+    const bool smooth = Fv(F_THF, x) > 0.5;
+    auto Davg = [&](int f) {
+        const double c0 = Fv(f, x);
+        double lap = 0;
 #pragma unroll
-        for (int c = 0; c < 8; ++c)
+        for (int t = 0; t < 2; ++t)
         {
-            double val = 0;
-            val += bmp[c]*adjX*1.0;
-            Fsum[c] = val;
+            const int tt = t == 0 ? t1 : t2;
+            lap += shift(f, tt, 1) - 2.0*c0 + shift(f, tt, -1);
         }
-    }
-    else
-    {
-        // angular faces: the mapped products of the averaged flux at the face
-        // and of its half radial difference (the angular metric is constant
-        // along the radius, so the latter is the radial difference of the
-        // mapped products)
-        double mp[8], dmp[8];
-        op.mapped(F_PFA, x, mp);
-        op.mappedRadialDiff(F_PFA, x, dmp);
-        const double dA = dAdj(x[0]);
-#pragma unroll
-        for (int c = 0; c < 8; ++c)
-        {
-            double val = 0;
-            val += dmp[c]*dA*(1.0/12.0);
-            val += mp[c]*adjX*1.0;
-            Fsum[c] = val;
-        }
-    }
+        return smooth ? c0 + lap/24.0 : c0; };
+    // Fsum of a component from its mapped products (radial: bertaRadialMapped's
+    // per-column terms; angular: FaceOps::mapped and mappedRadialDiff)
+    auto fsum = [&](double mp, double dmp) {
+        double val = 0;
+        if (D != 0) val += dmp*dA*(1.0/12.0);
+        val += mp*adjX*1.0;
+        return val; };
     // <p_tot adjDr> at y: radial tensor product of the averaged total pressure
     auto tmp3 = [&](const int y[3]) {
         double val = 0;
@@ -457,83 +458,115 @@ __global__ void __launch_bounds__(kB) k_b_flux(O4View v, double* __restrict__ S)
         }
         val += Fv(F_PAVG, y)*areaOf(v, b, D, y[0])*1.0;
         return val; };
-    const double t0v = tmp3(x);
-    double dT[2] = {0.0, 0.0};
-#pragma unroll
-    for (int t = D == 0 ? 0 : 1; t < 2; ++t)
+    double press[3];
     {
-        const int tt = t == 0 ? t1 : t2;
-        int y[3] = {x[0], x[1], x[2]}, z[3] = {x[0], x[1], x[2]};
-        y[tt] += 1; z[tt] -= 1;
-        dT[t] = 0.5*tmp3(y) - 0.5*tmp3(z);
-    }
+        const double t0v = tmp3(x);
+        double dT[2] = {0.0, 0.0};
 #pragma unroll
-    for (int r = 0; r < 3; ++r)
-    {
-        double val = 0;
-        if (D == 0) val += op.Dm(10+r, t1, x)*dT[0]*(1.0/12.0);
-        val += op.Dm(10+r, t2, x)*dT[1]*(1.0/12.0);
-        val += op.M(10+r, x)*t0v*1.0;
-        Fsum[1+r] = Fsum[1+r] + val;
-    }
-    // theta-aware averaged point dissipation and its metric products
-    const bool smooth = Fv(F_THF, x) > 0.5;
-    double Davg[8];
-#pragma unroll
-    for (int c = 0; c < 8; ++c)
-    {
-        double lap = 0;
-#pragma unroll
-        for (int t = 0; t < 2; ++t)
+        for (int t = D == 0 ? 0 : 1; t < 2; ++t)
         {
             const int tt = t == 0 ? t1 : t2;
             int y[3] = {x[0], x[1], x[2]}, z[3] = {x[0], x[1], x[2]};
             y[tt] += 1; z[tt] -= 1;
-            lap += Fv(F_DPT+c, y) - 2.0*Fv(F_DPT+c, x) + Fv(F_DPT+c, z);
+            dT[t] = 0.5*tmp3(y) - 0.5*tmp3(z);
         }
-        Davg[c] = smooth ? Fv(F_DPT+c, x) + lap/24.0 : Fv(F_DPT+c, x);
-    }
-    double Fd[8];
-    if constexpr (!ROE)
-    {
-        // Rusanov: the averaged jump mapped by the face metric averages, times the speed
-        double dAA[9];
 #pragma unroll
-        for (int a = 0; a < 9; ++a) dAA[a] = op.M(1+a, x);
-        rusanovMapped(Fd, Davg, op.M(0, x), dAA, Fv(F_SPD, x));
+        for (int r = 0; r < 3; ++r)
+        {
+            double val = 0;
+            if (D == 0) val += op.Dm(10+r, t1, x)*dT[0]*(1.0/12.0);
+            val += op.Dm(10+r, t2, x)*dT[1]*(1.0/12.0);
+            val += op.M(10+r, x)*t0v*1.0;
+            press[r] = val;
+        }
     }
-    else
-    {
-#pragma unroll
+    const double speed = ROE ? 0.0 : Fv(F_SPD, x);
+    // scalar components 0 and 4
+#pragma unroll 1
     for (int s = 0; s < 2; ++s)
     {
         const int c = s == 0 ? 0 : 4;
-        double val = 0;
-        val += op.Dm(0, t1, x)*op.Df(F_DPT+c, t1, x)*(1.0/12.0);
-        val += op.Dm(0, t2, x)*op.Df(F_DPT+c, t2, x)*(1.0/12.0);
-        val += op.M(0, x)*Davg[c];
-        Fd[c] = val;
-    }
-#pragma unroll
-    for (int r = 0; r < 3; ++r)
-#pragma unroll
-        for (int grp = 0; grp < 2; ++grp)
+        double mp = 0, dmp = 0;
+        if (D == 0)
         {
-            const int c0 = grp == 0 ? 1 : 5;
-            double val = 0;
+            double term = 0;
+            term += op.Dm(0, t1, x)*Df(F_PFA+c, t1)*(1.0/12.0);
+            term += op.Dm(0, t2, x)*Df(F_PFA+c, t2)*(1.0/12.0);
+            term += op.M(0, x)*Fv(F_PFA+c, x)*1.0;
+            mp += term;
+        }
+        else
+        {
+            mp += op.Dm(0, t2, x)*Df(F_PFA+c, t2)*(1.0/12.0);
+            mp += op.M(0, x)*Fv(F_PFA+c, x);
+            dmp += op.Dm(0, t2, x)*DGv(F_PFA+c)*(1.0/12.0);
+            dmp += op.M(0, x)*Gv0(F_PFA+c);
+        }
+        const double fs = fsum(mp, dmp);
+        const double da = Davg(F_DPT+c);
+        double fd;
+        if constexpr (ROE)
+        {
+            fd = 0;
+            fd += op.Dm(0, t1, x)*Df(F_DPT+c, t1)*(1.0/12.0);
+            fd += op.Dm(0, t2, x)*Df(F_DPT+c, t2)*(1.0/12.0);
+            fd += op.M(0, x)*da;
+        }
+        else fd = (op.M(0, x)*da)*speed;
+        P[I(F_FLUX+8*D+c, x[0], x[1], x[2])] = 0.5*(fs-fd);
+    }
+    // vector components: momentum (grp 0, with the pressure force) and field
+#pragma unroll 1
+    for (int grp = 0; grp < 2; ++grp)
+    {
+        const int c0 = grp == 0 ? 1 : 5;
+        double mp[3] = {0, 0, 0}, dmp[3] = {0, 0, 0}, fd[3] = {0, 0, 0};
+#pragma unroll 1
+        for (int col = 0; col < 3; ++col)
+        {
+            const int f = F_PFA+c0+col, g = F_DPT+c0+col;
+            const double fx = Fv(f, x), df2 = Df(f, t2);
+            const double df1 = D == 0 ? Df(f, t1) : 0.0;
+            const double gx = D == 0 ? 0.0 : Gv0(f), dg = D == 0 ? 0.0 : DGv(f);
+            const double da = Davg(g);
+            const double dd1 = ROE ? Df(g, t1) : 0.0, dd2 = ROE ? Df(g, t2) : 0.0;
 #pragma unroll
-            for (int c = 0; c < 3; ++c)
+            for (int r = 0; r < 3; ++r)
             {
-                val += op.Dm(1+3*r+c, t1, x)*op.Df(F_DPT+c0+c, t1, x)*(1.0/12.0);
-                val += op.Dm(1+3*r+c, t2, x)*op.Df(F_DPT+c0+c, t2, x)*(1.0/12.0);
-                val += op.M(1+3*r+c, x)*Davg[c0+c];
+                const int m = 1+3*r+col;
+                if (D == 0)
+                {
+                    double term = 0;
+                    term += op.Dm(m, t1, x)*df1*(1.0/12.0);
+                    term += op.Dm(m, t2, x)*df2*(1.0/12.0);
+                    term += op.M(m, x)*fx*1.0;
+                    mp[r] += term;
+                }
+                else
+                {
+                    mp[r] += op.Dm(m, t2, x)*df2*(1.0/12.0);
+                    mp[r] += op.M(m, x)*fx;
+                    dmp[r] += op.Dm(m, t2, x)*dg*(1.0/12.0);
+                    dmp[r] += op.M(m, x)*gx;
+                }
+                if constexpr (ROE)
+                {
+                    fd[r] += op.Dm(m, t1, x)*dd1*(1.0/12.0);
+                    fd[r] += op.Dm(m, t2, x)*dd2*(1.0/12.0);
+                    fd[r] += op.M(m, x)*da;
+                }
+                else fd[r] += op.M(m, x)*da;
             }
-            Fd[c0+r] = val;
+        }
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+        {
+            double fs = fsum(mp[r], dmp[r]);
+            if (grp == 0) fs = fs + press[r];
+            const double d = ROE ? fd[r] : fd[r]*speed;
+            P[I(F_FLUX+8*D+c0+r, x[0], x[1], x[2])] = 0.5*(fs-d);
         }
     }
-#pragma unroll
-    for (int c = 0; c < 8; ++c)
-        P[I(F_FLUX+8*D+c, x[0], x[1], x[2])] = 0.5*(Fsum[c]-Fd[c]);
 }
 
 // 4b. Powell face products of direction D (divergence_cleaning 1,

@@ -424,31 +424,6 @@ __device__ double rusanovSpeedO4(const double L[8], const double R[8], const dou
     return fabs(w_d) + af*Ad;
 }
 
-// Mapped Rusanov dissipation of an averaged conserved jump: det(A) A times the
-// vector parts (plain matrix-vector product), det A times the scalars, times
-// the speed.
-__device__ __forceinline__ void rusanovMapped(double out[8], const double diff[8], double detA,
-                                              const double* detAA, double speed)
-{
-    out[0] = detA*diff[0];
-    out[4] = detA*diff[4];
-#pragma unroll
-    for (int g = 0; g < 2; ++g)
-    {
-        const int c0 = g == 0 ? 1 : 5;
-#pragma unroll
-        for (int r = 0; r < 3; ++r)
-        {
-            double acc = 0.0;
-#pragma unroll
-            for (int j = 0; j < 3; ++j) acc += detAA[3*r+j]*diff[c0+j];
-            out[c0+r] = acc;
-        }
-    }
-#pragma unroll
-    for (int c = 0; c < 8; ++c) out[c] *= speed;
-}
-
 // ---- kernels ------------------------------------------------------------------
 // 1. Cell-average primitives WBar and their magnetic energy on box+5.
 __global__ void __launch_bounds__(kB) k_o4_wbar(O4View v, const double* __restrict__ U, double* __restrict__ S)
@@ -941,9 +916,12 @@ struct FaceOps
 // 6. Mapped face flux on the flux faces [lo_d, hi_d+1] x [lo, hi]: the
 // radial-area tensor product of the mapped products, the pressure force
 // through the cofactor row, the averaged Roe dissipation, and
-// F = (F_L + F_R - D)/2.
+// F = (F_L + F_R - D)/2. Evaluated one input component at a time (its
+// stencils, then its contributions to the outputs it enters), which bounds
+// the live registers; every output accumulates its terms in the same order
+// as the term-by-term form of FaceOps.
 template <int D, bool ROE>
-__global__ void __launch_bounds__(kB) k_o4_flux(O4View v, double* __restrict__ S)
+__global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(4))) k_o4_flux(O4View v, double* __restrict__ S)
 {
     const BoxDesc& b = blockBoxDesc(v);
     int L[3], H[3], x[3];
@@ -957,81 +935,71 @@ __global__ void __launch_bounds__(kB) k_o4_flux(O4View v, double* __restrict__ S
     auto M = [&](int c, const int y[3]) { return op.M(c, y); };
     auto Dm = [&](int c, int t, const int y[3]) { return op.Dm(c, t, y); };
     auto Fv = [&](int f, const int y[3]) { return op.Fv(f, y); };
-    auto Df = [&](int f, int t, const int y[3]) { return op.Df(f, t, y); };
+    auto shift = [&](int f, int t, int o) { int y[3] = {x[0], x[1], x[2]}; y[t] += o; return Fv(f, y); };
+    auto shift2 = [&](int f, int o0, int o2) { int y[3] = {x[0]+o0, x[1], x[2]}; y[t2] += o2; return Fv(f, y); };
+    auto Df = [&](int f, int t) { return 0.5*shift(f, t, 1) - 0.5*shift(f, t, -1); };
+    auto Gv0 = [&](int f) { return 0.5*shift(f, 0, 1) - 0.5*shift(f, 0, -1); };
+    auto DGv = [&](int f) {
+        const double gp = 0.5*shift2(f, 1, 1) - 0.5*shift2(f, -1, 1);
+        const double gm = 0.5*shift2(f, 1, -1) - 0.5*shift2(f, -1, -1);
+        return 0.5*gp - 0.5*gm; };
     const double adjX = areaOf(v, b, D, x[0]);
     // The radial area varies only with the radial index, so its transverse
     // difference is exactly zero unless the transverse direction is radial:
     // t2 is never radial and t1 is radial for D = 1, 2. The zero terms of the
     // product rule add +-0 and are left out (bit-identical for finite data).
     const double adjD1 = D == 0 ? 0.0 : 0.5*areaOf(v, b, D, x[0]+1) - 0.5*areaOf(v, b, D, x[0]-1);
-    // face-average total pressure at the transverse neighbours
-    double p0bD[2], radPD[2];
-#pragma unroll
-    for (int t = 0; t < 2; ++t)
-    {
-        const int tt = t == 0 ? t1 : t2;
-        int y[3] = {x[0], x[1], x[2]}, z[3] = {x[0], x[1], x[2]};
-        y[tt] += 1; z[tt] -= 1;
-        const double pa = Fv(F_P0B, y), pz = Fv(F_P0B, z);
-        p0bD[t] = 0.5*pa - 0.5*pz;
-        radPD[t] = 0.5*(pa*areaOf(v, b, D, y[0])) - 0.5*(pz*areaOf(v, b, D, z[0]));
-    }
     // Both sides at once: the mapped products and the pressure force are linear
     // in the point flux and the face-average total pressure, so they act on the
     // sums PHL + PHR and P0L + P0R that k_o4_faces stores; the terms of the
     // face-average total pressure p0Bar, common to both sides, count twice.
-    double Fsum[8];
-    if constexpr (D == 0)
+    auto fsum = [&](double mp, double dmp) {
+        double val = 0;
+        if (D != 0) val += adjD1*dmp*(1.0/12.0);
+        val += adjX*mp;
+        return val; };
+    // pressure force: p0 (face average) with p0Bar = pBar + meBar
+    double press[3];
     {
-        // radial faces: the mapped products at the face itself
-        double mp[8];
-        op.mapped(F_PHS, x, mp);
+        double p0bD[2], radPD[2];
 #pragma unroll
-        for (int c = 0; c < 8; ++c)
+        for (int t = 0; t < 2; ++t)
         {
-            double val = 0;
-            val += adjX*mp[c];
-            Fsum[c] = val;
+            const int tt = t == 0 ? t1 : t2;
+            int y[3] = {x[0], x[1], x[2]}, z[3] = {x[0], x[1], x[2]};
+            y[tt] += 1; z[tt] -= 1;
+            const double pa = Fv(F_P0B, y), pz = Fv(F_P0B, z);
+            p0bD[t] = 0.5*pa - 0.5*pz;
+            radPD[t] = 0.5*(pa*areaOf(v, b, D, y[0])) - 0.5*(pz*areaOf(v, b, D, z[0]));
         }
-    }
-    else
-    {
-        // angular faces: the mapped products at the face and the half radial
-        // difference of those at its radial neighbours
-        double mp[8], dmp[8];
-        op.mapped(F_PHS, x, mp);
-        op.mappedRadialDiff(F_PHS, x, dmp);
-#pragma unroll
-        for (int c = 0; c < 8; ++c)
-        {
-            double val = 0;
-            val += adjD1*dmp[c]*(1.0/12.0);
-            val += adjX*mp[c];
-            Fsum[c] = val;
-        }
-    }
-    {
-        // pressure force: p0 (face average) with p0Bar = pBar + meBar
         double tmp3 = 0;
         if (D != 0) tmp3 += 2.0*(adjD1*p0bD[0]*(1.0/12.0));
         tmp3 += adjX*Fv(F_P0S, x);
+#pragma unroll
         for (int r = 0; r < 3; ++r)
         {
             double pf = 0;
             pf += 2.0*(radPD[0]*Dm(10+r, t1, x)*(1.0/12.0));
             pf += 2.0*(radPD[1]*Dm(10+r, t2, x)*(1.0/12.0));
             pf += tmp3*M(10+r, x);
-            Fsum[1+r] += pf;
+            press[r] = pf;
         }
     }
-    double Fd[8];
-    if constexpr (!ROE)
-    {
-        // Rusanov: face-averaged conserved values of both sides, their
-        // difference mapped by the face metric averages, times the speed
-        double diff[8];
-#pragma unroll
-        for (int c = 0; c < 8; ++c)
+    // face-averaged dissipation input of component c: the averaged point Roe
+    // dissipation (ROE) or the difference of both sides' averaged conserved
+    // values (Rusanov)
+    auto dissAvg = [&](int c) {
+        if constexpr (ROE)
+        {
+            double lap = 0;
+            for (int t = 0; t < 2; ++t)
+            {
+                const int tt = t == 0 ? t1 : t2;
+                lap += shift(F_DPT+c, tt, 1) - 2.0*Fv(F_DPT+c, x) + shift(F_DPT+c, tt, -1);
+            }
+            return Fv(F_DPT+c, x) + lap/24.0;
+        }
+        else
         {
             double a2[2];
 #pragma unroll
@@ -1043,62 +1011,86 @@ __global__ void __launch_bounds__(kB) k_o4_flux(O4View v, double* __restrict__ S
                 for (int t = 0; t < 2; ++t)
                 {
                     const int tt = t == 0 ? t1 : t2;
-                    int y[3] = {x[0], x[1], x[2]}, z[3] = {x[0], x[1], x[2]};
-                    y[tt] += 1; z[tt] -= 1;
-                    lap += Fv(f, y) - 2.0*Fv(f, x) + Fv(f, z);
+                    lap += shift(f, tt, 1) - 2.0*Fv(f, x) + shift(f, tt, -1);
                 }
                 a2[s] = Fv(f, x) + lap/24.0;
             }
-            diff[c] = a2[1] - a2[0];
+            return a2[1] - a2[0];
         }
-        double dAA[9];
-#pragma unroll
-        for (int a = 0; a < 9; ++a) dAA[a] = M(1+a, x);
-        rusanovMapped(Fd, diff, M(0, x), dAA, Fv(F_SPD, x));
-    }
-    else
-    {
-    // averaged point dissipation and its metric products
-    double Davg[8];
-#pragma unroll
-    for (int c = 0; c < 8; ++c)
-    {
-        double lap = 0;
-        for (int t = 0; t < 2; ++t)
-        {
-            const int tt = t == 0 ? t1 : t2;
-            int y[3] = {x[0], x[1], x[2]}, z[3] = {x[0], x[1], x[2]};
-            y[tt] += 1; z[tt] -= 1;
-            lap += Fv(F_DPT+c, y) - 2.0*Fv(F_DPT+c, x) + Fv(F_DPT+c, z);
-        }
-        Davg[c] = Fv(F_DPT+c, x) + lap/24.0;
-    }
+    };
+    const double speed = ROE ? 0.0 : Fv(F_SPD, x);
+    // scalar components 0 and 4
+#pragma unroll 1
     for (int s = 0; s < 2; ++s)
     {
         const int c = s == 0 ? 0 : 4;
-        double val = 0;
-        val += Dm(0, t1, x)*Df(F_DPT+c, t1, x)*(1.0/12.0);
-        val += Dm(0, t2, x)*Df(F_DPT+c, t2, x)*(1.0/12.0);
-        val += M(0, x)*Davg[c];
-        Fd[c] = val;
-    }
-    for (int r = 0; r < 3; ++r)
-        for (int grp = 0; grp < 2; ++grp)
+        double mp = 0, dmp = 0;
+        if (D == 0) mp += Dm(0, t1, x)*Df(F_PHS+c, t1)*(1.0/12.0);
+        mp += Dm(0, t2, x)*Df(F_PHS+c, t2)*(1.0/12.0);
+        mp += M(0, x)*Fv(F_PHS+c, x);
+        if (D != 0)
         {
-            const int c0 = grp == 0 ? 1 : 5;
-            double val = 0;
-            for (int c = 0; c < 3; ++c)
+            dmp += Dm(0, t2, x)*DGv(F_PHS+c)*(1.0/12.0);
+            dmp += M(0, x)*Gv0(F_PHS+c);
+        }
+        const double fs = fsum(mp, dmp);
+        const double da = dissAvg(c);
+        double fd;
+        if constexpr (ROE)
+        {
+            fd = 0;
+            fd += Dm(0, t1, x)*Df(F_DPT+c, t1)*(1.0/12.0);
+            fd += Dm(0, t2, x)*Df(F_DPT+c, t2)*(1.0/12.0);
+            fd += M(0, x)*da;
+        }
+        else fd = (M(0, x)*da)*speed;
+        P[I(F_FLUX+8*D+c, x[0], x[1], x[2])] = 0.5*(fs-fd);
+    }
+    // vector components: momentum (grp 0, with the pressure force) and field
+#pragma unroll 1
+    for (int grp = 0; grp < 2; ++grp)
+    {
+        const int c0 = grp == 0 ? 1 : 5;
+        double mp[3] = {0, 0, 0}, dmp[3] = {0, 0, 0}, fd[3] = {0, 0, 0};
+#pragma unroll 1
+        for (int col = 0; col < 3; ++col)
+        {
+            const int f = F_PHS+c0+col, g = F_DPT+c0+col;
+            const double fx = Fv(f, x), df2 = Df(f, t2);
+            const double df1 = D == 0 ? Df(f, t1) : 0.0;
+            const double gx = D == 0 ? 0.0 : Gv0(f), dg = D == 0 ? 0.0 : DGv(f);
+            const double da = dissAvg(c0+col);
+            const double dd1 = ROE ? Df(g, t1) : 0.0, dd2 = ROE ? Df(g, t2) : 0.0;
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
             {
-                val += Dm(1+3*r+c, t1, x)*Df(F_DPT+c0+c, t1, x)*(1.0/12.0);
-                val += Dm(1+3*r+c, t2, x)*Df(F_DPT+c0+c, t2, x)*(1.0/12.0);
-                val += M(1+3*r+c, x)*Davg[c0+c];
+                const int m = 1+3*r+col;
+                if (D == 0) mp[r] += Dm(m, t1, x)*df1*(1.0/12.0);
+                mp[r] += Dm(m, t2, x)*df2*(1.0/12.0);
+                mp[r] += M(m, x)*fx;
+                if (D != 0)
+                {
+                    dmp[r] += Dm(m, t2, x)*dg*(1.0/12.0);
+                    dmp[r] += M(m, x)*gx;
+                }
+                if constexpr (ROE)
+                {
+                    fd[r] += Dm(m, t1, x)*dd1*(1.0/12.0);
+                    fd[r] += Dm(m, t2, x)*dd2*(1.0/12.0);
+                    fd[r] += M(m, x)*da;
+                }
+                else fd[r] += M(m, x)*da;
             }
-            Fd[c0+r] = val;
+        }
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+        {
+            double fs = fsum(mp[r], dmp[r]);
+            if (grp == 0) fs += press[r];
+            const double d = ROE ? fd[r] : fd[r]*speed;
+            P[I(F_FLUX+8*D+c0+r, x[0], x[1], x[2])] = 0.5*(fs-d);
         }
     }
-#pragma unroll
-    for (int c = 0; c < 8; ++c)
-        P[I(F_FLUX+8*D+c, x[0], x[1], x[2])] = 0.5*(Fsum[c]-Fd[c]);
 }
 
 // 6a. Seam flux reconciliation 1/2 (F_a + sign F_b) on both boxes' copies
