@@ -354,11 +354,19 @@ def bench_level(args, wl):
                             "unit": "TFLOP/s", "frac": tf / FP64_PEAK_TFS, "traffic": traffic,
                             "avg_launch_us": rhs_ms * 1e3, "flop_per_cell_stage": fpc, "flop_source": fpc_path,
                             "hbm": {k: roof[k] for k in ("achieved", "peak", "unit", "frac")}}
+        if traffic is not None:
+            # bytes the counters saw move per second (scratch round trips included)
+            line["roofline"]["hbm"]["counted_GBs"] = traffic / (rhs_ms / 1e3) / 1e9
+            line["roofline"]["traffic_source"] = pmc_path
     else:
         line["roofline"] = roof
     if pmc_flop is not None:
-        line["valu_fp64_counters"] = {"achieved": pmc_flop / (rhs_ms / 1e3) / 1e12, "peak": FP64_PEAK_TFS,
-                                      "unit": "TFLOP/s", "flop_per_launch": pmc_flop, "source": pmc_path}
+        # the counted FP64 instructions are those of the FP64 build: rate over its own stage RHS time
+        pf = res["fp64"]["prof"][dominant] if "fp64" in res else res[head]["prof"][dominant]
+        ms64 = pf[0] / pf[1]
+        line["valu_fp64_counters"] = {"achieved": pmc_flop / (ms64 / 1e3) / 1e12, "peak": FP64_PEAK_TFS,
+                                      "unit": "TFLOP/s", "flop_per_launch": pmc_flop, "precision": "fp64",
+                                      "avg_launch_us": ms64 * 1e3, "source": pmc_path}
 
     # one stage RHS from the same prepared stage input in both precisions
     if wl["order"] == 4 and len(res) == 2:
