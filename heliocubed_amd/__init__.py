@@ -98,6 +98,15 @@ _SIGNATURES = [
     ("hc_init_native_boxes", _I, [ctypes.POINTER(HcShell), ctypes.POINTER(HcOptions), _I, ctypes.c_ulonglong,
                                   ctypes.POINTER(HcBox), _I, _I, _P]),
     ("hc_init_level4", _I, [ctypes.POINTER(HcShell), ctypes.POINTER(HcOptions), _I, ctypes.c_ulonglong, _I, _P, _P]),
+    ("hc_init_ju_boxes", _I, [ctypes.POINTER(HcShell), ctypes.POINTER(HcOptions), _I, ctypes.c_ulonglong,
+                              ctypes.POINTER(HcBox), _I, _I, _P]),
+    ("hc_amr4_create", _I, [ctypes.POINTER(HcShell), _I, ctypes.POINTER(HcOptions), ctypes.POINTER(HcBox),
+                            ctypes.POINTER(ctypes.c_int), _I, ctypes.POINTER(_P)]),
+    ("hc_amr4_destroy", None, [_P]),
+    ("hc_amr4_count", ctypes.c_longlong, [_P, _I]),
+    ("hc_amr4_prepare", _I, [_P, _P, _P, _P, _P]),
+    ("hc_amr4_stage_rhs", _I, [_P, _P, _P, _P, _P]),
+    ("hc_amr4_rk4_advance", _I, [_P, _P, _P, _D, _I, _P]),
     ("hc_level_profile", _I, [_P, _I]),
     ("hc_level_profile_read", _I, [_P, _I, ctypes.c_char_p, _I, ctypes.POINTER(_D), ctypes.POINTER(ctypes.c_longlong)]),
     ("hc_last_error", ctypes.c_char_p, []),
@@ -494,6 +503,93 @@ def init_native_boxes(shell: Shell, options: Options, boxes, config: int = 3, se
     _check(lib().hc_init_native_boxes(ctypes.byref(sh), ctypes.byref(op), config, seed, tab, len(boxes), quad,
                                       out.ctypes.data))
     return out
+
+
+def init_ju_boxes(shell: Shell, options: Options, boxes, config: int = 3, seed: int = 20261203, quad: int = 3):
+    """<J U> of every box (valid cells, [8][n2][n1][n0] each), concatenated, as numpy."""
+    import numpy as np
+
+    total = sum(8 * (hi[0] - lo[0] + 1) * (hi[1] - lo[1] + 1) * (hi[2] - lo[2] + 1) for _, lo, hi in boxes)
+    out = np.zeros(total)
+    tab = box_table(boxes)
+    sh, op = shell.c(), options.c()
+    _check(lib().hc_init_ju_boxes(ctypes.byref(sh), ctypes.byref(op), config, seed, tab, len(boxes), quad,
+                                  out.ctypes.data))
+    return out
+
+
+class Amr4:
+    """Fixed AMR hierarchy of fourth-order levels with the synchronous
+    classical RK4 of CubedSphereAMR::advanceRungeKuttaSynchronous
+    (src/MHD_AMRRungeKutta.H:5-41) on the current HIP device.
+
+    shells: one Shell per level (level 0 = the whole shell); patches: the
+    refined patches as (level, (block, lo, hi)). State tensors hold the
+    level-0 valid level followed by every patch in table order."""
+
+    def __init__(self, shells, options: Options, patches):
+        self.options = options
+        self.shells = list(shells)
+        self.patches = [(int(l), (int(b), tuple(lo), tuple(hi))) for l, (b, lo, hi) in patches]
+        self._handle = ctypes.c_void_p()
+        op = options.c()
+        boxes = [b for _, b in self.patches]
+        tab = box_table(boxes) if boxes else None
+        shs = (HcShell * len(self.shells))(*[s.c() for s in self.shells])
+        lev = (ctypes.c_int * max(1, len(boxes)))(*[l for l, _ in self.patches])
+        _check(lib().hc_amr4_create(shs, len(self.shells), ctypes.byref(op), tab, lev, len(boxes),
+                                    ctypes.byref(self._handle)))
+
+    def __del__(self):
+        if getattr(self, "_handle", None) and self._handle.value and _lib is not None:
+            _lib.hc_amr4_destroy(self._handle)
+            self._handle = ctypes.c_void_p()
+
+    @property
+    def state_count(self) -> int:
+        return lib().hc_amr4_count(self._handle, 0)
+
+    @property
+    def stage_count(self) -> int:
+        return lib().hc_amr4_count(self._handle, 1)
+
+    @property
+    def level0_count(self) -> int:
+        return lib().hc_amr4_count(self._handle, 2)
+
+    def _check_args(self, pairs):
+        import torch
+
+        for t, n, name in pairs:
+            if t is None or t.dtype != torch.float64 or not t.is_cuda or not t.is_contiguous() or t.numel() != n:
+                raise ValueError(f"{name}: need a contiguous float64 device tensor of {n} elements")
+
+    def _inner_count(self):
+        sh = self.shells[0]
+        return 6 * 8 * (sh.n_ang + 10) ** 2 * 5
+
+    def prepare(self, JU, inner, USph):
+        import torch
+
+        self._check_args([(JU, self.state_count, "JU"), (inner, self._inner_count(), "inner"),
+                          (USph, self.stage_count, "USph")])
+        stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+        _check(lib().hc_amr4_prepare(self._handle, _ptr(JU), _ptr(inner), _ptr(USph), stream))
+
+    def stage_rhs(self, JU, inner, K):
+        import torch
+
+        self._check_args([(JU, self.state_count, "JU"), (inner, self._inner_count(), "inner"),
+                          (K, self.state_count, "K")])
+        stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+        _check(lib().hc_amr4_stage_rhs(self._handle, _ptr(JU), _ptr(inner), _ptr(K), stream))
+
+    def rk4_advance(self, JU, inner, dt: float, nsteps: int = 1):
+        import torch
+
+        self._check_args([(JU, self.state_count, "JU"), (inner, self._inner_count(), "inner")])
+        stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+        _check(lib().hc_amr4_rk4_advance(self._handle, _ptr(JU), _ptr(inner), float(dt), int(nsteps), stream))
 
 
 class BoxOp:

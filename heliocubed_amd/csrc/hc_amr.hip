@@ -387,6 +387,8 @@ bool validBox(const hc_box* b)
     return b && b->hi[0] >= b->lo[0] && b->hi[1] >= b->lo[1] && b->hi[2] >= b->lo[2];
 }
 } // namespace
+
+void amrChildWeights(int ratio, int child, double w[5]) { childWeights(ratio, child, w); }
 } // namespace hc
 
 namespace

@@ -37,7 +37,7 @@ inline const char* checkOperatorOptions(const hc_options& o)
 // Fourth-order level (defined with the fourth-order kernels): stage
 // preparation JU -> USph (ghost 5) and the box-table RHS over the level.
 struct Level4;
-Level4* level4Create(const hc_shell& shell, const hc_options& opt, const int tile[3]);
+Level4* level4Create(const hc_shell& shell, const hc_options& opt, const int tile[3], bool withTable = true);
 void level4Destroy(Level4* L);
 // Rebuilds the level's box table with tiles of tile[0] x tile[1] x tile[2]
 // cells (radial, alpha, beta; clipped to the panel; results unchanged).
@@ -46,6 +46,11 @@ void level4TileShape(const Level4& L, int tile[3]);
 void level4Prepare(Level4& L, const double* U0, const double* K, double a_dt, const double* inner, double* US,
                    hipStream_t s);
 void level4Rhs(Level4& L, const double* US, double* K, double* U, double b_dt, hipStream_t s);
+
+// Child weights of the fourth-order AMR interpolation (defined with the AMR
+// transfer kernels): the average over child `child` of `ratio` of the quartic
+// matching the five coarse averages around the parent.
+void amrChildWeights(int ratio, int child, double w[5]);
 
 // Level reductions (defined with their kernels): the CFL geometry tables and
 // reduction buffers; the CFL reduction returns a device pointer to 1 / max

@@ -1515,6 +1515,29 @@ int hc_init_level4(const hc_shell* shell, const hc_options* options, int config,
         for (int ig = 0; ig < g; ++ig)
             inner[((((size_t)p*8+c)*e1+kk)*e1+jj)*g+ig] =
                 tmp[(size_t)p*8*e0*e1*e1 + (((size_t)c*e1+kk)*e1+jj)*e0 + (ig+g)];
+    // <J U> of the six whole panels (the valid level layout is the panel
+    // boxes concatenated)
+    std::vector<hc_box> panels;
+    for (int p = 0; p < 6; ++p) { hc_box b{}; b.block = p; b.hi[0] = nr-1; b.hi[1] = na-1; b.hi[2] = na-1;
+        panels.push_back(b); }
+    return hc_init_ju_boxes(shell, options, config, seed, panels.data(), 6, quad, JU);
+}
+
+int hc_init_ju_boxes(const hc_shell* shell, const hc_options* options, int config, unsigned long long seed,
+                     const hc_box* boxes, int nbox, int quad, double* JU)
+{
+    if (!shell || !options || !boxes || !JU || nbox <= 0) return fail("hc_init_ju_boxes: bad argument");
+    if (config < 0 || config > 5) return fail("hc_init_ju_boxes: config must be 0..5");
+    if (quad < 1 || quad > 8) return fail("hc_init_ju_boxes: quad must be 1..8");
+    const int nr = shell->n_rad, na = shell->n_ang, g = 5;
+    for (int n = 0; n < nbox; ++n)
+    {
+        const hc_box& b = boxes[n];
+        if (b.block < 0 || b.block > 5) return fail("hc_init_ju_boxes: block out of range");
+        for (int d = 0; d < 3; ++d)
+            if (b.lo[d] < 0 || b.hi[d] < b.lo[d] || b.hi[d] >= (d == 0 ? nr : na))
+                return fail("hc_init_ju_boxes: box outside the level");
+    }
     // <J U>: J-weighted averages with Cartesian vector components
     const hc::ShellDims dims{nr, na, g, shell->inner_radius, shell->outer_radius, shell->radial_stretch};
     const WindState ws = makeWind(*shell, *options, config, seed);
@@ -1523,57 +1546,67 @@ int hc_init_level4(const hc_shell* shell, const hc_options* options, int config,
     const double h = 1.0/na, gamma = options->gamma;
     const double C = dims.stretch;
     const double span = C == 0.0 ? (dims.r1-dims.r0) : (dims.r1-dims.r0)/std::expm1(C);
-    for (int p = 0; p < 6; ++p) for (int k = 0; k < na; ++k) for (int j = 0; j < na; ++j)
+    size_t off = 0;
+    for (int n = 0; n < nbox; ++n)
     {
-        // per-column angular samples
-        std::vector<double> dirs(3*quad*quad), dets(quad*quad);
-        for (int qk = 0; qk < quad; ++qk) for (int qj = 0; qj < quad; ++qj)
+        const hc_box& bx = boxes[n];
+        const int p = bx.block;
+        const int n0 = bx.hi[0]-bx.lo[0]+1, n1 = bx.hi[1]-bx.lo[1]+1, n2 = bx.hi[2]-bx.lo[2]+1;
+        for (int k = bx.lo[2]; k <= bx.hi[2]; ++k) for (int j = bx.lo[1]; j <= bx.hi[1]; ++j)
         {
-            const double al = hc::xiToAngle((j+gx[qj])*h), be = hc::xiToAngle((k+gx[qk])*h);
-            double A[3][3];
-            hc::pointBasis(p, al, be, A);
-            dets[qk*quad+qj] = A[0][0]*(A[1][1]*A[2][2]-A[1][2]*A[2][1]) - A[0][1]*(A[1][0]*A[2][2]-A[1][2]*A[2][0])
-                             + A[0][2]*(A[1][0]*A[2][1]-A[1][1]*A[2][0]);
-            const hc::Vec3 d = hc::panelPoint(p, al, be);
-            for (int a = 0; a < 3; ++a) dirs[3*(qk*quad+qj)+a] = d.x[a];
-        }
-        for (int i = 0; i < nr; ++i)
-        {
-            double noise = 1.0;
-            if (config == 3 || config == 4)
+            // per-column angular samples
+            std::vector<double> dirs(3*quad*quad), dets(quad*quad);
+            for (int qk = 0; qk < quad; ++qk) for (int qj = 0; qj < quad; ++qj)
             {
-                SplitMix cr{seed ^ (0x9E3779B97F4A7C15ull*((((unsigned long long)p*4096+(k+8))*4096+(j+8))*65536+(i+8)))};
-                const double u1 = std::max(cr.uniform(), 1e-300), u2 = cr.uniform();
-                const double gn = std::sqrt(-2.0*std::log(u1))*std::cos(2*hc::kPi*u2);
-                noise = std::exp(0.05*gn-0.00125);
+                const double al = hc::xiToAngle((j+gx[qj])*h), be = hc::xiToAngle((k+gx[qk])*h);
+                double A[3][3];
+                hc::pointBasis(p, al, be, A);
+                dets[qk*quad+qj] = A[0][0]*(A[1][1]*A[2][2]-A[1][2]*A[2][1]) - A[0][1]*(A[1][0]*A[2][2]-A[1][2]*A[2][0])
+                                 + A[0][2]*(A[1][0]*A[2][1]-A[1][1]*A[2][0]);
+                const hc::Vec3 d = hc::panelPoint(p, al, be);
+                for (int a = 0; a < 3; ++a) dirs[3*(qk*quad+qj)+a] = d.x[a];
             }
-            double avg[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-            for (int qi = 0; qi < quad; ++qi)
+            for (int i = bx.lo[0]; i <= bx.hi[0]; ++i)
             {
-                const double eta = (i+gx[qi])/double(nr);
-                const double r = dims.r0 + span*(C == 0.0 ? eta : std::expm1(C*eta));
-                const double drdxi = C == 0.0 ? span : span*C*std::exp(C*eta);
-                for (int qa = 0; qa < quad*quad; ++qa)
+                double noise = 1.0;
+                if (config == 3 || config == 4)
                 {
-                    const double* d = &dirs[3*qa];
-                    const double X[3] = {r*d[0], r*d[1], r*d[2]};
-                    double w[8];
-                    ws.at(X, w);
-                    w[0] *= noise;
-                    double v2 = 0, b2 = 0;
-                    for (int a = 0; a < 3; ++a) { v2 += w[1+a]*w[1+a]; b2 += w[5+a]*w[5+a]; }
-                    const double J = r*r*drdxi*dets[qa];
-                    const double wt = gw[qi]*gw[qa % quad]*gw[qa / quad]*J;
-                    avg[0] += wt*w[0];
-                    for (int a = 0; a < 3; ++a) { avg[1+a] += wt*w[0]*w[1+a]; avg[5+a] += wt*w[5+a]; }
-                    avg[4] += wt*(w[4]/(gamma-1.0) + 0.5*w[0]*v2 + b2/(8*hc::kPi));
+                    SplitMix cr{seed ^ (0x9E3779B97F4A7C15ull*((((unsigned long long)p*4096+(k+8))*4096+(j+8))*65536+(i+8)))};
+                    const double u1 = std::max(cr.uniform(), 1e-300), u2 = cr.uniform();
+                    const double gn = std::sqrt(-2.0*std::log(u1))*std::cos(2*hc::kPi*u2);
+                    noise = std::exp(0.05*gn-0.00125);
                 }
+                double avg[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+                for (int qi = 0; qi < quad; ++qi)
+                {
+                    const double eta = (i+gx[qi])/double(nr);
+                    const double r = dims.r0 + span*(C == 0.0 ? eta : std::expm1(C*eta));
+                    const double drdxi = C == 0.0 ? span : span*C*std::exp(C*eta);
+                    for (int qa = 0; qa < quad*quad; ++qa)
+                    {
+                        const double* d = &dirs[3*qa];
+                        const double X[3] = {r*d[0], r*d[1], r*d[2]};
+                        double w[8];
+                        ws.at(X, w);
+                        w[0] *= noise;
+                        double v2 = 0, b2 = 0;
+                        for (int a = 0; a < 3; ++a) { v2 += w[1+a]*w[1+a]; b2 += w[5+a]*w[5+a]; }
+                        const double J = r*r*drdxi*dets[qa];
+                        const double wt = gw[qi]*gw[qa % quad]*gw[qa / quad]*J;
+                        avg[0] += wt*w[0];
+                        for (int a = 0; a < 3; ++a) { avg[1+a] += wt*w[0]*w[1+a]; avg[5+a] += wt*w[5+a]; }
+                        avg[4] += wt*(w[4]/(gamma-1.0) + 0.5*w[0]*v2 + b2/(8*hc::kPi));
+                    }
+                }
+                for (int c = 0; c < 8; ++c)
+                    JU[off + (((size_t)c*n2+(k-bx.lo[2]))*n1+(j-bx.lo[1]))*n0+(i-bx.lo[0])] = avg[c];
             }
-            for (int c = 0; c < 8; ++c) JU[((((size_t)p*8+c)*na+k)*na+j)*nr+i] = avg[c];
         }
+        off += (size_t)8*n0*n1*n2;
     }
     return 0;
 }
+
 
 double hc_geom_cell_detA(int n_ang, int j, int k) { return hc::cellDetA(n_ang, j, k); }
 

@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <map>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -1777,6 +1778,50 @@ static std::vector<AvgDonor> averageDonors(int na, int p, int j, int k)
     return out;
 }
 
+// Seam pairs between the face fluxes of the first nbox boxes of a table
+// (whole-level tiles of one fourth-order level): every seam face of the
+// level, its two sides in the scratch of the tiles holding them.
+static std::vector<SeamPair> panelSeamPairs(const std::vector<BoxDesc>& boxes, size_t nbox, int nr, int na)
+{
+    auto boxOf = [&](int p, int i, int j, int k) {
+        for (size_t n = 0; n < nbox; ++n)
+        {
+            const BoxDesc& b = boxes[n];
+            if (b.p == p && i >= b.lo[0] && i <= b.hi[0] && j >= b.lo[1] && j <= b.hi[1] && k >= b.lo[2] && k <= b.hi[2])
+                return (int)n;
+        }
+        return -1;
+    };
+    auto faceIndex = [&](int n, int d, int i, int jf, int kf, long long& ec) {
+        const BoxDesc& b = boxes[n];
+        const long long e0 = b.n[0]+2*kG, e1 = b.n[1]+2*kG, e2 = b.n[2]+2*kG;
+        ec = e0*e1*e2;
+        return b.sOff + (F_FLUX+8*d)*ec + ((long long)(kf-b.lo[2]+kG)*e1 + (jf-b.lo[1]+kG))*e0 + (i-b.lo[0]+kG);
+    };
+    std::vector<SeamPair> pairs;
+    for (int p = 0; p < 6; ++p) for (int d = 1; d < 3; ++d) for (int s = 0; s < 2; ++s)
+    {
+        const SeamEdge e = seamEdge(na, p, d, s);
+        if ((p*2+(d-1))*2+s > (e.q*2+(e.dq-1))*2+e.sq) continue;
+        const int f = s ? na : 0, fq = e.sq ? na : 0;
+        for (int t = 0; t < na; ++t) for (int i = 0; i < nr; ++i)
+        {
+            const int tq = e.rev ? na-1-t : t;
+            const int jf = d == 1 ? f : t, kf = d == 1 ? t : f;
+            const int jq = e.dq == 1 ? fq : tq, kq = e.dq == 1 ? tq : fq;
+            const int na1 = na-1;
+            const int na_ = boxOf(p, i, d == 1 ? (s ? na1 : 0) : t, d == 1 ? t : (s ? na1 : 0));
+            const int nb = boxOf(e.q, i, e.dq == 1 ? (e.sq ? na1 : 0) : tq, e.dq == 1 ? tq : (e.sq ? na1 : 0));
+            SeamPair sp;
+            sp.a = faceIndex(na_, d, i, jf, kf, sp.ea);
+            sp.b = faceIndex(nb, e.dq, i, jq, kq, sp.eb);
+            sp.sign = s != e.sq ? 1.0 : -1.0;
+            pairs.push_back(sp);
+        }
+    }
+    return pairs;
+}
+
 // Box table over a fourth-order level: tiles of tile[0] x tile[1] x tile[2]
 // cells (clipped to the panel) viewing the level arrays, with the seam pairs
 // of their face fluxes. Every tile evaluates the same per-cell arithmetic, so
@@ -1814,54 +1859,20 @@ static hc_boxop* levelBoxTable(const Level4& Lv, const int tile[3])
     HC_THROW(hipMemcpy(const_cast<BoxDesc*>(B->view.boxes), B->boxes.data(), B->boxes.size()*sizeof(BoxDesc),
                        hipMemcpyHostToDevice));
     // seam pairs between the boxes' face fluxes
-    auto boxOf = [&](int p, int i, int j, int k) {
-        for (size_t n = 0; n < B->boxes.size(); ++n)
-        {
-            const BoxDesc& b = B->boxes[n];
-            if (b.p == p && i >= b.lo[0] && i <= b.hi[0] && j >= b.lo[1] && j <= b.hi[1] && k >= b.lo[2] && k <= b.hi[2])
-                return (int)n;
-        }
-        return -1;
-    };
-    auto faceIndex = [&](int n, int d, int i, int jf, int kf, long long& ec) {
-        const BoxDesc& b = B->boxes[n];
-        const long long e0 = b.n[0]+2*kG, e1 = b.n[1]+2*kG, e2 = b.n[2]+2*kG;
-        ec = e0*e1*e2;
-        return b.sOff + (F_FLUX+8*d)*ec + ((long long)(kf-b.lo[2]+kG)*e1 + (jf-b.lo[1]+kG))*e0 + (i-b.lo[0]+kG);
-    };
-    std::vector<SeamPair> pairs;
-    for (int p = 0; p < 6; ++p) for (int d = 1; d < 3; ++d) for (int s = 0; s < 2; ++s)
-    {
-        const SeamEdge e = seamEdge(na, p, d, s);
-        if ((p*2+(d-1))*2+s > (e.q*2+(e.dq-1))*2+e.sq) continue;
-        const int f = s ? na : 0, fq = e.sq ? na : 0;
-        for (int t = 0; t < na; ++t) for (int i = 0; i < nr; ++i)
-        {
-            const int tq = e.rev ? na-1-t : t;
-            const int jf = d == 1 ? f : t, kf = d == 1 ? t : f;
-            const int jq = e.dq == 1 ? fq : tq, kq = e.dq == 1 ? tq : fq;
-            const int na1 = na-1;
-            const int na_ = boxOf(p, i, d == 1 ? (s ? na1 : 0) : t, d == 1 ? t : (s ? na1 : 0));
-            const int nb = boxOf(e.q, i, e.dq == 1 ? (e.sq ? na1 : 0) : tq, e.dq == 1 ? tq : (e.sq ? na1 : 0));
-            SeamPair sp;
-            sp.a = faceIndex(na_, d, i, jf, kf, sp.ea);
-            sp.b = faceIndex(nb, e.dq, i, jq, kq, sp.eb);
-            sp.sign = s != e.sq ? 1.0 : -1.0;
-            pairs.push_back(sp);
-        }
-    }
+    std::vector<SeamPair> pairs = panelSeamPairs(B->boxes, B->boxes.size(), nr, na);
     B->view.seams = upload(pairs, B->owned);
     B->view.nseams = (int)pairs.size();
     return B.release();
 }
 
-Level4* level4Create(const hc_shell& shell, const hc_options& opt, const int tile[3])
+// Metric tables of a fourth-order level's stage preparation: cell-averaged
+// det A, <A> and its inverse over the angular range -6 .. na+5, dVolr and the
+// mid radius over -6 .. nr+6.
+void prep4Tables(const hc_shell& shell, const hc_options& opt, Prep4View& v, std::vector<void*>& owned)
 {
-    std::unique_ptr<Level4> L(new Level4());
-    L->opt = opt;
-    L->dims = {shell.n_rad, shell.n_ang, kG, shell.inner_radius, shell.outer_radius, shell.radial_stretch};
+    const hc::ShellDims dims{shell.n_rad, shell.n_ang, kG, shell.inner_radius, shell.outer_radius,
+                             shell.radial_stretch};
     const int nr = shell.n_rad, na = shell.n_ang;
-    Prep4View& v = L->pv;
     v.nr = nr; v.na = na; v.NE = na+12;
     v.vcs = (long long)nr*na*na; v.vps = 8*v.vcs;
     v.cJ = na+2*kGC; v.ccs = v.cJ*v.cJ*nr; v.cps = 8*v.ccs;
@@ -1886,9 +1897,19 @@ Level4* level4Create(const hc_shell& shell, const hc_options& opt, const int til
         for (int r = 0; r < 3; ++r) for (int c = 0; c < 3; ++c) { Aavg[x+3*r+c] = M[r][c]; Ainv[x+3*r+c] = Mi[r][c]; }
     }
     std::vector<double> dvol(nr+13), rmid(nr+13);
-    for (int i = -6; i <= nr+6; ++i) { const RadialCell c = radialCell(L->dims, i); dvol[i+6] = c.dvol; rmid[i+6] = c.rmid; }
-    v.det = upload(det, L->owned); v.Aavg = upload(Aavg, L->owned); v.Ainv = upload(Ainv, L->owned);
-    v.dvol = upload(dvol, L->owned); v.rmid = upload(rmid, L->owned);
+    for (int i = -6; i <= nr+6; ++i) { const RadialCell c = radialCell(dims, i); dvol[i+6] = c.dvol; rmid[i+6] = c.rmid; }
+    v.det = upload(det, owned); v.Aavg = upload(Aavg, owned); v.Ainv = upload(Ainv, owned);
+    v.dvol = upload(dvol, owned); v.rmid = upload(rmid, owned);
+}
+
+Level4* level4Create(const hc_shell& shell, const hc_options& opt, const int tile[3], bool withTable)
+{
+    std::unique_ptr<Level4> L(new Level4());
+    L->opt = opt;
+    L->dims = {shell.n_rad, shell.n_ang, kG, shell.inner_radius, shell.outer_radius, shell.radial_stretch};
+    const int nr = shell.n_rad, na = shell.n_ang;
+    Prep4View& v = L->pv;
+    prep4Tables(shell, opt, v, L->owned);
     std::vector<long long> colDst, donorSrc;
     std::vector<int> colStart{0};
     std::vector<double> donorW;
@@ -1910,7 +1931,7 @@ Level4* level4Create(const hc_shell& shell, const hc_options& opt, const int til
     L->donorW = upload(donorW, L->owned);
     HC_THROW(hipMalloc(&L->cart, 6ull*v.cps*sizeof(double))); L->owned.push_back(L->cart);
     HC_THROW(hipMemset(L->cart, 0, 6ull*v.cps*sizeof(double)));
-    L->box = levelBoxTable(*L, tile);
+    if (withTable) L->box = levelBoxTable(*L, tile);
     for (int d = 0; d < 3; ++d) L->tile[d] = tile[d];
     return L.release();
 }
@@ -2034,3 +2055,5 @@ int hc_boxop_apply(hc_boxop* op, const double* USph, double* rhs, double* F0, do
 }
 
 } // extern "C"
+
+#include "hc_amr4.hpp"

@@ -248,6 +248,47 @@ long long hc_boxop_count(const hc_boxop* op, int what);
 int hc_boxop_apply(hc_boxop* op, const double* USph, double* rhs, double* F0, double* F1, double* F2,
                    hc_stream stream);
 
+/* Fixed AMR hierarchy of fourth-order levels advanced by the synchronous
+ * classical RK4 (advanceRungeKuttaSynchronous, src/MHD_AMRRungeKutta.H:5-41):
+ * per stage the stage state is restricted fine -> coarse (restrictTo,
+ * src/MHD_AMR.H:597-607), every level is prepared (prepareFourthOrder,
+ * src/MHD_AMRFourthOrder.H:5-187: refined patches take their ghost cells
+ * from the quartic interpolation of the parent's Cartesian averages and
+ * from the patches of their own level), every patch of every level is
+ * evaluated in one box table (evaluate, src/MHD_AMR.H:1417-1499), the coarse
+ * cells outside a finer level are refluxed with the mean fine face flux
+ * (the flux registers of src/MHD_AMR.H:651-663, 916-921), and the fine RHS
+ * is restricted onto the coarse RHS; after the stages the state is averaged
+ * down.
+ * shells[l]: cell counts of level l (same radii and stretch; the ratios
+ * n_rad(l)/n_rad(l-1) and n_ang(l)/n_ang(l-1) are integers 1..4). Level 0 is
+ * the whole shell. patches / patch_levels: the refined patches (levels >= 1),
+ * lo/hi in their level's index space; the patches of a level share one box
+ * size (a multiple of the ratio, >= 4 cells) and are aligned to it.
+ * Restrictions (rejected with an error otherwise): every refined patch keeps
+ * 6 cells from the radial boundaries and 8 from the panel edges (no seam
+ * interpolation or radial boundary on refined levels), lies inside one patch
+ * of its parent level (level >= 2), interpolates only from cells its parent
+ * level defines (level 0, or a parent patch grown by 6), and the levels are
+ * properly nested (every coarse-fine face borders a parent-level cell).
+ * Arrays (device, FP64): the state JU / RHS K is the level-0 valid level
+ * [6][8][na][na][nr] followed by every patch [8][n2][n1][n0] in table order
+ * (hc_amr4_count what 0; what 2 = the level-0 part); the stage input USph is
+ * the level-0 stage layout [6][8][na+10][na+10][nr+10] followed by every
+ * patch grown by 5 (what 1); inner_ghosts as for hc_prepare_stage at level 0.
+ * hc_amr4_prepare: USph of the restricted JU. hc_amr4_stage_rhs: K of one
+ * stage from JU (restriction, preparation, evaluation, reflux, restriction).
+ * hc_amr4_rk4_advance: nsteps RK4 steps of JU in place. */
+typedef struct hc_amr4 hc_amr4;
+int hc_amr4_create(const hc_shell* shells, int nlevels, const hc_options* options, const hc_box* patches,
+                   const int* patch_levels, int npatch, hc_amr4** out);
+void hc_amr4_destroy(hc_amr4* amr);
+long long hc_amr4_count(const hc_amr4* amr, int what);
+int hc_amr4_prepare(hc_amr4* amr, const double* JU, const double* inner_ghosts, double* USph, hc_stream stream);
+int hc_amr4_stage_rhs(hc_amr4* amr, const double* JU, const double* inner_ghosts, double* K, hc_stream stream);
+int hc_amr4_rk4_advance(hc_amr4* amr, double* JU, const double* inner_ghosts, double dt, int nsteps,
+                        hc_stream stream);
+
 /* Host-side seeded initial state (test problem data, not on the hot path):
  * config 0 = radial wind rho=(r0/r)^2, v_r=1, B_r=+-(r0/r)^2 by hemisphere
  * (sign of z), p=0.1 rho (1 + 0.2 exp(-|x-x_c|^2/w^2)), w = 0.1 (r1-r0), x_c at
@@ -281,6 +322,13 @@ int hc_init_native_boxes(const hc_shell* shell, const hc_options* options, int c
  * [6][8][na+10][na+10][5], for the states of hc_init_native_boxes. */
 int hc_init_level4(const hc_shell* shell, const hc_options* options, int config, unsigned long long seed,
                    int quad, double* JU_host, double* inner_host);
+
+/* Host-side <J U> initial data of boxes of a fourth-order level (HOST array,
+ * test problem data): the J-weighted averages of hc_init_level4 on each box,
+ * each box [8][n2][n1][n0] (radial index fastest), concatenated in table
+ * order. hc_init_level4's valid level is this over the six whole panels. */
+int hc_init_ju_boxes(const hc_shell* shell, const hc_options* options, int config, unsigned long long seed,
+                     const hc_box* boxes, int nbox, int quad, double* JU_host);
 
 /* Host-side geometry probes (no device use): cell-averaged det A of angular
  * cell (j,k), and the face-averaged cofactor row of face (d; j,k) of panel p. */

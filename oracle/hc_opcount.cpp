@@ -10,7 +10,10 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <cstdio>
 #include <limits>
+#include <memory>
+#include <string>
 #include <vector>
 
 #include "opcount.hpp"

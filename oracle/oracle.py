@@ -92,6 +92,13 @@ def lib() -> ctypes.CDLL:
             "orc_amr_ghosts2": (None, [P, P, P, P, P, P, P, P, D, P]),
             "orc_amr_average_down": (None, [P, P, P, I, P, P, P, P]),
             "orc_amr_prolong4": (I, [P, P, P, P, P, P, P, P, P]),
+            "orc_amr4_create": (P, [cfgp, I, P, I, ctypes.c_char_p, I]),
+            "orc_amr4_destroy": (None, [P]),
+            "orc_amr4_count": (ctypes.c_longlong, [P, I]),
+            "orc_amr4_prepare": (I, [P, P, P, P]),
+            "orc_amr4_stage_rhs": (I, [P, P, P, P]),
+            "orc_amr4_rk4": (I, [P, P, P, D, I]),
+            "orc_amr4_stage_rhs_opt": (I, [P, P, P, P, I]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -481,4 +488,54 @@ class Oracle4:
         u = np.ascontiguousarray(ju.copy())
         g = _c(inner)
         lib().orc4_rk4(self._h, _p(u), _p(g), dt, nsteps)
+        return u
+
+
+class Amr4Oracle:
+    """Fixed AMR hierarchy of fourth-order levels with synchronous classical
+    RK4 (advanceRungeKuttaSynchronous) on the CPU. levels: [(n_rad, n_ang)]
+    from the base shell up; patches: [(level, (panel, lo, hi))] for levels
+    >= 1. Combined arrays: the level-0 array, then every patch in table order."""
+
+    def __init__(self, levels, patches, limiter=1, entropy_fix=0.3, gamma=5.0 / 3.0, r0=21.5, r1=215.0,
+                 spatial_operator=1, eta_d=0.02, fallback=0, riemann=2, divergence_cleaning=0):
+        self.levels = [tuple(x) for x in levels]
+        self.patches = [(int(l), (int(b), tuple(lo), tuple(hi))) for l, (b, lo, hi) in patches]
+        arr = (OrcCfg * len(levels))()
+        for i, (nr, na) in enumerate(levels):
+            arr[i] = OrcCfg(nr, na, 5, r0, r1, 0.0, gamma, limiter, riemann, entropy_fix, spatial_operator, eta_d,
+                            fallback, divergence_cleaning)
+        self._cfgs = arr
+        tab = np.ascontiguousarray([[l, b, *lo, *hi] for l, (b, lo, hi) in self.patches], dtype=np.int32)
+        self._tab = tab
+        err = ctypes.create_string_buffer(256)
+        h = lib().orc_amr4_create(arr, len(levels), tab.ctypes.data if len(tab) else None, len(self.patches),
+                                  err, 256)
+        if not h:
+            raise ValueError(err.value.decode())
+        self._h = ctypes.c_void_p(h)
+        self.state_size = int(lib().orc_amr4_count(self._h, 0))
+        self.stage_size = int(lib().orc_amr4_count(self._h, 1))
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().orc_amr4_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def prepare(self, ju, inner):
+        us = np.zeros(self.stage_size)
+        a, b = _c(ju), _c(inner)
+        lib().orc_amr4_prepare(self._h, _p(a), _p(b), _p(us))
+        return us
+
+    def stage_rhs(self, ju, inner, reflux=True):
+        k = np.zeros(self.state_size)
+        a, b = _c(ju), _c(inner)
+        lib().orc_amr4_stage_rhs_opt(self._h, _p(a), _p(b), _p(k), int(reflux))
+        return k
+
+    def rk4(self, ju, inner, dt, nsteps=1):
+        u = np.ascontiguousarray(np.array(ju, dtype=np.float64).ravel().copy())
+        g = _c(inner)
+        lib().orc_amr4_rk4(self._h, _p(u), _p(g), dt, nsteps)
         return u

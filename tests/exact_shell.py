@@ -125,12 +125,16 @@ class Shell:
 
     def ju_averages(self, p, quad=4):
         """<J U> of every valid cell of panel p (Cartesian vectors), [8][k][j][i]."""
+        return self.ju_box(p, (0, 0, 0), (self.nr - 1, self.na - 1, self.na - 1), quad)
+
+    def ju_box(self, p, lo, hi, quad=4):
+        """<J U> of the cells lo..hi of panel p (Cartesian vectors), [8][k][j][i]."""
         M = panel_matrix(p)
         gx, gw = _gauss(quad)
         na, nr = self.na, self.nr
-        i = np.arange(nr)[None, None, :, None, None, None]
-        j = np.arange(na)[None, :, None, None, None, None]
-        k = np.arange(na)[:, None, None, None, None, None]
+        i = np.arange(lo[0], hi[0] + 1)[None, None, :, None, None, None]
+        j = np.arange(lo[1], hi[1] + 1)[None, :, None, None, None, None]
+        k = np.arange(lo[2], hi[2] + 1)[:, None, None, None, None, None]
         xi1 = (j + gx[None, None, None, :, None, None]) / na
         xi2 = (k + gx[None, None, None, None, :, None]) / na
         xi0 = (i + gx[None, None, None, None, None, :]) / nr
