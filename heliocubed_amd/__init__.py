@@ -104,6 +104,7 @@ _SIGNATURES = [
                             ctypes.POINTER(ctypes.c_int), _I, ctypes.POINTER(_P)]),
     ("hc_amr4_destroy", None, [_P]),
     ("hc_amr4_count", ctypes.c_longlong, [_P, _I]),
+    ("hc_amr4_set_tiles", _I, [_P, _P, _P]),
     ("hc_amr4_prepare", _I, [_P, _P, _P, _P, _P]),
     ("hc_amr4_stage_rhs", _I, [_P, _P, _P, _P, _P]),
     ("hc_amr4_rk4_advance", _I, [_P, _P, _P, _D, _I, _P]),
@@ -556,6 +557,19 @@ class Amr4:
     @property
     def level0_count(self) -> int:
         return lib().hc_amr4_count(self._handle, 2)
+
+    @property
+    def tile_count(self) -> int:
+        """Tiles over the refined levels (patches grouped into rectangular tiles)."""
+        return lib().hc_amr4_count(self._handle, 3)
+
+    def set_tiles(self, level0_tile=None, refined_max=None):
+        """Level-0 tile shape (radial, alpha, beta) and the largest refined tile
+        (cells per direction; the patch size gives one tile per patch). Results
+        do not depend on either; synchronizes the device."""
+        a = (ctypes.c_int * 3)(*level0_tile) if level0_tile is not None else None
+        b = (ctypes.c_int * 3)(*refined_max) if refined_max is not None else None
+        _check(lib().hc_amr4_set_tiles(self._handle, a, b))
 
     def _check_args(self, pairs):
         import torch

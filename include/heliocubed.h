@@ -284,6 +284,16 @@ int hc_amr4_create(const hc_shell* shells, int nlevels, const hc_options* option
                    const int* patch_levels, int npatch, hc_amr4** out);
 void hc_amr4_destroy(hc_amr4* amr);
 long long hc_amr4_count(const hc_amr4* amr, int what);
+/* Tiles (no reference counterpart: the reference evaluates each patch on its
+ * own ghosted box). The patches of every refined level are grouped into
+ * rectangular tiles of whole patches of at most refined_max cells per
+ * direction (default 128 x 64 x 64; the patch size gives one tile per
+ * patch), and level 0 is evaluated in tiles of level0_tile (default 32^3);
+ * each tile is prepared and evaluated as one box, which gives the per-patch
+ * results cell for cell with fewer ghost cells. NULL keeps a setting.
+ * Synchronizes the device. hc_amr4_count(what 3) is the number of refined
+ * tiles. */
+int hc_amr4_set_tiles(hc_amr4* amr, const int* level0_tile, const int* refined_max);
 int hc_amr4_prepare(hc_amr4* amr, const double* JU, const double* inner_ghosts, double* USph, hc_stream stream);
 int hc_amr4_stage_rhs(hc_amr4* amr, const double* JU, const double* inner_ghosts, double* K, hc_stream stream);
 int hc_amr4_rk4_advance(hc_amr4* amr, double* JU, const double* inner_ghosts, double dt, int nsteps,

@@ -102,3 +102,26 @@ def test_amr4_rk4_matches_oracle(device, operator, config):
     cov = g0[b, :, c[2]:c[2] + 4, c[1]:c[1] + 4, c[0]:c[0] + 4]
     scale = np.abs(avg).max(axis=(1, 2, 3))[:, None, None, None]
     assert (np.abs(cov - avg) <= 1e-14 * scale).all()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("operator", [1, 2])
+def test_amr4_tilings_are_identical(device, operator):
+    """Refined patches prepared and evaluated per patch (the reference's
+    structure), in tiles of whole patches, and with other level-0 tiles give
+    the same stage RHS and RK4 step bit for bit."""
+    print("case amr4 tilings", operator, flush=True)
+    shells, patches, ju, inner, A, O = _case(operator, 2, 3 if operator == 1 else 4)
+    dt = 0.2 * (C.R0 * np.pi / 2 / shells[-1][1]) / 1.6
+    out = []
+    for level0, refined in ((None, None), (None, (8, 8, 8)), ((24, 16, 16), (16, 8, 16)), ((16, 32, 16), None)):
+        A.set_tiles(level0, refined)
+        K = torch.zeros(A.state_count, dtype=torch.float64, device=device)
+        A.stage_rhs(_dev(ju), _dev(inner), K)
+        U = _dev(ju)
+        A.rk4_advance(U, _dev(inner), dt, 1)
+        torch.cuda.synchronize()
+        print("tiles", level0, refined, "refined tiles", A.tile_count)
+        out.append((K.cpu().numpy(), U.cpu().numpy()))
+    for k, u in out[1:]:
+        assert np.array_equal(k, out[0][0]) and np.array_equal(u, out[0][1])
