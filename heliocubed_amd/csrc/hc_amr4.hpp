@@ -503,7 +503,7 @@ struct hc_amr4
     std::vector<AmrPatchH> P;               // refined patches in table order
     std::vector<std::array<int,3>> bsize, ratio;
     std::vector<std::vector<int>> lev, grid;
-    int tile0[3] = {32, 32, 32};            // level-0 tile shape
+    int tile0[3] = {128, 64, 64};           // level-0 tile shape (clipped to the level)
     int fineMax[3] = {128, 64, 64};         // largest refined tile
     long long nState = 0, nStage = 0, n0 = 0, us0 = 0, usInternal = 0, kInternal = 0;
     int ntarget = 0;

@@ -288,7 +288,8 @@ long long hc_amr4_count(const hc_amr4* amr, int what);
  * own ghosted box). The patches of every refined level are grouped into
  * rectangular tiles of whole patches of at most refined_max cells per
  * direction (default 128 x 64 x 64; the patch size gives one tile per
- * patch), and level 0 is evaluated in tiles of level0_tile (default 32^3);
+ * patch), and level 0 is evaluated in tiles of level0_tile (default
+ * 128 x 64 x 64, clipped to the level);
  * each tile is prepared and evaluated as one box, which gives the per-patch
  * results cell for cell with fewer ghost cells. NULL keeps a setting.
  * Synchronizes the device. hc_amr4_count(what 3) is the number of refined
