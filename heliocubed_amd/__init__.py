@@ -70,6 +70,7 @@ _SIGNATURES = [
     ("hc_stage_rhs", _I, [_P, _P, _P, _P, _D, _P, _P, _P, _P]),
     ("hc_rk4_advance", _I, [_P, _P, _P, _D, _I, _P]),
     ("hc_level_use_graph", _I, [_P, _I]),
+    ("hc_level_set_cons_sum", _I, [_P, _P]),
     ("hc_level_dt_cfl", _I, [_P, _P, _D, _P, _P]),
     ("hc_level_set_tile", _I, [_P, _I]),
     ("hc_level_set_tile_shape", _I, [_P, _I, _I, _I]),
@@ -315,6 +316,19 @@ class Level:
 
     def use_graph(self, enable: bool = True):
         _check(lib().hc_level_use_graph(self._handle, int(bool(enable))))
+
+    def set_cons_sum(self, cons):
+        """Conservation sums (radialCons): `cons` a float64 device tensor of 8
+        elements that every stage RHS adds b_dt x the net radial boundary
+        inflow to (fourth order); None turns it off. Keep the tensor alive
+        while it is set."""
+        import torch
+
+        if cons is not None and (cons.dtype != torch.float64 or not cons.is_cuda or cons.numel() != 8
+                                 or not cons.is_contiguous()):
+            raise ValueError("cons: need a contiguous float64 device tensor of 8 elements")
+        self._cons = cons
+        _check(lib().hc_level_set_cons_sum(self._handle, _ptr(cons)))
 
     def set_tile(self, tile):
         """Fourth-order box-table tiles (results unchanged): an edge for cubes,

@@ -46,6 +46,9 @@ void level4TileShape(const Level4& L, int tile[3]);
 void level4Prepare(Level4& L, const double* U0, const double* K, double a_dt, const double* inner, double* US,
                    hipStream_t s);
 void level4Rhs(Level4& L, const double* US, double* K, double* U, double b_dt, hipStream_t s);
+// cons[c] += scale * (inner - outer radial boundary face fluxes) of the last
+// level4Rhs (the conservation sums of radialCons).
+void level4RadialCons(Level4& L, double scale, double* cons, hipStream_t s);
 
 // Child weights of the fourth-order AMR interpolation (defined with the AMR
 // transfer kernels): the average over child `child` of `ratio` of the quartic

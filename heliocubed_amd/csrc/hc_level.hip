@@ -646,6 +646,8 @@ struct hc_level
     const double* gU = nullptr;
     const double* gInner = nullptr;
     double gDt = 0.0;
+    double* consSum = nullptr;      // radialCons accumulator (device double[8]) or none
+    double* gCons = nullptr;
 
     void dropGraph()
     {
@@ -851,6 +853,9 @@ void stageRhs(hc_level& L, const double* W, double* K, double* U, double b_dt,
     {
         if (F0 || F1 || F2) throw std::runtime_error("face fluxes of the fourth-order level: use hc_boxop_apply");
         L.profile.run("k4_rhs", s, [&] { hc::level4Rhs(*L.l4, W, K, U, b_dt, s); });
+        if (L.consSum)
+            L.profile.run("k4_radcons", s, [&] {
+                hc::level4RadialCons(*L.l4, b_dt/((double)v.na*v.na), L.consSum, s); });
         return;
     }
     if (!F0 && !F1 && !F2)
@@ -1127,7 +1132,7 @@ int hc_rk4_advance(hc_level* level, double* U, const double* inner, double dt, i
             HC_CHECK(hipGetLastError());
             return 0;
         }
-        if (!L.graphExec || L.gU != U || L.gInner != inner || L.gDt != dt)
+        if (!L.graphExec || L.gU != U || L.gInner != inner || L.gDt != dt || L.gCons != L.consSum)
         {
             L.dropGraph();
             if (!L.capStream) HC_CHECK(hipStreamCreateWithFlags(&L.capStream, hipStreamNonBlocking));
@@ -1152,12 +1157,22 @@ int hc_rk4_advance(hc_level* level, double* U, const double* inner, double dt, i
             }
             HC_CHECK(hipStreamEndCapture(L.capStream, &L.graph));
             HC_CHECK(hipGraphInstantiate(&L.graphExec, L.graph, nullptr, nullptr, 0));
-            L.gU = U; L.gInner = inner; L.gDt = dt;
+            L.gU = U; L.gInner = inner; L.gDt = dt; L.gCons = L.consSum;
         }
         for (int n = 0; n < nsteps; ++n) HC_CHECK(hipGraphLaunch(L.graphExec, s));
         return 0;
     }
     catch (const std::exception& e) { return fail(std::string("hc_rk4_advance: ") + e.what()); }
+}
+
+int hc_level_set_cons_sum(hc_level* level, double* cons_sum)
+{
+    if (!level) return fail("hc_level_set_cons_sum: null level");
+    if (cons_sum && !level->l4)
+        return fail("hc_level_set_cons_sum: the conservation sums use the fourth-order level's face fluxes "
+                    "(spatial_order 4)");
+    level->consSum = cons_sum;
+    return 0;
 }
 
 int hc_level_use_graph(hc_level* level, int enable)

@@ -1703,6 +1703,50 @@ __global__ void __launch_bounds__(kB) k4_radial_bc(Prep4View v, const double* __
 }
 } // namespace
 
+namespace
+{
+// Radial boundary flux sums of a fourth-order level (radialCons): block b
+// sums the radial face fluxes of faces [b chunk, (b+1) chunk) of the table
+// per component, + on the inner boundary (the first half of the table), - on
+// the outer; each thread takes every blockDim-th face of the chunk, then a
+// tree reduction in LDS. Deterministic for a fixed table and launch.
+__global__ void __launch_bounds__(kB) k4_radcons_part(const long long* __restrict__ face, const long long* __restrict__ ec,
+                                                      int nrad, int chunk, const double* __restrict__ S,
+                                                      double* __restrict__ part)
+{
+    __shared__ double red[kB];
+    const int f0 = blockIdx.x*chunk, f1 = min(nrad, f0 + chunk);
+    const int half = nrad/2;
+    for (int c = 0; c < 8; ++c)
+    {
+        double acc = 0;
+        for (int f = f0 + (int)threadIdx.x; f < f1; f += blockDim.x)
+        {
+            const double v = S[face[f] + c*ec[f]];
+            acc += f < half ? v : -v;
+        }
+        red[threadIdx.x] = acc;
+        __syncthreads();
+        for (int w = kB/2; w > 0; w >>= 1)
+        {
+            if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) part[blockIdx.x*8 + c] = red[0];
+        __syncthreads();
+    }
+}
+// cons[c] += scale * (sum of the block partials in block order).
+__global__ void k4_radcons_final(const double* __restrict__ part, int nblk, double scale, double* __restrict__ cons)
+{
+    const int c = threadIdx.x;
+    if (c >= 8) return;
+    double sum = 0;
+    for (int b = 0; b < nblk; ++b) sum += part[b*8 + c];
+    cons[c] += scale*sum;
+}
+} // namespace
+
 namespace hc
 {
 struct Level4
@@ -1718,13 +1762,62 @@ struct Level4
     long long* donorSrc = nullptr;
     double* donorW = nullptr;
     double* cart = nullptr;
-    std::vector<void*> owned;
+    // radial boundary faces of the box table (inner, then outer): flux offsets
+    // in the tiles' scratch and their component strides
+    long long* radFace = nullptr;
+    long long* radEc = nullptr;
+    double* radPart = nullptr;
+    int nrad = 0, radBlocks = 0;
+    std::vector<void*> owned, radOwned;
     ~Level4()
     {
         delete box;
+        for (void* p : radOwned) (void)hipFree(p);
         for (void* p : owned) (void)hipFree(p);
     }
 };
+
+constexpr int kRadChunk = 2048;   // radial faces per block of k4_radcons_part
+
+// The radial boundary faces of the level's box table: face i = 0 (inner) and
+// i = n_r (outer) of every angular column, read from the tile holding the
+// column's first or last radial cell.
+static void radialFaceTable(Level4& L)
+{
+    for (void* p : L.radOwned) (void)hipFree(p);
+    L.radOwned.clear();
+    const int nr = L.pv.nr, na = L.pv.na;
+    const hc_boxop& B = *L.box;
+    std::vector<int> inner((size_t)6*na*na, -1), outer((size_t)6*na*na, -1);
+    for (size_t n = 0; n < B.boxes.size(); ++n)
+    {
+        const BoxDesc& b = B.boxes[n];
+        for (int k = b.lo[2]; k <= b.hi[2]; ++k) for (int j = b.lo[1]; j <= b.hi[1]; ++j)
+        {
+            if (b.lo[0] == 0) inner[((size_t)b.p*na + k)*na + j] = (int)n;
+            if (b.hi[0] == nr-1) outer[((size_t)b.p*na + k)*na + j] = (int)n;
+        }
+    }
+    std::vector<long long> face, ec;
+    for (int side = 0; side < 2; ++side)
+        for (int p = 0; p < 6; ++p) for (int k = 0; k < na; ++k) for (int j = 0; j < na; ++j)
+        {
+            const int n = (side ? outer : inner)[((size_t)p*na + k)*na + j];
+            if (n < 0) throw std::runtime_error("radial boundary column without a tile");
+            const BoxDesc& b = B.boxes[n];
+            const long long e0 = b.n[0]+2*kG, e1 = b.n[1]+2*kG, e2 = b.n[2]+2*kG;
+            const int i = side ? nr : 0;
+            face.push_back(b.sOff + (long long)F_FLUX*e0*e1*e2
+                           + ((long long)(k-b.lo[2]+kG)*e1 + (j-b.lo[1]+kG))*e0 + (i-b.lo[0]+kG));
+            ec.push_back(e0*e1*e2);
+        }
+    L.nrad = (int)face.size();
+    L.radBlocks = (L.nrad + kRadChunk - 1)/kRadChunk;
+    L.radFace = upload(face, L.radOwned);
+    L.radEc = upload(ec, L.radOwned);
+    HC_THROW(hipMalloc(&L.radPart, std::max(1, L.radBlocks)*8*sizeof(double)));
+    L.radOwned.push_back(L.radPart);
+}
 
 // Seam donors of a ghost column of the Cartesian averages. The ghost cell is
 // integrated by a 6x6 Gauss-Legendre rule in its own panel's xi; each
@@ -1931,7 +2024,7 @@ Level4* level4Create(const hc_shell& shell, const hc_options& opt, const int til
     L->donorW = upload(donorW, L->owned);
     HC_THROW(hipMalloc(&L->cart, 6ull*v.cps*sizeof(double))); L->owned.push_back(L->cart);
     HC_THROW(hipMemset(L->cart, 0, 6ull*v.cps*sizeof(double)));
-    if (withTable) L->box = levelBoxTable(*L, tile);
+    if (withTable) { L->box = levelBoxTable(*L, tile); radialFaceTable(*L); }
     for (int d = 0; d < 3; ++d) L->tile[d] = tile[d];
     return L.release();
 }
@@ -1941,7 +2034,18 @@ void level4Retile(Level4& L, const int tile[3])
     hc_boxop* B = levelBoxTable(L, tile);
     delete L.box;
     L.box = B;
+    radialFaceTable(L);
     for (int d = 0; d < 3; ++d) L.tile[d] = tile[d];
+}
+
+// radialCons of the last stage RHS: cons[c] += scale * (sum of the inner
+// radial face fluxes - sum of the outer ones), from the tiles' scratch.
+void level4RadialCons(Level4& L, double scale, double* cons, hipStream_t s)
+{
+    hipLaunchKernelGGL(k4_radcons_part, dim3(L.radBlocks), dim3(kB), 0, s, L.radFace, L.radEc, L.nrad, kRadChunk,
+                       L.box->scratch, L.radPart);
+    hipLaunchKernelGGL(k4_radcons_final, dim3(1), dim3(64), 0, s, L.radPart, L.radBlocks, scale, cons);
+    HC_THROW(hipGetLastError());
 }
 
 void level4Destroy(Level4* L) { delete L; }

@@ -143,6 +143,17 @@ int hc_rk4_advance(hc_level* level, double* U, const double* inner_ghosts, doubl
  * individually. No reference counterpart (Proto launches per box). */
 int hc_level_use_graph(hc_level* level, int enable);
 
+/* Conservation sums (radialCons, src/BoxOp_EulerCubedSphereImplem.H:1010 as
+ * StageLevel calls it with the stage weight times dt; Proto's sum restated):
+ * with cons_sum a device array of 8 doubles, every stage RHS of a
+ * fourth-order level (hc_stage_rhs, hc_rk4_advance) adds b_dt times the net
+ * radial inflow sum over the inner and outer boundary faces of
+ * (F0(inner) - F0(outer)) dxi1 dxi2 per component, so that over RK4 steps
+ * cons_sum gathers the change of sum(<JU>) dxi^3 the radial boundaries cause
+ * (all of it without the Powell source). NULL turns it off. The caller zeroes
+ * the array; the level only adds to it. Fourth order only. */
+int hc_level_set_cons_sum(hc_level* level, double* cons_sum);
+
 /* Tiles of the fourth-order level's box table: the level evaluates its RHS as
  * a table of t0 x t1 x t2 boxes (radial x alpha x beta cells, clipped to the
  * panel; 32^3 by default, the box size of the reference's production

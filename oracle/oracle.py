@@ -78,6 +78,7 @@ def lib() -> ctypes.CDLL:
             "orc4_prepare": (I, [P, P, P, P]),
             "orc4_rhs": (I, [P, P, P]),
             "orc4_rk4": (I, [P, P, P, D, I]),
+            "orc4_rk4_cons": (I, [P, P, P, D, I, P]),
             "orc4_ghost_donors": (I, [cfgp, I, I, I, P, P, P, P]),
             "orc_wenoz": (None, [P, P]),
             "orc_berta_theta": (None, [cfgp, I, P, P, P, P]),
@@ -489,6 +490,14 @@ class Oracle4:
         g = _c(inner)
         lib().orc4_rk4(self._h, _p(u), _p(g), dt, nsteps)
         return u
+
+    def rk4_cons(self, ju, inner, dt, nsteps=1):
+        """RK4 steps and the radialCons sums of the radial boundary fluxes."""
+        u = np.ascontiguousarray(ju.copy())
+        g = _c(inner)
+        cons = np.zeros(8)
+        lib().orc4_rk4_cons(self._h, _p(u), _p(g), dt, nsteps, _p(cons))
+        return u, cons
 
 
 class Amr4Oracle:
