@@ -80,6 +80,9 @@ _SIGNATURES = [
     ("hc_level_valid_state", _I, [_P, _P, _P, _P, _P]),
     ("hc_level_inner_ghosts_from_bc", _I, [_P, _P, _P, _P]),
     ("hc_level_inner_ghosts_outflow", _I, [_P, _D, _D, _P, _P]),
+    ("hc_bc_frame_at", _I, [_P, _I, _I, _I, _D, _I, _D, _P, _P]),
+    ("hc_level_bc_points", _I, [_P, _P, _I, _I, _P, _P, _P, _P]),
+    ("hc_level_bc_averages", _I, [_P, _P, _I, _I, _P, _P, _P, _P]),
     ("hc_amr_interp4", _I, [_P, _P, _I, _P, _P, _P, _P]),
     ("hc_amr_ghosts2", _I, [_P, _P, _P, _P, _P, _D, _P, _P]),
     ("hc_amr_average_down", _I, [_P, _P, _I, _P, _P, _P, _P]),
@@ -379,6 +382,28 @@ class Level:
         self._require(inner, (6, 8, n, n, g), "inner")
         _check(lib().hc_level_inner_ghosts_from_bc(self._handle, _ptr(bc), _ptr(inner), self._stream()))
 
+    def bc_points(self, data0, theta, phi, bc):
+        """fillPoints: the boundary map data0[8][nphi][ntheta] (hc_bc_frame_at)
+        at the inner strip's cell centres, bc[6][8][na+2g][na+2g]."""
+        g = self.ghost()
+        n = self.shell.n_ang + 2 * g
+        nt, npf = theta.numel(), phi.numel()
+        self._require(data0, (8, npf, nt), "data0")
+        self._require(bc, (6, 8, n, n), "bc")
+        _check(lib().hc_level_bc_points(self._handle, _ptr(data0), nt, npf, _ptr(theta), _ptr(phi), _ptr(bc),
+                                        self._stream()))
+
+    def bc_averages(self, data0, theta, phi, inner):
+        """fill (BC_conserved_averages 1): native conserved averages of the
+        inner ghosts inner[6][8][na+2g][na+2g][g] from the map data0."""
+        g = self.ghost()
+        n = self.shell.n_ang + 2 * g
+        nt, npf = theta.numel(), phi.numel()
+        self._require(data0, (8, npf, nt), "data0")
+        self._require(inner, (6, 8, n, n, g), "inner")
+        _check(lib().hc_level_bc_averages(self._handle, _ptr(data0), nt, npf, _ptr(theta), _ptr(phi), _ptr(inner),
+                                          self._stream()))
+
     def inner_ghosts_outflow(self, inner, strength: float, time: float):
         """Inner radial ghosts of the radial-outflow test boundary at `time` (s)."""
         g = self.ghost()
@@ -518,6 +543,22 @@ def init_native_boxes(shell: Shell, options: Options, boxes, config: int = 3, se
     _check(lib().hc_init_native_boxes(ctypes.byref(sh), ctypes.byref(op), config, seed, tab, len(boxes), quad,
                                       out.ctypes.data))
     return out
+
+
+def bc_frame_at(frames, cadence_hours: float, frame_rotate: int, time: float, data0):
+    """BoxData_to_BC's frame step: frames [nframes][8][nphi][ntheta] (device)
+    -> the rotated, time-interpolated map data0 [8][nphi][ntheta] at `time` (s)."""
+    import torch
+
+    if frames.dim() != 4 or frames.shape[1] != 8:
+        raise ValueError("frames: need a [nframes][8][nphi][ntheta] tensor")
+    nf, _, npf, nt = frames.shape
+    for t, shape, name in ((frames, tuple(frames.shape), "frames"), (data0, (8, npf, nt), "data0")):
+        if t.dtype != torch.float64 or not t.is_cuda or not t.is_contiguous() or tuple(t.shape) != shape:
+            raise ValueError(f"{name}: need a contiguous float64 device tensor of shape {shape}")
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    _check(lib().hc_bc_frame_at(_ptr(frames), nf, nt, npf, float(cadence_hours), int(frame_rotate), float(time),
+                                _ptr(data0), stream))
 
 
 def init_ju_boxes(shell: Shell, options: Options, boxes, config: int = 3, seed: int = 20261203, quad: int = 3):

@@ -71,6 +71,13 @@ BcTable* bcCreate(const ShellDims& dims, int order, double gamma);
 void bcDestroy(BcTable* T);
 void bcLaunch(BcTable& T, const double* bc, double* inner, hipStream_t s);
 void outflowLaunch(BcTable& T, double strength, double time, double* inner, hipStream_t s);
+// Boundary-file maps onto the inner boundary (defined with their kernels):
+// fillPoints (the strip of cell-centre values) and fill (native conserved
+// averages of the inner ghosts).
+void bcPoints(const ShellDims& d, const double* data0, int ntheta, int nphi, const double* theta, const double* phi,
+              double* bc, hipStream_t s);
+void bcAverages(const ShellDims& d, int order, double gamma, const double* data0, int ntheta, int nphi,
+                const double* theta, const double* phi, double* inner, hipStream_t s);
 } // namespace hc
 
 #define HC_THROW(call)                                                              \

@@ -1310,6 +1310,33 @@ int hc_level_dt_cfl(hc_level* level, const double* U, double cone_boundary_densi
     catch (const std::exception& e) { return fail(std::string("hc_level_dt_cfl: ") + e.what()); }
 }
 
+int hc_level_bc_points(hc_level* level, const double* data0, int ntheta, int nphi, const double* theta,
+                       const double* phi, double* bc, hc_stream stream)
+{
+    if (!level || !data0 || !theta || !phi || !bc || ntheta < 1 || nphi < 1)
+        return fail("hc_level_bc_points: bad argument");
+    try
+    {
+        hc::bcPoints(level->dims, data0, ntheta, nphi, theta, phi, bc, static_cast<hipStream_t>(stream));
+        return 0;
+    }
+    catch (const std::exception& e) { return fail(std::string("hc_level_bc_points: ") + e.what()); }
+}
+
+int hc_level_bc_averages(hc_level* level, const double* data0, int ntheta, int nphi, const double* theta,
+                         const double* phi, double* inner, hc_stream stream)
+{
+    if (!level || !data0 || !theta || !phi || !inner || ntheta < 1 || nphi < 1)
+        return fail("hc_level_bc_averages: bad argument");
+    try
+    {
+        hc::bcAverages(level->dims, level->opt.spatial_order, level->opt.gamma, data0, ntheta, nphi, theta, phi, inner,
+                       static_cast<hipStream_t>(stream));
+        return 0;
+    }
+    catch (const std::exception& e) { return fail(std::string("hc_level_bc_averages: ") + e.what()); }
+}
+
 int hc_level_inner_ghosts_from_bc(hc_level* level, const double* bc, double* inner, hc_stream stream)
 {
     if (!level || !bc || !inner) return fail("hc_level_inner_ghosts_from_bc: null argument");

@@ -204,6 +204,33 @@ int hc_level_dt_cfl(hc_level* level, const double* U, double cone_boundary_densi
  * src/BoxOp_SecondOrderPreStage.H:194-214. Stream-ordered. */
 int hc_level_inner_ghosts_from_bc(hc_level* level, const double* bc, double* inner_ghosts, hc_stream stream);
 
+/* Boundary-file frames (the arithmetic of MHDReader::BoxData_to_BC,
+ * src/MHDReader.H:221-331). frames: device [nframes][8][nphi][ntheta] of the
+ * file variables (n, v_r, v_phi, v_theta, p, B_r, B_phi, B_theta) on the
+ * longitude-latitude nodes; data0: device [8][nphi][ntheta], the map at
+ * time [s]: frame floor(time / cadence) (the last one past the end, the only
+ * one if nframes == 1) rotated in longitude by 360 time / (25.38 days)
+ * degrees, in whole cells of the integer 360 / nphi degrees plus a linear
+ * blend, and between two frames interpolated linearly in time; with
+ * frame_rotate 1 each frame rotates from its own start (BC_frame_rotate).
+ * nphi <= 360. Stream-ordered. */
+int hc_bc_frame_at(const double* frames, int nframes, int ntheta, int nphi, double cadence_hours, int frame_rotate,
+                   double time, double* data0, hc_stream stream);
+/* The map onto the level's inner boundary (MHD_BoundaryAverages,
+ * src/MHD_BoundaryAverages.H:17-266); theta (polar angle from +z, ascending)
+ * and phi (azimuth, ascending over one turn) are device arrays of the nodes.
+ * hc_level_bc_points (fillPoints): bilinear values at the strip cell centres,
+ * bc[6][8][na+2g][na+2g] (the input of hc_level_inner_ghosts_from_bc; the
+ * second-order default, BC_conserved_averages 0). hc_level_bc_averages
+ * (fill, BC_conserved_averages 1): native conserved averages of the inner
+ * ghosts, inner_ghosts[6][8][na+2g][na+2g][g], by an 8x8 Gauss-Legendre
+ * rule per angular cell and a 4-point rule per radial ghost layer (the
+ * midpoint at second order). Stream-ordered. */
+int hc_level_bc_points(hc_level* level, const double* data0, int ntheta, int nphi, const double* theta,
+                       const double* phi, double* bc, hc_stream stream);
+int hc_level_bc_averages(hc_level* level, const double* data0, int ntheta, int nphi, const double* theta,
+                         const double* phi, double* inner_ghosts, hc_stream stream);
+
 /* Inner radial ghosts of the radial-outflow test boundary (init_condition_type
  * 1, f_radialoutflowIBCs, src/BoxOp_PreStage.H:5-48 as applied at :216-227):
  * rho = 700 m_p (r_BC/r)^2, p = 1e-7 (r_BC/r)^(2 gamma), radial speed

@@ -103,6 +103,9 @@ inline Counted atan(Counted a) { OPC_TICK(other); return Counted(std::atan(a.v))
 inline Counted atan2(Counted a, Counted b) { OPC_TICK(other); return Counted(std::atan2(a.v, b.v)); }
 inline Counted acos(Counted a) { OPC_TICK(other); return Counted(std::acos(a.v)); }
 inline Counted fmax(Counted a, Counted b) { OPC_TICK(cmp); return Counted(std::fmax(a.v, b.v)); }
+inline Counted fmod(Counted a, Counted b) { OPC_TICK(other); return Counted(std::fmod(a.v, b.v)); }
+inline Counted fmod(Counted a, double b) { OPC_TICK(other); return Counted(std::fmod(a.v, b)); }
+inline Counted fmod(Counted a, int b) { OPC_TICK(other); return Counted(std::fmod(a.v, double(b))); }
 inline Counted fmin(Counted a, Counted b) { OPC_TICK(cmp); return Counted(std::fmin(a.v, b.v)); }
 inline bool isfinite(Counted a) { return std::isfinite(a.v); }
 inline bool isnan(Counted a) { return std::isnan(a.v); }

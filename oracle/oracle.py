@@ -94,6 +94,9 @@ def lib() -> ctypes.CDLL:
             "orc_amr_average_down": (None, [P, P, P, I, P, P, P, P]),
             "orc_amr_prolong4": (I, [P, P, P, P, P, P, P, P, P]),
             "orc_amr4_create": (P, [cfgp, I, P, I, ctypes.c_char_p, I]),
+            "orc_bc_frame_at": (None, [P, I, I, I, D, I, D, P]),
+            "orc_bc_points": (None, [cfgp, P, I, I, P, P, P]),
+            "orc_bc_averages": (None, [cfgp, I, P, I, I, P, P, P]),
             "orc_amr4_destroy": (None, [P]),
             "orc_amr4_count": (ctypes.c_longlong, [P, I]),
             "orc_amr4_prepare": (I, [P, P, P, P]),
@@ -548,3 +551,36 @@ class Amr4Oracle:
         g = _c(inner)
         lib().orc_amr4_rk4(self._h, _p(u), _p(g), dt, nsteps)
         return u
+
+
+def bc_frame_at(frames, cadence_hours, frame_rotate, time):
+    """BoxData_to_BC's frame step on [nframes][8][nphi][ntheta] host frames."""
+    f = _c(frames)
+    nf, _, npf, nt = f.shape
+    out = np.zeros((8, npf, nt))
+    lib().orc_bc_frame_at(_p(f), nf, nt, npf, float(cadence_hours), int(frame_rotate), float(time), _p(out))
+    return out
+
+
+def _bc_cfg(n_rad, n_ang, ghost, r0, r1, gamma):
+    return OrcCfg(n_rad, n_ang, ghost, r0, r1, 0.0, gamma, 1, 2, 0.3, 1, 0.02, 0, 0)
+
+
+def bc_points(n_rad, n_ang, ghost, data0, theta, phi, r0=21.5, r1=215.0, gamma=5.0 / 3.0):
+    """fillPoints onto the inner strip [6][8][na+2g][na+2g]."""
+    d, t, p = _c(data0), _c(theta), _c(phi)
+    n = n_ang + 2 * ghost
+    out = np.zeros((6, 8, n, n))
+    cfg = _bc_cfg(n_rad, n_ang, ghost, r0, r1, gamma)
+    lib().orc_bc_points(ctypes.byref(cfg), _p(d), t.size, p.size, _p(t), _p(p), _p(out))
+    return out
+
+
+def bc_averages(n_rad, n_ang, ghost, order, data0, theta, phi, r0=21.5, r1=215.0, gamma=5.0 / 3.0):
+    """fill: native conserved averages of the inner ghosts [6][8][na+2g][na+2g][g]."""
+    d, t, p = _c(data0), _c(theta), _c(phi)
+    n = n_ang + 2 * ghost
+    out = np.zeros((6, 8, n, n, ghost))
+    cfg = _bc_cfg(n_rad, n_ang, ghost, r0, r1, gamma)
+    lib().orc_bc_averages(ctypes.byref(cfg), int(order), _p(d), t.size, p.size, _p(t), _p(p), _p(out))
+    return out
