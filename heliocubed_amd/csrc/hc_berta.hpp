@@ -12,9 +12,9 @@
 //   per direction:
 //     BertaReconstructWithFallback (:747-1239): WENOZ / WENO3 / PLM face points
 //     Roe repair with the cell-average donors (:1817-1844)
-//     point fluxes of both sides summed, theta-aware face averages (:1246-1266, :1904-1946)
-//     BertaPrimPointToMappedFlux (:1334-1534) with the separable-metric shortcut (:1-98),
-//       evaluated inside the flux kernel
+//     point fluxes of both sides summed (:1904-1946)
+//     theta-aware face averages (:1246-1266) and BertaPrimPointToMappedFlux (:1334-1534)
+//       with the separable-metric shortcut (:1-98), in one LDS-tiled kernel
 //     primToMappedRoeFlux with theta-aware dissipation average (src/BoxOp_RoeCubedSphere.H:29-145),
 //     or BertaPrimPointToMappedRusanovFlux (:1541-1730) for riemann_solver 1
 //     F = (Sum - D)/2 (:2004-2026)
@@ -28,9 +28,7 @@ constexpr int F_WPT = F_W;                  // point primitives, box+4
 constexpr int F_BWL = F_PTL, F_BWR = F_PTR; // face points (k_o4_roe reads them)
 constexpr int F_THF = F_P0L;                // flux-average flag of a face
 constexpr int F_PTOT = F_P0R;               // point total pressure, both sides summed
-constexpr int F_PAVG = F_MEL;               // its theta-aware face average
 constexpr int F_FPS = F_PHL;                // point physical fluxes, both sides summed
-constexpr int F_PFA = F_PHR;                // their theta-aware face average
 
 __device__ __forceinline__ double refMax(double a, double b) { return a < b ? b : a; }
 __device__ __forceinline__ double refMin(double a, double b) { return b < a ? b : a; }
@@ -323,248 +321,313 @@ __global__ void __launch_bounds__(kB) k_b_faces(O4View v, double* __restrict__ S
     }
 }
 
-// 5. Theta-aware face averages of the summed point flux and total pressure
-// on the faces [lo_d, hi_d+1] x transverse [lo-1, hi+1] (:1246-1266).
-template <int D>
-__global__ void __launch_bounds__(kB) k_b_avg(O4View v, double* __restrict__ S)
+// 5. Face-average and face-flux kernel of direction D over rectangular tiles
+// of faces (theta-aware averages, :1246-1266; BertaPrimPointToMappedFlux,
+// :1334-1534, with the separable-metric shortcut, :1-98; Roe or Rusanov
+// dissipation averages; F = (Sum - D)/2, :2004-2026). A workgroup owns
+// FluxTile<D>::T0 x T1 x T2 faces of one box; per input component it stages
+// the summed point flux on the tile grown by 2 in the two transverse
+// directions and the point dissipation grown by 1 in LDS, forms the face
+// averages of the grown-by-1 tile there, and every face reads its stencils
+// (transverse differences, the 3x3 radial-transverse window of the angular
+// directions) from LDS. The flux-average flag and the face metric rows of
+// the tile are staged once. Each output accumulates its terms in the same
+// order as the term-by-term form of the operator.
+// The points of an LDS region org - h + [0, N0) x [0, N1) x [0, N2) that
+// this thread stages: their scratch offsets (field 0), -1 where the region
+// leaves [lo, hi] (the points where the staged fields are defined; staged as
+// 0). Computed once per tile, used for every component.
+template <int N0, int N1, int N2>
+struct StagePlan
 {
-    const BoxDesc& b = blockBoxDesc(v);
-    int L[3], H[3], x[3];
-    for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]-1; H[d] = b.hi[d]+1; }
-    L[D] = b.lo[D]; H[D] = b.hi[D]+1;
-    if (!regionPoint(b, L, H, boxThread(v, b), x)) return;
-    const BoxIdx I(b);
-    double* const P = S + b.sOff;
-    constexpr int t1 = D == 0 ? 1 : 0, t2 = D == 2 ? 1 : 2;
-    const bool smooth = P[I(F_THF, x[0], x[1], x[2])] > 0.5;
-    auto avg = [&](int f) {
-        const double c0 = P[I(f, x[0], x[1], x[2])];
-        double lap = 0;
-#pragma unroll
-        for (int t = 0; t < 2; ++t)
-        {
-            const int tt = t == 0 ? t1 : t2;
-            int y[3] = {x[0], x[1], x[2]}, z[3] = {x[0], x[1], x[2]};
-            y[tt] += 1; z[tt] -= 1;
-            lap += P[I(f, y[0], y[1], y[2])] - 2.0*c0 + P[I(f, z[0], z[1], z[2])];
-        }
-        return smooth ? c0 + lap/24.0 : c0; };
-#pragma unroll
-    for (int c = 0; c < 8; ++c) P[I(F_PFA+c, x[0], x[1], x[2])] = avg(F_FPS+c);
-    P[I(F_PAVG, x[0], x[1], x[2])] = avg(F_PTOT);
-}
-
-// 6. Mapped products of the averaged flux at a radial face (bertaMapHostBox,
-// :35-98): per matrix column one term (transverse corrections, then the
-// product), the terms summed over columns. The radial flux kernel evaluates
-// them at its own face; the angular directions use FaceOps::mapped.
-__device__ __forceinline__ void bertaRadialMapped(const FaceOps<0>& op, const int x[3], double bmp[8])
-{
-    constexpr int t1 = 1, t2 = 2;
-#pragma unroll
-    for (int s = 0; s < 2; ++s)
+    static constexpr int N = N0*N1*N2, K = (N + kFT - 1)/kFT;
+    int off[K];
+    __device__ StagePlan(const BoxIdx& I, const int org[3], const int h[3], const int lo[3], const int hi[3])
     {
-        const int c = s == 0 ? 0 : 4;
-        double term = 0;
-        term += op.Dm(0, t1, x)*op.Df(F_PFA+c, t1, x)*(1.0/12.0);
-        term += op.Dm(0, t2, x)*op.Df(F_PFA+c, t2, x)*(1.0/12.0);
-        term += op.M(0, x)*op.Fv(F_PFA+c, x)*1.0;
-        double sum = 0;
-        sum += term;
-        bmp[c] = sum;
-    }
 #pragma unroll
-    for (int grp = 0; grp < 2; ++grp)
-    {
-        const int c0 = grp == 0 ? 1 : 5;
-#pragma unroll
-        for (int r = 0; r < 3; ++r)
+        for (int k = 0; k < K; ++k)
         {
-            double sum = 0;
-#pragma unroll
-            for (int col = 0; col < 3; ++col)
-            {
-                double term = 0;
-                term += op.Dm(1+3*r+col, t1, x)*op.Df(F_PFA+c0+col, t1, x)*(1.0/12.0);
-                term += op.Dm(1+3*r+col, t2, x)*op.Df(F_PFA+c0+col, t2, x)*(1.0/12.0);
-                term += op.M(1+3*r+col, x)*op.Fv(F_PFA+c0+col, x)*1.0;
-                sum += term;
-            }
-            bmp[c0+r] = sum;
+            const int p = threadIdx.x + k*kFT;
+            const int a0 = p % N0, q = p / N0, a1 = q % N1, a2 = q / N1;
+            const int y0 = org[0]-h[0]+a0, y1 = org[1]-h[1]+a1, y2 = org[2]-h[2]+a2;
+            const bool in = p < N && y0 >= lo[0] && y0 <= hi[0] && y1 >= lo[1] && y1 <= hi[1]
+                            && y2 >= lo[2] && y2 <= hi[2];
+            off[k] = in ? I(0, y0, y1, y2) : -1;
         }
     }
-}
+    // load the field at scratch offset f0 (field index x field size) into
+    // registers, then store them into LDS
+    __device__ __forceinline__ void load(double (&val)[K], const double* __restrict__ P, int f0) const
+    {
+#pragma unroll
+        for (int k = 0; k < K; ++k) val[k] = off[k] >= 0 ? P[f0 + off[k]] : 0.0;
+    }
+    template <typename T>
+    __device__ __forceinline__ void store(T* __restrict__ dst, const double (&val)[K]) const
+    {
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+        {
+            const int p = threadIdx.x + k*kFT;
+            if (N % kFT == 0 || p < N) dst[p] = T(val[k]);
+        }
+    }
+};
 
-// 7. Face flux on [lo_d, hi_d+1] x [lo, hi]: radial-area tensor product of the
-// mapped products, pressure force ntrow <(p_tot averaged) adjDr>, the
-// theta-aware averaged Roe dissipation with its metric products, and
-// F = (Sum - D)/2. Evaluated one input component at a time (its stencil of
-// averaged fluxes and dissipation, then its contributions to the outputs it
-// enters), which bounds the live registers; every output accumulates its
-// terms in the same order as the term-by-term form.
 template <int D, bool ROE>
-__global__ void __launch_bounds__(kB) k_b_flux(O4View v, double* __restrict__ S)
+__global__ void __launch_bounds__(kFT) k_b_avgflux(O4View v, double* __restrict__ S)
 {
-    const BoxDesc& b = blockBoxDesc(v);
-    int L[3], H[3], x[3];
-    for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]; H[d] = b.hi[d]; }
-    H[D] = b.hi[D]+1;
-    if (!regionPoint(b, L, H, boxThread(v, b), x)) return;
+    using G = FluxTile<D>;
+    constexpr int t1 = D == 0 ? 1 : 0, t2 = D == 2 ? 1 : 2;
+    __shared__ double sF[G::NF];        // summed point flux (or total pressure) of one component
+    __shared__ double sA[G::NA];        // its face average
+    __shared__ double sD[G::NA];        // point dissipation of the component
+    __shared__ double sM[13*G::NM];     // face metric rows of the tile's angular columns
+    __shared__ float sT[G::NA];         // flux-average flag (0 or 1)
+    const FluxTileRef tl = v.ftiles[D][xcdOrder(blockIdx.x, v.nftiles[D])];
+    const BoxDesc& b = v.boxes[tl.box];
     const BoxIdx I(b);
     double* const P = S + b.sOff;
-    constexpr int t1 = D == 0 ? 1 : 0, t2 = D == 2 ? 1 : 2;
-    const FaceOps<D> op(v, b, I, P);
-    auto Fv = [&](int f, const int y[3]) { return op.Fv(f, y); };
-    const double adjX = areaOf(v, b, D, x[0]);
-    auto dAdj = [&](int i) { return 0.5*areaOf(v, b, D, i+1) - 0.5*areaOf(v, b, D, i-1); };
-    const double dA = D == 0 ? 0.0 : dAdj(x[0]);
-    auto shift = [&](int f, int t, int o) { int y[3] = {x[0], x[1], x[2]}; y[t] += o; return Fv(f, y); };
-    auto shift2 = [&](int f, int o0, int o2) { int y[3] = {x[0]+o0, x[1], x[2]}; y[t2] += o2; return Fv(f, y); };
-    // Df(f, t) at x and, for the angular radial difference, the half radial
-    // differences Gv(f, .) at x and x +- e_t2 (Gv = (F(+e_r) - F(-e_r))/2)
-    auto Df = [&](int f, int t) { return 0.5*shift(f, t, 1) - 0.5*shift(f, t, -1); };
-    auto Gv0 = [&](int f) { return 0.5*shift(f, 0, 1) - 0.5*shift(f, 0, -1); };
-    auto DGv = [&](int f) {
-        const double gp = 0.5*shift2(f, 1, 1) - 0.5*shift2(f, -1, 1);
-        const double gm = 0.5*shift2(f, 1, -1) - 0.5*shift2(f, -1, -1);
+    const int org[3] = {tl.o0, tl.o1, tl.o2};
+    const int hA[3] = {G::h0, G::h1, G::h2}, hF[3] = {2*G::h0, 2*G::h1, 2*G::h2};
+    // the face kernel defines the point fluxes on the faces [lo_d, hi_d+1] x
+    // transverse [lo-2, hi+2], the dissipation on transverse [lo-1, hi+1]
+    int LA[3], HA[3], LF[3], HF[3];
+    for (int d = 0; d < 3; ++d)
+    {
+        LA[d] = b.lo[d]-hA[d]; HA[d] = b.hi[d]+hA[d]+(d == D);
+        LF[d] = b.lo[d]-hF[d]; HF[d] = b.hi[d]+hF[d]+(d == D);
+    }
+    const StagePlan<G::F0, G::F1, G::F2> planF(I, org, hF, LF, HF);
+    const StagePlan<G::A0, G::A1, G::A2> planA(I, org, hA, LA, HA);
+    {
+        double t[StagePlan<G::A0, G::A1, G::A2>::K];
+        planA.load(t, P, F_THF*I.ecount);
+        planA.store(sT, t);
+    }
+    for (int p = threadIdx.x; p < 13*G::NM; p += kFT)
+    {
+        const int col = p / 13, c = p - 13*col, a1 = col % G::A1, a2 = col / G::A1;
+        const int j = org[1]-G::h1+a1, k = org[2]-G::h2+a2;
+        const bool in = j >= LA[1] && j <= HA[1] && k >= LA[2] && k <= HA[2];
+        sM[p] = in ? v.fmet[D][13*angIdx(b, j, k) + c] : 0.0;
+    }
+    // LDS index of the face-local point l + offsets in the averaged region
+    auto ia = [](int l0, int l1, int l2) { return ((l2+G::h2)*G::A1 + (l1+G::h1))*G::A0 + (l0+G::h0); };
+    auto mrow = [](int l1, int l2) { return 13*((l2+G::h2)*G::A1 + (l1+G::h1)); };
+    auto sh = [&](const double* s, const int l[3], int t, int o) {
+        int y[3] = {l[0], l[1], l[2]}; y[t] += o; return s[ia(y[0], y[1], y[2])]; };
+    auto Mv = [&](int c, const int l[3]) { return sM[mrow(l[1], l[2]) + c]; };
+    auto Dm = [&](int c, int t, const int l[3]) {
+        int a[3] = {l[0], l[1], l[2]}, z[3] = {l[0], l[1], l[2]};
+        a[t] += 1; z[t] -= 1;
+        return 0.5*sM[mrow(a[1], a[2]) + c] - 0.5*sM[mrow(z[1], z[2]) + c]; };
+    auto Df = [&](const double* s, int t, const int l[3]) { return 0.5*sh(s, l, t, 1) - 0.5*sh(s, l, t, -1); };
+    // radial half difference and its centred t2 difference (angular directions)
+    auto Gv0 = [&](const double* s, const int l[3]) { return 0.5*sh(s, l, 0, 1) - 0.5*sh(s, l, 0, -1); };
+    auto DGv = [&](const double* s, const int l[3]) {
+        auto s2 = [&](int o0, int o2) { int y[3] = {l[0]+o0, l[1], l[2]}; y[t2] += o2; return s[ia(y[0], y[1], y[2])]; };
+        const double gp = 0.5*s2(1, 1) - 0.5*s2(-1, 1);
+        const double gm = 0.5*s2(1, -1) - 0.5*s2(-1, -1);
         return 0.5*gp - 0.5*gm; };
-    // face-average Roe dissipation of component f (theta-aware)
-    const bool smooth = Fv(F_THF, x) > 0.5;
-    auto Davg = [&](int f) {
-        const double c0 = Fv(f, x);
+    auto Davg = [&](const double* s, bool smooth, const int l[3]) {
+        const double c0 = s[ia(l[0], l[1], l[2])];
         double lap = 0;
-#pragma unroll
-        for (int t = 0; t < 2; ++t)
-        {
-            const int tt = t == 0 ? t1 : t2;
-            lap += shift(f, tt, 1) - 2.0*c0 + shift(f, tt, -1);
-        }
+        lap += sh(s, l, t1, 1) - 2.0*c0 + sh(s, l, t1, -1);
+        lap += sh(s, l, t2, 1) - 2.0*c0 + sh(s, l, t2, -1);
         return smooth ? c0 + lap/24.0 : c0; };
-    // Fsum of a component from its mapped products (radial: bertaRadialMapped's
-    // per-column terms; angular: FaceOps::mapped and mappedRadialDiff)
-    auto fsum = [&](double mp, double dmp) {
-        double val = 0;
-        if (D != 0) val += dmp*dA*(1.0/12.0);
-        val += mp*adjX*1.0;
-        return val; };
-    // <p_tot adjDr> at y: radial tensor product of the averaged total pressure
-    auto tmp3 = [&](const int y[3]) {
-        double val = 0;
-        if (D != 0)
-        {
-            const int a[3] = {y[0]+1, y[1], y[2]}, z[3] = {y[0]-1, y[1], y[2]};
-            const double dv = 0.5*Fv(F_PAVG, a) - 0.5*Fv(F_PAVG, z);
-            val += dv*dAdj(y[0])*(1.0/12.0);
-        }
-        val += Fv(F_PAVG, y)*areaOf(v, b, D, y[0])*1.0;
-        return val; };
-    double press[3];
-    {
-        const double t0v = tmp3(x);
-        double dT[2] = {0.0, 0.0};
+
+    // the faces of this thread
+    int lf[kFPT][3];
+    bool ok[kFPT];
+    double adjX[kFPT], dA[kFPT], speed[kFPT];
 #pragma unroll
-        for (int t = D == 0 ? 0 : 1; t < 2; ++t)
-        {
-            const int tt = t == 0 ? t1 : t2;
-            int y[3] = {x[0], x[1], x[2]}, z[3] = {x[0], x[1], x[2]};
-            y[tt] += 1; z[tt] -= 1;
-            dT[t] = 0.5*tmp3(y) - 0.5*tmp3(z);
-        }
+    for (int k = 0; k < kFPT; ++k)
+    {
+        const int q = threadIdx.x + k*kFT;
+        lf[k][0] = q % G::T0; lf[k][1] = (q / G::T0) % G::T1; lf[k][2] = q / (G::T0*G::T1);
+        ok[k] = true;
 #pragma unroll
-        for (int r = 0; r < 3; ++r)
+        for (int d = 0; d < 3; ++d)
         {
-            double val = 0;
-            if (D == 0) val += op.Dm(10+r, t1, x)*dT[0]*(1.0/12.0);
-            val += op.Dm(10+r, t2, x)*dT[1]*(1.0/12.0);
-            val += op.M(10+r, x)*t0v*1.0;
-            press[r] = val;
+            const int x = org[d]+lf[k][d];
+            ok[k] = ok[k] && x >= b.lo[d] && x <= b.hi[d]+(d == D);
         }
+        const int x0 = org[0]+lf[k][0];
+        adjX[k] = ok[k] ? areaOf(v, b, D, x0) : 0.0;
+        dA[k] = (ok[k] && D != 0) ? 0.5*areaOf(v, b, D, x0+1) - 0.5*areaOf(v, b, D, x0-1) : 0.0;
+        speed[k] = (ok[k] && !ROE) ? P[I(F_SPD, x0, org[1]+lf[k][1], org[2]+lf[k][2])] : 0.0;
     }
-    const double speed = ROE ? 0.0 : Fv(F_SPD, x);
-    // scalar components 0 and 4
-#pragma unroll 1
-    for (int s = 0; s < 2; ++s)
+    // F-region index of the centre of each averaged point of this thread
+    constexpr int KA = (G::NA + kFT - 1)/kFT;
+    int fc[KA];
+#pragma unroll
+    for (int k = 0; k < KA; ++k)
     {
-        const int c = s == 0 ? 0 : 4;
-        double mp = 0, dmp = 0;
-        if (D == 0)
-        {
-            double term = 0;
-            term += op.Dm(0, t1, x)*Df(F_PFA+c, t1)*(1.0/12.0);
-            term += op.Dm(0, t2, x)*Df(F_PFA+c, t2)*(1.0/12.0);
-            term += op.M(0, x)*Fv(F_PFA+c, x)*1.0;
-            mp += term;
-        }
-        else
-        {
-            mp += op.Dm(0, t2, x)*Df(F_PFA+c, t2)*(1.0/12.0);
-            mp += op.M(0, x)*Fv(F_PFA+c, x);
-            dmp += op.Dm(0, t2, x)*DGv(F_PFA+c)*(1.0/12.0);
-            dmp += op.M(0, x)*Gv0(F_PFA+c);
-        }
-        const double fs = fsum(mp, dmp);
-        const double da = Davg(F_DPT+c);
-        double fd;
-        if constexpr (ROE)
-        {
-            fd = 0;
-            fd += op.Dm(0, t1, x)*Df(F_DPT+c, t1)*(1.0/12.0);
-            fd += op.Dm(0, t2, x)*Df(F_DPT+c, t2)*(1.0/12.0);
-            fd += op.M(0, x)*da;
-        }
-        else fd = (op.M(0, x)*da)*speed;
-        P[I(F_FLUX+8*D+c, x[0], x[1], x[2])] = 0.5*(fs-fd);
+        const int p = threadIdx.x + k*kFT;
+        const int a0 = p % G::A0, q = p / G::A0, a1 = q % G::A1, a2 = q / G::A1;
+        fc[k] = ((a2+G::h2)*G::F1 + (a1+G::h1))*G::F0 + (a0+G::h0);
     }
-    // vector components: momentum (grp 0, with the pressure force) and field
+    double press[kFPT][3], mp[kFPT][3], dmp[kFPT][3], fd[kFPT][3];
+    // components in the order their outputs are completed: total pressure
+    // (the momentum pressure force), 0, 4, momentum 1..3, field 5..7
+    auto component = [](int s) { return s == 0 ? -1 : s == 1 ? 0 : s == 2 ? 4 : s <= 5 ? s-2 : s-1; };
+    // the next component's point fluxes and dissipation are loaded into
+    // registers while the current one is evaluated
+    double nF[StagePlan<G::F0, G::F1, G::F2>::K], nD[StagePlan<G::A0, G::A1, G::A2>::K];
+    planF.load(nF, P, F_PTOT*I.ecount);
 #pragma unroll 1
-    for (int grp = 0; grp < 2; ++grp)
+    for (int s = 0; s < 9; ++s)
     {
-        const int c0 = grp == 0 ? 1 : 5;
-        double mp[3] = {0, 0, 0}, dmp[3] = {0, 0, 0}, fd[3] = {0, 0, 0};
-#pragma unroll 1
-        for (int col = 0; col < 3; ++col)
+        const int c = component(s);
+        __syncthreads();
+        planF.store(sF, nF);
+        if (c >= 0) planA.store(sD, nD);
+        __syncthreads();
+        if (s < 8)
         {
-            const int f = F_PFA+c0+col, g = F_DPT+c0+col;
-            const double fx = Fv(f, x), df2 = Df(f, t2);
-            const double df1 = D == 0 ? Df(f, t1) : 0.0;
-            const double gx = D == 0 ? 0.0 : Gv0(f), dg = D == 0 ? 0.0 : DGv(f);
-            const double da = Davg(g);
-            const double dd1 = ROE ? Df(g, t1) : 0.0, dd2 = ROE ? Df(g, t2) : 0.0;
+            const int cn = component(s+1);
+            planF.load(nF, P, (F_FPS+cn)*I.ecount);
+            planA.load(nD, P, (F_DPT+cn)*I.ecount);
+        }
+        // theta-aware face average on the tile grown by 1
+#pragma unroll
+        for (int k = 0; k < KA; ++k)
+        {
+            const int p = threadIdx.x + k*kFT;
+            if (G::NA % kFT != 0 && p >= G::NA) break;
+            auto F = [&](int t, int o) { return sF[fc[k] + (t == 0 ? o : t == 1 ? o*G::F0 : o*G::F0*G::F1)]; };
+            const double c0 = sF[fc[k]];
+            double lap = 0;
+            lap += F(t1, 1) - 2.0*c0 + F(t1, -1);
+            lap += F(t2, 1) - 2.0*c0 + F(t2, -1);
+            sA[p] = sT[p] > 0.5f ? c0 + lap/24.0 : c0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kFPT; ++k)
+        {
+            if (!ok[k]) continue;
+            const int* l = lf[k];
+            const int x0 = org[0]+l[0], x1 = org[1]+l[1], x2 = org[2]+l[2];
+            const bool smooth = sT[ia(l[0], l[1], l[2])] > 0.5f;
+            if (c < 0)
+            {
+                // <p_tot adjDr> at l + o e_t2 (angular) or l (radial faces:
+                // every point of the window shares the face's radius)
+                auto tmp3 = [&](int t, int o) {
+                    int y[3] = {l[0], l[1], l[2]}; y[t] += o;
+                    double val = 0;
+                    if constexpr (D != 0)
+                    {
+                        const double dv = 0.5*sh(sA, y, 0, 1) - 0.5*sh(sA, y, 0, -1);
+                        val += dv*dA[k]*(1.0/12.0);
+                    }
+                    val += sA[ia(y[0], y[1], y[2])]*adjX[k]*1.0;
+                    return val; };
+                const double t0v = tmp3(0, 0);
+                double dT[2] = {0.0, 0.0};
+                if constexpr (D == 0) dT[0] = 0.5*tmp3(t1, 1) - 0.5*tmp3(t1, -1);
+                dT[1] = 0.5*tmp3(t2, 1) - 0.5*tmp3(t2, -1);
+#pragma unroll
+                for (int r = 0; r < 3; ++r)
+                {
+                    double val = 0;
+                    if constexpr (D == 0) val += Dm(10+r, t1, l)*dT[0]*(1.0/12.0);
+                    val += Dm(10+r, t2, l)*dT[1]*(1.0/12.0);
+                    val += Mv(10+r, l)*t0v*1.0;
+                    press[k][r] = val;
+                }
+                continue;
+            }
+            auto fsum = [&](double m, double dm) {
+                double val = 0;
+                if constexpr (D != 0) val += dm*dA[k]*(1.0/12.0);
+                val += m*adjX[k]*1.0;
+                return val; };
+            if (c == 0 || c == 4)
+            {
+                double m = 0, dm = 0;
+                if constexpr (D == 0)
+                {
+                    double term = 0;
+                    term += Dm(0, t1, l)*Df(sA, t1, l)*(1.0/12.0);
+                    term += Dm(0, t2, l)*Df(sA, t2, l)*(1.0/12.0);
+                    term += Mv(0, l)*sA[ia(l[0], l[1], l[2])]*1.0;
+                    m += term;
+                }
+                else
+                {
+                    m += Dm(0, t2, l)*Df(sA, t2, l)*(1.0/12.0);
+                    m += Mv(0, l)*sA[ia(l[0], l[1], l[2])];
+                    dm += Dm(0, t2, l)*DGv(sA, l)*(1.0/12.0);
+                    dm += Mv(0, l)*Gv0(sA, l);
+                }
+                const double fs = fsum(m, dm);
+                const double da = Davg(sD, smooth, l);
+                double dd;
+                if constexpr (ROE)
+                {
+                    dd = 0;
+                    dd += Dm(0, t1, l)*Df(sD, t1, l)*(1.0/12.0);
+                    dd += Dm(0, t2, l)*Df(sD, t2, l)*(1.0/12.0);
+                    dd += Mv(0, l)*da;
+                }
+                else dd = (Mv(0, l)*da)*speed[k];
+                P[I(F_FLUX+8*D+c, x0, x1, x2)] = 0.5*(fs-dd);
+                continue;
+            }
+            // vector components: momentum (with the pressure force) and field
+            const int c0 = c <= 3 ? 1 : 5, col = c-c0;
+            if (col == 0)
+            {
+#pragma unroll
+                for (int r = 0; r < 3; ++r) { mp[k][r] = 0; dmp[k][r] = 0; fd[k][r] = 0; }
+            }
+            const double fx = sA[ia(l[0], l[1], l[2])], df2 = Df(sA, t2, l);
+            double df1 = 0.0, gx = 0.0, dg = 0.0;
+            if constexpr (D == 0) df1 = Df(sA, t1, l);
+            else { gx = Gv0(sA, l); dg = DGv(sA, l); }
+            const double da = Davg(sD, smooth, l);
+            const double dd1 = ROE ? Df(sD, t1, l) : 0.0, dd2 = ROE ? Df(sD, t2, l) : 0.0;
 #pragma unroll
             for (int r = 0; r < 3; ++r)
             {
                 const int m = 1+3*r+col;
-                if (D == 0)
+                if constexpr (D == 0)
                 {
                     double term = 0;
-                    term += op.Dm(m, t1, x)*df1*(1.0/12.0);
-                    term += op.Dm(m, t2, x)*df2*(1.0/12.0);
-                    term += op.M(m, x)*fx*1.0;
-                    mp[r] += term;
+                    term += Dm(m, t1, l)*df1*(1.0/12.0);
+                    term += Dm(m, t2, l)*df2*(1.0/12.0);
+                    term += Mv(m, l)*fx*1.0;
+                    mp[k][r] += term;
                 }
                 else
                 {
-                    mp[r] += op.Dm(m, t2, x)*df2*(1.0/12.0);
-                    mp[r] += op.M(m, x)*fx;
-                    dmp[r] += op.Dm(m, t2, x)*dg*(1.0/12.0);
-                    dmp[r] += op.M(m, x)*gx;
+                    mp[k][r] += Dm(m, t2, l)*df2*(1.0/12.0);
+                    mp[k][r] += Mv(m, l)*fx;
+                    dmp[k][r] += Dm(m, t2, l)*dg*(1.0/12.0);
+                    dmp[k][r] += Mv(m, l)*gx;
                 }
                 if constexpr (ROE)
                 {
-                    fd[r] += op.Dm(m, t1, x)*dd1*(1.0/12.0);
-                    fd[r] += op.Dm(m, t2, x)*dd2*(1.0/12.0);
-                    fd[r] += op.M(m, x)*da;
+                    fd[k][r] += Dm(m, t1, l)*dd1*(1.0/12.0);
+                    fd[k][r] += Dm(m, t2, l)*dd2*(1.0/12.0);
+                    fd[k][r] += Mv(m, l)*da;
                 }
-                else fd[r] += op.M(m, x)*da;
+                else fd[k][r] += Mv(m, l)*da;
             }
-        }
+            if (col == 2)
+            {
 #pragma unroll
-        for (int r = 0; r < 3; ++r)
-        {
-            double fs = fsum(mp[r], dmp[r]);
-            if (grp == 0) fs = fs + press[r];
-            const double d = ROE ? fd[r] : fd[r]*speed;
-            P[I(F_FLUX+8*D+c0+r, x[0], x[1], x[2])] = 0.5*(fs-d);
+                for (int r = 0; r < 3; ++r)
+                {
+                    double fs = fsum(mp[k][r], dmp[k][r]);
+                    if (c0 == 1) fs = fs + press[k][r];
+                    const double d = ROE ? fd[k][r] : fd[k][r]*speed[k];
+                    P[I(F_FLUX+8*D+c0+r, x0, x1, x2)] = 0.5*(fs-d);
+                }
+            }
         }
     }
 }
@@ -645,10 +708,10 @@ void bertaDirectionPass(const O4View& v, dim3 grid, hipStream_t s, const double*
     if (v.roe) hipLaunchKernelGGL((k_b_faces<D, S_, true>), grid, dim3(kB), 0, s, v, scratch);
     else hipLaunchKernelGGL((k_b_faces<D, S_, false>), grid, dim3(kB), 0, s, v, scratch);
     if (v.powell) hipLaunchKernelGGL(k_b_pw_face<D>, grid, dim3(kB), 0, s, v, U, scratch);
-    hipLaunchKernelGGL(k_b_avg<D>, grid, dim3(kB), 0, s, v, scratch);
     if (v.roe) hipLaunchKernelGGL((k_o4_roe<D, S_>), grid, dim3(kB), 0, s, v, scratch);
-    if (v.roe) hipLaunchKernelGGL((k_b_flux<D, true>), grid, dim3(kB), 0, s, v, scratch);
-    else hipLaunchKernelGGL((k_b_flux<D, false>), grid, dim3(kB), 0, s, v, scratch);
+    if (v.nftiles[D] == 0) return;
+    if (v.roe) hipLaunchKernelGGL((k_b_avgflux<D, true>), dim3(v.nftiles[D]), dim3(kFT), 0, s, v, scratch);
+    else hipLaunchKernelGGL((k_b_avgflux<D, false>), dim3(v.nftiles[D]), dim3(kFT), 0, s, v, scratch);
 }
 
 // Cell stage and the three direction passes of the Berta operator.

@@ -81,6 +81,29 @@ struct SeamPair
     double sign;
 };
 
+// Face tiles of the Berta face-average and flux kernel of direction D: T0 x
+// T1 x T2 faces (x0 unit-stride), the transverse halo of the face averages
+// (1; the point fluxes they are formed from need 2), and the LDS regions.
+template <int D>
+struct FluxTile
+{
+    static constexpr int T0 = D == 0 ? 16 : 32, T1 = D == 0 ? 4 : D == 1 ? 2 : 4, T2 = D == 0 ? 4 : D == 1 ? 4 : 2;
+    static constexpr int h0 = D == 0 ? 0 : 1, h1 = D == 1 ? 0 : 1, h2 = D == 2 ? 0 : 1;
+    static constexpr int F0 = T0+4*h0, F1 = T1+4*h1, F2 = T2+4*h2;
+    static constexpr int A0 = T0+2*h0, A1 = T1+2*h1, A2 = T2+2*h2;
+    static constexpr int NF = F0*F1*F2, NA = A0*A1*A2, NM = A1*A2, NT = T0*T1*T2;
+};
+constexpr int kFT = 256;   // threads of a flux tile
+constexpr int kFPT = 1;    // faces per thread
+static_assert(FluxTile<0>::NT == kFT*kFPT && FluxTile<1>::NT == kFT*kFPT && FluxTile<2>::NT == kFT*kFPT,
+              "a flux tile holds kFPT faces per thread");
+
+// One face tile: its box and first face.
+struct FluxTileRef
+{
+    int box, o0, o1, o2;
+};
+
 struct O4View
 {
     int nr, na, naE, limiter;
@@ -99,6 +122,8 @@ struct O4View
     unsigned nblocks;        // blocks of the flattened launch
     const SeamPair* seams;
     int nseams;
+    const FluxTileRef* ftiles[3];   // face tiles of each direction (Berta face-average and flux kernel)
+    unsigned nftiles[3];
 };
 
 // Entry of the angular tables (cellA, faceA, fmet) for cell / face (j, k) of the box's panel and level.
@@ -125,11 +150,12 @@ __device__ __forceinline__ double radialPoint(const O4View& v, const BoxDesc& b,
 // the contiguous logical range starting at start(x); neighbouring tiles, which
 // share stencil data, then run on the same XCD and hit in its L2.
 constexpr unsigned kXcd = 8;
-__device__ __forceinline__ unsigned logicalBlock(const O4View& v)
+__device__ __forceinline__ unsigned xcdOrder(unsigned b, unsigned n)
 {
-    const unsigned b = blockIdx.x, x = b % kXcd, q = v.nblocks / kXcd, r = v.nblocks % kXcd;
+    const unsigned x = b % kXcd, q = n / kXcd, r = n % kXcd;
     return x*q + (x < r ? x : r) + b / kXcd;
 }
+__device__ __forceinline__ unsigned logicalBlock(const O4View& v) { return xcdOrder(blockIdx.x, v.nblocks); }
 __device__ __forceinline__ int blockBoxIndex(const O4View& v)
 {
     const unsigned L = logicalBlock(v);
@@ -1447,6 +1473,26 @@ void buildBoxop(hc_boxop& B, const hc_box* boxes, const int* boxLevel, int nbox)
     for (int d = 0; d < 3; ++d) { B.fCount[d] = f[d]; B.fOff[d] = upload(B.fOffHost[d], B.owned); }
     v.boxes = upload(B.boxes, B.owned);
     v.blockBox = upload(blockBox, B.owned);
+    // face tiles of each direction, box by box, x0 fastest
+    for (int d = 0; d < 3; ++d)
+    {
+        const int T[3] = {d == 0 ? FluxTile<0>::T0 : d == 1 ? FluxTile<1>::T0 : FluxTile<2>::T0,
+                          d == 0 ? FluxTile<0>::T1 : d == 1 ? FluxTile<1>::T1 : FluxTile<2>::T1,
+                          d == 0 ? FluxTile<0>::T2 : d == 1 ? FluxTile<1>::T2 : FluxTile<2>::T2};
+        std::vector<FluxTileRef> tiles;
+        for (int n = 0; n < nbox; ++n)
+        {
+            const BoxDesc& bd = B.boxes[n];
+            const int m0 = bd.n[0]+(d == 0), m1 = bd.n[1]+(d == 1), m2 = bd.n[2]+(d == 2);
+            for (int k = 0; k < m2; k += T[2])
+                for (int j = 0; j < m1; j += T[1])
+                    for (int i = 0; i < m0; i += T[0])
+                        tiles.push_back({n, bd.lo[0]+i, bd.lo[1]+j, bd.lo[2]+k});
+        }
+        if (tiles.size() > 0x7fffffffULL) throw std::runtime_error("box table too large for one launch");
+        v.nftiles[d] = (unsigned)tiles.size();
+        v.ftiles[d] = upload(tiles, B.owned);
+    }
     v.seams = nullptr;
     v.nseams = 0;
     HC_THROW(hipMalloc(&B.scratch, std::max<long long>(1, sc)*sizeof(double)));
