@@ -87,7 +87,7 @@ struct SeamPair
 template <int D>
 struct FluxTile
 {
-    static constexpr int T0 = D == 0 ? 16 : 32, T1 = D == 0 ? 4 : D == 1 ? 2 : 4, T2 = D == 0 ? 4 : D == 1 ? 4 : 2;
+    static constexpr int T0 = D == 0 ? 8 : 32, T1 = D == 0 ? 8 : D == 1 ? 2 : 4, T2 = D == 0 ? 4 : D == 1 ? 4 : 2;
     static constexpr int h0 = D == 0 ? 0 : 1, h1 = D == 1 ? 0 : 1, h2 = D == 2 ? 0 : 1;
     static constexpr int F0 = T0+4*h0, F1 = T1+4*h1, F2 = T2+4*h2;
     static constexpr int A0 = T0+2*h0, A1 = T1+2*h1, A2 = T2+2*h2;
@@ -311,18 +311,16 @@ __device__ double etaTilde(const double* pb2, const double* pb, const double* pa
     return si;
 }
 
-// Mapped and factored Roe dissipation at a face point, src/MHD_Roe8Wave.H:634-740.
-template <typename S>
-__device__ void factoredRoe(double out[8], const double wl[8], const double wr[8], const double* Am,
-                            double radialArea, int dir, double gamma, double efix)
+// The face basis A and the orthonormal Roe frame of direction dir (rows: the
+// unit normal, the unit first tangent, their cross product) with the
+// normal's length, src/MHD_Roe8Wave.H:638-661.
+__device__ __forceinline__ void roeFrame(const double* Am, int dir, double A[3][3], double frame[3][3], double& nm)
 {
-    double A[3][3];
 #pragma unroll
     for (int a = 0; a < 3; ++a)
 #pragma unroll
         for (int b = 0; b < 3; ++b) A[a][b] = Am[3*a+b];
     const int t1 = (dir+1)%3, t2 = (dir+2)%3;
-    double frame[3][3];
     double normal2 = 0, tangent2 = 0;
 #pragma unroll
     for (int i = 0; i < 3; ++i)
@@ -333,15 +331,27 @@ __device__ void factoredRoe(double out[8], const double wl[8], const double wr[8
         normal2 += frame[0][i]*frame[0][i];
         tangent2 += frame[1][i]*frame[1][i];
     }
-    // the frame (and with it the face-frame states the Roe solver decides on)
-    // is the same in both precisions; only the back-transform below takes
-    // reciprocal products in mixed precision
-    constexpr bool FAST = !std::is_same<S, double>::value;
-    const double nm = sqrt(normal2), tm = sqrt(tangent2);
+    nm = sqrt(normal2);
+    const double tm = sqrt(tangent2);
 #pragma unroll
     for (int i = 0; i < 3; ++i) { frame[0][i] /= nm; frame[1][i] /= tm; }
 #pragma unroll
     for (int i = 0; i < 3; ++i) { const int j = (i+1)%3, k = (i+2)%3; frame[2][i] = frame[0][j]*frame[1][k]-frame[0][k]*frame[1][j]; }
+}
+
+// Mapped and factored Roe dissipation at a face point, src/MHD_Roe8Wave.H:634-740.
+// The basis and the frame are formed again after the Riemann solver (the same
+// operations, so the same values) instead of being held in registers across it.
+template <typename S>
+__device__ void factoredRoe(double out[8], const double wl[8], const double wr[8], const double* Am,
+                            double radialArea, int dir, double gamma, double efix)
+{
+    double A[3][3], frame[3][3], nm;
+    roeFrame(Am, dir, A, frame, nm);
+    // the frame (and with it the face-frame states the Roe solver decides on)
+    // is the same in both precisions; only the back-transform below takes
+    // reciprocal products in mixed precision
+    constexpr bool FAST = !std::is_same<S, double>::value;
     double loc[2][8];
 #pragma unroll
     for (int s = 0; s < 2; ++s)
@@ -366,6 +376,10 @@ __device__ void factoredRoe(double out[8], const double wl[8], const double wr[8
     double diss[8];
     if (!hc::roe8WaveDissipation<S>(diss, loc[0], loc[1], gamma, efix))
         hc::rusanovDissipation<double>(diss, loc[0], loc[1], gamma);
+    // the basis is read again and the frame recomputed: nothing of the
+    // frame stays live across the solver
+    asm volatile("" ::: "memory");
+    roeFrame(Am, dir, A, frame, nm);
     const double area = radialArea*nm;
     double mapped[8];
     mapped[0] = area*diss[0]; mapped[4] = area*diss[4];

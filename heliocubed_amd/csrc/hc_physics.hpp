@@ -8,6 +8,13 @@
 #include <type_traits>
 
 #define HC_HD __host__ __device__ __forceinline__
+// Scheduling fence of device code: instructions are not moved across it, which
+// bounds the live registers of long unrolled sequences (no-op on the host).
+#if defined(__HIP_DEVICE_COMPILE__)
+#define HC_SCHED_FENCE() __builtin_amdgcn_sched_barrier(0)
+#else
+#define HC_SCHED_FENCE() ((void)0)
+#endif
 
 // Plasma beta below which mixed precision evaluates the Roe eigen-system in FP64.
 #ifndef HC_MIXED_FP64_BETA
@@ -346,6 +353,7 @@ HC_HD bool roe8WaveDissipation(double out[8], const double WL[8], const double W
                     -kInv8Pi*(st[5]*st[5]+st[6]*st[6]+st[7]*st[7]);
                 if (!(th >= 0.0) || !isfinite(th)) return false;
             }
+            HC_SCHED_FENCE();
         }
     }
     S lu[8];
