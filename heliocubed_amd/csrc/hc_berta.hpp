@@ -64,14 +64,25 @@ __device__ double wenoNormalize(double (&v)[N])
     return scale;
 }
 
+// Quotient a / b of the WENO weights: FP64 division, or in FP32 (mixed
+// precision) the product with the hardware reciprocal, whose ulp-level error
+// is far below the FP32 rounding already accepted in the weights.
+__device__ __forceinline__ double wenoQuot(double a, double b) { return a/b; }
+__device__ __forceinline__ float wenoQuot(float a, float b) { return a*__builtin_amdgcn_rcpf(b); }
+
 // One side of f_wenoz_pointwise_faceF (:782-878) on normalized values. The
 // candidate values q_k and the undivided differences are formed in FP64; the
 // smoothness indicators and nonlinear weights in S (S = double reproduces
-// the reference operation for operation); the weighted sum in FP64.
+// the reference operation for operation); the weighted sum in FP64. The
+// reference's epsilon 1e-40 lies below the FP32 normal range: in FP32 it is
+// 1e-30, which keeps every quotient finite (a zero indicator next to a jump
+// gives a weight of at most ~1e30, as the FP64 form gives ~1e40) and changes
+// no weight of normalized smooth data, whose indicators are far above it.
 template <typename S>
 __device__ __forceinline__ double wenozSide(double m2, double m1, double c, double p1, double p2)
 {
-    const S eps = S(1.0e-40), d0 = S(1.0/16.0), d1 = S(5.0/8.0), d2 = S(5.0/16.0);
+    constexpr bool F32 = std::is_same<S, float>::value;
+    const S eps = F32 ? S(1.0e-30) : S(1.0e-40), d0 = S(1.0/16.0), d1 = S(5.0/8.0), d2 = S(5.0/16.0);
     const double q0 = (3.0*m2 - 10.0*m1 + 15.0*c)/8.0;
     const double q1 = (-m1 + 6.0*c + 3.0*p1)/8.0;
     const double q2 = (3.0*c + 6.0*p1 - p2)/8.0;
@@ -82,9 +93,9 @@ __device__ __forceinline__ double wenozSide(double m2, double m1, double c, doub
     const S beta1 = (S(13.0)/S(12.0))*b1a*b1a + S(0.25)*b1b*b1b;
     const S beta2 = (S(13.0)/S(12.0))*b2a*b2a + S(0.25)*b2b*b2b;
     const S tau5 = fabs(beta0 - beta2);
-    const double a0 = double(d0*(S(1.0) + tau5/(beta0 + eps)));
-    const double a1 = double(d1*(S(1.0) + tau5/(beta1 + eps)));
-    const double a2 = double(d2*(S(1.0) + tau5/(beta2 + eps)));
+    const double a0 = double(d0*(S(1.0) + wenoQuot(tau5, beta0 + eps)));
+    const double a1 = double(d1*(S(1.0) + wenoQuot(tau5, beta1 + eps)));
+    const double a2 = double(d2*(S(1.0) + wenoQuot(tau5, beta2 + eps)));
     const double asum = a0 + a1 + a2;
     return (a0*q0 + a1*q1 + a2*q2)/asum;
 }
@@ -289,12 +300,15 @@ __global__ void __launch_bounds__(kB) k_b_faces(O4View v, double* __restrict__ S
             thF = 0.0;
         }
     }
+    // mixed precision: constant divisions of the point energies and fluxes as
+    // reciprocal products (within an ulp)
+    constexpr bool FAST = !std::is_same<S_, double>::value;
     const double* A = v.faceA[D] + 9*angIdx(b, x[1], x[2]);
     double keL, meL, vbL, keR, meR, vbR, fl[8], fr[8];
-    energiesNative(A, l, keL, meL, vbL);
-    energiesNative(A, r, keR, meR, vbR);
-    primFlux(fl, l, keL, meL, vbL, v.gamma, D);
-    primFlux(fr, r, keR, meR, vbR, v.gamma, D);
+    energiesNative<FAST>(A, l, keL, meL, vbL);
+    energiesNative<FAST>(A, r, keR, meR, vbR);
+    primFlux<FAST>(fl, l, keL, meL, vbL, v.gamma, D);
+    primFlux<FAST>(fr, r, keR, meR, vbR, v.gamma, D);
 #pragma unroll
     for (int c = 0; c < 8; ++c)
     {

@@ -242,7 +242,8 @@ __device__ void primFlux(double f[8], const double w[8], double ke, double me, d
     f[d+1] = f[0]*vn - q4pi(bn*bn);
     f[t1+1] = f[0]*vt1 - q4pi(bn*bt1);
     f[t2+1] = f[0]*vt2 - q4pi(bn*bt2);
-    f[4] = (2.0*me + ke + w[4]*gamma/(gamma-1.0))*vn - q4pi(bn*vb);
+    const double enth = FAST ? w[4]*gamma*(1.0/(gamma-1.0)) : w[4]*gamma/(gamma-1.0);
+    f[4] = (2.0*me + ke + enth)*vn - q4pi(bn*vb);
     f[5] = vn*w[5] - w[1]*bn;
     f[6] = vn*w[6] - w[2]*bn;
     f[7] = vn*w[7] - w[3]*bn;
