@@ -229,7 +229,7 @@ __global__ void __launch_bounds__(kB) k_b_faces(O4View v, double* __restrict__ S
     for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]-2; H[d] = b.hi[d]+2; }
     // the Powell face products also take the faces lo_d-1 and hi_d+2
     L[D] = b.lo[D] - (v.powell ? 1 : 0); H[D] = b.hi[D] + (v.powell ? 2 : 1);
-    if (!regionPoint(b, L, H, boxThread(v, b), x)) return;
+    if (!(D == 2 ? regionPointX2(b, L, H, boxThread(v, b), x) : regionPoint(b, L, H, boxThread(v, b), x))) return;
     const BoxIdx I(b);
     double* const P = S + b.sOff;
     auto at = [&](int f, int o) {

@@ -196,6 +196,20 @@ __device__ __forceinline__ bool regionPoint(const BoxDesc& b, const int L[3], co
     return true;
 }
 
+// Region iteration with x2 before x1: consecutive blocks step through the x2
+// planes of an x0 row, so a stencil along x2 is read by blocks that run
+// together on one XCD (its L2 does not hold the planes of a whole box).
+__device__ __forceinline__ bool regionPointX2(const BoxDesc& b, const int L[3], const int H[3], unsigned t, int x[3])
+{
+    const unsigned n0 = H[0]-L[0]+1, n1 = H[1]-L[1]+1, n2 = H[2]-L[2]+1;
+    if (t >= n0*n1*n2) return false;
+    const unsigned q = t / n0;
+    x[0] = L[0] + (int)(t - q*n0);
+    x[2] = L[2] + (int)(q % n2);
+    x[1] = L[1] + (int)(q / n2);
+    return true;
+}
+
 // FAST (mixed precision only): constant divisions as products with the
 // reciprocal (within an ulp; the FP64 form keeps the divisions of the CPU
 // restatement).
@@ -1488,7 +1502,9 @@ void buildBoxop(hc_boxop& B, const hc_box* boxes, const int* boxLevel, int nbox)
     for (int d = 0; d < 3; ++d) { B.fCount[d] = f[d]; B.fOff[d] = upload(B.fOffHost[d], B.owned); }
     v.boxes = upload(B.boxes, B.owned);
     v.blockBox = upload(blockBox, B.owned);
-    // face tiles of each direction, box by box, x0 fastest
+    // face tiles of each direction, box by box, x0 fastest; then x2 for the
+    // r and alpha directions (whose stencils reach two planes across x2
+    // tiles of 4), x1 for beta
     for (int d = 0; d < 3; ++d)
     {
         const int T[3] = {d == 0 ? FluxTile<0>::T0 : d == 1 ? FluxTile<1>::T0 : FluxTile<2>::T0,
@@ -1499,10 +1515,20 @@ void buildBoxop(hc_boxop& B, const hc_box* boxes, const int* boxLevel, int nbox)
         {
             const BoxDesc& bd = B.boxes[n];
             const int m0 = bd.n[0]+(d == 0), m1 = bd.n[1]+(d == 1), m2 = bd.n[2]+(d == 2);
-            for (int k = 0; k < m2; k += T[2])
+            if (d == 2)
+            {
+                for (int k = 0; k < m2; k += T[2])
+                    for (int j = 0; j < m1; j += T[1])
+                        for (int i = 0; i < m0; i += T[0])
+                            tiles.push_back({n, bd.lo[0]+i, bd.lo[1]+j, bd.lo[2]+k});
+            }
+            else
+            {
                 for (int j = 0; j < m1; j += T[1])
-                    for (int i = 0; i < m0; i += T[0])
-                        tiles.push_back({n, bd.lo[0]+i, bd.lo[1]+j, bd.lo[2]+k});
+                    for (int k = 0; k < m2; k += T[2])
+                        for (int i = 0; i < m0; i += T[0])
+                            tiles.push_back({n, bd.lo[0]+i, bd.lo[1]+j, bd.lo[2]+k});
+            }
         }
         if (tiles.size() > 0x7fffffffULL) throw std::runtime_error("box table too large for one launch");
         v.nftiles[d] = (unsigned)tiles.size();
