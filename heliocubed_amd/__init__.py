@@ -99,6 +99,7 @@ _SIGNATURES = [
     ("hc_boxop_destroy", None, [_P]),
     ("hc_boxop_count", ctypes.c_longlong, [_P, _I]),
     ("hc_boxop_apply", _I, [_P, _P, _P, _P, _P, _P, _P]),
+    ("hc_boxop_apply_scaled", _I, [_P, _P, _P, _P, _P, _P, ctypes.c_double, _P]),
     ("hc_init_native_boxes", _I, [ctypes.POINTER(HcShell), ctypes.POINTER(HcOptions), _I, ctypes.c_ulonglong,
                                   ctypes.POINTER(HcBox), _I, _I, _P]),
     ("hc_init_level4", _I, [ctypes.POINTER(HcShell), ctypes.POINTER(HcOptions), _I, ctypes.c_ulonglong, _I, _P, _P]),
@@ -691,7 +692,9 @@ class BoxOp:
     def count(self, what: int) -> int:
         return lib().hc_boxop_count(self._handle, what)
 
-    def apply(self, USph, rhs, fluxes=None):
+    def apply(self, USph, rhs, fluxes=None, scale=1.0):
+        """rhs += scale * (mapped flux divergence) of every box; face fluxes
+        into `fluxes` (three tensors or None entries)."""
         import torch
 
         f = list(fluxes) if fluxes is not None else [None, None, None]
@@ -703,4 +706,5 @@ class BoxOp:
                     or t.numel() != self.count(what):
                 raise ValueError(f"{name}: need a contiguous float64 device tensor of {self.count(what)} elements")
         stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
-        _check(lib().hc_boxop_apply(self._handle, _ptr(USph), _ptr(rhs), _ptr(f[0]), _ptr(f[1]), _ptr(f[2]), stream))
+        _check(lib().hc_boxop_apply_scaled(self._handle, _ptr(USph), _ptr(rhs), _ptr(f[0]), _ptr(f[1]), _ptr(f[2]),
+                                           float(scale), stream))

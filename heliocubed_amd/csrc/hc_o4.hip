@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <map>
 #include <memory>
 #include <stdexcept>
@@ -1318,7 +1319,7 @@ __global__ void __launch_bounds__(kB) k_o4_pw_cell(O4View v, double* __restrict_
 // source on, the cell product <(-J div B) C_c> = sum_d D_d(-J div B) D_d C_c / 12
 // + (-J div B) C_c is added to components 1..7 after the divergence.
 __global__ void __launch_bounds__(kB) k_o4_div(O4View v, const double* __restrict__ S, double* __restrict__ R,
-                                               int overwrite, double* __restrict__ Uacc, double b_dt)
+                                               int overwrite, double* __restrict__ Uacc, double b_dt, double scale)
 {
     const BoxDesc& b = blockBoxDesc(v);
     int x[3];
@@ -1332,7 +1333,7 @@ __global__ void __launch_bounds__(kB) k_o4_div(O4View v, const double* __restric
         double acc = overwrite ? 0.0 : R[r + n];
         for (int d = 0; d < 3; ++d)
         {
-            const double dxInv = -1.0/(d == 0 ? 1.0/b.nr : 1.0/b.na);
+            const double dxInv = -scale/(d == 0 ? 1.0/b.nr : 1.0/b.na);
             int a[3] = {x[0], x[1], x[2]}; a[d] += 1;
             double inc = 0;
             inc += (-1.0*dxInv)*P[I(F_FLUX+8*d+c, x[0], x[1], x[2])];
@@ -1556,7 +1557,7 @@ void directionPass(const O4View& v, dim3 grid, hipStream_t s, double* scratch)
 
 template <typename S_>
 void applyBoxop(hc_boxop& B, const double* U, double* R, double* const F[3], hipStream_t s,
-                int overwrite = 0, double* Uacc = nullptr, double b_dt = 0.0)
+                int overwrite = 0, double* Uacc = nullptr, double b_dt = 0.0, double scale = 1.0)
 {
     const O4View& v = B.view;
     const dim3 grid((unsigned)B.totalBlocks);
@@ -1578,7 +1579,7 @@ void applyBoxop(hc_boxop& B, const double* U, double* R, double* const F[3], hip
         else hipLaunchKernelGGL(k_o4_pw_face, grid, dim3(kB), 0, s, v, B.scratch);
         hipLaunchKernelGGL(k_o4_pw_cell, grid, dim3(kB), 0, s, v, B.scratch, v.op == 2 ? (int)F_GBB : (int)F_GB);
     }
-    hipLaunchKernelGGL(k_o4_div, grid, dim3(kB), 0, s, v, B.scratch, R, overwrite, Uacc, b_dt);
+    hipLaunchKernelGGL(k_o4_div, grid, dim3(kB), 0, s, v, B.scratch, R, overwrite, Uacc, b_dt, scale);
     for (int d = 0; d < 3; ++d)
         if (F[d]) hipLaunchKernelGGL(k_o4_fluxout, grid, dim3(kB), 0, s, v, d, B.scratch, F[d], B.fOff[d]);
     HC_THROW(hipGetLastError());
@@ -2231,18 +2232,26 @@ long long hc_boxop_count(const hc_boxop* op, int what)
     }
 }
 
-int hc_boxop_apply(hc_boxop* op, const double* USph, double* rhs, double* F0, double* F1, double* F2,
-                   hc_stream stream)
+int hc_boxop_apply_scaled(hc_boxop* op, const double* USph, double* rhs, double* F0, double* F1, double* F2,
+                          double scale, hc_stream stream)
 {
     if (!op || !USph || !rhs) return hc::fail("hc_boxop_apply: null argument");
+    if (!std::isfinite(scale)) return hc::fail("hc_boxop_apply: the divergence scale must be finite");
     try
     {
         double* const F[3] = {F0, F1, F2};
-        if (op->opt.precision == 1) applyBoxop<float>(*op, USph, rhs, F, static_cast<hipStream_t>(stream));
-        else applyBoxop<double>(*op, USph, rhs, F, static_cast<hipStream_t>(stream));
+        const hipStream_t s = static_cast<hipStream_t>(stream);
+        if (op->opt.precision == 1) applyBoxop<float>(*op, USph, rhs, F, s, 0, nullptr, 0.0, scale);
+        else applyBoxop<double>(*op, USph, rhs, F, s, 0, nullptr, 0.0, scale);
         return 0;
     }
     catch (const std::exception& e) { return hc::fail(std::string("hc_boxop_apply: ") + e.what()); }
+}
+
+int hc_boxop_apply(hc_boxop* op, const double* USph, double* rhs, double* F0, double* F1, double* F2,
+                   hc_stream stream)
+{
+    return hc_boxop_apply_scaled(op, USph, rhs, F0, F1, F2, 1.0, stream);
 }
 
 } // extern "C"

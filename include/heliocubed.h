@@ -285,6 +285,12 @@ void hc_boxop_destroy(hc_boxop* op);
 long long hc_boxop_count(const hc_boxop* op, int what);
 int hc_boxop_apply(hc_boxop* op, const double* USph, double* rhs, double* F0, double* F1, double* F2,
                    hc_stream stream);
+/* hc_boxop_apply with the divergence scaled: rhs += scale * (-div F), the
+ * a_scale argument of BoxOp_EulerCubedSphere::operator() (dxInv =
+ * -a_scale/dx, src/BoxOp_EulerCubedSphereImplem.H:355); the Powell source is
+ * not scaled, as there (:386-390). hc_boxop_apply is scale 1. */
+int hc_boxop_apply_scaled(hc_boxop* op, const double* USph, double* rhs, double* F0, double* F1, double* F2,
+                          double scale, hc_stream stream);
 
 /* Fixed AMR hierarchy of fourth-order levels advanced by the synchronous
  * classical RK4 (advanceRungeKuttaSynchronous, src/MHD_AMRRungeKutta.H:5-41):
