@@ -1020,7 +1020,7 @@ int hc_amr4_create(const hc_shell* shells, int nlevels, const hc_options* option
     if (options->spatial_order != 4) return hc::fail("hc_amr4_create: spatial_order must be 4");
     if (options->riemann_solver != 1 && options->riemann_solver != 2)
         return hc::fail("hc_amr4_create: riemann_solver must be 1 (Rusanov) or 2 (Roe)");
-    if (options->limiter != 0 && options->limiter != 1) return hc::fail("hc_amr4_create: limiter must be 0 or 1");
+    if (options->limiter < 0 || options->limiter > 2) return hc::fail("hc_amr4_create: limiter must be 0, 1 or 2");
     if (options->precision != 0 && options->precision != 1) return hc::fail("hc_amr4_create: precision must be 0 or 1");
     if (const char* e = hc::checkOperatorOptions(*options)) return hc::fail(std::string("hc_amr4_create: ") + e);
     for (int l = 0; l < nlevels; ++l)

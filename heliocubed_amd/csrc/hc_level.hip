@@ -1051,8 +1051,8 @@ int hc_level_create(const hc_shell* shell, const hc_options* options, hc_level**
     if (options->riemann_solver != 1 && options->riemann_solver != 2)
         return fail("hc_level_create: riemann_solver must be 1 (Rusanov) or 2 (Roe)");
     if (options->limiter < 0 || options->limiter > 3) return fail("hc_level_create: limiter must be 0..3");
-    if (options->spatial_order == 4 && options->limiter > 1)
-        return fail("hc_level_create: the fourth-order operator takes limiter 0 or 1");
+    if (options->spatial_order == 4 && options->limiter > 2)
+        return fail("hc_level_create: the fourth-order operator takes limiter 0, 1 or 2");
     if (options->spatial_order == 4 && (shell->n_ang < 16 || shell->n_rad < 4))
         return fail("hc_level_create: the fourth-order level needs n_ang >= 16 (five ghost layers must stay "
                     "inside the panel's horizon) and n_rad >= 4");

@@ -117,7 +117,7 @@ struct O4View
     const double* cellA;     // [6][naE][naE][9]
     const double* faceA[3];  // [6][naE][naE][9]
     const double* fmet[3];   // [6][naE][naE][13]: detA, det(A) A, cofactor row
-    const double* radial;    // per level [nr+13][2]: adjDr_0 (area0), adjDr_t (areaT) for i = -6 ..
+    const double* radial;    // per level [nr+13][4]: adjDr_0 (area0), adjDr_t (areaT), dVolr, Dr_0 dxi_r for i = -6 ..
     const int* blockBox;     // box of each block of the flattened launch
     int blocksPerBox;        // > 0 when every box owns the same number of blocks (no table lookup)
     unsigned nblocks;        // blocks of the flattened launch
@@ -135,7 +135,7 @@ __device__ __forceinline__ long long angIdx(const BoxDesc& b, int j, int k)
 
 __device__ __forceinline__ double areaOf(const O4View& v, const BoxDesc& b, int d, int i)
 {
-    return v.radial[b.radOff + 2*(i+6) + (d == 0 ? 0 : 1)];
+    return v.radial[b.radOff + 4*(i+6) + (d == 0 ? 0 : 1)];
 }
 
 __device__ __forceinline__ double radialPoint(const O4View& v, const BoxDesc& b, int d, int i)
@@ -592,11 +592,74 @@ __global__ void __launch_bounds__(kB) k_o4_eta(O4View v, double* __restrict__ S)
 }
 
 // Face-average state WFaceBar at face x (direction d): (WBar_{x-e} + WBar_x)/2.
-__device__ __forceinline__ void faceBar(const BoxIdx& I, const double* P, int d, int i, int j, int k, double wf[8])
+__device__ __forceinline__ void faceBarAvg(const BoxIdx& I, const double* P, int d, int i, int j, int k, double wf[8])
 {
     const int a = i-(d == 0), b = j-(d == 1), c = k-(d == 2);
 #pragma unroll
     for (int q = 0; q < 8; ++q) wf[q] = 0.5*P[I(F_WB+q, a, b, c)] + 0.5*P[I(F_WB+q, i, j, k)];
+}
+
+// Width of cell y along D for the minmod limiter (limiter 2,
+// src/BoxOp_EulerCubedSphereImplem.H:220-240): Dr_0 dxi_r radially; along an
+// angular direction dVolr det A dxi_perp / adjDr_t / |adjA| with the face
+// metric of index y (the reference evaluates it on the face box).
+template <int D>
+__device__ __forceinline__ double minmodWidth(const O4View& v, const BoxDesc& b, int y0, int y1, int y2)
+{
+    const double* rr = v.radial + b.radOff + 4*(y0+6);
+    if (D == 0) return rr[3];
+    const double* m = v.fmet[D] + 13*angIdx(b, y1, y2);
+    const double adjA = sqrt(m[10]*m[10] + m[11]*m[11] + m[12]*m[12]);
+    return rr[2]*m[0]*(1.0/b.na)/rr[1]/adjA;
+}
+
+// Minmod face states of face (i,j,k) of direction D, Limiters_minmod /
+// minmod_calc (src/BoxOp_Limiters.H:360-446): slope dx minmod(dW+ / (dx +
+// dx_ahead), dW- / (dx + dx_behind)) of the point values W, left state from
+// the cell below the face (W + slope), right state from the cell above
+// (W - slope), pressures clipped at 0.
+template <int D>
+__device__ void minmodFaceStates(const O4View& v, const BoxDesc& b, const BoxIdx& I, const double* P, int i, int j, int k,
+                                 double l[8], double r[8])
+{
+    double dx[4];
+#pragma unroll
+    for (int o = -2; o <= 1; ++o)
+        dx[o+2] = minmodWidth<D>(v, b, i+(D == 0)*o, j+(D == 1)*o, k+(D == 2)*o);
+    auto Wv = [&](int c, int o) { return P[I(F_W+c, i+(D == 0)*o, j+(D == 1)*o, k+(D == 2)*o)]; };
+    // slope of the cell at offset m (-1: below the face, 0: above)
+    auto slope = [&](int c, int m, double wm) {
+        const double c1 = 1.0/(dx[m+2] + dx[m+3]), c2 = 1.0/(dx[m+2] + dx[m+1]);
+        const double dWR = Wv(c, m+1) - wm, dWL = wm - Wv(c, m-1);
+        const double A = dWR*c1, B = dWL*c2;
+        double mm;
+        if (A*B < 0) mm = 0;
+        else if (fabs(A) < fabs(B)) mm = A;
+        else mm = B;
+        return dx[m+2]*mm; };
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+    {
+        const double wl = Wv(c, -1), wr = Wv(c, 0);
+        l[c] = wl + slope(c, -1, wl);
+        r[c] = wr - slope(c, 0, wr);
+    }
+    if (l[4] < 0) l[4] = 0.;
+    if (r[4] < 0) r[4] = 0.;
+}
+
+// Face-average state WFaceBar of face (i,j,k): the second-order average of the
+// two cell averages, or with limiter 2 the mean of the two minmod states
+// (src/BoxOp_EulerCubedSphereImplem.H:242-245).
+template <int D>
+__device__ __forceinline__ void faceBar(const O4View& v, const BoxDesc& b, const BoxIdx& I, const double* P,
+                                        int i, int j, int k, double wf[8])
+{
+    if (v.limiter != 2) { faceBarAvg(I, P, D, i, j, k, wf); return; }
+    double l[8], r[8];
+    minmodFaceStates<D>(v, b, I, P, i, j, k, l, r);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) wf[q] = (l[q] + r[q])*0.5;
 }
 
 // Transverse extent of the face states and point fluxes of direction D: the
@@ -635,8 +698,9 @@ __global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(2))) k_
         int y[3] = {x[0], x[1], x[2]}; y[D] += o;
         return P[I(F_W+c, y[0], y[1], y[2])]; };
     double lf[8], rf[8];
+    if (v.limiter == 2) minmodFaceStates<D>(v, b, I, P, x[0], x[1], x[2], lf, rf);
 #pragma unroll 1
-    for (int c = 0; c < 8; ++c)
+    for (int c = 0; c < 8 && v.limiter != 2; ++c)
     {
         double w[8];   // W at offsets -4..+3 from the face's right cell
 #pragma unroll
@@ -670,7 +734,7 @@ __global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(2))) k_
     double corr[8], lapb[8];
     {
         double wfx[8], a[8], bq[8];
-        faceBar(I, P, D, x[0], x[1], x[2], wfx);
+        faceBar<D>(v, b, I, P, x[0], x[1], x[2], wfx);
 #pragma unroll
         for (int q = 0; q < 8; ++q) lapb[q] = 0.0;
         for (int t = 0; t < 2; ++t)
@@ -678,8 +742,8 @@ __global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(2))) k_
             const int tt = t == 0 ? t1 : t2;
             int y[3] = {x[0], x[1], x[2]}, z[3] = {x[0], x[1], x[2]};
             y[tt] += 1; z[tt] -= 1;
-            faceBar(I, P, D, y[0], y[1], y[2], a);
-            faceBar(I, P, D, z[0], z[1], z[2], bq);
+            faceBar<D>(v, b, I, P, y[0], y[1], y[2], a);
+            faceBar<D>(v, b, I, P, z[0], z[1], z[2], bq);
 #pragma unroll
             for (int q = 0; q < 8; ++q) lapb[q] += a[q] - 2.0*wfx[q] + bq[q];
         }
@@ -773,7 +837,7 @@ __device__ void barFlux(const O4View& v, const BoxDesc& b, const BoxIdx& I, cons
                         int i, int j, int k, double f[8], double& meBar, double& pBar)
 {
     double wf[8];
-    faceBar(I, P, D, i, j, k, wf);
+    faceBar<D>(v, b, I, P, i, j, k, wf);
     double ke, me, vb;
     energiesNative(v.faceA[D] + 9*angIdx(b, j, k), wf, ke, me, vb);
     primFlux(f, wf, ke, me, vb, v.gamma, D);
@@ -1453,7 +1517,9 @@ void buildBoxop(hc_boxop& B, const hc_box* boxes, const int* boxLevel, int nbox)
         for (int i = -6; i <= nr+6; ++i)
         {
             const hc::RadialCell c = hc::radialCell(L, i);
+            const double dxi = 1.0/nr;
             radial.push_back(c.area0); radial.push_back(c.areaT);
+            radial.push_back(c.dvol); radial.push_back((c.rhi - c.rlo)/dxi*dxi);
         }
     }
     v.cellA = upload(cellA, B.owned);
@@ -2179,7 +2245,7 @@ int hc_boxop_create_levels(const hc_shell* shells, int nlevels, const hc_options
     if (options->spatial_order != 4) return hc::fail("hc_boxop_create: spatial_order must be 4");
     if (options->riemann_solver != 1 && options->riemann_solver != 2)
         return hc::fail("hc_boxop_create: riemann_solver must be 1 (Rusanov) or 2 (Roe)");
-    if (options->limiter != 0 && options->limiter != 1) return hc::fail("hc_boxop_create: limiter must be 0 or 1");
+    if (options->limiter < 0 || options->limiter > 2) return hc::fail("hc_boxop_create: limiter must be 0, 1 or 2");
     if (options->precision != 0 && options->precision != 1) return hc::fail("hc_boxop_create: precision must be 0 or 1");
     if (const char* e = hc::checkOperatorOptions(*options)) return hc::fail(std::string("hc_boxop_create: ") + e);
     for (int l = 0; l < nlevels; ++l)

@@ -50,7 +50,9 @@ typedef struct hc_shell
  * (src/BoxOp_SecondOrder.H:155-370; limiter 0 none, 1 or 2 minmod, 3 MC;
  * riemann_solver 1 Rusanov, 2 Roe 8-wave); spatial_order 4 the standard
  * fourth-order operator (src/BoxOp_EulerCubedSphereImplem.H:97-402; limiter
- * 0 none or 1 Limiters_4O with flattening; riemann_solver 2) with
+ * 0 none, 1 Limiters_4O with flattening, 2 Limiters_minmod of the point
+ * values with metric cell widths (src/BoxOp_Limiters.H:360-446,
+ * src/BoxOp_EulerCubedSphereImplem.H:220-245); riemann_solver 2) with
  * state_conversion 1 (src/MHD_StateConversion.H:153-316); spatial_operator 2
  * replaces the standard operator by the Berta WENOZ operator
  * (src/BoxOp_BertaCubedSphereImplem.H:1737-2076: Jameson detector with
@@ -252,7 +254,7 @@ int hc_level_valid_state(hc_level* level, const double* U, int* valid, long long
                          hc_stream stream);
 
 /* Fourth-order box operator (spatial_order 4; the standard operator with
- * limiter 0 or 1, or the Berta WENOZ operator, spatial_operator 2; Roe 8-wave,
+ * limiter 0, 1 or 2, or the Berta WENOZ operator, spatial_operator 2; Roe 8-wave,
  * riemann_solver 2, or Rusanov, 1; optional Powell source) over a table of
  * boxes, evaluated for every box in the same launches. Replaces BoxOp_EulerCubedSphere::operator()
  * (src/BoxOp_EulerCubedSphere.H:412-421, src/BoxOp_EulerCubedSphereImplem.H:4-402)

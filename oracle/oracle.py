@@ -70,6 +70,8 @@ def lib() -> ctypes.CDLL:
             "orc_o4_rhs_table": (I, [cfgp, I, P, P, P]),
             "orc_limit_line": (None, [P, P]),
             "orc_eta_tilde": (D, [P, P]),
+            "orc_minmod_cell": (None, [D, D, D, D, D, D, P]),
+            "orc_minmod_width": (D, [cfgp, I, I, I, I, I]),
             "orc_prim_flux": (None, [P, P, D, I, P, P]),
             "orc_factored_roe": (None, [P, P, P, P, D, I, D, D]),
             "orc_face_average": (None, [cfgp, I, I, I, I, I, P]),
@@ -422,6 +424,19 @@ def limit_line(w7):
     out, w = np.zeros(2), np.ascontiguousarray(w7, dtype=np.float64)
     lib().orc_limit_line(_p(w), _p(out))
     return out[0], out[1]
+
+
+def minmod_cell(wb, w, wa, dxb, dx, dxa):
+    """(W + slope, W - slope) of Limiters_minmod's minmod_calc for one component."""
+    out = np.zeros(2)
+    lib().orc_minmod_cell(wb, w, wa, dxb, dx, dxa, _p(out))
+    return out[0], out[1]
+
+
+def minmod_width(n_rad, n_ang, p, d, i, j, k, r0=21.5, r1=215.0, stretch=0.0):
+    """Cell width along d of the minmod limiter at index (i, j, k) of panel p."""
+    cfg = OrcCfg(n_rad, n_ang, 5, r0, r1, stretch, 5.0 / 3.0, 2, 2, 0.3, 1, 0.02, 0, 0)
+    return lib().orc_minmod_width(ctypes.byref(cfg), p, d, i, j, k)
 
 
 def eta_tilde(p5, v5):

@@ -21,6 +21,9 @@ CASES = [
     (32, 16, [(3, (8, 0, 0), (15, 7, 7)), (0, (16, 4, 8), (23, 11, 15))], 1, 3),
     (32, 16, [(1, (12, 0, 4), (19, 7, 11))], 0, 1),
     (32, 16, [(p, (8, 0, 8), (15, 7, 15)) for p in range(6)], 1, 0),
+    # limiter 2 (Limiters_minmod with the metric cell widths), boxes at panel edges
+    (32, 16, [(3, (8, 0, 0), (15, 7, 7)), (0, (4, 4, 8), (11, 11, 15)), (4, (20, 8, 2), (27, 15, 9))], 2, 3),
+    (32, 16, [(2, (4, 4, 4), (11, 11, 11))], 2, 1),
 ]
 
 
@@ -69,6 +72,7 @@ RUSANOV_CASES = [
     (1, 1, 3, 0.02),
     (2, 1, 3, 0.02),
     (2, 1, 1, 10.0),
+    (1, 2, 3, 0.02),
 ]
 
 

@@ -42,7 +42,7 @@ def test_o4_prepare_matches_oracle(device, nr, na, config):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("nr,na,config,limiter", [(16, 16, 3, 1), (12, 20, 0, 1), (16, 16, 1, 0)])
+@pytest.mark.parametrize("nr,na,config,limiter", [(16, 16, 3, 1), (12, 20, 0, 1), (16, 16, 1, 0), (16, 16, 3, 2)])
 def test_o4_level_rhs_matches_oracle(device, nr, na, config, limiter):
     shell, opts, ju, inner, O = _setup(nr, na, config, limiter=limiter)
     W = O.prepare(ju, inner)
