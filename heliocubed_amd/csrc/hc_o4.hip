@@ -1703,48 +1703,81 @@ __device__ __forceinline__ unsigned xcdColumn()
     return x*q + (x < r ? x : r) + b / kXcd;
 }
 
+// Column (j, k) of the c-th column of an n x n panel in 4 x 4 patches: patches
+// row by row, columns row by row inside a patch (the last patch row and
+// column may be narrower). Consecutive blocks then share the columns of their
+// 5-point stencils inside a patch; row-major order puts the k +- 1 neighbours
+// a whole row (a full XCD L2 of data at 128 columns) apart.
+__device__ __forceinline__ void patchColumn(unsigned c, int n, int& j, int& k)
+{
+    constexpr int H = 4;
+    const unsigned full = (unsigned)(n / H);
+    unsigned pk = c / (unsigned)(H*n);
+    int h = H;
+    if (pk >= full) { pk = full; h = n - H*(int)full; }
+    const unsigned r = c - pk*(unsigned)(H*n);
+    unsigned pj = r / (unsigned)(H*h);
+    int w = H;
+    if (pj >= full) { pj = full; w = n - H*(int)full; }
+    const unsigned in = r - pj*(unsigned)(H*h);
+    j = (int)pj*H + (int)(in % (unsigned)w);
+    k = (int)pk*H + (int)(in / (unsigned)w);
+}
+
 // Angular quotient with the fourth-order correction (the metric terms are
 // uniform over a column), then the radial quotient with its correction, of
-// one angular column of valid cells: the angular quotient of each component
-// along the whole radial line goes to LDS, the radial quotient reads its
-// radial neighbours there and writes the Cartesian buffer. One block per
-// column in the XCD-aware order, threads along the radius; dynamic LDS of nr
-// doubles.
+// one angular column of valid cells: the angular quotients of G components
+// along the whole radial line go to LDS, the radial quotients read their
+// radial neighbours there and write the Cartesian buffer. One block per
+// column in the XCD-aware patch order, threads along the radius; dynamic LDS
+// of G*nr doubles. G components at a time put their loads in flight together.
+template <int G>
 __global__ void __launch_bounds__(kB) k4_angrad(Prep4View v, const double* __restrict__ U0, const double* __restrict__ K,
                                                 double a_dt, double* __restrict__ cart)
 {
     extern __shared__ double line[];
     const int nr = v.nr, na = v.na;
     const unsigned col = xcdColumn();
-    const int j = (int)(col % (unsigned)na), k = (int)((col / (unsigned)na) % (unsigned)na), p = (int)(col / (unsigned)(na*na));
+    const int p = (int)(col / (unsigned)(na*na));
+    int j, k;
+    patchColumn(col - (unsigned)p*(unsigned)(na*na), na, j, k);
     const double djd = 0.5*(det4(v, j+1, k)-det4(v, j-1, k));
     const double dkd = 0.5*(det4(v, j, k+1)-det4(v, j, k-1));
     const double idet = 1.0/det4(v, j, k);
     auto dv = [&](int ii) { return v.dvol[ii+6]; };
 #pragma unroll 1
-    for (int c = 0; c < 8; ++c)
+    for (int c0 = 0; c0 < 8; c0 += G)
     {
-        const long long base = p*v.vps + c*v.vcs;
         for (int i = threadIdx.x; i < nr; i += blockDim.x)
         {
-            auto ju = [&](int jj, int kk) {
-                const long long x = base + ((long long)kk*na + jj)*nr + i;
-                return K ? U0[x] + a_dt*K[x] : U0[x]; };
-            auto q1 = [&](int jj) { return ju(jj, k)/det4(v, jj, k); };
-            auto q2 = [&](int kk) { return ju(j, kk)/det4(v, j, kk); };
-            double corr = 0;
-            corr += deriv3(q1, j, 0, na-1)*djd;
-            corr += deriv3(q2, k, 0, na-1)*dkd;
-            line[i] = (ju(j, k)-corr/12.0)*idet;
+#pragma unroll
+            for (int g = 0; g < G; ++g)
+            {
+                const long long base = p*v.vps + (c0+g)*v.vcs;
+                auto ju = [&](int jj, int kk) {
+                    const long long x = base + ((long long)kk*na + jj)*nr + i;
+                    return K ? U0[x] + a_dt*K[x] : U0[x]; };
+                auto q1 = [&](int jj) { return ju(jj, k)/det4(v, jj, k); };
+                auto q2 = [&](int kk) { return ju(j, kk)/det4(v, j, kk); };
+                double corr = 0;
+                corr += deriv3(q1, j, 0, na-1)*djd;
+                corr += deriv3(q2, k, 0, na-1)*dkd;
+                line[g*nr + i] = (ju(j, k)-corr/12.0)*idet;
+            }
         }
         __syncthreads();
         for (int i = threadIdx.x; i < nr; i += blockDim.x)
         {
             const double dd = 0.5*(dv(i+1)-dv(i-1));
             const double idv = 1.0/dv(i);
-            auto rq = [&](int ii) { return line[ii]/dv(ii); };
-            const double dq = deriv3(rq, i, 0, nr-1);
-            cart[p*v.cps + c*v.ccs + ((long long)(k+kGC)*v.cJ + (j+kGC))*nr + i] = (line[i]-dq*dd/12.0)*idv;
+#pragma unroll
+            for (int g = 0; g < G; ++g)
+            {
+                const double* ln = line + g*nr;
+                auto rq = [&](int ii) { return ln[ii]/dv(ii); };
+                const double dq = deriv3(rq, i, 0, nr-1);
+                cart[p*v.cps + (c0+g)*v.ccs + ((long long)(k+kGC)*v.cJ + (j+kGC))*nr + i] = (ln[i]-dq*dd/12.0)*idv;
+            }
         }
         __syncthreads();
     }
@@ -1786,8 +1819,10 @@ __global__ void __launch_bounds__(kB) k4_native(Prep4View v, const double* __res
     const int NA = v.na+10;
     // 1-D grid over the 6 (na+10)^2 columns in the XCD-aware order
     const unsigned col = xcdColumn();
-    const int j = (int)(col % (unsigned)NA) - 5, k = (int)((col / (unsigned)NA) % (unsigned)NA) - 5,
-              p = (int)(col / (unsigned)(NA*NA));
+    const int p = (int)(col / (unsigned)(NA*NA));
+    int j, k;
+    patchColumn(col - (unsigned)p*(unsigned)(NA*NA), NA, j, k);
+    j -= 5; k -= 5;
     for (int i = threadIdx.x; i < v.nr; i += blockDim.x)
     {
         auto ca = [&](int c, int jj, int kk) {
@@ -2212,7 +2247,14 @@ void level4Prepare(Level4& L, const double* U0, const double* K, double a_dt, co
     // one block per angular column (j in x; k and the panel in y), threads
     // along the radius in multiples of the wavefront, at most kB
     const dim3 colBlock(std::min(kB, (v.nr+63)/64*64));
-    hipLaunchKernelGGL(k4_angrad, dim3(6u*v.na*v.na), colBlock, v.nr*sizeof(double), s, v, U0, K, a_dt, L.cart);
+    // components per LDS pass: all 8 while 8 radial lines fit in 64 KB
+    const dim3 cols(6u*v.na*v.na);
+    if (8*v.nr*sizeof(double) <= 65536)
+        hipLaunchKernelGGL(k4_angrad<8>, cols, colBlock, 8*v.nr*sizeof(double), s, v, U0, K, a_dt, L.cart);
+    else if (2*v.nr*sizeof(double) <= 65536)
+        hipLaunchKernelGGL(k4_angrad<2>, cols, colBlock, 2*v.nr*sizeof(double), s, v, U0, K, a_dt, L.cart);
+    else
+        hipLaunchKernelGGL(k4_angrad<1>, cols, colBlock, v.nr*sizeof(double), s, v, U0, K, a_dt, L.cart);
     if (L.ncol > 0)
         hipLaunchKernelGGL(k4_seam, dim3(L.ncol), colBlock, 0, s, v, L.colDst, L.colStart, L.donorSrc, L.donorW,
                            L.cart);
