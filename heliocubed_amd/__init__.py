@@ -98,6 +98,11 @@ _SIGNATURES = [
                                     ctypes.POINTER(ctypes.c_int), _I, ctypes.POINTER(_P)]),
     ("hc_boxop_destroy", None, [_P]),
     ("hc_boxop_count", ctypes.c_longlong, [_P, _I]),
+    ("hc_boxop2_create", _I, [ctypes.POINTER(HcShell), ctypes.POINTER(HcOptions), ctypes.POINTER(HcBox), _I,
+                              ctypes.POINTER(_P)]),
+    ("hc_boxop2_destroy", None, [_P]),
+    ("hc_boxop2_count", ctypes.c_longlong, [_P, _I]),
+    ("hc_boxop2_apply", _I, [_P, _P, _P, _P, _P, _P, ctypes.c_double, _P]),
     ("hc_boxop_apply", _I, [_P, _P, _P, _P, _P, _P, _P]),
     ("hc_boxop_apply_scaled", _I, [_P, _P, _P, _P, _P, _P, ctypes.c_double, _P]),
     ("hc_init_native_boxes", _I, [ctypes.POINTER(HcShell), ctypes.POINTER(HcOptions), _I, ctypes.c_ulonglong,
@@ -708,3 +713,40 @@ class BoxOp:
         stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
         _check(lib().hc_boxop_apply_scaled(self._handle, _ptr(USph), _ptr(rhs), _ptr(f[0]), _ptr(f[1]), _ptr(f[2]),
                                            float(scale), stream))
+
+
+class BoxOp2:
+    """Second-order per-box operator (secondOrderOperator with Cartesian
+    primitives) over a box table. apply(W, rhs) evaluates every box in the
+    same launches from the boxes' Cartesian primitives on the box grown by 2;
+    rhs is accumulated into (rhs += scale * (-div F), plus the Powell source)."""
+
+    def __init__(self, shell, options: Options, boxes):
+        self.shell, self.options, self.boxes = shell, options, list(boxes)
+        self._handle = ctypes.c_void_p()
+        sh, op = shell.c(), options.c()
+        _check(lib().hc_boxop2_create(ctypes.byref(sh), ctypes.byref(op), box_table(self.boxes), len(self.boxes),
+                                      ctypes.byref(self._handle)))
+
+    def __del__(self):
+        if getattr(self, "_handle", None) and self._handle.value and _lib is not None:
+            _lib.hc_boxop2_destroy(self._handle)
+            self._handle = ctypes.c_void_p()
+
+    def count(self, what: int) -> int:
+        return lib().hc_boxop2_count(self._handle, what)
+
+    def apply(self, W, rhs, fluxes=None, scale=1.0):
+        import torch
+
+        f = list(fluxes) if fluxes is not None else [None, None, None]
+        if len(f) != 3:
+            raise ValueError("fluxes: need three face-flux tensors (or None entries)")
+        checks = [(W, 0, "W"), (rhs, 1, "rhs")] + [(f[d], 2 + d, f"F{d}") for d in range(3) if f[d] is not None]
+        for t, what, name in checks:
+            if t is None or t.dtype != torch.float64 or not t.is_cuda or not t.is_contiguous() \
+                    or t.numel() != self.count(what):
+                raise ValueError(f"{name}: need a contiguous float64 device tensor of {self.count(what)} elements")
+        stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+        _check(lib().hc_boxop2_apply(self._handle, _ptr(W), _ptr(rhs), _ptr(f[0]), _ptr(f[1]), _ptr(f[2]),
+                                     float(scale), stream))

@@ -94,6 +94,7 @@ typedef struct hc_box
 
 typedef struct hc_level hc_level;
 typedef struct hc_boxop hc_boxop;
+typedef struct hc_boxop2 hc_boxop2;
 typedef void* hc_stream; /* hipStream_t; NULL is the default stream */
 
 /* Builds the level geometry (radial lines, face frames, seam gather tables)
@@ -293,6 +294,27 @@ int hc_boxop_apply(hc_boxop* op, const double* USph, double* rhs, double* F0, do
  * not scaled, as there (:386-390). hc_boxop_apply is scale 1. */
 int hc_boxop_apply_scaled(hc_boxop* op, const double* USph, double* rhs, double* F0, double* F1, double* F2,
                           double scale, hc_stream stream);
+
+/* Second-order operator over a table of boxes: secondOrderOperator
+ * (src/BoxOp_SecondOrder.H:155-370) as StageLevel calls it per box at order 2
+ * with Cartesian primitives (src/BoxOp_EulerCubedSphereImplem.H:1003-1004;
+ * PLM with limiter 0 none, 1 or 2 minmod, 3 MC; riemann_solver 1 Rusanov or 2
+ * Roe; optional Powell source), FP64. Arrays concatenate the boxes in table
+ * order, each box component-major with radial index fastest:
+ *   W    Cartesian primitives (rho, v, p, B) on the box grown by 2,
+ *        8*(n0+4)*(n1+4)*(n2+4) per box (the caller fills the ghost cells);
+ *   rhs  the valid box, ACCUMULATED: rhs += scale * (-div F) (+ the Powell
+ *        source), the divergence direction by direction;
+ *   F_d  face fluxes of the valid box extruded in d (NULL: kept internally).
+ * Without the shared panel states of a whole level every face, seam faces
+ * included, takes the PLM states of the box's own cells and ghost cells.
+ * hc_boxop2_count returns the element counts of W (0), rhs (1), F_0..F_2 (2..4). */
+int hc_boxop2_create(const hc_shell* shell, const hc_options* options, const hc_box* boxes, int nbox,
+                     hc_boxop2** out);
+void hc_boxop2_destroy(hc_boxop2* op);
+long long hc_boxop2_count(const hc_boxop2* op, int what);
+int hc_boxop2_apply(hc_boxop2* op, const double* W, double* rhs, double* F0, double* F1, double* F2, double scale,
+                    hc_stream stream);
 
 /* Fixed AMR hierarchy of fourth-order levels advanced by the synchronous
  * classical RK4 (advanceRungeKuttaSynchronous, src/MHD_AMRRungeKutta.H:5-41):

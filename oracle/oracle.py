@@ -67,6 +67,7 @@ def lib() -> ctypes.CDLL:
             "orc_ghost_donors": (I, [cfgp, I, I, I, P, P, P, P]),
             "orc_seam_edge": (None, [cfgp, I, I, I, P]),
             "orc_o4_rhs": (I, [cfgp, I, P, P, P, P, P, P, P]),
+            "orc_o2_box_rhs": (I, [cfgp, I, P, P, P, P, P, P, P, D]),
             "orc_o4_rhs_table": (I, [cfgp, I, P, P, P]),
             "orc_limit_line": (None, [P, P]),
             "orc_eta_tilde": (D, [P, P]),
@@ -382,6 +383,35 @@ def o4_rhs_boxes(n_rad, n_ang, boxes, USph, limiter=1, entropy_fix=0.3, gamma=5.
         F = [np.zeros(8 * (n[0] + (d == 0)) * (n[1] + (d == 1)) * (n[2] + (d == 2))) for d in range(3)]
         fp = [_p(F[d]) if with_fluxes else None for d in range(3)]
         lib().orc_o4_rhs(ctypes.byref(cfg), blk, lo_a.ctypes.data, hi_a.ctypes.data, _p(u), _p(r), *fp)
+        rhs_parts.append(r)
+        for d in range(3):
+            flux_parts[d].append(F[d])
+    rhs = np.concatenate(rhs_parts)
+    if with_fluxes:
+        return rhs, [np.concatenate(flux_parts[d]) for d in range(3)]
+    return rhs
+
+
+def o2_rhs_boxes(n_rad, n_ang, boxes, W, limiter=1, riemann=1, entropy_fix=0.3, gamma=5.0 / 3.0, r0=21.5, r1=215.0,
+                 scale=1.0, with_fluxes=False, divergence_cleaning=0):
+    """Second-order per-box RHS (secondOrderOperator with Cartesian primitives):
+    W concatenates the boxes' Cartesian primitives on the box grown by 2; the
+    RHS of every box (accumulated from zero) and optionally the face fluxes."""
+    cfg = OrcCfg(n_rad, n_ang, 2, r0, r1, 0.0, gamma, limiter, riemann, entropy_fix, 1, 0.02, 0,
+                 divergence_cleaning)
+    g = 2
+    rhs_parts, flux_parts, off = [], [[], [], []], 0
+    for blk, lo, hi in boxes:
+        n = [hi[d] - lo[d] + 1 for d in range(3)]
+        size = 8 * (n[0] + 2 * g) * (n[1] + 2 * g) * (n[2] + 2 * g)
+        w = np.ascontiguousarray(W[off:off + size], dtype=np.float64)
+        off += size
+        r = np.zeros(8 * n[0] * n[1] * n[2])
+        lo_a, hi_a = np.array(lo, dtype=np.int32), np.array(hi, dtype=np.int32)
+        F = [np.zeros(8 * (n[0] + (d == 0)) * (n[1] + (d == 1)) * (n[2] + (d == 2))) for d in range(3)]
+        fp = [_p(F[d]) if with_fluxes else None for d in range(3)]
+        lib().orc_o2_box_rhs(ctypes.byref(cfg), blk, lo_a.ctypes.data, hi_a.ctypes.data, _p(w), _p(r), *fp,
+                             float(scale))
         rhs_parts.append(r)
         for d in range(3):
             flux_parts[d].append(F[d])
