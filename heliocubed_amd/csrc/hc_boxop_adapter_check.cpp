@@ -1,12 +1,14 @@
 // Host program that drives the C++ adapter (include/heliocubed_boxop.hpp) the
 // way HelioCubed's StageLevel drives its per-box operator: one operator for
 // one box, a_Rhs accumulated, a_fluxes written, a_scale applied. It reads the
-// box's native conserved averages (ghost 5, raw doubles) from a file, runs the
-// operator on the device and writes Rhs, F0, F1, F2 (raw doubles) to a file;
-// it then checks that a view on the wrong box is rejected.
+// box's ghosted input (raw doubles) from a file, runs the operator on the
+// device and writes Rhs, F0, F1, F2 (raw doubles) to a file; it then checks
+// that a view on the wrong box is rejected.
 //
-// usage: hc_boxop_adapter_check n_rad n_ang spatial_operator limiter riemann
-//            precision block lo0 lo1 lo2 hi0 hi1 hi2 scale in.bin out.bin
+// usage: hc_boxop_adapter_check order n_rad n_ang spatial_operator limiter
+//            riemann precision block lo0 lo1 lo2 hi0 hi1 hi2 scale in.bin out.bin
+// order 4 reads native conserved averages (ghost 5), order 2 Cartesian
+// primitives (ghost 2, the a_cartesianPrimitives call).
 #include <hip/hip_runtime_api.h>
 
 #include <array>
@@ -24,10 +26,11 @@ static void check(hipError_t e, const char* what)
 
 int main(int argc, char** argv)
 {
-    if (argc != 17) { std::fprintf(stderr, "usage: see the source header\n"); return 2; }
-    hc_shell shell{std::atoi(argv[1]), std::atoi(argv[2]), 21.5, 215.0, 0.0};
+    if (argc != 18) { std::fprintf(stderr, "usage: see the source header\n"); return 2; }
+    ++argv;   // argv[0] is the order from here on; the rest as before
     hc_options opt{};
-    opt.spatial_order = 4;
+    opt.spatial_order = std::atoi(argv[0]);
+    hc_shell shell{std::atoi(argv[1]), std::atoi(argv[2]), 21.5, 215.0, 0.0};
     opt.spatial_operator = std::atoi(argv[3]);
     opt.limiter = std::atoi(argv[4]);
     opt.riemann_solver = std::atoi(argv[5]);
@@ -41,7 +44,7 @@ int main(int argc, char** argv)
     const int lo[3] = {std::atoi(argv[8]), std::atoi(argv[9]), std::atoi(argv[10])};
     const int hi[3] = {std::atoi(argv[11]), std::atoi(argv[12]), std::atoi(argv[13])};
     const double scale = std::atof(argv[14]);
-    constexpr int g = heliocubed::BoxOp_EulerCubedSphere::ghost();
+    const int g = opt.spatial_order == 2 ? 2 : 5;
     long long n[3], e[3];
     for (int d = 0; d < 3; ++d) { n[d] = hi[d]-lo[d]+1; e[d] = n[d]+2*g; }
     const size_t nu = 8*e[0]*e[1]*e[2], nr = 8*n[0]*n[1]*n[2];
@@ -76,13 +79,14 @@ int main(int argc, char** argv)
     try
     {
         heliocubed::BoxOp_EulerCubedSphere op(shell, opt, block, lo, hi);
-        op(rhs, fluxes, usph, 0.1, scale);
+        if (op.ghost() != g) { std::fprintf(stderr, "ghost %d, expected %d\n", op.ghost(), g); return 1; }
+        op(rhs, fluxes, usph, 0.1, scale, -1, opt.spatial_order == 2);
         check(hipDeviceSynchronize(), "hipDeviceSynchronize");
         // a view on the wrong box is rejected before any device work
         heliocubed::DeviceBoxData bad = rhs;
         bad.hi[0] += 1;
         bool rejected = false;
-        try { op(bad, fluxes, usph, 0.1, scale); }
+        try { op(bad, fluxes, usph, 0.1, scale, -1, opt.spatial_order == 2); }
         catch (const std::runtime_error&) { rejected = true; }
         if (!rejected) { std::fprintf(stderr, "a Rhs view on the wrong box was accepted\n"); return 1; }
     }

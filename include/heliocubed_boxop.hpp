@@ -13,12 +13,18 @@
  *             (a_Rhs += rhsIncr, src/BoxOp_EulerCubedSphereImplem.H:355-378);
  *   a_fluxes  face fluxes of the valid box extruded in each direction (a view
  *             with data == nullptr is not written);
- *   a_USph    native conserved cell averages on the valid box grown by ghost();
- *   a_dt, a_counter  accepted and unused at fourth order, as in the reference;
- *   a_cartesianPrimitives, a_activeFaces  second-order arguments, ignored at
- *             fourth order (src/BoxOp_EulerCubedSphereImplem.H:73-76).
- * The options select the standard operator or, with spatial_operator 2, the
- * Berta operator, which operator() calls as the reference does (:83-86).
+ *   a_USph    at fourth order the native conserved cell averages on the valid
+ *             box grown by ghost() (5); at second order the Cartesian
+ *             primitives on the box grown by 2 (a_cartesianPrimitives true,
+ *             the StageLevel call, src/BoxOp_EulerCubedSphereImplem.H:1003-1004;
+ *             the native-conserved form is not built and is rejected);
+ *   a_dt, a_counter  accepted and unused, as in the reference;
+ *   a_activeFaces  the composite-AMR face mask of the second-order operator:
+ *             not built (nullptr required), ignored at fourth order.
+ * spatial_order 2 selects secondOrderOperator (src/BoxOp_SecondOrder.H:155-370);
+ * at order 4 the options select the standard operator or, with
+ * spatial_operator 2, the Berta operator, which operator() calls as the
+ * reference does (src/BoxOp_EulerCubedSphereImplem.H:73-86).
  * Errors (a view whose box does not match, a failed device call) throw
  * std::runtime_error with the library's message: the reference aborts
  * through MayDay on the same conditions.
@@ -55,13 +61,13 @@ public:
                            const int hi[3], hc_stream stream = nullptr)
         : m_options(options), m_block(block), m_stream(stream)
     {
-        if (options.spatial_order != 4)
-            throw std::runtime_error("BoxOp_EulerCubedSphere: only the fourth-order operators are built per box");
         hc_box box{(int)block, {lo[0], lo[1], lo[2]}, {hi[0], hi[1], hi[2]}};
         for (int d = 0; d < 3; ++d) { m_lo[d] = lo[d]; m_hi[d] = hi[d]; }
-        if (hc_boxop_create(&shell, &options, &box, 1, &m_op) != 0) throw std::runtime_error(hc_last_error());
+        const int rc = options.spatial_order == 2 ? hc_boxop2_create(&shell, &options, &box, 1, &m_op2)
+                                                  : hc_boxop_create(&shell, &options, &box, 1, &m_op);
+        if (rc != 0) throw std::runtime_error(hc_last_error());
     }
-    ~BoxOp_EulerCubedSphere() { if (m_op) hc_boxop_destroy(m_op); }
+    ~BoxOp_EulerCubedSphere() { release(); }
     BoxOp_EulerCubedSphere(const BoxOp_EulerCubedSphere&) = delete;
     BoxOp_EulerCubedSphere& operator=(const BoxOp_EulerCubedSphere&) = delete;
     BoxOp_EulerCubedSphere(BoxOp_EulerCubedSphere&& o) noexcept { *this = static_cast<BoxOp_EulerCubedSphere&&>(o); }
@@ -69,18 +75,20 @@ public:
     {
         if (this != &o)
         {
-            if (m_op) hc_boxop_destroy(m_op);
+            release();
             m_op = o.m_op; o.m_op = nullptr;
+            m_op2 = o.m_op2; o.m_op2 = nullptr;
             m_options = o.m_options; m_block = o.m_block; m_stream = o.m_stream;
             for (int d = 0; d < 3; ++d) { m_lo[d] = o.m_lo[d]; m_hi[d] = o.m_hi[d]; }
         }
         return *this;
     }
 
-    /* src/BoxOp_EulerCubedSphere.H:376-396 */
+    /* src/BoxOp_EulerCubedSphere.H:376-396: ghost 2 at second order, 5 at
+     * fourth order (the Jameson detector of the Berta operator) */
     static constexpr int numState() { return 8; }
-    static constexpr int ghost() { return 5; }
-    static constexpr int order() { return 4; }
+    int ghost() const { return m_options.spatial_order == 2 ? 2 : 5; }
+    int order() const { return m_options.spatial_order == 2 ? 2 : 4; }
     unsigned block() const { return m_block; }
     void setStream(hc_stream stream) { m_stream = stream; }
 
@@ -88,7 +96,16 @@ public:
                     double a_dt, double a_scale = 1.0, int a_counter = -1, bool a_cartesianPrimitives = false,
                     const void* a_activeFaces = nullptr) const
     {
-        (void)a_cartesianPrimitives; (void)a_activeFaces;
+        if (m_options.spatial_order == 2)
+        {
+            if (!a_cartesianPrimitives)
+                throw std::runtime_error("BoxOp_EulerCubedSphere: the second-order operator takes Cartesian primitives "
+                                         "(a_cartesianPrimitives true)");
+            if (a_activeFaces)
+                throw std::runtime_error("BoxOp_EulerCubedSphere: the composite-AMR face mask is not built");
+            apply(a_Rhs, a_fluxes, a_USph, a_scale);
+            return;
+        }
         if (m_options.spatial_operator == 2) { BertaOperator(a_Rhs, a_fluxes, a_USph, a_dt, a_scale, a_counter); return; }
         apply(a_Rhs, a_fluxes, a_USph, a_scale);
     }
@@ -97,7 +114,7 @@ public:
                        double a_dt, double a_scale = 1.0, int a_counter = -1) const
     {
         (void)a_dt; (void)a_counter;
-        if (m_options.spatial_operator != 2)
+        if (m_options.spatial_order != 4 || m_options.spatial_operator != 2)
             throw std::runtime_error("BoxOp_EulerCubedSphere::BertaOperator: the operator was built with spatial_operator "
                                      + std::to_string(m_options.spatial_operator));
         apply(a_Rhs, a_fluxes, a_USph, a_scale);
@@ -126,11 +143,19 @@ private:
             expect(fluxes[d], m_lo, fhi, "a flux");
             F[d] = fluxes[d].data;
         }
-        if (hc_boxop_apply_scaled(m_op, usph.data, rhs.data, F[0], F[1], F[2], scale, m_stream) != 0)
-            throw std::runtime_error(hc_last_error());
+        const int rc = m_op2 ? hc_boxop2_apply(m_op2, usph.data, rhs.data, F[0], F[1], F[2], scale, m_stream)
+                             : hc_boxop_apply_scaled(m_op, usph.data, rhs.data, F[0], F[1], F[2], scale, m_stream);
+        if (rc != 0) throw std::runtime_error(hc_last_error());
+    }
+    void release()
+    {
+        if (m_op) hc_boxop_destroy(m_op);
+        if (m_op2) hc_boxop2_destroy(m_op2);
+        m_op = nullptr; m_op2 = nullptr;
     }
 
     hc_boxop* m_op = nullptr;
+    hc_boxop2* m_op2 = nullptr;
     hc_options m_options{};
     unsigned m_block = 0;
     int m_lo[3] = {0, 0, 0}, m_hi[3] = {-1, -1, -1};
