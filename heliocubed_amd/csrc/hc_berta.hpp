@@ -182,40 +182,6 @@ __global__ void __launch_bounds__(kB) k_b_theta(O4View v, double* __restrict__ S
     P[I(F_THETA, x[0], x[1], x[2])] = t;
 }
 
-// 3. Point primitives on box+4: UPoint = <U> - theta Lap<U>/24, V(UPoint)
-// with the pressure clipped at 0 (:696-741, :212-275).
-__global__ void __launch_bounds__(kB) k_b_wp(O4View v, const double* __restrict__ U, double* __restrict__ S)
-{
-    const BoxDesc& b = blockBoxDesc(v);
-    int L[3], H[3], x[3];
-    for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]-4; H[d] = b.hi[d]+4; }
-    if (!regionPoint(b, L, H, boxThread(v, b), x)) return;
-    const BoxIdx I(b);
-    double* const P = S + b.sOff;
-    auto ui = [&](int i, int j, int k) {
-        return b.uOff + (k-b.lo[2]+kG)*b.us[1] + (j-b.lo[1]+kG)*b.us[0] + (i-b.lo[0]+kG); };
-    const bool smooth = P[I(F_THETA, x[0], x[1], x[2])] > 0.5;
-    double up[8];
-#pragma unroll
-    for (int c = 0; c < 8; ++c)
-    {
-        const long long o = c*b.us[2];
-        const double uc = U[ui(x[0],x[1],x[2])+o];
-        const double lap = U[ui(x[0]+1,x[1],x[2])+o]+U[ui(x[0]-1,x[1],x[2])+o]+U[ui(x[0],x[1]+1,x[2])+o]
-                          +U[ui(x[0],x[1]-1,x[2])+o]+U[ui(x[0],x[1],x[2]+1)+o]+U[ui(x[0],x[1],x[2]-1)+o]
-                          -6.0*uc;
-        up[c] = smooth ? uc - lap/24.0 : uc;
-    }
-    double w[8];
-    sphToPrimPoint(w, up);
-    double ke, me, vb;
-    energiesNative(v.cellA + 9*angIdx(b, x[1], x[2]), w, ke, me, vb);
-    w[4] = (w[4]-ke-me)*(v.gamma-1.0);
-    if (w[4] < 0.0) w[4] = 0.0;
-#pragma unroll
-    for (int c = 0; c < 8; ++c) P[I(F_WPT+c, x[0], x[1], x[2])] = w[c];
-}
-
 // 4. Face points of direction D on the faces [lo_d, hi_d+1] x transverse
 // [lo-2, hi+2]: WENOZ where the six-cell flag minimum is 1, else WENO3 (or PLM),
 // floors, the Roe repair, then the point fluxes and total pressures of both
@@ -348,15 +314,18 @@ __global__ void __launch_bounds__(kB) k_b_faces(O4View v, double* __restrict__ S
 // the tile are staged once. Each output accumulates its terms in the same
 // order as the term-by-term form of the operator.
 // The points of an LDS region org - h + [0, N0) x [0, N1) x [0, N2) that
-// this thread stages: their scratch offsets (field 0), -1 where the region
-// leaves [lo, hi] (the points where the staged fields are defined; staged as
-// 0). Computed once per tile, used for every component.
+// this thread stages: their offsets in one field of an array whose point
+// (o0, o1, o2) is at offset 0 with strides 1, st1, st2 (a box's scratch
+// block, or the box's view of its <U>), -1 where the region leaves [lo, hi]
+// (the points where the staged fields are defined; staged as 0). Computed
+// once per tile, used for every component.
 template <int N0, int N1, int N2>
 struct StagePlan
 {
     static constexpr int N = N0*N1*N2, K = (N + kFT - 1)/kFT;
     int off[K];
-    __device__ StagePlan(const BoxIdx& I, const int org[3], const int h[3], const int lo[3], const int hi[3])
+    __device__ StagePlan(const int org[3], const int h[3], const int lo[3], const int hi[3],
+                         const int o[3], long long st1, long long st2)
     {
 #pragma unroll
         for (int k = 0; k < K; ++k)
@@ -366,12 +335,15 @@ struct StagePlan
             const int y0 = org[0]-h[0]+a0, y1 = org[1]-h[1]+a1, y2 = org[2]-h[2]+a2;
             const bool in = p < N && y0 >= lo[0] && y0 <= hi[0] && y1 >= lo[1] && y1 <= hi[1]
                             && y2 >= lo[2] && y2 <= hi[2];
-            off[k] = in ? I(0, y0, y1, y2) : -1;
+            off[k] = in ? (int)((y2-o[2])*st2 + (y1-o[1])*st1 + (y0-o[0])) : -1;
         }
     }
-    // load the field at scratch offset f0 (field index x field size) into
+    // offsets in a box's scratch block (BoxIdx layout)
+    __device__ StagePlan(const BoxIdx& I, const int org[3], const int h[3], const int lo[3], const int hi[3])
+        : StagePlan(org, h, lo, hi, (const int[3]){I.o0, I.o1, I.o2}, I.e0, (long long)I.e0*I.e1) {}
+    // load the field at offset f0 (field index x field stride) into
     // registers, then store them into LDS
-    __device__ __forceinline__ void load(double (&val)[K], const double* __restrict__ P, int f0) const
+    __device__ __forceinline__ void load(double (&val)[K], const double* __restrict__ P, long long f0) const
     {
 #pragma unroll
         for (int k = 0; k < K; ++k) val[k] = off[k] >= 0 ? P[f0 + off[k]] : 0.0;
@@ -387,6 +359,56 @@ struct StagePlan
         }
     }
 };
+
+// 3. Point primitives on box+4 over tiles of 32 x 4 x 2 cells: per component
+// the box's <U> on the tile grown by 1 goes to LDS (the next component's
+// loads in flight meanwhile) and UPoint = <U> - theta Lap<U>/24 is formed
+// there; then V(UPoint) with the pressure clipped at 0 (:696-741, :212-275).
+__global__ void __launch_bounds__(kFT) k_b_wp(O4View v, const double* __restrict__ U, double* __restrict__ S)
+{
+    using G = CellTile;
+    __shared__ double sU[G::NA];
+    const FluxTileRef tl = v.ctiles[xcdOrder(blockIdx.x, v.nctiles)];
+    const BoxDesc& b = v.boxes[tl.box];
+    const BoxIdx I(b);
+    double* const P = S + b.sOff;
+    const double* const Ub = U + b.uOff;    // <U>(lo-5) of the box: its own array or its view of a level
+    const int org[3] = {tl.o0, tl.o1, tl.o2};
+    const int h[3] = {1, 1, 1};
+    int L[3], H[3];
+    for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]-kG; H[d] = b.hi[d]+kG; }
+    const StagePlan<G::A0, G::A1, G::A2> plan(org, h, L, H, L, b.us[0], b.us[1]);
+    const int l0 = threadIdx.x % G::T0, l1 = (threadIdx.x / G::T0) % G::T1, l2 = threadIdx.x / (G::T0*G::T1);
+    const int x[3] = {org[0]+l0, org[1]+l1, org[2]+l2};
+    bool ok = true;
+    for (int d = 0; d < 3; ++d) ok = ok && x[d] >= b.lo[d]-4 && x[d] <= b.hi[d]+4;
+    const bool smooth = ok && P[I(F_THETA, x[0], x[1], x[2])] > 0.5;
+    const int c0 = ((l2+1)*G::A1 + (l1+1))*G::A0 + (l0+1);
+    constexpr int s1 = G::A0, s2 = G::A0*G::A1;
+    double val[StagePlan<G::A0, G::A1, G::A2>::K];
+    plan.load(val, Ub, 0);
+    double up[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+    {
+        __syncthreads();
+        plan.store(sU, val);
+        __syncthreads();
+        if (c < 7) plan.load(val, Ub, (c+1)*b.us[2]);
+        const double uc = sU[c0];
+        const double lap = sU[c0+1]+sU[c0-1]+sU[c0+s1]+sU[c0-s1]+sU[c0+s2]+sU[c0-s2]-6.0*uc;
+        up[c] = smooth ? uc - lap/24.0 : uc;
+    }
+    if (!ok) return;
+    double w[8];
+    sphToPrimPoint(w, up);
+    double ke, me, vb;
+    energiesNative(v.cellA + 9*angIdx(b, x[1], x[2]), w, ke, me, vb);
+    w[4] = (w[4]-ke-me)*(v.gamma-1.0);
+    if (w[4] < 0.0) w[4] = 0.0;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) P[I(F_WPT+c, x[0], x[1], x[2])] = w[c];
+}
 
 template <int D, bool ROE>
 __global__ void __launch_bounds__(kFT) k_b_avgflux(O4View v, double* __restrict__ S)
@@ -735,7 +757,7 @@ void bertaFluxes(const O4View& v, dim3 grid, hipStream_t s, const double* U, dou
     hipLaunchKernelGGL(k_o4_wbar, grid, dim3(kB), 0, s, v, U, scratch);
     hipLaunchKernelGGL(k_b_thraw, grid, dim3(kB), 0, s, v, scratch);
     hipLaunchKernelGGL(k_b_theta, grid, dim3(kB), 0, s, v, scratch);
-    hipLaunchKernelGGL(k_b_wp, grid, dim3(kB), 0, s, v, U, scratch);
+    if (v.nctiles > 0) hipLaunchKernelGGL(k_b_wp, dim3(v.nctiles), dim3(kFT), 0, s, v, U, scratch);
     bertaDirectionPass<0, S_>(v, grid, s, U, scratch);
     bertaDirectionPass<1, S_>(v, grid, s, U, scratch);
     bertaDirectionPass<2, S_>(v, grid, s, U, scratch);

@@ -99,7 +99,16 @@ constexpr int kFPT = 1;    // faces per thread
 static_assert(FluxTile<0>::NT == kFT*kFPT && FluxTile<1>::NT == kFT*kFPT && FluxTile<2>::NT == kFT*kFPT,
               "a flux tile holds kFPT faces per thread");
 
-// One face tile: its box and first face.
+// Cell tiles of the Berta point-primitive kernel: 32 x 4 x 2 cells of box+4,
+// the box's <U> staged on the tile grown by 1.
+struct CellTile
+{
+    static constexpr int T0 = 32, T1 = 4, T2 = 2;
+    static constexpr int A0 = T0+2, A1 = T1+2, A2 = T2+2, NA = A0*A1*A2, NT = T0*T1*T2;
+};
+static_assert(CellTile::NT == 256, "one cell per thread of a 256-thread tile");
+
+// One face tile (or cell tile): its box and first face (cell).
 struct FluxTileRef
 {
     int box, o0, o1, o2;
@@ -125,6 +134,8 @@ struct O4View
     int nseams;
     const FluxTileRef* ftiles[3];   // face tiles of each direction (Berta face-average and flux kernel)
     unsigned nftiles[3];
+    const FluxTileRef* ctiles;      // cell tiles of box+4 (Berta point primitives)
+    unsigned nctiles;
 };
 
 // Entry of the angular tables (cellA, faceA, fmet) for cell / face (j, k) of the box's panel and level.
@@ -1600,6 +1611,21 @@ void buildBoxop(hc_boxop& B, const hc_box* boxes, const int* boxLevel, int nbox)
         if (tiles.size() > 0x7fffffffULL) throw std::runtime_error("box table too large for one launch");
         v.nftiles[d] = (unsigned)tiles.size();
         v.ftiles[d] = upload(tiles, B.owned);
+    }
+    // cell tiles of box+4, box by box, x0 fastest, then x2
+    {
+        std::vector<FluxTileRef> tiles;
+        for (int n = 0; n < nbox; ++n)
+        {
+            const BoxDesc& bd = B.boxes[n];
+            for (int j = 0; j < bd.n[1]+8; j += CellTile::T1)
+                for (int k = 0; k < bd.n[2]+8; k += CellTile::T2)
+                    for (int i = 0; i < bd.n[0]+8; i += CellTile::T0)
+                        tiles.push_back({n, bd.lo[0]-4+i, bd.lo[1]-4+j, bd.lo[2]-4+k});
+        }
+        if (tiles.size() > 0x7fffffffULL) throw std::runtime_error("box table too large for one launch");
+        v.nctiles = (unsigned)tiles.size();
+        v.ctiles = upload(tiles, B.owned);
     }
     v.seams = nullptr;
     v.nseams = 0;
