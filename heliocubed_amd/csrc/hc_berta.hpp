@@ -6,7 +6,7 @@
 // Restates the host path of BoxOp_EulerCubedSphere::BertaOperator
 // (src/BoxOp_BertaCubedSphereImplem.H:1737-2076; src/ = HelioCubed reference
 // repository) with the Jameson detector and Roe 8-wave dissipation:
-//   sphAvgToPrim (:278-340)          WAvg = V(<U>) on box+5   (k_o4_wbar)
+//   sphAvgToPrim (:278-340)          WAvg = V(<U>) and its native |B|^2/2 on box+5   (k_b_wbar)
 //   BertaComputeThetaJameson (:346-507)  raw flag on box+4, dilated on box+3
 //   BertaAverageToPoint (:696-741) + sphPointToPrim (:212-275)  point W on box+4
 //   per direction:
@@ -22,6 +22,8 @@
 
 // Scratch slots of the Berta operator (the standard operator's, reused).
 constexpr int F_WA = F_WB;                  // V(<U>), box+5
+constexpr int F_BME = F_PTL;                // |B|^2/2 of the native field of V(<U>), box+5 (read by
+                                            // k_b_thraw before the face kernels take the slot)
 constexpr int F_THR = F_MEB;                // raw Jameson flag, box+4
 constexpr int F_THETA = F_ETA;              // dilated flag, box+5 (0 outside box+3)
 constexpr int F_WPT = F_W;                  // point primitives, box+4
@@ -124,6 +126,37 @@ __device__ __forceinline__ S minmod3(S a, S b, S c)
     return S(0);
 }
 
+// 0. Cell-average primitives V(<U>) on box+5 (sphAvgToPrim, :278-340), the
+// Roe repair's donors, and the native |B|^2/2 the Jameson detector takes.
+__device__ __forceinline__ void bertaCellPrim(const O4View& v, const BoxDesc& b, const double* __restrict__ U,
+                                              int i, int j, int k, double w[8])
+{
+    const long long u = b.uOff + (long long)(k-b.lo[2]+kG)*b.us[1] + (long long)(j-b.lo[1]+kG)*b.us[0] + (i-b.lo[0]+kG);
+    double uu[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) uu[c] = U[u + c*b.us[2]];
+    sphToPrimPoint(w, uu);
+    double ke, me, vb;
+    energiesNative(v.cellA + 9*angIdx(b, j, k), w, ke, me, vb);
+    w[4] = (w[4]-ke-me)*(v.gamma-1.0);
+    if (w[4] < 0) w[4] = 0.0;
+}
+
+__global__ void __launch_bounds__(kB) k_b_wbar(O4View v, const double* __restrict__ U, double* __restrict__ S)
+{
+    const BoxDesc& b = blockBoxDesc(v);
+    int L[3], H[3], x[3];
+    for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]-kG; H[d] = b.hi[d]+kG; }
+    if (!regionPoint(b, L, H, boxThread(v, b), x)) return;
+    const BoxIdx I(b);
+    double* const P = S + b.sOff;
+    double w[8];
+    bertaCellPrim(v, b, U, x[0], x[1], x[2], w);
+#pragma unroll
+    for (int c = 0; c < 8; ++c) P[I(F_WA+c, x[0], x[1], x[2])] = w[c];
+    P[I(F_BME, x[0], x[1], x[2])] = 0.5*(w[5]*w[5] + w[6]*w[6] + w[7]*w[7]);
+}
+
 // 1. Raw Jameson flag on box+4 (f_berta_jameson_theta_rawF, :346-407): the
 // normalized second difference of rho, p and the native |B|^2/2 per direction,
 // the largest of the three, the 2-norm over directions, 1 below eta_d.
@@ -139,9 +172,7 @@ __global__ void __launch_bounds__(kB) k_b_thraw(O4View v, double* __restrict__ S
         const double num = fabs(qp - 2.0*q0 + qm);
         const double den = fabs(qp) + 2.0*fabs(q0) + fabs(qm) + 1.0e-40;
         return num/den; };
-    auto magp = [&](int i, int j, int k) {
-        const double bx = P[I(F_WA+5, i, j, k)], by = P[I(F_WA+6, i, j, k)], bz = P[I(F_WA+7, i, j, k)];
-        return 0.5*(bx*bx + by*by + bz*bz); };
+    auto magp = [&](int i, int j, int k) { return P[I(F_BME, i, j, k)]; };
     double et[3];
 #pragma unroll
     for (int dd = 0; dd < 3; ++dd)
@@ -754,7 +785,7 @@ void bertaDirectionPass(const O4View& v, dim3 grid, hipStream_t s, const double*
 template <typename S_>
 void bertaFluxes(const O4View& v, dim3 grid, hipStream_t s, const double* U, double* scratch)
 {
-    hipLaunchKernelGGL(k_o4_wbar, grid, dim3(kB), 0, s, v, U, scratch);
+    hipLaunchKernelGGL(k_b_wbar, grid, dim3(kB), 0, s, v, U, scratch);
     hipLaunchKernelGGL(k_b_thraw, grid, dim3(kB), 0, s, v, scratch);
     hipLaunchKernelGGL(k_b_theta, grid, dim3(kB), 0, s, v, scratch);
     if (v.nctiles > 0) hipLaunchKernelGGL(k_b_wp, dim3(v.nctiles), dim3(kFT), 0, s, v, U, scratch);
