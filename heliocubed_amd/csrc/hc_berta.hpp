@@ -699,6 +699,216 @@ __global__ void __launch_bounds__(kFT) k_b_avgflux(O4View v, double* __restrict_
     }
 }
 
+// 6. Mapped face flux of the standard operator on the flux faces
+// [lo_d, hi_d+1] x [lo, hi]: the radial-area tensor product of the mapped
+// products <det A F> (scalars) and <det(A) A F> (vectors), the pressure force
+// through the cofactor rows, the face-averaged Roe dissipation, and
+// F = (F_L + F_R - D)/2. The mapped products and the pressure force are
+// linear in the point flux and the face-average total pressure, so they act
+// on the sums PHL + PHR and P0L + P0R that k_o4_faces stores (the p0Bar terms,
+// common to both sides, count twice). The radial area varies only with the
+// radial index, so its transverse difference is exactly zero unless the
+// transverse direction is radial; those +-0 terms are left out
+// (bit-identical for finite data). Work runs over the face tiles of
+// FluxTile<D>: per input component the summed point flux and the point Roe
+// dissipation (Rusanov: both sides' conserved point values) on the tile grown
+// by 1 in the transverse directions are staged in LDS (the next component's
+// loads in flight), p0Bar and the face metric rows once; every face reads its
+// transverse differences and the 3x3 radial-transverse window from LDS. Each
+// output accumulates its terms in the oracle's term-by-term order
+// (primToMappedFlux, src/BoxOp_EulerCubedSphereImplem.H:409-470; the sum,
+// :297-353).
+template <int D, bool ROE>
+__global__ void __launch_bounds__(kFT) k_o4_fluxt(O4View v, double* __restrict__ S)
+{
+    using G = FluxTile<D>;
+    constexpr int t1 = D == 0 ? 1 : 0, t2 = D == 2 ? 1 : 2;
+    __shared__ double sF[G::NA];            // summed point flux of the component
+    __shared__ double sD[G::NA];            // point Roe dissipation (Rusanov: left conserved point values)
+    __shared__ double sR[ROE ? 1 : G::NA];  // Rusanov: right conserved point values
+    __shared__ double sB[G::NA];            // face-average total pressure p0Bar
+    __shared__ double sM[13*G::NM];         // face metric rows of the tile's angular columns
+    const FluxTileRef tl = v.ftiles[D][xcdOrder(blockIdx.x, v.nftiles[D])];
+    const BoxDesc& b = v.boxes[tl.box];
+    const BoxIdx I(b);
+    double* const P = S + b.sOff;
+    const int org[3] = {tl.o0, tl.o1, tl.o2};
+    const int hA[3] = {G::h0, G::h1, G::h2};
+    // k_o4_faces defines the face fields on [lo_d, hi_d+1] x transverse
+    // [lo-1, hi+1]; k_o4_bar p0Bar on a wider region
+    int LA[3], HA[3];
+    for (int d = 0; d < 3; ++d) { LA[d] = b.lo[d]-hA[d]; HA[d] = b.hi[d]+hA[d]+(d == D); }
+    const StagePlan<G::A0, G::A1, G::A2> plan(I, org, hA, LA, HA);
+    {
+        double t[StagePlan<G::A0, G::A1, G::A2>::K];
+        plan.load(t, P, (long long)F_P0B*I.ecount);
+        plan.store(sB, t);
+    }
+    for (int p = threadIdx.x; p < 13*G::NM; p += kFT)
+    {
+        const int col = p / 13, c = p - 13*col, a1 = col % G::A1, a2 = col / G::A1;
+        const int j = org[1]-G::h1+a1, k = org[2]-G::h2+a2;
+        const bool in = j >= LA[1] && j <= HA[1] && k >= LA[2] && k <= HA[2];
+        sM[p] = in ? v.fmet[D][13*angIdx(b, j, k) + c] : 0.0;
+    }
+    auto ia = [](int l0, int l1, int l2) { return ((l2+G::h2)*G::A1 + (l1+G::h1))*G::A0 + (l0+G::h0); };
+    auto mrow = [](int l1, int l2) { return 13*((l2+G::h2)*G::A1 + (l1+G::h1)); };
+    const int q = threadIdx.x;
+    const int l[3] = {q % G::T0, (q / G::T0) % G::T1, q / (G::T0*G::T1)};
+    const int x0 = org[0]+l[0], x1 = org[1]+l[1], x2 = org[2]+l[2];
+    const bool ok = x0 >= b.lo[0] && x0 <= b.hi[0]+(D == 0) && x1 >= b.lo[1] && x1 <= b.hi[1]+(D == 1)
+                    && x2 >= b.lo[2] && x2 <= b.hi[2]+(D == 2);
+    auto sh = [&](const double* s_, int t, int o) {
+        int y[3] = {l[0], l[1], l[2]}; y[t] += o; return s_[ia(y[0], y[1], y[2])]; };
+    auto at = [&](const double* s_) { return s_[ia(l[0], l[1], l[2])]; };
+    auto Mv = [&](int c) { return sM[mrow(l[1], l[2]) + c]; };
+    auto Dm = [&](int c, int t) {
+        int a[3] = {l[0], l[1], l[2]}, z[3] = {l[0], l[1], l[2]};
+        a[t] += 1; z[t] -= 1;
+        return 0.5*sM[mrow(a[1], a[2]) + c] - 0.5*sM[mrow(z[1], z[2]) + c]; };
+    auto Df = [&](const double* s_, int t) { return 0.5*sh(s_, t, 1) - 0.5*sh(s_, t, -1); };
+    auto Gv0 = [&](const double* s_) { return 0.5*sh(s_, 0, 1) - 0.5*sh(s_, 0, -1); };
+    auto DGv = [&](const double* s_) {
+        auto s2 = [&](int o0, int o2) { int y[3] = {l[0]+o0, l[1], l[2]}; y[t2] += o2; return s_[ia(y[0], y[1], y[2])]; };
+        const double gp = 0.5*s2(1, 1) - 0.5*s2(-1, 1);
+        const double gm = 0.5*s2(1, -1) - 0.5*s2(-1, -1);
+        return 0.5*gp - 0.5*gm; };
+    auto lapAvg = [&](const double* s_) {
+        double lap = 0;
+        for (int t = 0; t < 2; ++t)
+        {
+            const int tt = t == 0 ? t1 : t2;
+            lap += sh(s_, tt, 1) - 2.0*at(s_) + sh(s_, tt, -1);
+        }
+        return at(s_) + lap/24.0; };
+    const double adjX = ok ? areaOf(v, b, D, x0) : 0.0;
+    const double adjD1 = (ok && D != 0) ? 0.5*areaOf(v, b, D, x0+1) - 0.5*areaOf(v, b, D, x0-1) : 0.0;
+    const double speed = (ok && !ROE) ? P[I(F_SPD, x0, x1, x2)] : 0.0;
+    auto fsum = [&](double mp, double dmp) {
+        double val = 0;
+        if constexpr (D != 0) val += adjD1*dmp*(1.0/12.0);
+        val += adjX*mp;
+        return val; };
+    auto component = [](int s_) { return s_ == 0 ? 0 : s_ == 1 ? 4 : s_ <= 4 ? s_-1 : s_; };
+    double nF[StagePlan<G::A0, G::A1, G::A2>::K], nD[StagePlan<G::A0, G::A1, G::A2>::K];
+    double nR[ROE ? 1 : StagePlan<G::A0, G::A1, G::A2>::K];
+    auto loadComp = [&](int c) {
+        plan.load(nF, P, (long long)(F_PHS+c)*I.ecount);
+        plan.load(nD, P, (long long)((ROE ? F_DPT : F_QL)+c)*I.ecount);
+        if constexpr (!ROE) plan.load(nR, P, (long long)(F_QR+c)*I.ecount);
+    };
+    loadComp(0);
+    __syncthreads();
+    // pressure force: p0 (face average) with p0Bar = pBar + meBar
+    double press[3] = {0, 0, 0};
+    if (ok)
+    {
+        double p0bD[2], radPD[2];
+#pragma unroll
+        for (int t = 0; t < 2; ++t)
+        {
+            const int tt = t == 0 ? t1 : t2;
+            const double pa = sh(sB, tt, 1), pz = sh(sB, tt, -1);
+            p0bD[t] = 0.5*pa - 0.5*pz;
+            const int ya = x0 + (tt == 0), yz = x0 - (tt == 0);
+            radPD[t] = 0.5*(pa*areaOf(v, b, D, ya)) - 0.5*(pz*areaOf(v, b, D, yz));
+        }
+        double tmp3 = 0;
+        if constexpr (D != 0) tmp3 += 2.0*(adjD1*p0bD[0]*(1.0/12.0));
+        tmp3 += adjX*P[I(F_P0S, x0, x1, x2)];
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+        {
+            double pf = 0;
+            pf += 2.0*(radPD[0]*Dm(10+r, t1)*(1.0/12.0));
+            pf += 2.0*(radPD[1]*Dm(10+r, t2)*(1.0/12.0));
+            pf += tmp3*Mv(10+r);
+            press[r] = pf;
+        }
+    }
+    double mp[3], dmp[3], fd[3];
+#pragma unroll 1
+    for (int s_ = 0; s_ < 8; ++s_)
+    {
+        const int c = component(s_);
+        __syncthreads();
+        plan.store(sF, nF);
+        plan.store(sD, nD);
+        if constexpr (!ROE) plan.store(sR, nR);
+        __syncthreads();
+        if (s_ < 7) loadComp(component(s_+1));
+        if (!ok) continue;
+        // face-averaged dissipation input of the component
+        const double da = ROE ? lapAvg(sD) : lapAvg(sR) - lapAvg(sD);
+        if (c == 0 || c == 4)
+        {
+            double m = 0, dm = 0;
+            if constexpr (D == 0) m += Dm(0, t1)*Df(sF, t1)*(1.0/12.0);
+            m += Dm(0, t2)*Df(sF, t2)*(1.0/12.0);
+            m += Mv(0)*at(sF);
+            if constexpr (D != 0)
+            {
+                dm += Dm(0, t2)*DGv(sF)*(1.0/12.0);
+                dm += Mv(0)*Gv0(sF);
+            }
+            const double fs = fsum(m, dm);
+            double dd;
+            if constexpr (ROE)
+            {
+                dd = 0;
+                dd += Dm(0, t1)*Df(sD, t1)*(1.0/12.0);
+                dd += Dm(0, t2)*Df(sD, t2)*(1.0/12.0);
+                dd += Mv(0)*da;
+            }
+            else dd = (Mv(0)*da)*speed;
+            P[I(F_FLUX+8*D+c, x0, x1, x2)] = 0.5*(fs-dd);
+            continue;
+        }
+        const int c0 = c <= 3 ? 1 : 5, col = c-c0;
+        if (col == 0)
+        {
+#pragma unroll
+            for (int r = 0; r < 3; ++r) { mp[r] = 0; dmp[r] = 0; fd[r] = 0; }
+        }
+        const double fx = at(sF), df2 = Df(sF, t2);
+        double df1 = 0.0, gx = 0.0, dg = 0.0;
+        if constexpr (D == 0) df1 = Df(sF, t1);
+        else { gx = Gv0(sF); dg = DGv(sF); }
+        const double dd1 = ROE ? Df(sD, t1) : 0.0, dd2 = ROE ? Df(sD, t2) : 0.0;
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+        {
+            const int m = 1+3*r+col;
+            if constexpr (D == 0) mp[r] += Dm(m, t1)*df1*(1.0/12.0);
+            mp[r] += Dm(m, t2)*df2*(1.0/12.0);
+            mp[r] += Mv(m)*fx;
+            if constexpr (D != 0)
+            {
+                dmp[r] += Dm(m, t2)*dg*(1.0/12.0);
+                dmp[r] += Mv(m)*gx;
+            }
+            if constexpr (ROE)
+            {
+                fd[r] += Dm(m, t1)*dd1*(1.0/12.0);
+                fd[r] += Dm(m, t2)*dd2*(1.0/12.0);
+                fd[r] += Mv(m)*da;
+            }
+            else fd[r] += Mv(m)*da;
+        }
+        if (col == 2)
+        {
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+            {
+                double fs = fsum(mp[r], dmp[r]);
+                if (c0 == 1) fs += press[r];
+                const double d = ROE ? fd[r] : fd[r]*speed;
+                P[I(F_FLUX+8*D+c0+r, x0, x1, x2)] = 0.5*(fs-d);
+            }
+        }
+    }
+}
+
 // 4b. Powell face products of direction D (divergence_cleaning 1,
 // src/BoxOp_BertaCubedSphereImplem.H:1911-1929 with src/BoxOp_DivBcleaning.H:31-71)
 // on the faces [lo_d-1, hi_d+2] x [lo-1, hi+1]: G_D = <adjDr_D <det A B>>, with

@@ -935,296 +935,6 @@ __global__ void __launch_bounds__(kB) k_o4_rusq(O4View v, double* __restrict__ S
     }
 }
 
-// Face-product helpers of direction D at one box: metric values M, centred
-// transverse differences Dm/Df of metrics and scratch fields, and the mapped
-// products <det A F> (scalars) and <det(A) A F> (vectors) of a point flux.
-template <int D>
-struct FaceOps
-{
-    static constexpr int t1 = D == 0 ? 1 : 0, t2 = D == 2 ? 1 : 2;
-    const O4View& v;
-    const BoxDesc& b;
-    const BoxIdx& I;
-    const double* P;          // the box's scratch block
-    const double* met;
-    __device__ FaceOps(const O4View& v_, const BoxDesc& b_, const BoxIdx& I_, const double* P_)
-        : v(v_), b(b_), I(I_), P(P_), met(v_.fmet[D]) {}
-    __device__ double M(int c, const int y[3]) const { return met[13*angIdx(b, y[1], y[2]) + c]; }
-    __device__ double Dm(int c, int t, const int y[3]) const
-    {
-        int a[3] = {y[0], y[1], y[2]}, z[3] = {y[0], y[1], y[2]};
-        a[t] += 1; z[t] -= 1;
-        return 0.5*M(c, a) - 0.5*M(c, z);
-    }
-    __device__ double Fv(int f, const int y[3]) const { return P[I(f, y[0], y[1], y[2])]; }
-    __device__ double Df(int f, int t, const int y[3]) const
-    {
-        int a[3] = {y[0], y[1], y[2]}, z[3] = {y[0], y[1], y[2]};
-        a[t] += 1; z[t] -= 1;
-        return 0.5*Fv(f, a) - 0.5*Fv(f, z);
-    }
-    // <det A F> and <det(A) A F> of the point flux in fields base.. at y.
-    // For an angular direction t1 is radial, along which the angular metric
-    // is constant: that term of the product rule is exactly +-0 and is left
-    // out.
-    __device__ void mapped(int base, const int y[3], double out[8]) const
-    {
-#pragma unroll
-        for (int r = 0; r < 3; ++r)
-        {
-#pragma unroll
-            for (int grp = 0; grp < 2; ++grp)
-            {
-                const int c0 = grp == 0 ? 1 : 5;
-                double val = 0;
-#pragma unroll
-                for (int c = 0; c < 3; ++c)
-                {
-                    if (D == 0) val += Dm(1+3*r+c, t1, y)*Df(base+c0+c, t1, y)*(1.0/12.0);
-                    val += Dm(1+3*r+c, t2, y)*Df(base+c0+c, t2, y)*(1.0/12.0);
-                    val += M(1+3*r+c, y)*Fv(base+c0+c, y);
-                }
-                out[c0+r] = val;
-            }
-        }
-#pragma unroll
-        for (int s = 0; s < 2; ++s)
-        {
-            const int c = s == 0 ? 0 : 4;
-            double val = 0;
-            if (D == 0) val += Dm(0, t1, y)*Df(base+c, t1, y)*(1.0/12.0);
-            val += Dm(0, t2, y)*Df(base+c, t2, y)*(1.0/12.0);
-            val += M(0, y)*Fv(base+c, y);
-            out[c] = val;
-        }
-    }
-    // For an angular direction: half the difference of the mapped products at
-    // the radial neighbours y + e_r and y - e_r. The angular metric does not
-    // vary with the radius (its radial differences Dm(., t1) are exactly
-    // zero), so this is the mapped product of the fluxes' radial difference.
-    __device__ double Gv(int f, const int y[3]) const
-    {
-        const int a[3] = {y[0]+1, y[1], y[2]}, z[3] = {y[0]-1, y[1], y[2]};
-        return 0.5*Fv(f, a) - 0.5*Fv(f, z);
-    }
-    __device__ double DGv(int f, const int y[3]) const
-    {
-        int a[3] = {y[0], y[1], y[2]}, z[3] = {y[0], y[1], y[2]};
-        a[t2] += 1; z[t2] -= 1;
-        return 0.5*Gv(f, a) - 0.5*Gv(f, z);
-    }
-    __device__ void mappedRadialDiff(int base, const int y[3], double out[8]) const
-    {
-        static_assert(D != 0, "radial direction has a radius-dependent metric");
-#pragma unroll
-        for (int r = 0; r < 3; ++r)
-        {
-#pragma unroll
-            for (int grp = 0; grp < 2; ++grp)
-            {
-                const int c0 = grp == 0 ? 1 : 5;
-                double val = 0;
-#pragma unroll
-                for (int c = 0; c < 3; ++c)
-                {
-                    val += Dm(1+3*r+c, t2, y)*DGv(base+c0+c, y)*(1.0/12.0);
-                    val += M(1+3*r+c, y)*Gv(base+c0+c, y);
-                }
-                out[c0+r] = val;
-            }
-        }
-#pragma unroll
-        for (int s = 0; s < 2; ++s)
-        {
-            const int c = s == 0 ? 0 : 4;
-            double val = 0;
-            val += Dm(0, t2, y)*DGv(base+c, y)*(1.0/12.0);
-            val += M(0, y)*Gv(base+c, y);
-            out[c] = val;
-        }
-    }
-};
-
-// 6. Mapped face flux on the flux faces [lo_d, hi_d+1] x [lo, hi]: the
-// radial-area tensor product of the mapped products, the pressure force
-// through the cofactor row, the averaged Roe dissipation, and
-// F = (F_L + F_R - D)/2. Evaluated one input component at a time (its
-// stencils, then its contributions to the outputs it enters), which bounds
-// the live registers; every output accumulates its terms in the same order
-// as the term-by-term form of FaceOps.
-template <int D, bool ROE>
-__global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(4))) k_o4_flux(O4View v, double* __restrict__ S)
-{
-    const BoxDesc& b = blockBoxDesc(v);
-    int L[3], H[3], x[3];
-    for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]; H[d] = b.hi[d]; }
-    H[D] = b.hi[D]+1;
-    if (!regionPoint(b, L, H, boxThread(v, b), x)) return;
-    const BoxIdx I(b);
-    double* const P = S + b.sOff;
-    constexpr int t1 = D == 0 ? 1 : 0, t2 = D == 2 ? 1 : 2;
-    const FaceOps<D> op(v, b, I, P);
-    auto M = [&](int c, const int y[3]) { return op.M(c, y); };
-    auto Dm = [&](int c, int t, const int y[3]) { return op.Dm(c, t, y); };
-    auto Fv = [&](int f, const int y[3]) { return op.Fv(f, y); };
-    auto shift = [&](int f, int t, int o) { int y[3] = {x[0], x[1], x[2]}; y[t] += o; return Fv(f, y); };
-    auto shift2 = [&](int f, int o0, int o2) { int y[3] = {x[0]+o0, x[1], x[2]}; y[t2] += o2; return Fv(f, y); };
-    auto Df = [&](int f, int t) { return 0.5*shift(f, t, 1) - 0.5*shift(f, t, -1); };
-    auto Gv0 = [&](int f) { return 0.5*shift(f, 0, 1) - 0.5*shift(f, 0, -1); };
-    auto DGv = [&](int f) {
-        const double gp = 0.5*shift2(f, 1, 1) - 0.5*shift2(f, -1, 1);
-        const double gm = 0.5*shift2(f, 1, -1) - 0.5*shift2(f, -1, -1);
-        return 0.5*gp - 0.5*gm; };
-    const double adjX = areaOf(v, b, D, x[0]);
-    // The radial area varies only with the radial index, so its transverse
-    // difference is exactly zero unless the transverse direction is radial:
-    // t2 is never radial and t1 is radial for D = 1, 2. The zero terms of the
-    // product rule add +-0 and are left out (bit-identical for finite data).
-    const double adjD1 = D == 0 ? 0.0 : 0.5*areaOf(v, b, D, x[0]+1) - 0.5*areaOf(v, b, D, x[0]-1);
-    // Both sides at once: the mapped products and the pressure force are linear
-    // in the point flux and the face-average total pressure, so they act on the
-    // sums PHL + PHR and P0L + P0R that k_o4_faces stores; the terms of the
-    // face-average total pressure p0Bar, common to both sides, count twice.
-    auto fsum = [&](double mp, double dmp) {
-        double val = 0;
-        if (D != 0) val += adjD1*dmp*(1.0/12.0);
-        val += adjX*mp;
-        return val; };
-    // pressure force: p0 (face average) with p0Bar = pBar + meBar
-    double press[3];
-    {
-        double p0bD[2], radPD[2];
-#pragma unroll
-        for (int t = 0; t < 2; ++t)
-        {
-            const int tt = t == 0 ? t1 : t2;
-            int y[3] = {x[0], x[1], x[2]}, z[3] = {x[0], x[1], x[2]};
-            y[tt] += 1; z[tt] -= 1;
-            const double pa = Fv(F_P0B, y), pz = Fv(F_P0B, z);
-            p0bD[t] = 0.5*pa - 0.5*pz;
-            radPD[t] = 0.5*(pa*areaOf(v, b, D, y[0])) - 0.5*(pz*areaOf(v, b, D, z[0]));
-        }
-        double tmp3 = 0;
-        if (D != 0) tmp3 += 2.0*(adjD1*p0bD[0]*(1.0/12.0));
-        tmp3 += adjX*Fv(F_P0S, x);
-#pragma unroll
-        for (int r = 0; r < 3; ++r)
-        {
-            double pf = 0;
-            pf += 2.0*(radPD[0]*Dm(10+r, t1, x)*(1.0/12.0));
-            pf += 2.0*(radPD[1]*Dm(10+r, t2, x)*(1.0/12.0));
-            pf += tmp3*M(10+r, x);
-            press[r] = pf;
-        }
-    }
-    // face-averaged dissipation input of component c: the averaged point Roe
-    // dissipation (ROE) or the difference of both sides' averaged conserved
-    // values (Rusanov)
-    auto dissAvg = [&](int c) {
-        if constexpr (ROE)
-        {
-            double lap = 0;
-            for (int t = 0; t < 2; ++t)
-            {
-                const int tt = t == 0 ? t1 : t2;
-                lap += shift(F_DPT+c, tt, 1) - 2.0*Fv(F_DPT+c, x) + shift(F_DPT+c, tt, -1);
-            }
-            return Fv(F_DPT+c, x) + lap/24.0;
-        }
-        else
-        {
-            double a2[2];
-#pragma unroll
-            for (int s = 0; s < 2; ++s)
-            {
-                const int f = (s == 0 ? F_QL : F_QR) + c;
-                double lap = 0;
-#pragma unroll
-                for (int t = 0; t < 2; ++t)
-                {
-                    const int tt = t == 0 ? t1 : t2;
-                    lap += shift(f, tt, 1) - 2.0*Fv(f, x) + shift(f, tt, -1);
-                }
-                a2[s] = Fv(f, x) + lap/24.0;
-            }
-            return a2[1] - a2[0];
-        }
-    };
-    const double speed = ROE ? 0.0 : Fv(F_SPD, x);
-    // scalar components 0 and 4
-#pragma unroll 1
-    for (int s = 0; s < 2; ++s)
-    {
-        const int c = s == 0 ? 0 : 4;
-        double mp = 0, dmp = 0;
-        if (D == 0) mp += Dm(0, t1, x)*Df(F_PHS+c, t1)*(1.0/12.0);
-        mp += Dm(0, t2, x)*Df(F_PHS+c, t2)*(1.0/12.0);
-        mp += M(0, x)*Fv(F_PHS+c, x);
-        if (D != 0)
-        {
-            dmp += Dm(0, t2, x)*DGv(F_PHS+c)*(1.0/12.0);
-            dmp += M(0, x)*Gv0(F_PHS+c);
-        }
-        const double fs = fsum(mp, dmp);
-        const double da = dissAvg(c);
-        double fd;
-        if constexpr (ROE)
-        {
-            fd = 0;
-            fd += Dm(0, t1, x)*Df(F_DPT+c, t1)*(1.0/12.0);
-            fd += Dm(0, t2, x)*Df(F_DPT+c, t2)*(1.0/12.0);
-            fd += M(0, x)*da;
-        }
-        else fd = (M(0, x)*da)*speed;
-        P[I(F_FLUX+8*D+c, x[0], x[1], x[2])] = 0.5*(fs-fd);
-    }
-    // vector components: momentum (grp 0, with the pressure force) and field
-#pragma unroll 1
-    for (int grp = 0; grp < 2; ++grp)
-    {
-        const int c0 = grp == 0 ? 1 : 5;
-        double mp[3] = {0, 0, 0}, dmp[3] = {0, 0, 0}, fd[3] = {0, 0, 0};
-#pragma unroll 1
-        for (int col = 0; col < 3; ++col)
-        {
-            const int f = F_PHS+c0+col, g = F_DPT+c0+col;
-            const double fx = Fv(f, x), df2 = Df(f, t2);
-            const double df1 = D == 0 ? Df(f, t1) : 0.0;
-            const double gx = D == 0 ? 0.0 : Gv0(f), dg = D == 0 ? 0.0 : DGv(f);
-            const double da = dissAvg(c0+col);
-            const double dd1 = ROE ? Df(g, t1) : 0.0, dd2 = ROE ? Df(g, t2) : 0.0;
-#pragma unroll
-            for (int r = 0; r < 3; ++r)
-            {
-                const int m = 1+3*r+col;
-                if (D == 0) mp[r] += Dm(m, t1, x)*df1*(1.0/12.0);
-                mp[r] += Dm(m, t2, x)*df2*(1.0/12.0);
-                mp[r] += M(m, x)*fx;
-                if (D != 0)
-                {
-                    dmp[r] += Dm(m, t2, x)*dg*(1.0/12.0);
-                    dmp[r] += M(m, x)*gx;
-                }
-                if constexpr (ROE)
-                {
-                    fd[r] += Dm(m, t1, x)*dd1*(1.0/12.0);
-                    fd[r] += Dm(m, t2, x)*dd2*(1.0/12.0);
-                    fd[r] += M(m, x)*da;
-                }
-                else fd[r] += M(m, x)*da;
-            }
-        }
-#pragma unroll
-        for (int r = 0; r < 3; ++r)
-        {
-            double fs = fsum(mp[r], dmp[r]);
-            if (grp == 0) fs += press[r];
-            const double d = ROE ? fd[r] : fd[r]*speed;
-            P[I(F_FLUX+8*D+c0+r, x[0], x[1], x[2])] = 0.5*(fs-d);
-        }
-    }
-}
-
 // 6a. Seam flux reconciliation 1/2 (F_a + sign F_b) on both boxes' copies
 // of each physical seam face, src/MHD_AMR.H:1484-1496.
 __global__ void __launch_bounds__(kB) k_o4_seams(O4View v, double* __restrict__ S)
@@ -1643,8 +1353,9 @@ void directionPass(const O4View& v, dim3 grid, hipStream_t s, double* scratch)
     if (v.roe) hipLaunchKernelGGL((k_o4_faces<D, true, S_>), grid, dim3(kB), 0, s, v, scratch);
     else hipLaunchKernelGGL((k_o4_faces<D, false, double>), grid, dim3(kB), 0, s, v, scratch);
     if (!v.roe) hipLaunchKernelGGL(k_o4_rusq<D>, grid, dim3(kB), 0, s, v, scratch);
-    if (v.roe) hipLaunchKernelGGL((k_o4_flux<D, true>), grid, dim3(kB), 0, s, v, scratch);
-    else hipLaunchKernelGGL((k_o4_flux<D, false>), grid, dim3(kB), 0, s, v, scratch);
+    if (v.nftiles[D] == 0) return;
+    if (v.roe) hipLaunchKernelGGL((k_o4_fluxt<D, true>), dim3(v.nftiles[D]), dim3(kFT), 0, s, v, scratch);
+    else hipLaunchKernelGGL((k_o4_fluxt<D, false>), dim3(v.nftiles[D]), dim3(kFT), 0, s, v, scratch);
 }
 
 template <typename S_>
