@@ -36,13 +36,15 @@ __device__ __forceinline__ double refMax(double a, double b) { return a < b ? b 
 __device__ __forceinline__ double refMin(double a, double b) { return b < a ? b : a; }
 
 // bertaNormalizeWenoStencil (:747-773): divides by the largest magnitude and
-// returns it (NaN-aware as the reference: a NaN is kept, never zeroed).
+// returns it (NaN-aware as the reference: a NaN is kept, never zeroed). The
+// running maximum skips NaN magnitudes as the reference's (a < b ? b : a)
+// does, which is what fmax does with one NaN operand.
 template <int N>
 __device__ double wenoNormalize(double (&v)[N])
 {
     double scale = 0;
 #pragma unroll
-    for (int i = 0; i < N; ++i) scale = refMax(scale, fabs(v[i]));
+    for (int i = 0; i < N; ++i) scale = fmax(scale, fabs(v[i]));
     if (scale == 0.0)
     {
 #pragma unroll
@@ -64,6 +66,27 @@ __device__ double wenoNormalize(double (&v)[N])
         }
     }
     return scale;
+}
+
+// Mixed precision: the same normalization by a power of two, 2^e with
+// max|v| < 2^e <= 2 max|v|, so that dividing and multiplying back are exact
+// and no FP64 division is needed. Zero and non-finite maxima take the
+// reference form above. The normalized maximum lies in [0.5, 1) instead of
+// being 1; the WENO weights are homogeneous of degree 0 in the values up to
+// the epsilon, which lies far below the indicators of normalized data.
+template <int N>
+__device__ double wenoNormalizePow2(double (&v)[N])
+{
+    double scale = 0;
+#pragma unroll
+    for (int i = 0; i < N; ++i) scale = fmax(scale, fabs(v[i]));
+    if (!(scale > 0.0 && scale <= 1.7976931348623157e308)) return wenoNormalize(v);
+    int e;
+    (void)frexp(scale, &e);
+    const double inv = ldexp(1.0, -e);
+#pragma unroll
+    for (int i = 0; i < N; ++i) v[i] *= inv;
+    return ldexp(1.0, e);
 }
 
 // Quotient a / b of the WENO weights: FP64 division, or in FP32 (mixed
@@ -100,6 +123,33 @@ __device__ __forceinline__ double wenozSide(double m2, double m1, double c, doub
     const double a2 = double(d2*(S(1.0) + wenoQuot(tau5, beta2 + eps)));
     const double asum = a0 + a1 + a2;
     return (a0*q0 + a1*q1 + a2*q2)/asum;
+}
+
+// Mixed-precision form of wenozSide<float>: the weighted sum
+// (a0 q0 + a1 q1 + a2 q2)/(a0 + a1 + a2) written as q1 + w0 (q0 - q1) +
+// w2 (q2 - q1) with the FP32 normalized weights w = a/sum (hardware
+// reciprocal). q0 - q1 = 3/8 (b0a - b1a) and q2 - q1 = (b1a - b2a)/8 are
+// formed in FP64 from the second differences and rounded to FP32; q1 stays
+// FP64. The weights' rounding then enters only through the candidates'
+// spread, as in the quotient form, and no FP64 division is left.
+__device__ __forceinline__ double wenozSideMixed(double m2, double m1, double c, double p1, double p2)
+{
+    const float eps = 1.0e-30f, d0 = 1.0f/16.0f, d1 = 5.0f/8.0f, d2 = 5.0f/16.0f;
+    const double b0a = m2 - 2.0*m1 + c, b1a = m1 - 2.0*c + p1, b2a = c - 2.0*p1 + p2;
+    const float f0a = float(b0a), f0b = float(m2 - 4.0*m1 + 3.0*c);
+    const float f1a = float(b1a), f1b = float(m1 - p1);
+    const float f2a = float(b2a), f2b = float(3.0*c - 4.0*p1 + p2);
+    const float beta0 = (13.0f/12.0f)*f0a*f0a + 0.25f*f0b*f0b;
+    const float beta1 = (13.0f/12.0f)*f1a*f1a + 0.25f*f1b*f1b;
+    const float beta2 = (13.0f/12.0f)*f2a*f2a + 0.25f*f2b*f2b;
+    const float tau5 = fabsf(beta0 - beta2);
+    const float a0 = d0*(1.0f + wenoQuot(tau5, beta0 + eps));
+    const float a1 = d1*(1.0f + wenoQuot(tau5, beta1 + eps));
+    const float a2 = d2*(1.0f + wenoQuot(tau5, beta2 + eps));
+    const float inv = __builtin_amdgcn_rcpf(a0 + a1 + a2);
+    const float e0 = 0.375f*float(b0a - b1a), e2 = 0.125f*float(b1a - b2a);
+    const double q1 = (-m1 + 6.0*c + 3.0*p1)/8.0;
+    return q1 + double((a0*inv)*e0 + (a2*inv)*e2);
 }
 
 // One side of f_weno3_pointwise_faceF (:881-946), eps = dx^2; differences and
@@ -247,12 +297,25 @@ __global__ void __launch_bounds__(kB) k_b_faces(O4View v, double* __restrict__ S
             double s6[6];
 #pragma unroll
             for (int o = -3; o <= 2; ++o) s6[o+3] = at(F_WPT+c, o);
-            const double scale = wenoNormalize(s6);
-            if (scale == 0.0) { l[c] = 0.0; r[c] = 0.0; }
+            if constexpr (std::is_same<S_, float>::value)
+            {
+                const double scale = wenoNormalizePow2(s6);
+                if (scale == 0.0) { l[c] = 0.0; r[c] = 0.0; }
+                else
+                {
+                    l[c] = scale*wenozSideMixed(s6[0], s6[1], s6[2], s6[3], s6[4]);
+                    r[c] = scale*wenozSideMixed(s6[5], s6[4], s6[3], s6[2], s6[1]);
+                }
+            }
             else
             {
-                l[c] = scale*wenozSide<S_>(s6[0], s6[1], s6[2], s6[3], s6[4]);
-                r[c] = scale*wenozSide<S_>(s6[5], s6[4], s6[3], s6[2], s6[1]);
+                const double scale = wenoNormalize(s6);
+                if (scale == 0.0) { l[c] = 0.0; r[c] = 0.0; }
+                else
+                {
+                    l[c] = scale*wenozSide<S_>(s6[0], s6[1], s6[2], s6[3], s6[4]);
+                    r[c] = scale*wenozSide<S_>(s6[5], s6[4], s6[3], s6[2], s6[1]);
+                }
             }
         }
         else

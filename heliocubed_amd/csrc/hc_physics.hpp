@@ -173,11 +173,24 @@ HC_HD void rusanovDissipation(double out[8], const double l[8], const double r[8
     for (int c = 0; c < 8; ++c) out[c] = double(speed)*(cons[1][c]-cons[0][c]);
 }
 
+// Quotient a / b in the eigen-system precision S: an FP64 division, or in
+// FP32 the product with the hardware reciprocal (within 2 ulp, the size of
+// the FP32 roundings the eigen-system already carries).
+HC_HD double quotS(double a, double b) { return a/b; }
+HC_HD float quotS(float a, float b)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return a*__builtin_amdgcn_rcpf(b);
+#else
+    return a/b;
+#endif
+}
+
 template <typename S>
 HC_HD S entropyFixedSpeed(S lambda, S del)
 {
     const S z = fabs(lambda);
-    return z >= del ? z : S(0.5)*(z*z+del*del)/del;
+    return z >= del ? z : quotS(S(0.5)*(z*z+del*del), del);
 }
 
 // Roe 8-wave dissipation D = |A| (U_R - U_L), arithmetic averages, eight-wave
@@ -291,7 +304,7 @@ HC_HD bool roe8WaveDissipation(double out[8], const double WL[8], const double W
     // in the same order as the dense products.
     const S aa = alf*af, sa = als*as, rc = Rho*cc;
     const S fy = S(2)*sqPiRho*c*hyy, fz = S(2)*sqPiRho*c*hzz;
-    const S ih = S(0.5)*sih/sq2PiRho, qy = S(0.25)*hyy/(c*sqPiRho), qz = S(0.25)*hzz/(c*sqPiRho);
+    const S ih = quotS(S(0.5)*sih, sq2PiRho), qy = quotS(S(0.25)*hyy, c*sqPiRho), qz = quotS(S(0.25)*hzz, c*sqPiRho);
     const S hzr = -hzz*rs2, hyr = hyy*rs2;
     const S zq = hzz*sq2PiRho*sih, yq = hyy*sq2PiRho*sih;
     const S ayh = aa*hyy*sih, azh = aa*hzz*sih, syh = sa*hyy*sih, szh = sa*hzz*sih;
@@ -312,11 +325,11 @@ HC_HD bool roe8WaveDissipation(double out[8], const double WL[8], const double W
     // Projections of the jump on the left eigenvectors.
     S Ldu[8];
     {
-        const S a2 = S(0.5)*aa/cc, s2 = S(0.5)*sa/cc;
-        const S sy2 = S(0.5)*sa*hyy*sih/cc, sz2 = S(0.5)*sa*hzz*sih/cc;
-        const S ay2 = S(0.5)*aa*hyy*sih/cc, az2 = S(0.5)*aa*hzz*sih/cc;
-        const S f4 = S(0.5)*alf/(cc*Rho), f6 = S(0.5)*als/(Rho*cc);
-        Ldu[0] = du[0] + (-S(1)/cc)*du[4];
+        const S a2 = quotS(S(0.5)*aa, cc), s2 = quotS(S(0.5)*sa, cc);
+        const S sy2 = quotS(S(0.5)*sa*hyy*sih, cc), sz2 = quotS(S(0.5)*sa*hzz*sih, cc);
+        const S ay2 = quotS(S(0.5)*aa*hyy*sih, cc), az2 = quotS(S(0.5)*aa*hzz*sih, cc);
+        const S f4 = quotS(S(0.5)*alf, cc*Rho), f6 = quotS(S(0.5)*als, Rho*cc);
+        Ldu[0] = du[0] + quotS(-S(1), cc)*du[4];
         Ldu[1] = du[5];
         Ldu[2] = hzr*du[2] + hyr*du[3] + (hzz*ih)*du[6] + (-hyy*ih)*du[7];
         Ldu[3] = hzr*du[2] + hyr*du[3] + (-hzz*ih)*du[6] + (hyy*ih)*du[7];
@@ -349,9 +362,20 @@ HC_HD bool roe8WaveDissipation(double out[8], const double WL[8], const double W
             if (lam[order[wv+1]] > lam[k])
             {
                 if (!(st[0] > 0.0) || !isfinite(st[0])) return false;
-                const double th = st[4]-0.5*(st[1]*st[1]+st[2]*st[2]+st[3]*st[3])/st[0]
-                    -kInv8Pi*(st[5]*st[5]+st[6]*st[6]+st[7]*st[7]);
-                if (!(th >= 0.0) || !isfinite(th)) return false;
+                if constexpr (FAST)
+                {
+                    // the same test times the positive density: no FP64
+                    // division (the fan states carry FP32 roundings here)
+                    const double th = st[4]*st[0]-0.5*(st[1]*st[1]+st[2]*st[2]+st[3]*st[3])
+                        -(kInv8Pi*(st[5]*st[5]+st[6]*st[6]+st[7]*st[7]))*st[0];
+                    if (!(th >= 0.0) || !isfinite(th)) return false;
+                }
+                else
+                {
+                    const double th = st[4]-0.5*(st[1]*st[1]+st[2]*st[2]+st[3]*st[3])/st[0]
+                        -kInv8Pi*(st[5]*st[5]+st[6]*st[6]+st[7]*st[7]);
+                    if (!(th >= 0.0) || !isfinite(th)) return false;
+                }
             }
             HC_SCHED_FENCE();
         }
