@@ -27,7 +27,7 @@ constexpr int F_BME = F_PTL;                // |B|^2/2 of the native field of V(
 constexpr int F_THR = F_MEB;                // raw Jameson flag, box+4
 constexpr int F_THETA = F_ETA;              // dilated flag, box+5 (0 outside box+3)
 constexpr int F_WPT = F_W;                  // point primitives, box+4
-constexpr int F_BWL = F_PTL, F_BWR = F_PTR; // face points (k_o4_roe reads them)
+constexpr int F_BWL = F_PTL, F_BWR = F_PTR; // face points (the Powell products read them)
 constexpr int F_THF = F_P0L;                // flux-average flag of a face
 constexpr int F_PTOT = F_P0R;               // point total pressure, both sides summed
 constexpr int F_FPS = F_PHL;                // point physical fluxes, both sides summed
@@ -266,7 +266,10 @@ __global__ void __launch_bounds__(kB) k_b_theta(O4View v, double* __restrict__ S
 // 4. Face points of direction D on the faces [lo_d, hi_d+1] x transverse
 // [lo-2, hi+2]: WENOZ where the six-cell flag minimum is 1, else WENO3 (or PLM),
 // floors, the Roe repair, then the point fluxes and total pressures of both
-// sides summed. Reconstruction arithmetic in S (the normalization and the
+// sides summed, and on the faces [lo_d, hi_d+1] x [lo-1, hi+1] the point Roe
+// dissipation of the two face points (factoredRoe, src/MHD_Roe8Wave.H:634-740),
+// so the face points never leave registers (Rusanov: the conserved jump and
+// the speed). Reconstruction arithmetic in S (the normalization and the
 // rescaling stay FP64).
 template <int D, typename S_, bool ROE>
 __global__ void __launch_bounds__(kB) k_b_faces(O4View v, double* __restrict__ S)
@@ -289,14 +292,22 @@ __global__ void __launch_bounds__(kB) k_b_faces(O4View v, double* __restrict__ S
     const double dx = D == 0 ? 1.0/b.nr : 1.0/b.na;
     const double dx2 = dx*dx;
     double l[8], r[8];
+    // the six stencil values of the next component are loaded while the
+    // current one is reconstructed
+    double nx[6];
+    auto load6 = [&](int c) {
+#pragma unroll
+        for (int o = -3; o <= 2; ++o) nx[o+3] = at(F_WPT+c, o); };
+    load6(0);
 #pragma unroll 1
     for (int c = 0; c < 8; ++c)
     {
+        double s6[6];
+#pragma unroll
+        for (int o = 0; o < 6; ++o) s6[o] = nx[o];
+        if (c < 7) load6(c+1);
         if (tr > 0.5)
         {
-            double s6[6];
-#pragma unroll
-            for (int o = -3; o <= 2; ++o) s6[o+3] = at(F_WPT+c, o);
             if constexpr (std::is_same<S_, float>::value)
             {
                 const double scale = wenoNormalizePow2(s6);
@@ -320,9 +331,7 @@ __global__ void __launch_bounds__(kB) k_b_faces(O4View v, double* __restrict__ S
         }
         else
         {
-            double s4[4];
-#pragma unroll
-            for (int o = -2; o <= 1; ++o) s4[o+2] = at(F_WPT+c, o);
+            double s4[4] = {s6[1], s6[2], s6[3], s6[4]};
             // the fallbacks run where the detector flags a non-smooth
             // stencil: FP64 in both precisions (their weights and slopes act
             // on the largest jumps)
@@ -370,15 +379,33 @@ __global__ void __launch_bounds__(kB) k_b_faces(O4View v, double* __restrict__ S
     primFlux<FAST>(fl, l, keL, meL, vbL, v.gamma, D);
     primFlux<FAST>(fr, r, keR, meR, vbR, v.gamma, D);
 #pragma unroll
-    for (int c = 0; c < 8; ++c)
-    {
-        P[I(F_BWL+c, x[0], x[1], x[2])] = l[c];
-        P[I(F_BWR+c, x[0], x[1], x[2])] = r[c];
-        P[I(F_FPS+c, x[0], x[1], x[2])] = fl[c] + fr[c];
-    }
+    for (int c = 0; c < 8; ++c) P[I(F_FPS+c, x[0], x[1], x[2])] = fl[c] + fr[c];
     P[I(F_THF, x[0], x[1], x[2])] = thF;
     P[I(F_PTOT, x[0], x[1], x[2])] = (l[4] + meL) + (r[4] + meR);
-    if constexpr (!ROE)
+    // the Powell face products take the normal field of both face points
+    if (v.powell)
+    {
+        P[I(F_BWL+5+D, x[0], x[1], x[2])] = l[5+D];
+        P[I(F_BWR+5+D, x[0], x[1], x[2])] = r[5+D];
+    }
+    if constexpr (ROE)
+    {
+        // point Roe dissipation of the two face points on the faces
+        // [lo_d, hi_d+1] x transverse [lo-1, hi+1] (the faces whose averages
+        // the flux kernel forms)
+        bool roeFace = x[D] >= b.lo[D] && x[D] <= b.hi[D]+1;
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+            if (d != D) roeFace = roeFace && x[d] >= b.lo[d]-1 && x[d] <= b.hi[d]+1;
+        if (roeFace)
+        {
+            double out[8];
+            factoredRoe<S_>(out, l, r, A, radialPoint(v, b, D, x[0]), D, v.gamma, v.entropy);
+#pragma unroll
+            for (int q = 0; q < 8; ++q) P[I(F_DPT+q, x[0], x[1], x[2])] = out[q];
+        }
+    }
+    else
     {
         // Rusanov: pointwise conserved jump R - L times the radial point area
         // and the speed of the point states (:1589-1613, :1653-1725)
@@ -1048,7 +1075,7 @@ void bertaDirectionPass(const O4View& v, dim3 grid, hipStream_t s, const double*
     if (v.roe) hipLaunchKernelGGL((k_b_faces<D, S_, true>), grid, dim3(kB), 0, s, v, scratch);
     else hipLaunchKernelGGL((k_b_faces<D, S_, false>), grid, dim3(kB), 0, s, v, scratch);
     if (v.powell) hipLaunchKernelGGL(k_b_pw_face<D>, grid, dim3(kB), 0, s, v, U, scratch);
-    if (v.roe) hipLaunchKernelGGL((k_o4_roe<D, S_>), grid, dim3(kB), 0, s, v, scratch);
+
     if (v.nftiles[D] == 0) return;
     if (v.roe) hipLaunchKernelGGL((k_b_avgflux<D, true>), dim3(v.nftiles[D]), dim3(kFT), 0, s, v, scratch);
     else hipLaunchKernelGGL((k_b_avgflux<D, false>), dim3(v.nftiles[D]), dim3(kFT), 0, s, v, scratch);

@@ -877,28 +877,6 @@ __global__ void __launch_bounds__(kB) k_o4_bar(O4View v, double* __restrict__ S)
     P[I(F_P0B, x[0], x[1], x[2])] = p + me;
 }
 
-// 5c. Point Roe dissipation of the stored face states on the faces
-// [lo_d, hi_d+1] x transverse [lo-1, hi+1] (the Berta operator's pass; the
-// standard operator evaluates it inside k_o4_faces).
-template <int D, typename S_>
-__global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(2))) k_o4_roe(O4View v, double* __restrict__ S)
-{
-    const BoxDesc& b = blockBoxDesc(v);
-    int L[3], H[3], x[3];
-    for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]-1; H[d] = b.hi[d]+1; }
-    L[D] = b.lo[D]; H[D] = b.hi[D]+1;
-    if (!regionPoint(b, L, H, boxThread(v, b), x)) return;
-    const BoxIdx I(b);
-    double* const P = S + b.sOff;
-    const double* A = v.faceA[D] + 9*angIdx(b, x[1], x[2]);
-    double wl[8], wr[8], out[8];
-#pragma unroll
-    for (int q = 0; q < 8; ++q) { wl[q] = P[I(F_PTL+q, x[0], x[1], x[2])]; wr[q] = P[I(F_PTR+q, x[0], x[1], x[2])]; }
-    factoredRoe<S_>(out, wl, wr, A, radialPoint(v, b, D, x[0]), D, v.gamma, v.entropy);
-#pragma unroll
-    for (int q = 0; q < 8; ++q) P[I(F_DPT+q, x[0], x[1], x[2])] = out[q];
-}
-
 // 5c'. Rusanov (riemann_solver 1): conserved point values of both face
 // states times the radial point area, primToMappedRusanovFlux
 // (src/BoxOp_EulerCubedSphereImplem.H:586-640). Faces [lo_d, hi_d+1] x
