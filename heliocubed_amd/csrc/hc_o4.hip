@@ -1500,13 +1500,14 @@ __global__ void __launch_bounds__(kB) k4_angrad(Prep4View v, const double* __res
 
 // Seam ghost columns of the Cartesian averages: weighted sums of donor
 // columns (valid cells of the panels the ghost cell overlaps). One block per
-// ghost column (blockIdx.x), threads along the radius: the donor table entries
-// are uniform over the block.
+// ghost column, threads along the radius: the donor table entries are
+// uniform over the block. The table runs along the seams, so the XCD-aware
+// order keeps the shared donors of neighbouring ghost columns in one L2.
 __global__ void __launch_bounds__(kB) k4_seam(Prep4View v, const long long* __restrict__ dst,
                                               const int* __restrict__ start, const long long* __restrict__ src,
                                               const double* __restrict__ w, double* __restrict__ cart)
 {
-    const int col = blockIdx.x;
+    const int col = (int)xcdColumn();
     const int d0 = start[col], d1 = start[col+1];
     for (int i = threadIdx.x; i < v.nr; i += blockDim.x)
     {
