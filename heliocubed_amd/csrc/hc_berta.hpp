@@ -31,6 +31,9 @@ constexpr int F_BWL = F_PTL, F_BWR = F_PTR; // face points (the Powell products 
 constexpr int F_THF = F_P0L;                // flux-average flag of a face
 constexpr int F_PTOT = F_P0R;               // point total pressure, both sides summed
 constexpr int F_FPS = F_PHL;                // point physical fluxes, both sides summed
+// Marker in F_DPT of a face whose Roe dissipation k_b_roe64 evaluates (a quiet
+// NaN with a payload no arithmetic produces).
+constexpr long long kRoeDeferred = 0x7ffc0de0defe4a11LL;
 
 __device__ __forceinline__ double refMax(double a, double b) { return a < b ? b : a; }
 __device__ __forceinline__ double refMin(double a, double b) { return b < a ? b : a; }
@@ -195,9 +198,16 @@ __device__ __forceinline__ void bertaCellPrim(const O4View& v, const BoxDesc& b,
 __global__ void __launch_bounds__(kB) k_b_wbar(O4View v, const double* __restrict__ U, double* __restrict__ S)
 {
     const BoxDesc& b = blockBoxDesc(v);
+    const unsigned t = boxThread(v, b);
+    // the box's flags of deferred FP64 Roe faces start the stage clear
+    if (t == 0)
+    {
+        const int n = blockBoxIndex(v);
+        v.dflag[3*n] = 0u; v.dflag[3*n+1] = 0u; v.dflag[3*n+2] = 0u;
+    }
     int L[3], H[3], x[3];
     for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]-kG; H[d] = b.hi[d]+kG; }
-    if (!regionPoint(b, L, H, boxThread(v, b), x)) return;
+    if (!regionPoint(b, L, H, t, x)) return;
     const BoxIdx I(b);
     double* const P = S + b.sOff;
     double w[8];
@@ -272,7 +282,7 @@ __global__ void __launch_bounds__(kB) k_b_theta(O4View v, double* __restrict__ S
 // the speed). Reconstruction arithmetic in S (the normalization and the
 // rescaling stay FP64).
 template <int D, typename S_, bool ROE>
-__global__ void __launch_bounds__(kB) k_b_faces(O4View v, double* __restrict__ S)
+__device__ __forceinline__ void bertaFaces(const O4View& v, double* __restrict__ S)
 {
     const BoxDesc& b = blockBoxDesc(v);
     int L[3], H[3], x[3];
@@ -399,10 +409,26 @@ __global__ void __launch_bounds__(kB) k_b_faces(O4View v, double* __restrict__ S
             if (d != D) roeFace = roeFace && x[d] >= b.lo[d]-1 && x[d] <= b.hi[d]+1;
         if (roeFace)
         {
+            // mixed precision: a face whose eigen-system needs FP64 is left
+            // to k_b_roe64 with its face points and the marker in F_DPT
+            constexpr bool DEFER = std::is_same<S_, float>::value;
             double out[8];
-            factoredRoe<S_>(out, l, r, A, radialPoint(v, b, D, x[0]), D, v.gamma, v.entropy);
+            if (factoredRoe<S_, DEFER>(out, l, r, A, radialPoint(v, b, D, x[0]), D, v.gamma, v.entropy))
+            {
 #pragma unroll
-            for (int q = 0; q < 8; ++q) P[I(F_DPT+q, x[0], x[1], x[2])] = out[q];
+                for (int c = 0; c < 8; ++c)
+                {
+                    P[I(F_BWL+c, x[0], x[1], x[2])] = l[c];
+                    P[I(F_BWR+c, x[0], x[1], x[2])] = r[c];
+                }
+                P[I(F_DPT, x[0], x[1], x[2])] = __longlong_as_double(kRoeDeferred);
+                v.dflag[3*blockBoxIndex(v) + D] = 1u;
+            }
+            else
+            {
+#pragma unroll
+                for (int q = 0; q < 8; ++q) P[I(F_DPT+q, x[0], x[1], x[2])] = out[q];
+            }
         }
     }
     else
@@ -420,6 +446,43 @@ __global__ void __launch_bounds__(kB) k_b_faces(O4View v, double* __restrict__ S
         P[I(F_DPT+4, x[0], x[1], x[2])] = rp*(keR + meR + r[4]/(v.gamma-1.0)) - rp*(keL + meL + l[4]/(v.gamma-1.0));
         P[I(F_SPD, x[0], x[1], x[2])] = rusanovSpeedO4(l, r, A, v.gamma, D, true);
     }
+}
+
+// The FP64 build holds the FP64 Roe solver (2 waves per SIMD); the mixed
+// build defers the FP64 faces and fits 3 waves per SIMD.
+template <int D, typename S_, bool ROE>
+__global__ void __launch_bounds__(kB) k_b_faces(O4View v, double* __restrict__ S) { bertaFaces<D, S_, ROE>(v, S); }
+template <int D, bool ROE>
+__global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(3))) k_b_facesm(O4View v, double* __restrict__ S)
+{
+    bertaFaces<D, float, ROE>(v, S);
+}
+
+// 4c. Mixed precision: the point Roe dissipation of the faces k_b_faces left
+// marked for the FP64 eigen-system (plasma beta below HC_MIXED_FP64_BETA),
+// from the face points it stored there; the faces [lo_d, hi_d+1] x
+// transverse [lo-1, hi+1]. The blocks of a box whose flag the face kernel
+// left clear return at once (k_b_wbar clears the flags at the start of the
+// next stage).
+template <int D>
+__global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(2))) k_b_roe64(O4View v, double* __restrict__ S)
+{
+    if (v.dflag[3*blockBoxIndex(v) + D] == 0u) return;
+    const BoxDesc& b = blockBoxDesc(v);
+    int L[3], H[3], x[3];
+    for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]-1; H[d] = b.hi[d]+1; }
+    L[D] = b.lo[D]; H[D] = b.hi[D]+1;
+    if (!regionPoint(b, L, H, boxThread(v, b), x)) return;
+    const BoxIdx I(b);
+    double* const P = S + b.sOff;
+    if (__double_as_longlong(P[I(F_DPT, x[0], x[1], x[2])]) != kRoeDeferred) return;
+    const double* A = v.faceA[D] + 9*angIdx(b, x[1], x[2]);
+    double wl[8], wr[8], out[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) { wl[q] = P[I(F_BWL+q, x[0], x[1], x[2])]; wr[q] = P[I(F_BWR+q, x[0], x[1], x[2])]; }
+    factoredRoe<float>(out, wl, wr, A, radialPoint(v, b, D, x[0]), D, v.gamma, v.entropy);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) P[I(F_DPT+q, x[0], x[1], x[2])] = out[q];
 }
 
 // 5. Face-average and face-flux kernel of direction D over rectangular tiles
@@ -1072,8 +1135,14 @@ __global__ void __launch_bounds__(kB) k_b_pw_coef(O4View v, double* __restrict__
 template <int D, typename S_>
 void bertaDirectionPass(const O4View& v, dim3 grid, hipStream_t s, const double* U, double* scratch)
 {
-    if (v.roe) hipLaunchKernelGGL((k_b_faces<D, S_, true>), grid, dim3(kB), 0, s, v, scratch);
+    if constexpr (std::is_same<S_, float>::value)
+    {
+        if (v.roe) hipLaunchKernelGGL((k_b_facesm<D, true>), grid, dim3(kB), 0, s, v, scratch);
+        else hipLaunchKernelGGL((k_b_facesm<D, false>), grid, dim3(kB), 0, s, v, scratch);
+    }
+    else if (v.roe) hipLaunchKernelGGL((k_b_faces<D, S_, true>), grid, dim3(kB), 0, s, v, scratch);
     else hipLaunchKernelGGL((k_b_faces<D, S_, false>), grid, dim3(kB), 0, s, v, scratch);
+    if (v.roe && std::is_same<S_, float>::value) hipLaunchKernelGGL(k_b_roe64<D>, grid, dim3(kB), 0, s, v, scratch);
     if (v.powell) hipLaunchKernelGGL(k_b_pw_face<D>, grid, dim3(kB), 0, s, v, U, scratch);
 
     if (v.nftiles[D] == 0) return;

@@ -136,6 +136,7 @@ struct O4View
     unsigned nftiles[3];
     const FluxTileRef* ctiles;      // cell tiles of box+4 (Berta point primitives)
     unsigned nctiles;
+    unsigned* dflag;                // [box][3]: the box has faces of the direction deferred to k_b_roe64
 };
 
 // Entry of the angular tables (cellA, faceA, fmet) for cell / face (j, k) of the box's panel and level.
@@ -369,8 +370,10 @@ __device__ __forceinline__ void roeFrame(const double* Am, int dir, double A[3][
 // Mapped and factored Roe dissipation at a face point, src/MHD_Roe8Wave.H:634-740.
 // The basis and the frame are formed again after the Riemann solver (the same
 // operations, so the same values) instead of being held in registers across it.
-template <typename S>
-__device__ void factoredRoe(double out[8], const double wl[8], const double wr[8], const double* Am,
+// With DEFER, a face that the mixed path evaluates in FP64 is left to the
+// caller: the function then returns true and writes nothing.
+template <typename S, bool DEFER = false>
+__device__ bool factoredRoe(double out[8], const double wl[8], const double wr[8], const double* Am,
                             double radialArea, int dir, double gamma, double efix)
 {
     double A[3][3], frame[3][3], nm;
@@ -401,8 +404,9 @@ __device__ void factoredRoe(double out[8], const double wl[8], const double wr[8
     // the Rusanov fallback of an inadmissible Roe fan is FP64 in both
     // precisions: such faces carry the largest jumps
     double diss[8];
-    if (!hc::roe8WaveDissipation<S>(diss, loc[0], loc[1], gamma, efix))
-        hc::rusanovDissipation<double>(diss, loc[0], loc[1], gamma);
+    const int rs = hc::roe8WaveDissipation<S, DEFER>(diss, loc[0], loc[1], gamma, efix);
+    if (DEFER && rs == 2) return true;
+    if (rs == 0) hc::rusanovDissipation<double>(diss, loc[0], loc[1], gamma);
     // the basis is read again and the frame recomputed: nothing of the
     // frame stays live across the solver
     asm volatile("" ::: "memory");
@@ -437,7 +441,7 @@ __device__ void factoredRoe(double out[8], const double wl[8], const double wr[8
 #pragma unroll
             for (int i = 0; i < 3; ++i) { out[1+d] += adj[d][i]*mapped[1+i]*idet2; out[5+d] += adj[d][i]*mapped[5+i]*idet2; }
         }
-        return;
+        return false;
     }
     out[0] = mapped[0]/det; out[4] = mapped[4]/det;
 #pragma unroll
@@ -447,6 +451,7 @@ __device__ void factoredRoe(double out[8], const double wl[8], const double wr[8
 #pragma unroll
         for (int i = 0; i < 3; ++i) { out[1+d] += adj[d][i]*mapped[1+i]/(det*det); out[5+d] += adj[d][i]*mapped[5+i]/(det*det); }
     }
+    return false;
 }
 
 // Rusanov speed |w_d| + c_f |A e_d| of a face from its two states,
@@ -1268,6 +1273,7 @@ void buildBoxop(hc_boxop& B, const hc_box* boxes, const int* boxLevel, int nbox)
     for (int d = 0; d < 3; ++d) { B.fCount[d] = f[d]; B.fOff[d] = upload(B.fOffHost[d], B.owned); }
     v.boxes = upload(B.boxes, B.owned);
     v.blockBox = upload(blockBox, B.owned);
+    v.dflag = upload(std::vector<unsigned>((size_t)3*nbox, 0u), B.owned);
     // face tiles of each direction, box by box, x0 fastest; then x2 for the
     // r and alpha directions (whose stencils reach two planes across x2
     // tiles of 4), x1 for beta

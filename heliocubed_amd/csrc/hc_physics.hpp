@@ -195,16 +195,18 @@ HC_HD S entropyFixedSpeed(S lambda, S del)
 
 // Roe 8-wave dissipation D = |A| (U_R - U_L), arithmetic averages, eight-wave
 // entropy fix and the wave-fan admissibility walk, src/MHD_Roe8Wave.H:16-600.
-// Returns false when the caller must fall back to Rusanov. The averages, the
-// conserved jump, the walked fan states and the result are FP64; the
-// eigen-system and the projections are evaluated in S (S = double: all FP64).
-// The result stays FP64 in both cases, so a face evaluated in FP64 by the
-// mixed path carries no FP32 rounding.
-template <typename S>
-HC_HD bool roe8WaveDissipation(double out[8], const double WL[8], const double WR[8], double gamma_d, double efix_d)
+// Returns 0 when the caller must fall back to Rusanov, 1 with the result in
+// out. The averages, the conserved jump, the walked fan states and the result
+// are FP64; the eigen-system and the projections are evaluated in S (S =
+// double: all FP64). The result stays FP64 in both cases, so a face evaluated
+// in FP64 by the mixed path carries no FP32 rounding. With DEFER, a face the
+// mixed path must evaluate in FP64 returns 2 (nothing written) for the caller
+// to evaluate later, which keeps the FP64 solver out of the caller's code.
+template <typename S, bool DEFER = false>
+HC_HD int roe8WaveDissipation(double out[8], const double WL[8], const double WR[8], double gamma_d, double efix_d)
 {
     const S rs2 = S(0.707106781186547524400844362104849039);
-    if (!(WL[0] > 0.0 && WR[0] > 0.0 && WL[4] > 0.0 && WR[4] > 0.0)) return false;
+    if (!(WL[0] > 0.0 && WR[0] > 0.0 && WL[4] > 0.0 && WR[4] > 0.0)) return 0;
     const double hgd = gamma_d-1.0;
     // mixed precision (S = float): the FP64 divisions by gamma-1 and by the
     // average density become products with their reciprocals
@@ -233,7 +235,7 @@ HC_HD bool roe8WaveDissipation(double out[8], const double WL[8], const double W
     const double ccmi = a2L > a2R ? a2R : a2L, ccma = a2L > a2R ? a2L : a2R;
     if (ccd < ccmi) ccd = ccmi;
     if (ccd > ccma) ccd = ccma;
-    if (!(ccd > 0.0) || !isfinite(ccd)) return false;
+    if (!(ccd > 0.0) || !isfinite(ccd)) return 0;
     if constexpr (!std::is_same<S, double>::value)
     {
         // Below plasma beta 1/2 (of either side or of the average state) the
@@ -244,7 +246,8 @@ HC_HD bool roe8WaveDissipation(double out[8], const double WL[8], const double W
                                              fabs(WL[4]-WR[4]) > jumpFp64*fmin(WL[4], WR[4]));
         if (jump || ccd*Rhod < betaFp64*gamma_d*BxBHd || WL[4] < betaFp64*bbL || WR[4] < betaFp64*bbR)
         {
-            return roe8WaveDissipation<double>(out, WL, WR, gamma_d, efix_d);
+            if constexpr (DEFER) return 2;
+            else return roe8WaveDissipation<double>(out, WL, WR, gamma_d, efix_d);
         }
     }
     // Jump in primitive-like variables (left minus right), FP64.
@@ -361,20 +364,20 @@ HC_HD bool roe8WaveDissipation(double out[8], const double WL[8], const double W
             for (int q = 0; q < 8; ++q) st[q] += dl[q];
             if (lam[order[wv+1]] > lam[k])
             {
-                if (!(st[0] > 0.0) || !isfinite(st[0])) return false;
+                if (!(st[0] > 0.0) || !isfinite(st[0])) return 0;
                 if constexpr (FAST)
                 {
                     // the same test times the positive density: no FP64
                     // division (the fan states carry FP32 roundings here)
                     const double th = st[4]*st[0]-0.5*(st[1]*st[1]+st[2]*st[2]+st[3]*st[3])
                         -(kInv8Pi*(st[5]*st[5]+st[6]*st[6]+st[7]*st[7]))*st[0];
-                    if (!(th >= 0.0) || !isfinite(th)) return false;
+                    if (!(th >= 0.0) || !isfinite(th)) return 0;
                 }
                 else
                 {
                     const double th = st[4]-0.5*(st[1]*st[1]+st[2]*st[2]+st[3]*st[3])/st[0]
                         -kInv8Pi*(st[5]*st[5]+st[6]*st[6]+st[7]*st[7]);
-                    if (!(th >= 0.0) || !isfinite(th)) return false;
+                    if (!(th >= 0.0) || !isfinite(th)) return 0;
                 }
             }
             HC_SCHED_FENCE();
@@ -404,7 +407,7 @@ HC_HD bool roe8WaveDissipation(double out[8], const double WL[8], const double W
     bool finite = true;
 #pragma unroll
     for (int q = 0; q < 8; ++q) { out[q] = -rd[q]; finite = finite && isfinite(rd[q]); }
-    return finite;
+    return finite ? 1 : 0;
 }
 
 // Face frame: rows are the unit normal and two orthonormal tangents, plus
