@@ -548,6 +548,7 @@ struct StagePlan
 // the box's <U> on the tile grown by 1 goes to LDS (the next component's
 // loads in flight meanwhile) and UPoint = <U> - theta Lap<U>/24 is formed
 // there; then V(UPoint) with the pressure clipped at 0 (:696-741, :212-275).
+template <bool FAST>
 __global__ void __launch_bounds__(kFT) k_b_wp(O4View v, const double* __restrict__ U, double* __restrict__ S)
 {
     using G = CellTile;
@@ -581,13 +582,22 @@ __global__ void __launch_bounds__(kFT) k_b_wp(O4View v, const double* __restrict
         if (c < 7) plan.load(val, Ub, (c+1)*b.us[2]);
         const double uc = sU[c0];
         const double lap = sU[c0+1]+sU[c0-1]+sU[c0+s1]+sU[c0-s1]+sU[c0+s2]+sU[c0-s2]-6.0*uc;
-        up[c] = smooth ? uc - lap/24.0 : uc;
+        up[c] = smooth ? uc - (FAST ? lap*(1.0/24.0) : lap/24.0) : uc;
     }
     if (!ok) return;
     double w[8];
-    sphToPrimPoint(w, up);
+    if constexpr (FAST)
+    {
+        // mixed precision: the point primitives feed only the reconstruction
+        // (the detector takes the cell averages), so they take reciprocal products
+        w[0] = fmax(up[0], hc::kRhoFloor);
+        const double ir = 1.0/w[0];
+        w[1] = up[1]*ir; w[2] = up[2]*ir; w[3] = up[3]*ir;
+        w[4] = up[4]; w[5] = up[5]; w[6] = up[6]; w[7] = up[7];
+    }
+    else sphToPrimPoint(w, up);
     double ke, me, vb;
-    energiesNative(v.cellA + 9*angIdx(b, x[1], x[2]), w, ke, me, vb);
+    energiesNative<FAST>(v.cellA + 9*angIdx(b, x[1], x[2]), w, ke, me, vb);
     w[4] = (w[4]-ke-me)*(v.gamma-1.0);
     if (w[4] < 0.0) w[4] = 0.0;
 #pragma unroll
@@ -1157,7 +1167,8 @@ void bertaFluxes(const O4View& v, dim3 grid, hipStream_t s, const double* U, dou
     hipLaunchKernelGGL(k_b_wbar, grid, dim3(kB), 0, s, v, U, scratch);
     hipLaunchKernelGGL(k_b_thraw, grid, dim3(kB), 0, s, v, scratch);
     hipLaunchKernelGGL(k_b_theta, grid, dim3(kB), 0, s, v, scratch);
-    if (v.nctiles > 0) hipLaunchKernelGGL(k_b_wp, dim3(v.nctiles), dim3(kFT), 0, s, v, U, scratch);
+    if (v.nctiles > 0)
+        hipLaunchKernelGGL((k_b_wp<std::is_same<S_, float>::value>), dim3(v.nctiles), dim3(kFT), 0, s, v, U, scratch);
     bertaDirectionPass<0, S_>(v, grid, s, U, scratch);
     bertaDirectionPass<1, S_>(v, grid, s, U, scratch);
     bertaDirectionPass<2, S_>(v, grid, s, U, scratch);

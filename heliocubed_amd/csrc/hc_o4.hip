@@ -341,7 +341,9 @@ __device__ double etaTilde(const double* pb2, const double* pb, const double* pa
 
 // The face basis A and the orthonormal Roe frame of direction dir (rows: the
 // unit normal, the unit first tangent, their cross product) with the
-// normal's length, src/MHD_Roe8Wave.H:638-661.
+// normal's length, src/MHD_Roe8Wave.H:638-661. FAST (mixed precision)
+// normalizes each row by the product with its reciprocal length.
+template <bool FAST = false>
 __device__ __forceinline__ void roeFrame(const double* Am, int dir, double A[3][3], double frame[3][3], double& nm)
 {
 #pragma unroll
@@ -361,8 +363,17 @@ __device__ __forceinline__ void roeFrame(const double* Am, int dir, double A[3][
     }
     nm = sqrt(normal2);
     const double tm = sqrt(tangent2);
+    if constexpr (FAST)
+    {
+        const double inm = 1.0/nm, itm = 1.0/tm;
 #pragma unroll
-    for (int i = 0; i < 3; ++i) { frame[0][i] /= nm; frame[1][i] /= tm; }
+        for (int i = 0; i < 3; ++i) { frame[0][i] *= inm; frame[1][i] *= itm; }
+    }
+    else
+    {
+#pragma unroll
+        for (int i = 0; i < 3; ++i) { frame[0][i] /= nm; frame[1][i] /= tm; }
+    }
 #pragma unroll
     for (int i = 0; i < 3; ++i) { const int j = (i+1)%3, k = (i+2)%3; frame[2][i] = frame[0][j]*frame[1][k]-frame[0][k]*frame[1][j]; }
 }
@@ -376,12 +387,11 @@ template <typename S, bool DEFER = false>
 __device__ bool factoredRoe(double out[8], const double wl[8], const double wr[8], const double* Am,
                             double radialArea, int dir, double gamma, double efix)
 {
-    double A[3][3], frame[3][3], nm;
-    roeFrame(Am, dir, A, frame, nm);
-    // the frame (and with it the face-frame states the Roe solver decides on)
-    // is the same in both precisions; only the back-transform below takes
-    // reciprocal products in mixed precision
+    // mixed precision: the frame's normalization and the back-transform take
+    // reciprocal products (ulp-level changes next to the FP32 eigen-system)
     constexpr bool FAST = !std::is_same<S, double>::value;
+    double A[3][3], frame[3][3], nm;
+    roeFrame<FAST>(Am, dir, A, frame, nm);
     double loc[2][8];
 #pragma unroll
     for (int s = 0; s < 2; ++s)
@@ -410,7 +420,7 @@ __device__ bool factoredRoe(double out[8], const double wl[8], const double wr[8
     // the basis is read again and the frame recomputed: nothing of the
     // frame stays live across the solver
     asm volatile("" ::: "memory");
-    roeFrame(Am, dir, A, frame, nm);
+    roeFrame<FAST>(Am, dir, A, frame, nm);
     const double area = radialArea*nm;
     double mapped[8];
     mapped[0] = area*diss[0]; mapped[4] = area*diss[4];
