@@ -6,9 +6,10 @@
 // Restates the host path of BoxOp_EulerCubedSphere::BertaOperator
 // (src/BoxOp_BertaCubedSphereImplem.H:1737-2076; src/ = HelioCubed reference
 // repository) with the Jameson detector and Roe 8-wave dissipation:
-//   sphAvgToPrim (:278-340)          WAvg = V(<U>) and its native |B|^2/2 on box+5   (k_b_wbar)
-//   BertaComputeThetaJameson (:346-507)  raw flag on box+4, dilated on box+3
-//   BertaAverageToPoint (:696-741) + sphPointToPrim (:212-275)  point W on box+4
+//   sphAvgToPrim (:278-340) + BertaComputeThetaJameson (:346-507): raw flag on box+4
+//     from rho, p and the native |B|^2/2 of V(<U>) formed in LDS      (k_b_detect)
+//   dilated flag on box+3 + BertaAverageToPoint (:696-741) + sphPointToPrim (:212-275):
+//     point W on box+4                                                 (k_b_wp)
 //   per direction:
 //     BertaReconstructWithFallback (:747-1239): WENOZ / WENO3 / PLM face points
 //     Roe repair with the cell-average donors (:1817-1844)
@@ -21,11 +22,8 @@
 // Max/min follow the reference's std::max / std::min ((a < b) comparisons).
 
 // Scratch slots of the Berta operator (the standard operator's, reused).
-constexpr int F_WA = F_WB;                  // V(<U>), box+5
-constexpr int F_BME = F_PTL;                // |B|^2/2 of the native field of V(<U>), box+5 (read by
-                                            // k_b_thraw before the face kernels take the slot)
 constexpr int F_THR = F_MEB;                // raw Jameson flag, box+4
-constexpr int F_THETA = F_ETA;              // dilated flag, box+5 (0 outside box+3)
+constexpr int F_THETA = F_ETA;              // dilated flag, box+4 (0 outside box+3)
 constexpr int F_WPT = F_W;                  // point primitives, box+4
 constexpr int F_BWL = F_PTL, F_BWR = F_PTR; // face points (the Powell products read them)
 constexpr int F_THF = F_P0L;                // flux-average flag of a face
@@ -179,8 +177,8 @@ __device__ __forceinline__ S minmod3(S a, S b, S c)
     return S(0);
 }
 
-// 0. Cell-average primitives V(<U>) on box+5 (sphAvgToPrim, :278-340), the
-// Roe repair's donors, and the native |B|^2/2 the Jameson detector takes.
+// Cell-average primitives V(<U>) of a cell of box+5 (sphAvgToPrim, :278-340):
+// the Jameson detector's inputs and the Roe repair's donors.
 __device__ __forceinline__ void bertaCellPrim(const O4View& v, const BoxDesc& b, const double* __restrict__ U,
                                               int i, int j, int k, double w[8])
 {
@@ -195,82 +193,67 @@ __device__ __forceinline__ void bertaCellPrim(const O4View& v, const BoxDesc& b,
     if (w[4] < 0) w[4] = 0.0;
 }
 
-__global__ void __launch_bounds__(kB) k_b_wbar(O4View v, const double* __restrict__ U, double* __restrict__ S)
-{
-    const BoxDesc& b = blockBoxDesc(v);
-    const unsigned t = boxThread(v, b);
-    // the box's flags of deferred FP64 Roe faces start the stage clear
-    if (t == 0)
-    {
-        const int n = blockBoxIndex(v);
-        v.dflag[3*n] = 0u; v.dflag[3*n+1] = 0u; v.dflag[3*n+2] = 0u;
-    }
-    int L[3], H[3], x[3];
-    for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]-kG; H[d] = b.hi[d]+kG; }
-    if (!regionPoint(b, L, H, t, x)) return;
-    const BoxIdx I(b);
-    double* const P = S + b.sOff;
-    double w[8];
-    bertaCellPrim(v, b, U, x[0], x[1], x[2], w);
-#pragma unroll
-    for (int c = 0; c < 8; ++c) P[I(F_WA+c, x[0], x[1], x[2])] = w[c];
-    P[I(F_BME, x[0], x[1], x[2])] = 0.5*(w[5]*w[5] + w[6]*w[6] + w[7]*w[7]);
-}
-
 // 1. Raw Jameson flag on box+4 (f_berta_jameson_theta_rawF, :346-407): the
-// normalized second difference of rho, p and the native |B|^2/2 per direction,
-// the largest of the three, the 2-norm over directions, 1 below eta_d.
-__global__ void __launch_bounds__(kB) k_b_thraw(O4View v, double* __restrict__ S)
+// normalized second difference of rho, p and the native |B|^2/2 of V(<U>) per
+// direction, the largest of the three, the 2-norm over directions, 1 below
+// eta_d. Over the cell tiles of box+4: rho, p and |B|^2/2 of V(<U>) on the
+// tile grown by 1 are formed once per point into LDS (the eight components'
+// loads of a thread's points in flight together), then every cell of the
+// tile takes its second differences from LDS; only the flag is stored.
+// The first tile of a box also clears the box's flags of deferred FP64 Roe
+// faces for the stage's direction passes.
+__global__ void __launch_bounds__(kFT) k_b_detect(O4View v, const double* __restrict__ U, double* __restrict__ S)
 {
-    const BoxDesc& b = blockBoxDesc(v);
-    int L[3], H[3], x[3];
-    for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]-4; H[d] = b.hi[d]+4; }
-    if (!regionPoint(b, L, H, boxThread(v, b), x)) return;
+    using G = CellTile;
+    __shared__ double sR[G::NA], sP[G::NA], sB[G::NA];
+    const FluxTileRef tl = v.ctiles[xcdOrder(blockIdx.x, v.nctiles)];
+    const BoxDesc& b = v.boxes[tl.box];
     const BoxIdx I(b);
     double* const P = S + b.sOff;
+    const int org[3] = {tl.o0, tl.o1, tl.o2};
+    if (threadIdx.x == 0 && org[0] == b.lo[0]-4 && org[1] == b.lo[1]-4 && org[2] == b.lo[2]-4)
+    {
+        v.dflag[3*tl.box] = 0u; v.dflag[3*tl.box+1] = 0u; v.dflag[3*tl.box+2] = 0u;
+    }
+    constexpr int KA = (G::NA + kFT - 1)/kFT;
+#pragma unroll
+    for (int k = 0; k < KA; ++k)
+    {
+        const int p = threadIdx.x + k*kFT;
+        const int a0 = p % G::A0, q = p / G::A0, a1 = q % G::A1, a2 = q / G::A1;
+        const int y0 = org[0]-1+a0, y1 = org[1]-1+a1, y2 = org[2]-1+a2;
+        // V(<U>) is defined on box+5, which holds every neighbour of box+4
+        if (p < G::NA && y0 >= b.lo[0]-kG && y0 <= b.hi[0]+kG && y1 >= b.lo[1]-kG && y1 <= b.hi[1]+kG
+            && y2 >= b.lo[2]-kG && y2 <= b.hi[2]+kG)
+        {
+            double w[8];
+            bertaCellPrim(v, b, U, y0, y1, y2, w);
+            sR[p] = w[0]; sP[p] = w[4];
+            sB[p] = 0.5*(w[5]*w[5] + w[6]*w[6] + w[7]*w[7]);
+        }
+    }
+    __syncthreads();
+    const int l0 = threadIdx.x % G::T0, l1 = (threadIdx.x / G::T0) % G::T1, l2 = threadIdx.x / (G::T0*G::T1);
+    const int x[3] = {org[0]+l0, org[1]+l1, org[2]+l2};
+    for (int d = 0; d < 3; ++d)
+        if (x[d] < b.lo[d]-4 || x[d] > b.hi[d]+4) return;
     auto eta1 = [](double qm, double q0, double qp) {
         const double num = fabs(qp - 2.0*q0 + qm);
         const double den = fabs(qp) + 2.0*fabs(q0) + fabs(qm) + 1.0e-40;
         return num/den; };
-    auto magp = [&](int i, int j, int k) { return P[I(F_BME, i, j, k)]; };
+    const int c0 = ((l2+1)*G::A1 + (l1+1))*G::A0 + (l0+1);
     double et[3];
 #pragma unroll
     for (int dd = 0; dd < 3; ++dd)
     {
-        const int m[3] = {x[0]-(dd == 0), x[1]-(dd == 1), x[2]-(dd == 2)};
-        const int p[3] = {x[0]+(dd == 0), x[1]+(dd == 1), x[2]+(dd == 2)};
-        const double er = eta1(P[I(F_WA, m[0], m[1], m[2])], P[I(F_WA, x[0], x[1], x[2])], P[I(F_WA, p[0], p[1], p[2])]);
-        const double ep = eta1(P[I(F_WA+4, m[0], m[1], m[2])], P[I(F_WA+4, x[0], x[1], x[2])],
-                               P[I(F_WA+4, p[0], p[1], p[2])]);
-        const double em = eta1(magp(m[0], m[1], m[2]), magp(x[0], x[1], x[2]), magp(p[0], p[1], p[2]));
+        const int st = dd == 0 ? 1 : dd == 1 ? G::A0 : G::A0*G::A1;
+        const double er = eta1(sR[c0-st], sR[c0], sR[c0+st]);
+        const double ep = eta1(sP[c0-st], sP[c0], sP[c0+st]);
+        const double em = eta1(sB[c0-st], sB[c0], sB[c0+st]);
         et[dd] = refMax(er, refMax(ep, em));
     }
     const double eta = sqrt(et[0]*et[0] + et[1]*et[1] + et[2]*et[2]);
     P[I(F_THR, x[0], x[1], x[2])] = eta < v.etaD ? 1.0 : 0.0;
-}
-
-// 2. Dilated flag on box+5: the minimum over the cell and its six neighbours
-// on box+3 (f_berta_dilate_thetaF, :410-434), 0 in the two outer layers (:500-503).
-__global__ void __launch_bounds__(kB) k_b_theta(O4View v, double* __restrict__ S)
-{
-    const BoxDesc& b = blockBoxDesc(v);
-    int L[3], H[3], x[3];
-    for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]-kG; H[d] = b.hi[d]+kG; }
-    if (!regionPoint(b, L, H, boxThread(v, b), x)) return;
-    const BoxIdx I(b);
-    double* const P = S + b.sOff;
-    bool inner = true;
-#pragma unroll
-    for (int d = 0; d < 3; ++d) inner = inner && x[d] >= b.lo[d]-3 && x[d] <= b.hi[d]+3;
-    double t = 0.0;
-    if (inner)
-    {
-        t = P[I(F_THR, x[0], x[1], x[2])];
-        t = refMin(t, P[I(F_THR, x[0]-1, x[1], x[2])]); t = refMin(t, P[I(F_THR, x[0]+1, x[1], x[2])]);
-        t = refMin(t, P[I(F_THR, x[0], x[1]-1, x[2])]); t = refMin(t, P[I(F_THR, x[0], x[1]+1, x[2])]);
-        t = refMin(t, P[I(F_THR, x[0], x[1], x[2]-1)]); t = refMin(t, P[I(F_THR, x[0], x[1], x[2]+1)]);
-    }
-    P[I(F_THETA, x[0], x[1], x[2])] = t;
 }
 
 // 4. Face points of direction D on the faces [lo_d, hi_d+1] x transverse
@@ -282,7 +265,7 @@ __global__ void __launch_bounds__(kB) k_b_theta(O4View v, double* __restrict__ S
 // the speed). Reconstruction arithmetic in S (the normalization and the
 // rescaling stay FP64).
 template <int D, typename S_, bool ROE>
-__device__ __forceinline__ void bertaFaces(const O4View& v, double* __restrict__ S)
+__device__ __forceinline__ void bertaFaces(const O4View& v, const double* __restrict__ U, double* __restrict__ S)
 {
     const BoxDesc& b = blockBoxDesc(v);
     int L[3], H[3], x[3];
@@ -369,9 +352,10 @@ __device__ __forceinline__ void bertaFaces(const O4View& v, double* __restrict__
     l[4] = refMax(l[4], 1.0e-12); r[4] = refMax(r[4], 1.0e-12);
     if (ROE && !(usablePrimitive(l) && usablePrimitive(r)))
     {
+        // the cell-average donors V(<U>) of the two cells of the face
         double cl[8], cr[8];
-#pragma unroll
-        for (int c = 0; c < 8; ++c) { cl[c] = at(F_WA+c, -1); cr[c] = at(F_WA+c, 0); }
+        bertaCellPrim(v, b, U, x[0]-(D == 0), x[1]-(D == 1), x[2]-(D == 2), cl);
+        bertaCellPrim(v, b, U, x[0], x[1], x[2], cr);
         if (usablePrimitive(cl) && usablePrimitive(cr))
         {
 #pragma unroll
@@ -451,11 +435,15 @@ __device__ __forceinline__ void bertaFaces(const O4View& v, double* __restrict__
 // The FP64 build holds the FP64 Roe solver (2 waves per SIMD); the mixed
 // build defers the FP64 faces and fits 3 waves per SIMD.
 template <int D, typename S_, bool ROE>
-__global__ void __launch_bounds__(kB) k_b_faces(O4View v, double* __restrict__ S) { bertaFaces<D, S_, ROE>(v, S); }
-template <int D, bool ROE>
-__global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(3))) k_b_facesm(O4View v, double* __restrict__ S)
+__global__ void __launch_bounds__(kB) k_b_faces(O4View v, const double* __restrict__ U, double* __restrict__ S)
 {
-    bertaFaces<D, float, ROE>(v, S);
+    bertaFaces<D, S_, ROE>(v, U, S);
+}
+template <int D, bool ROE>
+__global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(3))) k_b_facesm(O4View v, const double* __restrict__ U,
+                                                                                    double* __restrict__ S)
+{
+    bertaFaces<D, float, ROE>(v, U, S);
 }
 
 // 4c. Mixed precision: the point Roe dissipation of the faces k_b_faces left
@@ -544,10 +532,12 @@ struct StagePlan
     }
 };
 
-// 3. Point primitives on box+4 over tiles of 32 x 4 x 2 cells: per component
-// the box's <U> on the tile grown by 1 goes to LDS (the next component's
-// loads in flight meanwhile) and UPoint = <U> - theta Lap<U>/24 is formed
-// there; then V(UPoint) with the pressure clipped at 0 (:696-741, :212-275).
+// 2-3. Dilated flag and point primitives on box+4 over tiles of 32 x 4 x 2
+// cells: the cell's dilated flag from the raw flags of k_b_detect (stored for
+// the face kernels), then per component the box's <U> on the tile grown by 1
+// goes to LDS (the next component's loads in flight meanwhile) and UPoint =
+// <U> - theta Lap<U>/24 is formed there; then V(UPoint) with the pressure
+// clipped at 0 (:696-741, :212-275).
 template <bool FAST>
 __global__ void __launch_bounds__(kFT) k_b_wp(O4View v, const double* __restrict__ U, double* __restrict__ S)
 {
@@ -565,9 +555,25 @@ __global__ void __launch_bounds__(kFT) k_b_wp(O4View v, const double* __restrict
     const StagePlan<G::A0, G::A1, G::A2> plan(org, h, L, H, L, b.us[0], b.us[1]);
     const int l0 = threadIdx.x % G::T0, l1 = (threadIdx.x / G::T0) % G::T1, l2 = threadIdx.x / (G::T0*G::T1);
     const int x[3] = {org[0]+l0, org[1]+l1, org[2]+l2};
-    bool ok = true;
-    for (int d = 0; d < 3; ++d) ok = ok && x[d] >= b.lo[d]-4 && x[d] <= b.hi[d]+4;
-    const bool smooth = ok && P[I(F_THETA, x[0], x[1], x[2])] > 0.5;
+    bool ok = true, inner = true;
+    for (int d = 0; d < 3; ++d)
+    {
+        ok = ok && x[d] >= b.lo[d]-4 && x[d] <= b.hi[d]+4;
+        inner = inner && x[d] >= b.lo[d]-3 && x[d] <= b.hi[d]+3;
+    }
+    // dilated flag (f_berta_dilate_thetaF, :410-434): the minimum of the raw
+    // flag over the cell and its six neighbours on box+3, 0 in the outer
+    // layer of box+4 (:500-503)
+    double th = 0.0;
+    if (ok && inner)
+    {
+        th = P[I(F_THR, x[0], x[1], x[2])];
+        th = refMin(th, P[I(F_THR, x[0]-1, x[1], x[2])]); th = refMin(th, P[I(F_THR, x[0]+1, x[1], x[2])]);
+        th = refMin(th, P[I(F_THR, x[0], x[1]-1, x[2])]); th = refMin(th, P[I(F_THR, x[0], x[1]+1, x[2])]);
+        th = refMin(th, P[I(F_THR, x[0], x[1], x[2]-1)]); th = refMin(th, P[I(F_THR, x[0], x[1], x[2]+1)]);
+    }
+    if (ok) P[I(F_THETA, x[0], x[1], x[2])] = th;
+    const bool smooth = ok && th > 0.5;
     const int c0 = ((l2+1)*G::A1 + (l1+1))*G::A0 + (l0+1);
     constexpr int s1 = G::A0, s2 = G::A0*G::A1;
     double val[StagePlan<G::A0, G::A1, G::A2>::K];
@@ -1147,11 +1153,11 @@ void bertaDirectionPass(const O4View& v, dim3 grid, hipStream_t s, const double*
 {
     if constexpr (std::is_same<S_, float>::value)
     {
-        if (v.roe) hipLaunchKernelGGL((k_b_facesm<D, true>), grid, dim3(kB), 0, s, v, scratch);
-        else hipLaunchKernelGGL((k_b_facesm<D, false>), grid, dim3(kB), 0, s, v, scratch);
+        if (v.roe) hipLaunchKernelGGL((k_b_facesm<D, true>), grid, dim3(kB), 0, s, v, U, scratch);
+        else hipLaunchKernelGGL((k_b_facesm<D, false>), grid, dim3(kB), 0, s, v, U, scratch);
     }
-    else if (v.roe) hipLaunchKernelGGL((k_b_faces<D, S_, true>), grid, dim3(kB), 0, s, v, scratch);
-    else hipLaunchKernelGGL((k_b_faces<D, S_, false>), grid, dim3(kB), 0, s, v, scratch);
+    else if (v.roe) hipLaunchKernelGGL((k_b_faces<D, S_, true>), grid, dim3(kB), 0, s, v, U, scratch);
+    else hipLaunchKernelGGL((k_b_faces<D, S_, false>), grid, dim3(kB), 0, s, v, U, scratch);
     if (v.roe && std::is_same<S_, float>::value) hipLaunchKernelGGL(k_b_roe64<D>, grid, dim3(kB), 0, s, v, scratch);
     if (v.powell) hipLaunchKernelGGL(k_b_pw_face<D>, grid, dim3(kB), 0, s, v, U, scratch);
 
@@ -1164,11 +1170,11 @@ void bertaDirectionPass(const O4View& v, dim3 grid, hipStream_t s, const double*
 template <typename S_>
 void bertaFluxes(const O4View& v, dim3 grid, hipStream_t s, const double* U, double* scratch)
 {
-    hipLaunchKernelGGL(k_b_wbar, grid, dim3(kB), 0, s, v, U, scratch);
-    hipLaunchKernelGGL(k_b_thraw, grid, dim3(kB), 0, s, v, scratch);
-    hipLaunchKernelGGL(k_b_theta, grid, dim3(kB), 0, s, v, scratch);
     if (v.nctiles > 0)
+    {
+        hipLaunchKernelGGL(k_b_detect, dim3(v.nctiles), dim3(kFT), 0, s, v, U, scratch);
         hipLaunchKernelGGL((k_b_wp<std::is_same<S_, float>::value>), dim3(v.nctiles), dim3(kFT), 0, s, v, U, scratch);
+    }
     bertaDirectionPass<0, S_>(v, grid, s, U, scratch);
     bertaDirectionPass<1, S_>(v, grid, s, U, scratch);
     bertaDirectionPass<2, S_>(v, grid, s, U, scratch);
