@@ -714,9 +714,10 @@ __global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(2))) k_
     double* const P = S + b.sOff;
     constexpr int t1 = D == 0 ? 1 : 0, t2 = D == 2 ? 1 : 2;
     // mixed precision: the point fluxes take reciprocal products (FAST); the
-    // point states stay operation for operation with the FP64 path, so every
-    // decision taken on them (the Roe repair, the Roe fan admissibility and
-    // its FP64 fallback below plasma beta 1/2) is the same in both precisions
+    // point states stay operation for operation with the FP64 path, so the Roe
+    // repair decides on the same values in both precisions (the Roe solver's
+    // own decisions are taken on its face-frame and fan states, which carry
+    // the mixed path's reciprocal and FP32 roundings)
     constexpr bool FAST = !std::is_same<S_, double>::value;
     auto by24 = [](double x) { return x/24.0; };
     auto by24f = [](double x) { return FAST ? x*(1.0/24.0) : x/24.0; };
