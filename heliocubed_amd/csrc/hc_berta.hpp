@@ -29,9 +29,13 @@ constexpr int F_BWL = F_PTL, F_BWR = F_PTR; // face points (the Powell products 
 constexpr int F_THF = F_P0L;                // flux-average flag of a face
 constexpr int F_PTOT = F_P0R;               // point total pressure, both sides summed
 constexpr int F_FPS = F_PHL;                // point physical fluxes, both sides summed
-// Marker in F_DPT of a face whose Roe dissipation k_b_roe64 evaluates (a quiet
-// NaN with a payload no arithmetic produces).
-constexpr long long kRoeDeferred = 0x7ffc0de0defe4a11LL;
+// Mixed precision stores the point Roe dissipation in FP32 (8 components of
+// FP32 in the first half of the F_DPT slots): it enters the face flux only as
+// the dissipation term, whose relative rounding the FP32 eigen-system already
+// carries. Marker in its component 0 of a face whose Roe dissipation
+// k_b_roe64 evaluates (a quiet NaN with a payload no arithmetic produces).
+constexpr unsigned kRoeDeferred = 0x7fdefe4au;
+__device__ __forceinline__ float* dissF32(double* P, const BoxIdx& I) { return reinterpret_cast<float*>(P + (long long)F_DPT*I.ecount); }
 
 __device__ __forceinline__ double refMax(double a, double b) { return a < b ? b : a; }
 __device__ __forceinline__ double refMin(double a, double b) { return b < a ? b : a; }
@@ -395,18 +399,29 @@ __device__ __forceinline__ void bertaFaces(const O4View& v, const double* __rest
         {
             // mixed precision: a face whose eigen-system needs FP64 is left
             // to k_b_roe64 with its face points and the marker in F_DPT
-            constexpr bool DEFER = std::is_same<S_, float>::value;
+            constexpr bool MIXED = std::is_same<S_, float>::value;
             double out[8];
-            if (factoredRoe<S_, DEFER>(out, l, r, A, radialPoint(v, b, D, x[0]), D, v.gamma, v.entropy))
+            const bool deferred = factoredRoe<S_, MIXED>(out, l, r, A, radialPoint(v, b, D, x[0]), D, v.gamma, v.entropy);
+            if constexpr (MIXED)
             {
-#pragma unroll
-                for (int c = 0; c < 8; ++c)
+                float* const Df = dissF32(P, I);
+                const int cell = I(0, x[0], x[1], x[2]);
+                if (deferred)
                 {
-                    P[I(F_BWL+c, x[0], x[1], x[2])] = l[c];
-                    P[I(F_BWR+c, x[0], x[1], x[2])] = r[c];
+#pragma unroll
+                    for (int c = 0; c < 8; ++c)
+                    {
+                        P[I(F_BWL+c, x[0], x[1], x[2])] = l[c];
+                        P[I(F_BWR+c, x[0], x[1], x[2])] = r[c];
+                    }
+                    Df[cell] = __uint_as_float(kRoeDeferred);
+                    v.dflag[3*blockBoxIndex(v) + D] = 1u;
                 }
-                P[I(F_DPT, x[0], x[1], x[2])] = __longlong_as_double(kRoeDeferred);
-                v.dflag[3*blockBoxIndex(v) + D] = 1u;
+                else
+                {
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) Df[q*I.ecount + cell] = float(out[q]);
+                }
             }
             else
             {
@@ -463,14 +478,16 @@ __global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(2))) k_
     if (!regionPoint(b, L, H, boxThread(v, b), x)) return;
     const BoxIdx I(b);
     double* const P = S + b.sOff;
-    if (__double_as_longlong(P[I(F_DPT, x[0], x[1], x[2])]) != kRoeDeferred) return;
+    float* const Df = dissF32(P, I);
+    const int cell = I(0, x[0], x[1], x[2]);
+    if (__float_as_uint(Df[cell]) != kRoeDeferred) return;
     const double* A = v.faceA[D] + 9*angIdx(b, x[1], x[2]);
     double wl[8], wr[8], out[8];
 #pragma unroll
     for (int q = 0; q < 8; ++q) { wl[q] = P[I(F_BWL+q, x[0], x[1], x[2])]; wr[q] = P[I(F_BWR+q, x[0], x[1], x[2])]; }
     factoredRoe<float>(out, wl, wr, A, radialPoint(v, b, D, x[0]), D, v.gamma, v.entropy);
 #pragma unroll
-    for (int q = 0; q < 8; ++q) P[I(F_DPT+q, x[0], x[1], x[2])] = out[q];
+    for (int q = 0; q < 8; ++q) Df[q*I.ecount + cell] = float(out[q]);
 }
 
 // 5. Face-average and face-flux kernel of direction D over rectangular tiles
@@ -513,15 +530,17 @@ struct StagePlan
     // offsets in a box's scratch block (BoxIdx layout)
     __device__ StagePlan(const BoxIdx& I, const int org[3], const int h[3], const int lo[3], const int hi[3])
         : StagePlan(org, h, lo, hi, (const int[3]){I.o0, I.o1, I.o2}, I.e0, (long long)I.e0*I.e1) {}
-    // load the field at offset f0 (field index x field stride) into
-    // registers, then store them into LDS
-    __device__ __forceinline__ void load(double (&val)[K], const double* __restrict__ P, long long f0) const
+    // load the field at offset f0 (field index x field stride, in elements)
+    // into registers of its own type, then store them into LDS (converted
+    // there, so that a prefetch does not wait on its loads)
+    template <typename V>
+    __device__ __forceinline__ void load(V (&val)[K], const V* __restrict__ P, long long f0) const
     {
 #pragma unroll
-        for (int k = 0; k < K; ++k) val[k] = off[k] >= 0 ? P[f0 + off[k]] : 0.0;
+        for (int k = 0; k < K; ++k) val[k] = off[k] >= 0 ? P[f0 + off[k]] : V(0);
     }
-    template <typename T>
-    __device__ __forceinline__ void store(T* __restrict__ dst, const double (&val)[K]) const
+    template <typename T, typename V>
+    __device__ __forceinline__ void store(T* __restrict__ dst, const V (&val)[K]) const
     {
 #pragma unroll
         for (int k = 0; k < K; ++k)
@@ -610,7 +629,7 @@ __global__ void __launch_bounds__(kFT) k_b_wp(O4View v, const double* __restrict
     for (int c = 0; c < 8; ++c) P[I(F_WPT+c, x[0], x[1], x[2])] = w[c];
 }
 
-template <int D, bool ROE>
+template <int D, bool ROE, bool F32D>
 __global__ void __launch_bounds__(kFT) k_b_avgflux(O4View v, double* __restrict__ S)
 {
     using G = FluxTile<D>;
@@ -710,7 +729,8 @@ __global__ void __launch_bounds__(kFT) k_b_avgflux(O4View v, double* __restrict_
     auto component = [](int s) { return s == 0 ? -1 : s == 1 ? 0 : s == 2 ? 4 : s <= 5 ? s-2 : s-1; };
     // the next component's point fluxes and dissipation are loaded into
     // registers while the current one is evaluated
-    double nF[StagePlan<G::F0, G::F1, G::F2>::K], nD[StagePlan<G::A0, G::A1, G::A2>::K];
+    double nF[StagePlan<G::F0, G::F1, G::F2>::K];
+    std::conditional_t<F32D, float, double> nD[StagePlan<G::A0, G::A1, G::A2>::K];
     planF.load(nF, P, F_PTOT*I.ecount);
 #pragma unroll 1
     for (int s = 0; s < 9; ++s)
@@ -724,7 +744,8 @@ __global__ void __launch_bounds__(kFT) k_b_avgflux(O4View v, double* __restrict_
         {
             const int cn = component(s+1);
             planF.load(nF, P, (F_FPS+cn)*I.ecount);
-            planA.load(nD, P, (F_DPT+cn)*I.ecount);
+            if constexpr (F32D) planA.load(nD, dissF32(P, I), (long long)cn*I.ecount);
+            else planA.load(nD, P, (F_DPT+cn)*I.ecount);
         }
         // theta-aware face average on the tile grown by 1
 #pragma unroll
@@ -1162,8 +1183,9 @@ void bertaDirectionPass(const O4View& v, dim3 grid, hipStream_t s, const double*
     if (v.powell) hipLaunchKernelGGL(k_b_pw_face<D>, grid, dim3(kB), 0, s, v, U, scratch);
 
     if (v.nftiles[D] == 0) return;
-    if (v.roe) hipLaunchKernelGGL((k_b_avgflux<D, true>), dim3(v.nftiles[D]), dim3(kFT), 0, s, v, scratch);
-    else hipLaunchKernelGGL((k_b_avgflux<D, false>), dim3(v.nftiles[D]), dim3(kFT), 0, s, v, scratch);
+    constexpr bool F32D = std::is_same<S_, float>::value;
+    if (v.roe) hipLaunchKernelGGL((k_b_avgflux<D, true, F32D>), dim3(v.nftiles[D]), dim3(kFT), 0, s, v, scratch);
+    else hipLaunchKernelGGL((k_b_avgflux<D, false, false>), dim3(v.nftiles[D]), dim3(kFT), 0, s, v, scratch);
 }
 
 // Cell stage and the three direction passes of the Berta operator.
