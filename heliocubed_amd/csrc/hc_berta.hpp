@@ -6,8 +6,8 @@
 // Restates the host path of BoxOp_EulerCubedSphere::BertaOperator
 // (src/BoxOp_BertaCubedSphereImplem.H:1737-2076; src/ = HelioCubed reference
 // repository) with the Jameson detector and Roe 8-wave dissipation:
-//   sphAvgToPrim (:278-340) + BertaComputeThetaJameson (:346-507): raw flag on box+4
-//     from rho, p and the native |B|^2/2 of V(<U>) formed in LDS      (k_b_detect)
+//   sphAvgToPrim (:278-340): rho, p and the native |B|^2/2 of V(<U>) on box+5  (k_b_wbar)
+//   BertaComputeThetaJameson (:346-507): raw flag on box+4            (k_b_thraw)
 //   dilated flag on box+3 + BertaAverageToPoint (:696-741) + sphPointToPrim (:212-275):
 //     point W on box+4                                                 (k_b_wp)
 //   per direction:
@@ -22,6 +22,7 @@
 // Max/min follow the reference's std::max / std::min ((a < b) comparisons).
 
 // Scratch slots of the Berta operator (the standard operator's, reused).
+constexpr int F_DRHO = F_WB, F_DP = F_WB+1, F_DBME = F_WB+2;   // rho, p, |B|^2/2 of V(<U>), box+5
 constexpr int F_THR = F_MEB;                // raw Jameson flag, box+4
 constexpr int F_THETA = F_ETA;              // dilated flag, box+4 (0 outside box+3)
 constexpr int F_WPT = F_W;                  // point primitives, box+4
@@ -197,63 +198,57 @@ __device__ __forceinline__ void bertaCellPrim(const O4View& v, const BoxDesc& b,
     if (w[4] < 0) w[4] = 0.0;
 }
 
-// 1. Raw Jameson flag on box+4 (f_berta_jameson_theta_rawF, :346-407): the
-// normalized second difference of rho, p and the native |B|^2/2 of V(<U>) per
-// direction, the largest of the three, the 2-norm over directions, 1 below
-// eta_d. Over the cell tiles of box+4: rho, p and |B|^2/2 of V(<U>) on the
-// tile grown by 1 are formed once per point into LDS (the eight components'
-// loads of a thread's points in flight together), then every cell of the
-// tile takes its second differences from LDS; only the flag is stored.
-// The first tile of a box also clears the box's flags of deferred FP64 Roe
-// faces for the stage's direction passes.
-__global__ void __launch_bounds__(kFT) k_b_detect(O4View v, const double* __restrict__ U, double* __restrict__ S)
+// 0. The Jameson detector's inputs on box+5: rho, p and the native |B|^2/2 of
+// the cell-average primitives V(<U>) (sphAvgToPrim, :278-340). The first
+// thread of a box also clears the box's flags of deferred FP64 Roe faces for
+// the stage's direction passes.
+__global__ void __launch_bounds__(kB) k_b_wbar(O4View v, const double* __restrict__ U, double* __restrict__ S)
 {
-    using G = CellTile;
-    __shared__ double sR[G::NA], sP[G::NA], sB[G::NA];
-    const FluxTileRef tl = v.ctiles[xcdOrder(blockIdx.x, v.nctiles)];
-    const BoxDesc& b = v.boxes[tl.box];
+    const BoxDesc& b = blockBoxDesc(v);
+    const unsigned t = boxThread(v, b);
+    if (t == 0)
+    {
+        const int n = blockBoxIndex(v);
+        v.dflag[3*n] = 0u; v.dflag[3*n+1] = 0u; v.dflag[3*n+2] = 0u;
+    }
+    int L[3], H[3], x[3];
+    for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]-kG; H[d] = b.hi[d]+kG; }
+    if (!regionPoint(b, L, H, t, x)) return;
     const BoxIdx I(b);
     double* const P = S + b.sOff;
-    const int org[3] = {tl.o0, tl.o1, tl.o2};
-    if (threadIdx.x == 0 && org[0] == b.lo[0]-4 && org[1] == b.lo[1]-4 && org[2] == b.lo[2]-4)
-    {
-        v.dflag[3*tl.box] = 0u; v.dflag[3*tl.box+1] = 0u; v.dflag[3*tl.box+2] = 0u;
-    }
-    constexpr int KA = (G::NA + kFT - 1)/kFT;
-#pragma unroll
-    for (int k = 0; k < KA; ++k)
-    {
-        const int p = threadIdx.x + k*kFT;
-        const int a0 = p % G::A0, q = p / G::A0, a1 = q % G::A1, a2 = q / G::A1;
-        const int y0 = org[0]-1+a0, y1 = org[1]-1+a1, y2 = org[2]-1+a2;
-        // V(<U>) is defined on box+5, which holds every neighbour of box+4
-        if (p < G::NA && y0 >= b.lo[0]-kG && y0 <= b.hi[0]+kG && y1 >= b.lo[1]-kG && y1 <= b.hi[1]+kG
-            && y2 >= b.lo[2]-kG && y2 <= b.hi[2]+kG)
-        {
-            double w[8];
-            bertaCellPrim(v, b, U, y0, y1, y2, w);
-            sR[p] = w[0]; sP[p] = w[4];
-            sB[p] = 0.5*(w[5]*w[5] + w[6]*w[6] + w[7]*w[7]);
-        }
-    }
-    __syncthreads();
-    const int l0 = threadIdx.x % G::T0, l1 = (threadIdx.x / G::T0) % G::T1, l2 = threadIdx.x / (G::T0*G::T1);
-    const int x[3] = {org[0]+l0, org[1]+l1, org[2]+l2};
-    for (int d = 0; d < 3; ++d)
-        if (x[d] < b.lo[d]-4 || x[d] > b.hi[d]+4) return;
+    double w[8];
+    bertaCellPrim(v, b, U, x[0], x[1], x[2], w);
+    P[I(F_DRHO, x[0], x[1], x[2])] = w[0];
+    P[I(F_DP, x[0], x[1], x[2])] = w[4];
+    P[I(F_DBME, x[0], x[1], x[2])] = 0.5*(w[5]*w[5] + w[6]*w[6] + w[7]*w[7]);
+}
+
+// 1. Raw Jameson flag on box+4 (f_berta_jameson_theta_rawF, :346-407): the
+// normalized second difference of rho, p and the native |B|^2/2 per
+// direction, the largest of the three, the 2-norm over directions, 1 below
+// eta_d.
+__global__ void __launch_bounds__(kB) k_b_thraw(O4View v, double* __restrict__ S)
+{
+    const BoxDesc& b = blockBoxDesc(v);
+    int L[3], H[3], x[3];
+    for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]-4; H[d] = b.hi[d]+4; }
+    if (!regionPoint(b, L, H, boxThread(v, b), x)) return;
+    const BoxIdx I(b);
+    double* const P = S + b.sOff;
     auto eta1 = [](double qm, double q0, double qp) {
         const double num = fabs(qp - 2.0*q0 + qm);
         const double den = fabs(qp) + 2.0*fabs(q0) + fabs(qm) + 1.0e-40;
         return num/den; };
-    const int c0 = ((l2+1)*G::A1 + (l1+1))*G::A0 + (l0+1);
     double et[3];
 #pragma unroll
     for (int dd = 0; dd < 3; ++dd)
     {
-        const int st = dd == 0 ? 1 : dd == 1 ? G::A0 : G::A0*G::A1;
-        const double er = eta1(sR[c0-st], sR[c0], sR[c0+st]);
-        const double ep = eta1(sP[c0-st], sP[c0], sP[c0+st]);
-        const double em = eta1(sB[c0-st], sB[c0], sB[c0+st]);
+        const int m[3] = {x[0]-(dd == 0), x[1]-(dd == 1), x[2]-(dd == 2)};
+        const int p[3] = {x[0]+(dd == 0), x[1]+(dd == 1), x[2]+(dd == 2)};
+        auto q = [&](int f, const int y[3]) { return P[I(f, y[0], y[1], y[2])]; };
+        const double er = eta1(q(F_DRHO, m), q(F_DRHO, x), q(F_DRHO, p));
+        const double ep = eta1(q(F_DP, m), q(F_DP, x), q(F_DP, p));
+        const double em = eta1(q(F_DBME, m), q(F_DBME, x), q(F_DBME, p));
         et[dd] = refMax(er, refMax(ep, em));
     }
     const double eta = sqrt(et[0]*et[0] + et[1]*et[1] + et[2]*et[2]);
@@ -552,7 +547,7 @@ struct StagePlan
 };
 
 // 2-3. Dilated flag and point primitives on box+4 over tiles of 32 x 4 x 2
-// cells: the cell's dilated flag from the raw flags of k_b_detect (stored for
+// cells: the cell's dilated flag from the raw flags of k_b_thraw (stored for
 // the face kernels), then per component the box's <U> on the tile grown by 1
 // goes to LDS (the next component's loads in flight meanwhile) and UPoint =
 // <U> - theta Lap<U>/24 is formed there; then V(UPoint) with the pressure
@@ -1192,11 +1187,10 @@ void bertaDirectionPass(const O4View& v, dim3 grid, hipStream_t s, const double*
 template <typename S_>
 void bertaFluxes(const O4View& v, dim3 grid, hipStream_t s, const double* U, double* scratch)
 {
+    hipLaunchKernelGGL(k_b_wbar, grid, dim3(kB), 0, s, v, U, scratch);
+    hipLaunchKernelGGL(k_b_thraw, grid, dim3(kB), 0, s, v, scratch);
     if (v.nctiles > 0)
-    {
-        hipLaunchKernelGGL(k_b_detect, dim3(v.nctiles), dim3(kFT), 0, s, v, U, scratch);
         hipLaunchKernelGGL((k_b_wp<std::is_same<S_, float>::value>), dim3(v.nctiles), dim3(kFT), 0, s, v, U, scratch);
-    }
     bertaDirectionPass<0, S_>(v, grid, s, U, scratch);
     bertaDirectionPass<1, S_>(v, grid, s, U, scratch);
     bertaDirectionPass<2, S_>(v, grid, s, U, scratch);
