@@ -291,7 +291,9 @@ __device__ __forceinline__ void bertaFaces(const O4View& v, const double* __rest
 #pragma unroll
         for (int o = -3; o <= 2; ++o) nx[o+3] = at(F_WPT+c, o); };
     load6(0);
-#pragma unroll 1
+    // unrolled: the results land in registers of their own (no indexed
+    // register moves per component)
+#pragma unroll
     for (int c = 0; c < 8; ++c)
     {
         double s6[6];

@@ -85,10 +85,13 @@ struct SeamPair
 // Face tiles of the Berta face-average and flux kernel of direction D: T0 x
 // T1 x T2 faces (x0 unit-stride), the transverse halo of the face averages
 // (1; the point fluxes they are formed from need 2), and the LDS regions.
+// The angular tiles are one face deep along their normal and 32 x 8 across
+// it, so that the averaged region grown by the transverse halo is 1.33x the
+// tile (32 x 2 x 4 tiles: 1.59x).
 template <int D>
 struct FluxTile
 {
-    static constexpr int T0 = D == 0 ? 8 : 32, T1 = D == 0 ? 8 : D == 1 ? 2 : 4, T2 = D == 0 ? 4 : D == 1 ? 4 : 2;
+    static constexpr int T0 = D == 0 ? 8 : 32, T1 = D == 0 ? 8 : D == 1 ? 1 : 8, T2 = D == 0 ? 4 : D == 1 ? 8 : 1;
     static constexpr int h0 = D == 0 ? 0 : 1, h1 = D == 1 ? 0 : 1, h2 = D == 2 ? 0 : 1;
     static constexpr int F0 = T0+4*h0, F1 = T1+4*h1, F2 = T2+4*h2;
     static constexpr int A0 = T0+2*h0, A1 = T1+2*h1, A2 = T2+2*h2;
@@ -1286,8 +1289,8 @@ void buildBoxop(hc_boxop& B, const hc_box* boxes, const int* boxLevel, int nbox)
     v.blockBox = upload(blockBox, B.owned);
     v.dflag = upload(std::vector<unsigned>((size_t)3*nbox, 0u), B.owned);
     // face tiles of each direction, box by box, x0 fastest; then x2 for the
-    // r and alpha directions (whose stencils reach two planes across x2
-    // tiles of 4), x1 for beta
+    // r and alpha directions (whose stencils reach two planes across the
+    // x2 tiles), x1 for beta
     for (int d = 0; d < 3; ++d)
     {
         const int T[3] = {d == 0 ? FluxTile<0>::T0 : d == 1 ? FluxTile<1>::T0 : FluxTile<2>::T0,
