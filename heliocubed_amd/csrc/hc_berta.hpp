@@ -461,30 +461,44 @@ __global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(3))) k_
 // 4c. Mixed precision: the point Roe dissipation of the faces k_b_faces left
 // marked for the FP64 eigen-system (plasma beta below HC_MIXED_FP64_BETA),
 // from the face points it stored there; the faces [lo_d, hi_d+1] x
-// transverse [lo-1, hi+1]. The blocks of a box whose flag the face kernel
-// left clear return at once (k_b_wbar clears the flags at the start of the
+// transverse [lo-1, hi+1]. The pass runs on at most kRoeBlocks blocks. Each
+// block first reads every box's flag of direction D with its threads in
+// parallel and returns when none is set (the common case: the solver's
+// registers leave room for only two resident blocks per CU, so a full-size
+// grid of blocks that each wait on one flag load took 50-60 us). Otherwise
+// it steps through the face kernel's blocks (same box and thread mapping)
+// with a stride of its grid (k_b_wbar clears the flags at the start of the
 // next stage).
+constexpr unsigned kRoeBlocks = 2048;
 template <int D>
 __global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(2))) k_b_roe64(O4View v, double* __restrict__ S)
 {
-    if (v.dflag[3*blockBoxIndex(v) + D] == 0u) return;
-    const BoxDesc& b = blockBoxDesc(v);
-    int L[3], H[3], x[3];
-    for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]-1; H[d] = b.hi[d]+1; }
-    L[D] = b.lo[D]; H[D] = b.hi[D]+1;
-    if (!regionPoint(b, L, H, boxThread(v, b), x)) return;
-    const BoxIdx I(b);
-    double* const P = S + b.sOff;
-    float* const Df = dissF32(P, I);
-    const int cell = I(0, x[0], x[1], x[2]);
-    if (__float_as_uint(Df[cell]) != kRoeDeferred) return;
-    const double* A = v.faceA[D] + 9*angIdx(b, x[1], x[2]);
-    double wl[8], wr[8], out[8];
+    int any = 0;
+    for (int n = threadIdx.x; n < v.nboxes; n += blockDim.x) any |= v.dflag[3*n + D] != 0u;
+    if (!__syncthreads_or(any)) return;
+    for (unsigned hb = blockIdx.x; hb < v.nblocks; hb += gridDim.x)
+    {
+        const unsigned lb = xcdOrder(hb, v.nblocks);
+        const int box = v.blocksPerBox > 0 ? (int)(lb / (unsigned)v.blocksPerBox) : v.blockBox[lb];
+        if (v.dflag[3*box + D] == 0u) continue;
+        const BoxDesc& b = v.boxes[box];
+        int L[3], H[3], x[3];
+        for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]-1; H[d] = b.hi[d]+1; }
+        L[D] = b.lo[D]; H[D] = b.hi[D]+1;
+        if (!regionPoint(b, L, H, (lb - (unsigned)b.block0)*blockDim.x + threadIdx.x, x)) continue;
+        const BoxIdx I(b);
+        double* const P = S + b.sOff;
+        float* const Df = dissF32(P, I);
+        const int cell = I(0, x[0], x[1], x[2]);
+        if (__float_as_uint(Df[cell]) != kRoeDeferred) continue;
+        const double* A = v.faceA[D] + 9*angIdx(b, x[1], x[2]);
+        double wl[8], wr[8], out[8];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) { wl[q] = P[I(F_BWL+q, x[0], x[1], x[2])]; wr[q] = P[I(F_BWR+q, x[0], x[1], x[2])]; }
-    factoredRoe<float>(out, wl, wr, A, radialPoint(v, b, D, x[0]), D, v.gamma, v.entropy);
+        for (int q = 0; q < 8; ++q) { wl[q] = P[I(F_BWL+q, x[0], x[1], x[2])]; wr[q] = P[I(F_BWR+q, x[0], x[1], x[2])]; }
+        factoredRoe<float>(out, wl, wr, A, radialPoint(v, b, D, x[0]), D, v.gamma, v.entropy);
 #pragma unroll
-    for (int q = 0; q < 8; ++q) Df[q*I.ecount + cell] = float(out[q]);
+        for (int q = 0; q < 8; ++q) Df[q*I.ecount + cell] = float(out[q]);
+    }
 }
 
 // 5. Face-average and face-flux kernel of direction D over rectangular tiles
@@ -1176,7 +1190,8 @@ void bertaDirectionPass(const O4View& v, dim3 grid, hipStream_t s, const double*
     }
     else if (v.roe) hipLaunchKernelGGL((k_b_faces<D, S_, true>), grid, dim3(kB), 0, s, v, U, scratch);
     else hipLaunchKernelGGL((k_b_faces<D, S_, false>), grid, dim3(kB), 0, s, v, U, scratch);
-    if (v.roe && std::is_same<S_, float>::value) hipLaunchKernelGGL(k_b_roe64<D>, grid, dim3(kB), 0, s, v, scratch);
+    if (v.roe && std::is_same<S_, float>::value)
+        hipLaunchKernelGGL(k_b_roe64<D>, dim3(grid.x < kRoeBlocks ? grid.x : kRoeBlocks), dim3(kB), 0, s, v, scratch);
     if (v.powell) hipLaunchKernelGGL(k_b_pw_face<D>, grid, dim3(kB), 0, s, v, U, scratch);
 
     if (v.nftiles[D] == 0) return;

@@ -140,6 +140,7 @@ struct O4View
     const FluxTileRef* ctiles;      // cell tiles of box+4 (Berta point primitives)
     unsigned nctiles;
     unsigned* dflag;                // [box][3]: the box has faces of the direction deferred to k_b_roe64
+    int nboxes;                     // boxes of the table (entries of dflag / 3)
 };
 
 // Entry of the angular tables (cellA, faceA, fmet) for cell / face (j, k) of the box's panel and level.
@@ -1288,6 +1289,7 @@ void buildBoxop(hc_boxop& B, const hc_box* boxes, const int* boxLevel, int nbox)
     v.boxes = upload(B.boxes, B.owned);
     v.blockBox = upload(blockBox, B.owned);
     v.dflag = upload(std::vector<unsigned>((size_t)3*nbox, 0u), B.owned);
+    v.nboxes = nbox;
     // face tiles of each direction, box by box, x0 fastest; then x2 for the
     // r and alpha directions (whose stencils reach two planes across the
     // x2 tiles), x1 for beta
