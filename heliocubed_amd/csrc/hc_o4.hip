@@ -365,6 +365,9 @@ __device__ __forceinline__ void roeFrame(const double* Am, int dir, double A[3][
         normal2 += frame[0][i]*frame[0][i];
         tangent2 += frame[1][i]*frame[1][i];
     }
+    // correctly rounded lengths in both precisions: the deferred FP64 faces
+    // of the mixed path rotate their states in this frame, and their fan
+    // admissibility tests can sit at round-off (near-zero thermal energy)
     nm = sqrt(normal2);
     const double tm = sqrt(tangent2);
     if constexpr (FAST)
@@ -446,7 +449,7 @@ __device__ bool factoredRoe(double out[8], const double wl[8], const double wr[8
     const double det = A[0][0]*adj[0][0]+A[1][0]*adj[0][1]+A[2][0]*adj[0][2];
     if constexpr (FAST)
     {
-        const double idet = 1.0/det, idet2 = 1.0/(det*det);
+        const double idet = hc::quotN(1.0, det), idet2 = idet*idet;
         out[0] = mapped[0]*idet; out[4] = mapped[4]*idet;
 #pragma unroll
         for (int d = 0; d < 3; ++d)

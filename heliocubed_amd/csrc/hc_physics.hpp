@@ -173,6 +173,50 @@ HC_HD void rusanovDissipation(double out[8], const double l[8], const double r[8
     for (int c = 0; c < 8; ++c) out[c] = double(speed)*(cons[1][c]-cons[0][c]);
 }
 
+// Mixed precision only: FP64 square root, reciprocal square root and
+// quotient from the FP32 hardware estimate and one FP64 Newton step. The
+// estimate's relative error e (below 2^-22) becomes about e^2/2 + 1e-16, so
+// the results lie within ~1e-13 of the correctly rounded ones. Arguments
+// outside [2^-100, 2^100] (zero, FP32-subnormal or FP32-infinite estimates,
+// non-finite values) take the exact FP64 form.
+HC_HD bool fp32Seedable(double x) { return x >= 7.888609052210118e-31 && x <= 1.2676506002282294e30; }
+HC_HD double rsqrtN(double x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    if (!fp32Seedable(x)) return 1.0/sqrt(x);
+    const double r = double(__builtin_amdgcn_rsqf(float(x)));
+    // r (1 + (1 - x r^2)/2)
+    return r + (0.5*r)*(1.0 - x*r*r);
+#else
+    return 1.0/sqrt(x);
+#endif
+}
+HC_HD double sqrtN(double x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    if (!fp32Seedable(x)) return sqrt(x);
+    const double r = double(__builtin_amdgcn_rsqf(float(x)));
+    const double y = x*r;
+    // y + (r/2) (x - y^2)
+    return y + (0.5*r)*(x - y*y);
+#else
+    return sqrt(x);
+#endif
+}
+HC_HD double quotN(double a, double b)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    const double m = fabs(b);
+    if (!fp32Seedable(m)) return a/b;
+    const double r = double(__builtin_amdgcn_rcpf(float(b)));
+    const double q = a*r;
+    // q + r (a - b q)
+    return q + r*(a - b*q);
+#else
+    return a/b;
+#endif
+}
+
 // Quotient a / b in the eigen-system precision S: an FP64 division, or in
 // FP32 the product with the hardware reciprocal (within 2 ulp, the size of
 // the FP32 roundings the eigen-system already carries).
@@ -211,19 +255,19 @@ HC_HD int roe8WaveDissipation(double out[8], const double WL[8], const double WR
     // mixed precision (S = float): the FP64 divisions by gamma-1 and by the
     // average density become products with their reciprocals
     constexpr bool FAST = !std::is_same<S, double>::value;
-    const double ihgd = FAST ? 1.0/hgd : 0.0;
+    const double ihgd = FAST ? quotN(1.0, hgd) : 0.0;
     auto byH = [&](double x) { return FAST ? x*ihgd : x/hgd; };
     const double rL = WL[0], rR = WR[0];
     const double mL0 = rL*WL[1], mL1 = rL*WL[2], mL2 = rL*WL[3];
     const double mR0 = rR*WR[1], mR1 = rR*WR[2], mR2 = rR*WR[3];
     const double bbL = kInv8Pi*(WL[5]*WL[5]+WL[6]*WL[6]+WL[7]*WL[7]);
     const double keL = 0.5*(mL0*WL[1]+mL1*WL[2]+mL2*WL[3]);
-    const double EL = byH(WL[4])+keL+bbL, a2L = gamma_d*WL[4]/rL;
+    const double EL = byH(WL[4])+keL+bbL, a2L = FAST ? quotN(gamma_d*WL[4], rL) : gamma_d*WL[4]/rL;
     const double bbR = kInv8Pi*(WR[5]*WR[5]+WR[6]*WR[6]+WR[7]*WR[7]);
     const double keR = 0.5*(mR0*WR[1]+mR1*WR[2]+mR2*WR[3]);
-    const double ER = byH(WR[4])+keR+bbR, a2R = gamma_d*WR[4]/rR;
+    const double ER = byH(WR[4])+keR+bbR, a2R = FAST ? quotN(gamma_d*WR[4], rR) : gamma_d*WR[4]/rR;
     const double Rhod = 0.5*(rL+rR);
-    const double iRhod = FAST ? 1.0/Rhod : 0.0;
+    const double iRhod = FAST ? quotN(1.0, Rhod) : 0.0;
     auto byR = [&](double x) { return FAST ? x*iRhod : x/Rhod; };
     const double Ud = 0.5*(WL[1]+WR[1]), Vd = 0.5*(WL[2]+WR[2]), Wd = 0.5*(WL[3]+WR[3]);
     const double BXd = 0.5*(WL[5]+WR[5]), BYd = 0.5*(WL[6]+WR[6]), BZd = 0.5*(WL[7]+WR[7]);
@@ -260,8 +304,12 @@ HC_HD int roe8WaveDissipation(double out[8], const double WL[8], const double WR
     // fast/slow mixing factors cancel catastrophically at low beta); the
     // eigenvector entries, projections and products below are in S.
     const double rs2d = 0.707106781186547524400844362104849039, smallBd = 1.0e-9;
-    const double bbd = byR(kInv4Pi*BX2d), bd = sqrt(bbd);
-    const double cd = sqrt(ccd);
+    // mixed precision: square roots and quotients from FP32 estimates and a
+    // Newton step (within ~1e-13)
+    auto sqrtS = [](double x) { return FAST ? sqrtN(x) : sqrt(x); };
+    auto quotD = [](double a, double b) { return FAST ? quotN(a, b) : a/b; };
+    const double bbd = byR(kInv4Pi*BX2d), bd = sqrtS(bbd);
+    const double cd = sqrtS(ccd);
     double hyyd, hzzd, aafd, aasd, afd, asd;
     if (BYZ2d < smallBd*smallBd)
     {
@@ -271,17 +319,25 @@ HC_HD int roe8WaveDissipation(double out[8], const double WL[8], const double WR
     }
     else
     {
-        const double hyz = sqrt(BYZ2d);
-        hyyd = BYd/hyz; hzzd = BZd/hyz;
+        if constexpr (FAST)
+        {
+            const double ihyz = rsqrtN(BYZ2d);
+            hyyd = BYd*ihyz; hzzd = BZd*ihyz;
+        }
+        else
+        {
+            const double hyz = sqrt(BYZ2d);
+            hyyd = BYd/hyz; hzzd = BZd/hyz;
+        }
         const double szb = ccd+byR(BxBHd+BxBHd);
-        const double hh = sqrt(fmax(0.0, szb*szb-4.0*ccd*bbd));
+        const double hh = sqrtS(fmax(0.0, szb*szb-4.0*ccd*bbd));
         aasd = 0.5*(szb-hh);
         if (aasd <= 0.0) { aasd = 0; asd = 0; }
         else if (aasd >= ccd) { aasd = ccd; asd = cd; }
-        else asd = sqrt(aasd);
+        else asd = sqrtS(aasd);
         aafd = 0.5*(szb+hh);
         if (aafd <= ccd) { aafd = ccd; afd = cd; }
-        else afd = sqrt(aafd);
+        else afd = sqrtS(aafd);
     }
     const double deld = efix_d*(fabs(Ud)+fabs(Vd)+fabs(Wd)+afd);
     const double dfsd = aafd-aasd;
@@ -289,7 +345,7 @@ HC_HD int roe8WaveDissipation(double out[8], const double WL[8], const double WR
     if (dfsd < 1.0e-8) { alfd = 1; alsd = 0; }
     else if ((ccd-aasd) <= 0.0) { alfd = 0; alsd = 1; }
     else if ((aafd-ccd) <= 0.0) { alfd = 1; alsd = 0; }
-    else { alfd = sqrt((ccd-aasd)/dfsd); alsd = sqrt((aafd-ccd)/dfsd); }
+    else { alfd = sqrtS(quotD(ccd-aasd, dfsd)); alsd = sqrtS(quotD(aafd-ccd, dfsd)); }
     const S Rho = S(Rhod), BX = S(BXd);
     const S cc = S(ccd), c = S(cd), hyy = S(hyyd), hzz = S(hzzd), af = S(afd), as = S(asd);
     const S alf = S(alfd), als = S(alsd);
