@@ -519,9 +519,16 @@ __device__ double rusanovSpeedO4(const double L[8], const double R[8], const dou
 __global__ void __launch_bounds__(kB) k_o4_wbar(O4View v, const double* __restrict__ U, double* __restrict__ S)
 {
     const BoxDesc& b = blockBoxDesc(v);
+    const unsigned t = boxThread(v, b);
+    if (t == 0)
+    {
+        // the box's flags of deferred FP64 Roe faces, for this stage's passes
+        const int n = blockBoxIndex(v);
+        v.dflag[3*n] = 0u; v.dflag[3*n+1] = 0u; v.dflag[3*n+2] = 0u;
+    }
     int L[3], H[3], x[3];
     for (int d = 0; d < 3; ++d) { L[d] = b.lo[d]-kG; H[d] = b.hi[d]+kG; }
-    if (!regionPoint(b, L, H, boxThread(v, b), x)) return;
+    if (!regionPoint(b, L, H, t, x)) return;
     const BoxIdx I(b);
     double* const P = S + b.sOff;
     const long long u = b.uOff + (x[2]-L[2])*b.us[1] + (x[1]-L[1])*b.us[0] + (x[0]-L[0]);
@@ -710,8 +717,14 @@ __device__ __forceinline__ void pointFaceRegion(const BoxDesc& b, int grow, int 
 // deconvolution; the sums of both sides' physical fluxes and face-average
 // total pressures, on the point-flux faces (pointFaceRegion). Runs after
 // k_o4_bar.
-template <int D, bool ROE, typename S_>
-__global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(2))) k_o4_faces(O4View v, double* __restrict__ S)
+// Mixed precision (DEFER): a face whose Roe eigen-system must be FP64 (plasma
+// beta below HC_MIXED_FP64_BETA) leaves its two point states in F_PTL / F_PTR
+// (free during a Roe direction pass) and kO4Deferred in F_DPT, and sets its
+// box's flag; k_o4_roe64 evaluates it. A quiet NaN whose payload no
+// arithmetic produces.
+constexpr unsigned long long kO4Deferred = 0x7ffdefe4a0d5e0a4ULL;
+template <int D, bool ROE, typename S_, bool DEFER>
+__device__ __forceinline__ void o4Faces(const O4View& v, double* __restrict__ S)
 {
     const BoxDesc& b = blockBoxDesc(v);
     int L[3], H[3], x[3];
@@ -858,10 +871,30 @@ __global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(2))) k_
         double wl[8], wr[8], out[8];
 #pragma unroll
         for (int q = 0; q < 8; ++q) { wl[q] = lf[q]-by24(lapb[q]); wr[q] = rf[q]-by24(lapb[q]); }
-        factoredRoe<S_>(out, wl, wr, A, radialPoint(v, b, D, x[0]), D, v.gamma, v.entropy);
+        if (factoredRoe<S_, DEFER>(out, wl, wr, A, radialPoint(v, b, D, x[0]), D, v.gamma, v.entropy))
+        {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) { P[I(F_PTL+q, x[0], x[1], x[2])] = wl[q]; P[I(F_PTR+q, x[0], x[1], x[2])] = wr[q]; }
+            P[I(F_DPT, x[0], x[1], x[2])] = __longlong_as_double((long long)kO4Deferred);
+            v.dflag[3*blockBoxIndex(v) + D] = 1u;
+            return;
+        }
 #pragma unroll
         for (int q = 0; q < 8; ++q) P[I(F_DPT+q, x[0], x[1], x[2])] = out[q];
     }
+}
+
+// The FP64 build and Rusanov hold their solvers in the face kernel (2 waves
+// per SIMD); the mixed Roe build defers the FP64 faces and fits 3 waves.
+template <int D, bool ROE, typename S_>
+__global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(2))) k_o4_faces(O4View v, double* __restrict__ S)
+{
+    o4Faces<D, ROE, S_, false>(v, S);
+}
+template <int D>
+__global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(3))) k_o4_facesm(O4View v, double* __restrict__ S)
+{
+    o4Faces<D, true, float, true>(v, S);
 }
 
 // Energies (ke, me, v.B) and the physical flux of the face-average state at
@@ -1349,12 +1382,51 @@ void buildBoxop(hc_boxop& B, const hc_box* boxes, const int* boxLevel, int nbox)
 
 #include "hc_berta.hpp"
 
+// Mixed precision: the point Roe dissipation of the faces k_o4_facesm left
+// marked for the FP64 eigen-system, from the point states it stored; the
+// point-flux faces of the box. At most kRoeBlocks blocks, each returning
+// when no box's flag of the direction is set (k_o4_wbar clears the flags
+// at the start of each stage), else stepping through the face kernel's
+// blocks with a stride of its grid.
+template <int D>
+__global__ void __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(2))) k_o4_roe64(O4View v, double* __restrict__ S)
+{
+    int any = 0;
+    for (int n = threadIdx.x; n < v.nboxes; n += blockDim.x) any |= v.dflag[3*n + D] != 0u;
+    if (!__syncthreads_or(any)) return;
+    for (unsigned hb = blockIdx.x; hb < v.nblocks; hb += gridDim.x)
+    {
+        const unsigned lb = xcdOrder(hb, v.nblocks);
+        const int box = v.blocksPerBox > 0 ? (int)(lb / (unsigned)v.blocksPerBox) : v.blockBox[lb];
+        if (v.dflag[3*box + D] == 0u) continue;
+        const BoxDesc& b = v.boxes[box];
+        int L[3], H[3], x[3];
+        pointFaceRegion<D>(b, 0, L, H);
+        if (!regionPoint(b, L, H, (lb - (unsigned)b.block0)*blockDim.x + threadIdx.x, x)) continue;
+        const BoxIdx I(b);
+        double* const P = S + b.sOff;
+        if ((unsigned long long)__double_as_longlong(P[I(F_DPT, x[0], x[1], x[2])]) != kO4Deferred) continue;
+        const double* A = v.faceA[D] + 9*angIdx(b, x[1], x[2]);
+        double wl[8], wr[8], out[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) { wl[q] = P[I(F_PTL+q, x[0], x[1], x[2])]; wr[q] = P[I(F_PTR+q, x[0], x[1], x[2])]; }
+        factoredRoe<float>(out, wl, wr, A, radialPoint(v, b, D, x[0]), D, v.gamma, v.entropy);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) P[I(F_DPT+q, x[0], x[1], x[2])] = out[q];
+    }
+}
+
 template <int D, typename S_>
 void directionPass(const O4View& v, dim3 grid, hipStream_t s, double* scratch)
 {
     hipLaunchKernelGGL(k_o4_bar<D>, grid, dim3(kB), 0, s, v, scratch);
-    if (v.roe) hipLaunchKernelGGL((k_o4_faces<D, true, S_>), grid, dim3(kB), 0, s, v, scratch);
-    else hipLaunchKernelGGL((k_o4_faces<D, false, double>), grid, dim3(kB), 0, s, v, scratch);
+    if (!v.roe) hipLaunchKernelGGL((k_o4_faces<D, false, double>), grid, dim3(kB), 0, s, v, scratch);
+    else if constexpr (std::is_same<S_, float>::value)
+    {
+        hipLaunchKernelGGL(k_o4_facesm<D>, grid, dim3(kB), 0, s, v, scratch);
+        hipLaunchKernelGGL(k_o4_roe64<D>, dim3(grid.x < kRoeBlocks ? grid.x : kRoeBlocks), dim3(kB), 0, s, v, scratch);
+    }
+    else hipLaunchKernelGGL((k_o4_faces<D, true, S_>), grid, dim3(kB), 0, s, v, scratch);
     if (!v.roe) hipLaunchKernelGGL(k_o4_rusq<D>, grid, dim3(kB), 0, s, v, scratch);
     if (v.nftiles[D] == 0) return;
     if (v.roe) hipLaunchKernelGGL((k_o4_fluxt<D, true>), dim3(v.nftiles[D]), dim3(kFT), 0, s, v, scratch);
