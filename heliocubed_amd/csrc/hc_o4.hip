@@ -101,6 +101,10 @@ constexpr int kFT = 256;   // threads of a flux tile
 constexpr int kFPT = 1;    // faces per thread
 static_assert(FluxTile<0>::NT == kFT*kFPT && FluxTile<1>::NT == kFT*kFPT && FluxTile<2>::NT == kFT*kFPT,
               "a flux tile holds kFPT faces per thread");
+// Only one face per thread is validated: with two (8x8x8 and 32x1x16 tiles)
+// the low-beta cone's mixed RK4 increment was off by O(1), and the kernels
+// were slower (2 waves per SIMD at 178-180 VGPRs).
+static_assert(kFPT == 1, "the face-average and flux kernels are validated with one face per thread");
 
 // Cell tiles of the Berta point-primitive kernel: 32 x 4 x 2 cells of box+4,
 // the box's <U> staged on the tile grown by 1.
