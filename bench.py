@@ -264,6 +264,31 @@ def bench_level(args, wl):
     cells = 6 * wl["n_rad"] * wl["n_ang"] ** 2
     precisions = ["fp64", "mixed"] if args.precision == "both" else [args.precision]
 
+    if args.drift_steps > 0:
+        # state drift over many RK4 steps: mixed against FP64, and FP64 from
+        # an input perturbed by a relative 1e-9 (the size of one mixed step's
+        # state error) against FP64, from the same initial state
+        import numpy as np
+        n = args.drift_steps
+        runs = {}
+        rng = np.random.default_rng(20261203)
+        pert = torch.from_numpy(1 + 1e-9 * (2 * rng.random(ju.shape) - 1)).to(dev)
+        for name, o, U in (("fp64", o64, U0.clone()), ("mixed", omix, U0.clone()),
+                           ("fp64_perturbed", o64, U0 * pert)):
+            L = hc.Level(shell, o)
+            L.rk4_advance(U, G, dt, n)
+            torch.cuda.synchronize()
+            if not torch.isfinite(U).all():
+                raise RuntimeError(f"non-finite state after {n} steps ({name})")
+            runs[name] = U
+            del L
+        out = {"workload": args.workload, "steps": n, "dt": dt, "cells": cells}
+        for name in ("mixed", "fp64_perturbed"):
+            linf, l2 = rel_err(runs[name], runs["fp64"])
+            out[f"{name}_vs_fp64"] = {"rel_linf": linf, "rel_l2": l2, "max_rel_linf": max(linf), "max_rel_l2": max(l2)}
+        print(json.dumps({"drift": out}), flush=True)
+        return
+
     res = {}
     for prec in precisions:
         L = hc.Level(shell, o64 if prec == "fp64" else omix)
@@ -691,6 +716,9 @@ def main():
     ap.add_argument("--cpu-boxes", type=int, default=0, help="boxes in the CPU sample (0: two per thread)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the one-call-per-box FP64 measurement")
+    ap.add_argument("--drift-steps", type=int, default=0,
+                    help="instead of timing: the state after this many RK4 steps, mixed and FP64 from an input "
+                         "perturbed by 1e-9, each against FP64 (per-variable relative L-inf and L2)")
     ap.add_argument("--only-timed", action="store_true",
                     help="run the warmup and timed steps only: the command profiled under rocprofv3")
     args = ap.parse_args()
